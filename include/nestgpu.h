@@ -1,0 +1,100 @@
+/*
+ * nestgpu.h -- C ABI of the MI355X point-neuron simulation step.
+ *
+ * The reference exposes its simulation kernel to Python through a Cython binding
+ * (pynest/nestkernel_api.pyx) over the C++ API in nestkernel/nest.h. This header is the
+ * drop-in counterpart of that binding surface: the same calls (create, connect,
+ * connect_arrays, simulate, prepare/run/cleanup, kernel status, node status,
+ * GetConnections, spike_recorder events) with dictionaries passed as JSON text, plain
+ * pointers and sizes, integer status returns (0 = success) and the error message of the
+ * last failing call readable through nestgpu_last_error(). Node ids are NEST's 1-based
+ * global ids. No exception crosses this boundary. One host thread per kernel handle;
+ * several handles may share a device.
+ */
+#ifndef NESTGPU_H
+#define NESTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nestgpu_kernel nestgpu_kernel;
+
+/* nest::init_nest + ResetKernel (nestkernel/nest.cpp:54; pynest nestkernel_api.pyx llapi_reset_kernel).
+ * Binds the kernel to a HIP device; device memory is allocated at the first Prepare. */
+int nestgpu_kernel_new( int device, nestgpu_kernel** out );
+void nestgpu_kernel_free( nestgpu_kernel* k );
+/* Message of the last failing call on k (or of the last failing call without a kernel when k is NULL). */
+const char* nestgpu_last_error( const nestgpu_kernel* k );
+/* nest::reset_kernel (nestkernel/nest.cpp:62). */
+int nestgpu_reset_kernel( nestgpu_kernel* k );
+
+/* nest::set_kernel_status / get_kernel_status (nestkernel/nest.cpp:118-131):
+ * keys resolution, rng_seed, total_num_virtual_procs, local_num_threads, rng_type. */
+int nestgpu_set_kernel_status( nestgpu_kernel* k, const char* json );
+/* Writes the status dictionary as JSON into buf (NUL-terminated); *len receives the full length. */
+int nestgpu_get_kernel_status( nestgpu_kernel* k, char* buf, size_t cap, size_t* len );
+
+/* nest::create (nestkernel/nest.cpp:362): models iaf_psc_delta, iaf_psc_exp, aeif_cond_exp,
+ * spike_generator, spike_recorder. *first_id receives the id of the first new node. */
+int nestgpu_create( nestgpu_kernel* k, const char* model, int64_t n, const char* params_json, int64_t* first_id );
+/* nest::set_node_status with one dictionary for every node (NodeCollection.set(dict)). */
+int nestgpu_set_status( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* params_json );
+/* NodeCollection.set(key=array): one value per node. */
+int nestgpu_set_values( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* key, const double* values );
+/* NodeCollection.get(key): one value per node (state is read back from the device after Simulate). */
+int nestgpu_get_values( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* key, double* out );
+
+/* nest::connect (nestkernel/nest.cpp:424): conn_spec rules one_to_one, all_to_all,
+ * fixed_indegree, pairwise_bernoulli; syn_spec static_synapse with weight/delay numbers or
+ * parameter objects ({"normal": {"mean", "std"}}, {"uniform": {"min", "max"}},
+ * {"uniform_int": {"max"}}, {"sum": [a, b]}, {"product": [a, b]}, {"max": [a, v]},
+ * {"min": [a, v]}, {"redraw": [a, min, max]}). Random draws follow NEST's per-VP streams. */
+int nestgpu_connect( nestgpu_kernel* k, const int64_t* pre, int64_t n_pre, const int64_t* post, int64_t n_post,
+  const char* conn_spec_json, const char* syn_spec_json );
+/* nest::connect_arrays (nestkernel/nest.cpp:454); weights/delays may be NULL (defaults). */
+int nestgpu_connect_arrays( nestgpu_kernel* k, const int64_t* sources, const int64_t* targets, const double* weights,
+  const double* delays_ms, int64_t n );
+/* spike_generator input as dense multiplicities: counts[s * n + i] spikes of generator
+ * first_id + i with stamp (first_step + s + 1) * resolution. Equivalent to spike_times and
+ * spike_multiplicities of models/spike_generator.cpp:85-140 for on-grid times. */
+int nestgpu_set_spike_counts( nestgpu_kernel* k, int64_t first_id, int64_t n, int64_t first_step, int64_t n_steps,
+  const uint8_t* counts );
+
+/* GetConnections: per VP in creation order, then spike_generator connections. */
+int64_t nestgpu_num_connections( nestgpu_kernel* k );
+int nestgpu_get_connections( nestgpu_kernel* k, int64_t* sources, int64_t* targets, double* weights,
+  int64_t* delay_steps, int32_t* vps );
+
+/* nest::prepare / run / cleanup / simulate (nestkernel/nest.cpp:499-534). */
+int nestgpu_prepare( nestgpu_kernel* k );
+int nestgpu_run( nestgpu_kernel* k, double t_ms );
+int nestgpu_cleanup( nestgpu_kernel* k );
+int nestgpu_simulate( nestgpu_kernel* k, double t_ms );
+
+/* spike_recorder events (models/spike_recorder.cpp:100-112), sorted by (time, sender).
+ * Call with senders == NULL to get the count in *n. */
+int nestgpu_recorder_events( nestgpu_kernel* k, int64_t recorder_id, int64_t* senders, double* times, int64_t cap,
+  int64_t* n );
+
+/* Batched form: advances several prepared kernels on one device together, one launch per
+ * phase per min_delay slice for all of them (no counterpart in the reference, which runs
+ * one kernel per process). t_ms[i] is the simulation time of kernel i. */
+int nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms );
+
+/* Seeded Poisson drive counts, the input the spike_generator option poisson_rate produces:
+ * out[s * n + i] for streams stream0 + i and emission steps step0 + s, lambda per step. */
+int nestgpu_poisson_counts( uint64_t seed, double lambda, uint64_t stream0, int64_t n, int64_t step0, int64_t n_steps,
+  uint8_t* out );
+
+/* Number of visible HIP devices (0 without a GPU). */
+int nestgpu_device_count( int* n );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NESTGPU_H */

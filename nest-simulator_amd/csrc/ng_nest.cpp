@@ -1,0 +1,1664 @@
+// ng_nest.cpp -- host kernel: NEST's Create/Connect/Simulate surface over the device engine.
+#include "ng_nest.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <thread>
+
+namespace nestgpu
+{
+
+// ------------------------------------------------------------------ time base
+void
+TimeBase::set_resolution( double ms )
+{
+  // Time::set_resolution (nest_time.cpp:88-103)
+  tics_per_step = static_cast< long >( std::floor( tics_per_ms * ms + 0.5 ) );
+  tics_per_step_inv = 1. / static_cast< double >( tics_per_step );
+  tics_per_step_rnd = tics_per_step - 1;
+  ms_per_step = tics_per_step / tics_per_ms;
+  steps_per_ms = 1 / ms_per_step;
+}
+
+long
+TimeBase::delay_ms_to_steps( double ms ) const
+{
+  // Time::delay_ms_to_steps = ld_round( ms * STEPS_PER_MS ) (nest_time.h:539-542, numerics.cpp:41-44)
+  return static_cast< long >( std::floor( ms * steps_per_ms + 0.5 ) );
+}
+
+// ------------------------------------------------------------------ RNG
+SeedSeqFe128::SeedSeqFe128( const std::vector< uint32_t >& entropy )
+{
+  // hash each entropy word into the mixer, cross-mix all ordered pairs, then fold in any
+  // remaining words (randutils seed_seq_fe::mix_entropy, count = 4, one round)
+  uint32_t hash_const = 0x43b0d7e5u;
+  auto hash = [ &hash_const ]( uint32_t value )
+  {
+    value ^= hash_const;
+    hash_const *= 0x931e8875u;
+    value *= hash_const;
+    value ^= value >> 16;
+    return value;
+  };
+  auto mix = []( uint32_t x, uint32_t y )
+  {
+    uint32_t result = 0xca01f9ddu * x - 0x4973f715u * y;
+    result ^= result >> 16;
+    return result;
+  };
+  size_t next = 0;
+  for ( uint32_t& e : mixer )
+    e = hash( next < entropy.size() ? entropy[ next++ ] : 0u );
+  for ( int a = 0; a < 4; ++a )
+    for ( int b = 0; b < 4; ++b )
+      if ( a != b )
+        mixer[ b ] = mix( mixer[ b ], hash( mixer[ a ] ) );
+  for ( ; next < entropy.size(); ++next )
+    for ( uint32_t& e : mixer )
+      e = mix( e, hash( entropy[ next ] ) );
+}
+
+VpRng::VpRng( const std::vector< uint32_t >& seed )
+{
+  SeedSeqFe128 sseq( seed );
+  engine.seed( sseq );
+}
+
+// ------------------------------------------------------------------ parameters
+std::unique_ptr< Param >
+Param::from_json( const Json& j, int n_vp )
+{
+  auto p = std::make_unique< Param >();
+  if ( j.kind == Json::NUM or j.kind == Json::BOOL )
+  {
+    p->op_ = CONST;
+    p->a_ = j.number();
+    return p;
+  }
+  if ( j.kind != Json::OBJ or j.obj.size() != 1 )
+    throw std::runtime_error( "BadProperty: a parameter is a number or a one-key object" );
+  const std::string& k = j.obj[ 0 ].first;
+  const Json& v = j.obj[ 0 ].second;
+  if ( k == "uniform" )
+  {
+    // UniformParameter (parameter.h:180-221)
+    p->op_ = UNIFORM;
+    p->a_ = v.has( "min" ) ? v.at( "min" ).number() : 0.0;
+    double mx = v.has( "max" ) ? v.at( "max" ).number() : 1.0;
+    if ( p->a_ >= mx )
+      throw std::runtime_error( "BadProperty: nest::UniformParameter: min < max required." );
+    p->b_ = mx - p->a_;
+  }
+  else if ( k == "uniform_int" )
+  {
+    p->op_ = UNIFORM_INT;
+    p->a_ = v.has( "max" ) ? v.at( "max" ).number() : 1.0;
+    if ( p->a_ <= 0 )
+      throw std::runtime_error( "BadProperty: nest::UniformIntParameter: max > 0 required." );
+  }
+  else if ( k == "normal" )
+  {
+    p->op_ = NORMAL;
+    p->a_ = v.has( "mean" ) ? v.at( "mean" ).number() : 0.0;
+    p->b_ = v.has( "std" ) ? v.at( "std" ).number() : 1.0;
+    if ( p->b_ < 0 )
+      throw std::runtime_error( "BadProperty: nest::NormalParameter: std >= 0 required." );
+    for ( int i = 0; i < n_vp; ++i )
+      p->normals_.emplace_back( p->a_, p->b_ );
+  }
+  else if ( k == "sum" or k == "product" )
+  {
+    if ( v.kind != Json::ARR or v.arr.size() != 2 )
+      throw std::runtime_error( "BadProperty: sum/product take two operands" );
+    p->op_ = k == "sum" ? SUM : PRODUCT;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->r_ = from_json( v.arr[ 1 ], n_vp );
+  }
+  else if ( k == "max" or k == "min" )
+  {
+    if ( v.kind != Json::ARR or v.arr.size() != 2 )
+      throw std::runtime_error( "BadProperty: max/min take a parameter and a value" );
+    p->op_ = k == "max" ? MAX : MIN;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->a_ = v.arr[ 1 ].number();
+  }
+  else if ( k == "redraw" )
+  {
+    if ( v.kind != Json::ARR or v.arr.size() != 3 )
+      throw std::runtime_error( "BadProperty: redraw takes a parameter, min and max" );
+    p->op_ = REDRAW;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->a_ = v.arr[ 1 ].number();
+    p->b_ = v.arr[ 2 ].number();
+  }
+  else
+    throw std::runtime_error( "BadProperty: unknown parameter type '" + k + "'" );
+  return p;
+}
+
+double
+Param::value( VpRng& rng, int vp )
+{
+  switch ( op_ )
+  {
+  case CONST:
+    return a_;
+  case UNIFORM:
+    return a_ + rng.drand() * b_;
+  case UNIFORM_INT:
+    return static_cast< double >( rng.ulrand( static_cast< unsigned long >( a_ ) ) );
+  case NORMAL:
+    if ( b_ == 0 )
+      return a_;
+    return normals_[ vp ]( rng.engine );
+  case SUM:
+  {
+    const double x = l_->value( rng, vp );
+    return x + r_->value( rng, vp );
+  }
+  case PRODUCT:
+  {
+    const double x = l_->value( rng, vp );
+    return x * r_->value( rng, vp );
+  }
+  case MAX:
+    return std::max( l_->value( rng, vp ), a_ );
+  case MIN:
+    return std::min( l_->value( rng, vp ), a_ );
+  case REDRAW:
+    for ( int i = 0; i < 1000; ++i )
+    {
+      const double x = l_->value( rng, vp );
+      if ( a_ <= x and x <= b_ )
+        return x;
+    }
+    throw std::runtime_error( "KernelException: Number of redraws exceeded limit." );
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ Poisson drive
+PoissonTable
+make_poisson_table( double lambda, uint64_t seed )
+{
+  PoissonTable t {};
+  t.seed = seed;
+  double p = std::exp( -lambda );
+  double c = 0;
+  int k = 0;
+  for ( ; k < 63; ++k )
+  {
+    c += p;
+    t.cdf[ k ] = c;
+    if ( 1.0 - c < 1e-17 )
+      break;
+    p *= lambda / ( k + 1 );
+  }
+  t.kmax = k;
+  for ( int q = k + 1; q < 64; ++q )
+    t.cdf[ q ] = 2.0;
+  if ( lambda > 20.0 )
+    throw std::runtime_error( "BadProperty: Poisson drive rate per step must be <= 20" );
+  return t;
+}
+
+static inline uint32_t
+poisson_count_host( const PoissonTable& T, uint64_t stream, int64_t step )
+{
+  uint64_t z = T.seed + stream * 0x9E3779B97F4A7C15ull + static_cast< uint64_t >( step ) * 0xD1B54A32D192ED03ull;
+  z = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+  z = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
+  z = z ^ ( z >> 31 );
+  const double u = static_cast< double >( z >> 11 ) * ( 1.0 / 9007199254740992.0 );
+  uint32_t k = 0;
+  while ( static_cast< int32_t >( k ) < T.kmax and u >= T.cdf[ k ] )
+    ++k;
+  return k;
+}
+
+void
+poisson_counts( uint64_t seed, double lambda, uint64_t stream0, int64_t n, int64_t step0, int64_t n_steps, uint8_t* out )
+{
+  const PoissonTable T = make_poisson_table( lambda, seed );
+  const int nt = static_cast< int >( std::max< int64_t >( 1, std::min< int64_t >( 16, n_steps / 64 ) ) );
+  std::vector< std::thread > th;
+  for ( int t = 0; t < nt; ++t )
+    th.emplace_back(
+      [ & ]( int tid )
+      {
+        for ( int64_t s = tid; s < n_steps; s += nt )
+          for ( int64_t i = 0; i < n; ++i )
+            out[ s * n + i ] = static_cast< uint8_t >( poisson_count_host( T, stream0 + i, step0 + s ) );
+      },
+      t );
+  for ( auto& x : th )
+    x.join();
+}
+
+// ------------------------------------------------------------------ kernel
+Kernel::Kernel( int device )
+  : device_( device )
+{
+  reset();
+}
+
+Kernel::~Kernel() = default;
+
+void
+Kernel::reset()
+{
+  tb_ = TimeBase();
+  n_vp_ = 1;
+  seed_ = 143202461;
+  rngs_.clear();
+  rng_dirty_ = true;
+  nodes_.clear();
+  params_.clear();
+  state_.clear();
+  neuron_gid_.clear();
+  gens_.clear();
+  gen_gid_.clear();
+  dense_.clear();
+  recorders_.clear();
+  rec_gid_.clear();
+  conns_.assign( 1, ConnList() );
+  dev_conns_.clear();
+  d_min_seen_ = -1;
+  d_max_seen_ = 1;
+  min_delay_ = max_delay_ = 1;
+  simulated_ = prepared_ = false;
+  store_dirty_ = true;
+  state_dirty_ = false;
+  engine_.reset();
+  gid_to_dev_.clear();
+  dev_to_gid_.clear();
+  spike_sender_.clear();
+  spike_step_.clear();
+  spike_count_ = 0;
+}
+
+void
+Kernel::init_rngs()
+{
+  // RandomManager::initialize (random_manager.cpp:63-100): vp-specific engines seeded with
+  // seed_seq_fe128{ base_seed, THREAD_SPECIFIC_SEEDER_, vp }
+  rngs_.clear();
+  for ( int vp = 0; vp < n_vp_; ++vp )
+    rngs_.emplace_back( std::vector< uint32_t > { seed_, 0xb84c9baeu, static_cast< uint32_t >( vp ) } );
+  rng_dirty_ = false;
+}
+
+void
+Kernel::set_kernel_status( const Json& d )
+{
+  for ( const auto& kv : d.obj )
+  {
+    const std::string& k = kv.first;
+    if ( k == "resolution" )
+    {
+      if ( not nodes_.empty() )
+        throw std::runtime_error( "KernelException: Cannot change resolution after nodes have been created." );
+      const double r = kv.second.number();
+      if ( r <= 0 )
+        throw std::runtime_error( "BadProperty: resolution must be greater than zero." );
+      tb_.set_resolution( r );
+    }
+    else if ( k == "rng_seed" )
+    {
+      const double s = kv.second.number();
+      if ( s < 1 or s > 4294967295.0 )
+        throw std::runtime_error( "BadProperty: RNG seed must be in (0, 2^32-1]." );
+      seed_ = static_cast< uint32_t >( s );
+      rng_dirty_ = true;
+    }
+    else if ( k == "total_num_virtual_procs" or k == "local_num_threads" )
+    {
+      if ( not nodes_.empty() )
+        throw std::runtime_error( "KernelException: Nodes exist: number of threads cannot be changed." );
+      const int n = static_cast< int >( kv.second.number() );
+      if ( n < 1 )
+        throw std::runtime_error( "BadProperty: number of virtual processes must be positive." );
+      n_vp_ = n;
+      conns_.assign( n_vp_, ConnList() );
+      rng_dirty_ = true;
+    }
+    else if ( k == "slices_per_chunk" )
+    {
+      slices_per_chunk_ = std::max( 1, static_cast< int >( kv.second.number() ) );
+      own_runner_.reset();
+    }
+    else if ( k == "rng_type" )
+    {
+      if ( kv.second.string() != "mt19937_64" )
+        throw std::runtime_error( "BadProperty: only rng_type mt19937_64 is supported." );
+    }
+    else if ( k == "print_time" or k == "overwrite_files" or k == "data_path" )
+      continue;
+    else
+      throw std::runtime_error( "UnaccessedDictionaryEntry: kernel status key '" + k + "'" );
+  }
+}
+
+std::string
+Kernel::get_kernel_status_json() const
+{
+  std::ostringstream o;
+  o.precision( 17 );
+  const int64_t clk = engine_ ? engine_->clock : 0;
+  const int32_t fr = engine_ ? engine_->from_step : 0;
+  o << "{\"resolution\": " << tb_.resolution_ms() << ", \"tics_per_step\": " << tb_.tics_per_step
+    << ", \"rng_seed\": " << seed_ << ", \"total_num_virtual_procs\": " << n_vp_
+    << ", \"local_num_threads\": " << n_vp_ << ", \"min_delay\": " << min_delay_ * tb_.ms_per_step
+    << ", \"max_delay\": " << max_delay_ * tb_.ms_per_step << ", \"min_delay_steps\": " << min_delay_
+    << ", \"max_delay_steps\": " << max_delay_ << ", \"num_connections\": " << num_connections()
+    << ", \"network_size\": " << nodes_.size() << ", \"biological_time\": " << ( clk + fr ) * tb_.ms_per_step
+    << ", \"clock_steps\": " << clk << ", \"from_step\": " << fr << ", \"local_spike_counter\": " << spike_count_
+    << ", \"time_simulate_kernel_ms\": " << last_kernel_ms_ << ", \"time_simulate_ms\": " << last_wall_ms_
+    << ", \"device_bytes\": " << ( engine_ ? engine_->device_bytes() : 0 ) << "}";
+  return o.str();
+}
+
+Kernel::NodeRec&
+Kernel::node( int64_t gid )
+{
+  if ( gid < 1 or gid > static_cast< int64_t >( nodes_.size() ) )
+    throw std::runtime_error( "UnknownNode: node id " + std::to_string( gid ) );
+  return nodes_[ gid - 1 ];
+}
+
+const Kernel::NodeRec&
+Kernel::node( int64_t gid ) const
+{
+  if ( gid < 1 or gid > static_cast< int64_t >( nodes_.size() ) )
+    throw std::runtime_error( "UnknownNode: node id " + std::to_string( gid ) );
+  return nodes_[ gid - 1 ];
+}
+
+std::string
+Kernel::model_name( int64_t gid ) const
+{
+  const NodeRec& r = node( gid );
+  if ( r.kind == 1 )
+    return "spike_generator";
+  if ( r.kind == 2 )
+    return "spike_recorder";
+  static const char* names[] = { "iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp" };
+  return names[ params_[ r.idx ].model ];
+}
+
+NeuronParams
+Kernel::default_params( int model )
+{
+  NeuronParams p {};
+  p.model = model;
+  if ( model == M_IAF_PSC_DELTA )  // models/iaf_psc_delta.cpp:69-88
+  {
+    p.tau_m = 10.0;
+    p.c_m = 250.0;
+    p.t_ref = 2.0;
+    p.E_L = -70.0;
+    p.I_e = 0.0;
+    p.V_th = -55.0 - p.E_L;
+    p.V_min = -std::numeric_limits< double >::max();
+    p.V_reset = -70.0 - p.E_L;
+  }
+  else if ( model == M_IAF_PSC_EXP )  // models/iaf_psc_exp.cpp:71-94
+  {
+    p.tau_m = 10.0;
+    p.c_m = 250.0;
+    p.t_ref = 2.0;
+    p.E_L = -70.0;
+    p.I_e = 0.0;
+    p.V_th = -55.0 - p.E_L;
+    p.V_reset = -70.0 - p.E_L;
+    p.tau_ex = 2.0;
+    p.tau_in = 2.0;
+    p.rho = 0.01;
+    p.delta = 0.0;
+  }
+  else  // models/aeif_cond_exp.cpp:132-151
+  {
+    p.V_peak = 0.0;
+    p.V_reset = -60.0;
+    p.t_ref = 0.0;
+    p.g_L = 30.0;
+    p.c_m = 281.0;
+    p.E_ex = 0.0;
+    p.E_in = -85.0;
+    p.E_L = -70.6;
+    p.Delta_T = 2.0;
+    p.tau_w = 144.0;
+    p.a = 4.0;
+    p.b = 80.5;
+    p.V_th = -50.4;
+    p.tau_ex = 0.2;
+    p.tau_in = 2.0;
+    p.I_e = 0.0;
+    p.tol = 1e-6;
+  }
+  return p;
+}
+
+// Parameters_::set and State_::set of the three models: E_L first, then the values stored
+// relative to E_L (iaf), then the checks the reference makes.
+void
+Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
+{
+  std::set< std::string > used;
+  auto has = [ & ]( const char* k )
+  {
+    if ( d.has( k ) )
+    {
+      used.insert( k );
+      return true;
+    }
+    return false;
+  };
+  auto get = [ & ]( const char* k ) { return d.at( k ).number(); };
+  if ( p.model == M_AEIF_COND_EXP )
+  {
+    const char* keys[] = { "V_peak", "V_reset", "t_ref", "g_L", "C_m", "E_ex", "E_in", "E_L", "Delta_T", "tau_w", "a",
+      "b", "V_th", "tau_syn_ex", "tau_syn_in", "I_e", "gsl_error_tol" };
+    double* dst[] = { &p.V_peak, &p.V_reset, &p.t_ref, &p.g_L, &p.c_m, &p.E_ex, &p.E_in, &p.E_L, &p.Delta_T, &p.tau_w,
+      &p.a, &p.b, &p.V_th, &p.tau_ex, &p.tau_in, &p.I_e, &p.tol };
+    NeuronParams q = p;
+    double* qdst[] = { &q.V_peak, &q.V_reset, &q.t_ref, &q.g_L, &q.c_m, &q.E_ex, &q.E_in, &q.E_L, &q.Delta_T,
+      &q.tau_w, &q.a, &q.b, &q.V_th, &q.tau_ex, &q.tau_in, &q.I_e, &q.tol };
+    for ( size_t i = 0; i < sizeof( keys ) / sizeof( keys[ 0 ] ); ++i )
+      if ( has( keys[ i ] ) )
+        *qdst[ i ] = get( keys[ i ] );
+    (void) dst;
+    if ( q.V_peak < q.V_th )
+      throw std::runtime_error( "BadProperty: V_peak >= V_th required." );
+    if ( q.V_reset >= q.V_peak )
+      throw std::runtime_error( "BadProperty: Ensure that: V_reset < V_peak ." );
+    if ( q.Delta_T < 0. )
+      throw std::runtime_error( "BadProperty: Delta_T must be positive." );
+    if ( q.c_m <= 0 )
+      throw std::runtime_error( "BadProperty: Ensure that C_m > 0" );
+    if ( q.t_ref < 0 )
+      throw std::runtime_error( "BadProperty: Ensure that t_ref >= 0" );
+    if ( q.tau_ex <= 0 or q.tau_in <= 0 or q.tau_w <= 0 )
+      throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
+    if ( q.tol <= 0. )
+      throw std::runtime_error( "BadProperty: The gsl_error_tol must be strictly positive." );
+    p = q;
+    if ( has( "V_m" ) )
+      s.v0 = get( "V_m" );
+    if ( has( "g_ex" ) )
+      s.v1 = get( "g_ex" );
+    if ( has( "g_in" ) )
+      s.v2 = get( "g_in" );
+    if ( has( "w" ) )
+      s.v3 = get( "w" );
+  }
+  else
+  {
+    NeuronParams q = p;
+    const double ELold = q.E_L;
+    if ( has( "E_L" ) )
+      q.E_L = get( "E_L" );
+    const double delta_EL = q.E_L - ELold;
+    if ( has( "V_reset" ) )
+      q.V_reset = get( "V_reset" ) - q.E_L;
+    else
+      q.V_reset -= delta_EL;
+    if ( has( "V_th" ) )
+      q.V_th = get( "V_th" ) - q.E_L;
+    else
+      q.V_th -= delta_EL;
+    if ( q.model == M_IAF_PSC_DELTA )
+    {
+      if ( has( "V_min" ) )
+        q.V_min = get( "V_min" ) - q.E_L;
+      else
+        q.V_min -= delta_EL;
+    }
+    if ( has( "I_e" ) )
+      q.I_e = get( "I_e" );
+    if ( has( "C_m" ) )
+      q.c_m = get( "C_m" );
+    if ( has( "tau_m" ) )
+      q.tau_m = get( "tau_m" );
+    if ( q.model == M_IAF_PSC_EXP )
+    {
+      if ( has( "tau_syn_ex" ) )
+        q.tau_ex = get( "tau_syn_ex" );
+      if ( has( "tau_syn_in" ) )
+        q.tau_in = get( "tau_syn_in" );
+    }
+    if ( has( "t_ref" ) )
+      q.t_ref = get( "t_ref" );
+    if ( q.V_reset >= q.V_th )
+      throw std::runtime_error( "BadProperty: Reset potential must be smaller than threshold." );
+    if ( q.c_m <= 0 )
+      throw std::runtime_error( "BadProperty: Capacitance must be > 0." );
+    if ( q.t_ref < 0 )
+      throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
+    if ( q.tau_m <= 0 or ( q.model == M_IAF_PSC_EXP and ( q.tau_ex <= 0 or q.tau_in <= 0 ) ) )
+      throw std::runtime_error( "BadProperty: Membrane and synapse time constants must be strictly positive." );
+    if ( q.model == M_IAF_PSC_DELTA and has( "refractory_input" ) )
+    {
+      q.refr_input = d.at( "refractory_input" ).number() != 0;
+      if ( q.refr_input )
+        throw std::runtime_error( "NotImplemented: refractory_input is not supported on the device path." );
+    }
+    if ( q.model == M_IAF_PSC_EXP )
+    {
+      if ( has( "rho" ) )
+        q.rho = get( "rho" );
+      if ( has( "delta" ) )
+        q.delta = get( "delta" );
+      if ( q.rho < 0 )
+        throw std::runtime_error( "BadProperty: Stochastic firing intensity must not be negative." );
+      if ( q.delta < 0 )
+        throw std::runtime_error( "BadProperty: Width of threshold region must not be negative." );
+      if ( q.delta > 1e-10 )
+        throw std::runtime_error( "NotImplemented: stochastic threshold (delta > 0) is not supported on the device path." );
+    }
+    p = q;
+    if ( has( "V_m" ) )
+      s.v0 = get( "V_m" ) - p.E_L;
+    else
+      s.v0 -= delta_EL;
+  }
+  for ( const auto& kv : d.obj )
+    if ( not used.count( kv.first ) )
+      throw std::runtime_error( "UnaccessedDictionaryEntry: '" + kv.first + "' is not a parameter of this model" );
+}
+
+int64_t
+Kernel::create( const std::string& model, int64_t n, const Json& params )
+{
+  if ( n < 1 )
+    throw std::runtime_error( "BadProperty: Create requires n >= 1." );
+  if ( simulated_ )
+    throw std::runtime_error( "NotImplemented: creating nodes after Simulate is not supported on the device path." );
+  const int64_t first = static_cast< int64_t >( nodes_.size() ) + 1;
+  int model_id = -1;
+  if ( model == "iaf_psc_delta" )
+    model_id = M_IAF_PSC_DELTA;
+  else if ( model == "iaf_psc_exp" )
+    model_id = M_IAF_PSC_EXP;
+  else if ( model == "aeif_cond_exp" )
+    model_id = M_AEIF_COND_EXP;
+  if ( model_id >= 0 )
+  {
+    NeuronParams p = default_params( model_id );
+    NeuronState s {};
+    if ( model_id == M_AEIF_COND_EXP )
+      s.v0 = p.E_L;  // State_( const Parameters_& ): y[V_M] = E_L
+    apply_params( p, s, params, true );
+    for ( int64_t i = 0; i < n; ++i )
+    {
+      nodes_.push_back( NodeRec { 0, static_cast< uint32_t >( params_.size() ) } );
+      params_.push_back( p );
+      state_.push_back( s );
+      neuron_gid_.push_back( first + i );
+    }
+  }
+  else if ( model == "spike_generator" )
+  {
+    for ( int64_t i = 0; i < n; ++i )
+    {
+      nodes_.push_back( NodeRec { 1, static_cast< uint32_t >( gens_.size() ) } );
+      gens_.emplace_back();
+      gen_gid_.push_back( first + i );
+    }
+    if ( not params.obj.empty() )
+    {
+      std::vector< int64_t > g( n );
+      for ( int64_t i = 0; i < n; ++i )
+        g[ i ] = first + i;
+      set_status( g, params );
+    }
+  }
+  else if ( model == "spike_recorder" )
+  {
+    for ( int64_t i = 0; i < n; ++i )
+    {
+      nodes_.push_back( NodeRec { 2, static_cast< uint32_t >( recorders_.size() ) } );
+      recorders_.emplace_back();
+      rec_gid_.push_back( first + i );
+    }
+  }
+  else
+    throw std::runtime_error( "UnknownModelName: '" + model + "' is not available on the device path." );
+  store_dirty_ = true;
+  return first;
+}
+
+void
+Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
+{
+  for ( int64_t g : gids )
+  {
+    NodeRec& r = node( g );
+    if ( r.kind == 0 )
+    {
+      if ( simulated_ )
+      {
+        // after Simulate only the membrane potential can be set (it lives on the device)
+        if ( d.obj.size() != 1 or not d.has( "V_m" ) )
+          throw std::runtime_error( "NotImplemented: only V_m can be changed after Simulate on the device path." );
+        const double v = d.at( "V_m" ).number();
+        set_values( { g }, "V_m", &v );
+        continue;
+      }
+      NeuronParams p = params_[ r.idx ];
+      NeuronState s = state_[ r.idx ];
+      apply_params( p, s, d, false );
+      params_[ r.idx ] = p;
+      state_[ r.idx ] = s;
+      state_dirty_ = true;
+      store_dirty_ = true;
+    }
+    else if ( r.kind == 1 )
+    {
+      SpikeGen& G = gens_[ r.idx ];
+      for ( const auto& kv : d.obj )
+      {
+        const std::string& k = kv.first;
+        if ( k == "spike_times" )
+        {
+          G.stamps.clear();
+          for ( const Json& t : kv.second.arr )
+          {
+            // spike_generator::Parameters_::assert_valid_spike_time_and_insert_ (spike_generator.cpp:85-140)
+            const double tm = t.number();
+            if ( tm == 0.0 )
+              throw std::runtime_error( "BadProperty: spike time cannot be set to 0." );
+            const long tics = tb_.ms_to_tics( tm );
+            if ( tics % tb_.tics_per_step != 0 )
+              throw std::runtime_error(
+                "BadProperty: spike_generator: Time point " + std::to_string( tm ) + " is not representable in current resolution." );
+            G.stamps.push_back( tics / tb_.tics_per_step );
+          }
+          for ( size_t i = 1; i < G.stamps.size(); ++i )
+            if ( G.stamps[ i ] < G.stamps[ i - 1 ] )
+              throw std::runtime_error( "BadProperty: spike times must be sorted in non-descending order." );
+        }
+        else if ( k == "spike_multiplicities" )
+        {
+          G.mults.clear();
+          for ( const Json& m : kv.second.arr )
+            G.mults.push_back( static_cast< int64_t >( m.number() ) );
+        }
+        else if ( k == "poisson_rate" or k == "poisson_seed" or k == "poisson_start" or k == "poisson_stop" )
+          continue;
+        else if ( k == "precise_times" or k == "allow_offgrid_times" or k == "shift_now_spikes" )
+        {
+          if ( kv.second.number() != 0 )
+            throw std::runtime_error( "NotImplemented: off-grid spike_generator options are not supported." );
+        }
+        else
+          throw std::runtime_error( "UnaccessedDictionaryEntry: '" + k + "' is not a spike_generator parameter" );
+      }
+      if ( d.has( "poisson_rate" ) )
+      {
+        // seeded Poisson spike train: counts per step ~ Poisson(rate * h), pre-generated
+        const double rate = d.at( "poisson_rate" ).number();
+        const double start = d.has( "poisson_start" ) ? d.at( "poisson_start" ).number() : 0.0;
+        const double stop = d.has( "poisson_stop" ) ? d.at( "poisson_stop" ).number() : 1e9;
+        G.poisson_lambda = rate * tb_.resolution_ms() * 1e-3;
+        G.poisson_seed = d.has( "poisson_seed" ) ? static_cast< uint64_t >( d.at( "poisson_seed" ).number() ) : 1;
+        G.poisson_t0 = tb_.ms_to_steps( start );
+        G.poisson_steps = tb_.ms_to_steps( stop ) - G.poisson_t0;
+      }
+      if ( not G.mults.empty() and G.mults.size() != G.stamps.size() )
+        throw std::runtime_error( "BadProperty: spike_multiplicities must have the same length as spike_times." );
+      store_dirty_ = true;
+    }
+    else
+      throw std::runtime_error( "BadProperty: spike_recorder has no settable parameters on the device path" );
+  }
+}
+
+void
+Kernel::collect_log()
+{
+  if ( not engine_ or not engine_->built() )
+    return;
+  engine_->drain_log();
+  auto& src = engine_->log_src;
+  auto& st = engine_->log_step;
+  for ( size_t i = 0; i < src.size(); ++i )
+  {
+    spike_sender_.push_back( dev_to_gid_[ src[ i ] ] );
+    spike_step_.push_back( st[ i ] );
+  }
+  spike_count_ += src.size();
+  src.clear();
+  st.clear();
+}
+
+void
+Kernel::set_values( const std::vector< int64_t >& gids, const std::string& key, const double* v )
+{
+  // per-node values (NodeCollection.set(key=array)); live state goes to the device directly
+  const bool live = engine_ and engine_->built() and simulated_;
+  for ( size_t i = 0; i < gids.size(); ++i )
+  {
+    NodeRec& r = node( gids[ i ] );
+    if ( r.kind != 0 )
+      throw std::runtime_error( "BadProperty: per-node values are only supported for neurons" );
+    NeuronParams& p = params_[ r.idx ];
+    NeuronState& s = state_[ r.idx ];
+    if ( key == "V_m" and live )
+    {
+      const int dev = gid_to_dev_[ gids[ i ] ];
+      const uint32_t j = static_cast< uint32_t >( dev ) - engine_->net().model_begin[ p.model ];
+      engine_->set_state( p.model, 0, j, p.model == M_AEIF_COND_EXP ? v[ i ] : v[ i ] - p.E_L );
+      continue;
+    }
+    Json d;
+    d.kind = Json::OBJ;
+    Json x;
+    x.kind = Json::NUM;
+    x.num = v[ i ];
+    d.obj.emplace_back( key, x );
+    if ( live )
+      throw std::runtime_error( "NotImplemented: only V_m can be changed after Simulate on the device path." );
+    apply_params( p, s, d, false );
+    store_dirty_ = true;
+  }
+}
+
+void
+Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, double* out )
+{
+  const bool live = engine_ and engine_->built() and simulated_;
+  std::vector< double > cache[ N_MODELS ][ 5 ];
+  std::vector< int32_t > rcache[ N_MODELS ];
+  for ( size_t i = 0; i < gids.size(); ++i )
+  {
+    const NodeRec& r = node( gids[ i ] );
+    if ( r.kind != 0 )
+      throw std::runtime_error( "BadProperty: '" + key + "' is not a property of " + model_name( gids[ i ] ) );
+    const NeuronParams& p = params_[ r.idx ];
+    const NeuronState& s = state_[ r.idx ];
+    const bool aeif = p.model == M_AEIF_COND_EXP;
+    int field = -1;
+    if ( key == "V_m" )
+      field = 0;
+    else if ( key == "I_syn_ex" and p.model == M_IAF_PSC_EXP )
+      field = 1;
+    else if ( key == "I_syn_in" and p.model == M_IAF_PSC_EXP )
+      field = 2;
+    else if ( key == "g_ex" and aeif )
+      field = 1;
+    else if ( key == "g_in" and aeif )
+      field = 2;
+    else if ( key == "w" and aeif )
+      field = 3;
+    if ( field >= 0 or key == "refractory_steps" )
+    {
+      double v;
+      int32_t rr;
+      if ( live )
+      {
+        const int dev = gid_to_dev_[ gids[ i ] ];
+        const uint32_t j = static_cast< uint32_t >( dev ) - engine_->net().model_begin[ p.model ];
+        if ( field >= 0 )
+        {
+          if ( cache[ p.model ][ field ].empty() )
+            engine_->get_state( p.model, field, cache[ p.model ][ field ] );
+          v = cache[ p.model ][ field ][ j ];
+        }
+        else
+        {
+          if ( rcache[ p.model ].empty() )
+            engine_->get_r( p.model, rcache[ p.model ] );
+          rr = rcache[ p.model ][ j ];
+        }
+      }
+      else
+      {
+        const double vals[] = { s.v0, s.v1, s.v2, s.v3, s.v4 };
+        v = field >= 0 ? vals[ field ] : 0;
+        rr = s.r;
+      }
+      if ( field < 0 )
+        out[ i ] = rr;
+      else
+        out[ i ] = ( field == 0 and not aeif ) ? v + p.E_L : v;
+      continue;
+    }
+    if ( key == "E_L" )
+      out[ i ] = p.E_L;
+    else if ( key == "V_th" )
+      out[ i ] = aeif ? p.V_th : p.V_th + p.E_L;
+    else if ( key == "V_reset" )
+      out[ i ] = aeif ? p.V_reset : p.V_reset + p.E_L;
+    else if ( key == "C_m" )
+      out[ i ] = p.c_m;
+    else if ( key == "tau_m" and not aeif )
+      out[ i ] = p.tau_m;
+    else if ( key == "t_ref" )
+      out[ i ] = p.t_ref;
+    else if ( key == "I_e" )
+      out[ i ] = p.I_e;
+    else if ( key == "tau_syn_ex" and p.model != M_IAF_PSC_DELTA )
+      out[ i ] = p.tau_ex;
+    else if ( key == "tau_syn_in" and p.model != M_IAF_PSC_DELTA )
+      out[ i ] = p.tau_in;
+    else if ( key == "global_id" )
+      out[ i ] = static_cast< double >( gids[ i ] );
+    else
+      throw std::runtime_error( "BadProperty: unknown property '" + key + "'" );
+  }
+}
+
+void
+Kernel::check_delay( long steps )
+{
+  // DelayChecker::assert_valid_delay_ms (delay_checker.cpp:146-171)
+  if ( steps < 1 )
+    throw std::runtime_error( "BadDelay: Delay must be greater than or equal to resolution" );
+  if ( simulated_ and ( steps < min_delay_ or steps > max_delay_ ) )
+    throw std::runtime_error( "BadDelay: Minimum and maximum delay cannot be changed after Simulate has been called." );
+}
+
+void
+Kernel::single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp )
+{
+  // BipartiteConnBuilder::single_connect_ (conn_builder.cpp:379-437): delay is drawn before
+  // the weight when both are given; defaults of static_synapse are weight 1, delay 1 ms
+  double delay = 1.0, weight = 1.0;
+  if ( wp and dp )
+  {
+    delay = dp->value( rng, vp );
+    weight = wp->value( rng, vp );
+  }
+  else if ( dp )
+    delay = dp->value( rng, vp );
+  else if ( wp )
+    weight = wp->value( rng, vp );
+  const long ds = tb_.delay_ms_to_steps( delay );
+  check_delay( ds );
+  ConnList& C = conns_[ vp ];
+  C.src.push_back( static_cast< uint32_t >( s ) );
+  C.tgt.push_back( static_cast< uint32_t >( t ) );
+  C.w.push_back( weight );
+  if ( ds > 65535 )
+    throw std::runtime_error( "BadDelay: delays above 65535 steps are not supported" );
+  C.d.push_back( static_cast< uint16_t >( ds ) );
+}
+
+void
+Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const Json& conn_spec,
+  const Json& syn_spec )
+{
+  if ( rng_dirty_ )
+    init_rngs();
+  std::string rule = "all_to_all";
+  bool autapses = true, multapses = true;
+  double indegree = 0, p = 0;
+  if ( conn_spec.kind == Json::STR )
+    rule = conn_spec.str;
+  else if ( conn_spec.kind == Json::OBJ )
+  {
+    for ( const auto& kv : conn_spec.obj )
+    {
+      if ( kv.first == "rule" )
+        rule = kv.second.string();
+      else if ( kv.first == "allow_autapses" )
+        autapses = kv.second.number() != 0;
+      else if ( kv.first == "allow_multapses" )
+        multapses = kv.second.number() != 0;
+      else if ( kv.first == "indegree" )
+        indegree = kv.second.number();
+      else if ( kv.first == "p" )
+        p = kv.second.number();
+      else
+        throw std::runtime_error( "UnaccessedDictionaryEntry: conn_spec key '" + kv.first + "'" );
+    }
+  }
+  std::unique_ptr< Param > wp, dp;
+  for ( const auto& kv : syn_spec.obj )
+  {
+    if ( kv.first == "synapse_model" )
+    {
+      const std::string& sm = kv.second.string();
+      if ( sm != "static_synapse" and sm != "static_synapse_hpc" )
+        throw std::runtime_error( "UnknownSynapseType: '" + sm + "' is not available on the device path." );
+    }
+    else if ( kv.first == "weight" )
+      wp = Param::from_json( kv.second, n_vp_ );
+    else if ( kv.first == "delay" )
+      dp = Param::from_json( kv.second, n_vp_ );
+    else
+      throw std::runtime_error( "UnaccessedDictionaryEntry: syn_spec key '" + kv.first + "'" );
+  }
+  if ( pre.empty() or post.empty() )
+    return;
+  const int pre_kind = node( pre[ 0 ] ).kind;
+  const int post_kind = node( post[ 0 ] ).kind;
+  for ( int64_t g : pre )
+    if ( node( g ).kind != pre_kind )
+      throw std::runtime_error( "IllegalConnection: mixed node kinds in source collection" );
+  for ( int64_t g : post )
+    if ( node( g ).kind != post_kind )
+      throw std::runtime_error( "IllegalConnection: mixed node kinds in target collection" );
+  if ( rule == "one_to_one" and pre.size() != post.size() )
+    throw std::runtime_error( "DimensionMismatch: one_to_one requires equal numbers of sources and targets." );
+
+  if ( post_kind == 2 )
+  {
+    // connections to a spike_recorder: the recorder observes these senders
+    if ( pre_kind != 0 )
+      throw std::runtime_error( "IllegalConnection: spike_recorder records neurons only" );
+    for ( int64_t r : post )
+    {
+      auto& v = recorders_[ node( r ).idx ];
+      v.insert( v.end(), pre.begin(), pre.end() );
+      std::sort( v.begin(), v.end() );
+      v.erase( std::unique( v.begin(), v.end() ), v.end() );
+    }
+    return;
+  }
+  if ( pre_kind == 2 )
+    throw std::runtime_error( "IllegalConnection: spike_recorder cannot be a source" );
+  if ( post_kind == 1 )
+    throw std::runtime_error( "IllegalConnection: spike_generator cannot be a target" );
+
+  if ( rule == "fixed_indegree" and pre_kind == 0 )
+  {
+    if ( indegree < 0 )
+      throw std::runtime_error( "BadProperty: Indegree cannot be less than zero." );
+    if ( not multapses and static_cast< size_t >( indegree ) > pre.size() )
+      throw std::runtime_error( "BadProperty: Indegree cannot be larger than population size." );
+  }
+  if ( rule == "pairwise_bernoulli" and ( p < 0 or p > 1 ) )
+    throw std::runtime_error( "BadProperty: Connection probability 0 <= p <= 1 required." );
+  if ( rule != "one_to_one" and rule != "all_to_all" and rule != "fixed_indegree" and rule != "pairwise_bernoulli" )
+    throw std::runtime_error( "BadProperty: unknown or unsupported connection rule '" + rule + "'" );
+
+  if ( pre_kind == 1 )
+  {
+    // spike_generator -> neurons (connect_from_device_): generators exist on every thread;
+    // each target's VP runs the builder with its own stream
+    if ( rule != "one_to_one" and rule != "all_to_all" )
+      throw std::runtime_error( "NotImplemented: spike_generator connections support one_to_one and all_to_all" );
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      for ( size_t ti = 0; ti < post.size(); ++ti )
+      {
+        const int64_t t = post[ ti ];
+        if ( vp_of( t ) != vp )
+          continue;
+        auto one = [ & ]( int64_t s )
+        {
+          double delay = 1.0, weight = 1.0;
+          if ( wp and dp )
+          {
+            delay = dp->value( rngs_[ vp ], vp );
+            weight = wp->value( rngs_[ vp ], vp );
+          }
+          else if ( dp )
+            delay = dp->value( rngs_[ vp ], vp );
+          else if ( wp )
+            weight = wp->value( rngs_[ vp ], vp );
+          const long ds = tb_.delay_ms_to_steps( delay );
+          check_delay( ds );
+          if ( d_min_seen_ < 0 or ds < d_min_seen_ )
+            d_min_seen_ = ds;
+          d_max_seen_ = std::max( d_max_seen_, ds );
+          dev_conns_.push_back( DevConn { s, t, weight, ds } );
+        };
+        if ( rule == "one_to_one" )
+          one( pre[ ti ] );
+        else
+          for ( int64_t s : pre )
+            one( s );
+      }
+    store_dirty_ = true;
+    return;
+  }
+
+  // neuron -> neuron: every VP runs the builder for its local targets with its own stream
+  // (FixedInDegreeBuilder::inner_connect_ conn_builder.cpp:1508-1545, BernoulliBuilder
+  // :1937-1965, OneToOne :960-1031, AllToAll :1170-1255)
+  std::vector< std::string > errors( n_vp_ );
+  std::vector< long > dmin( n_vp_, -1 ), dmax( n_vp_, 1 );
+  std::vector< size_t > start( n_vp_ );
+  for ( int vp = 0; vp < n_vp_; ++vp )
+    start[ vp ] = conns_[ vp ].size();
+  auto worker = [ & ]( int vp )
+  {
+    try
+    {
+      VpRng& rng = rngs_[ vp ];
+      const long K = static_cast< long >( std::round( indegree ) );
+      std::vector< unsigned long > chosen;
+      for ( size_t ti = 0; ti < post.size(); ++ti )
+      {
+        const int64_t t = post[ ti ];
+        if ( vp_of( t ) != vp )
+          continue;
+        if ( rule == "one_to_one" )
+        {
+          const int64_t s = pre[ ti ];
+          if ( s == t and not autapses )
+            continue;
+          single_connect( s, t, vp, rng, wp.get(), dp.get() );
+        }
+        else if ( rule == "all_to_all" )
+        {
+          for ( int64_t s : pre )
+          {
+            if ( s == t and not autapses )
+              continue;
+            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+        else if ( rule == "fixed_indegree" )
+        {
+          chosen.clear();
+          for ( long j = 0; j < K; ++j )
+          {
+            unsigned long s_id;
+            int64_t s;
+            bool skip_autapse, skip_multapse;
+            do
+            {
+              s_id = rng.ulrand( pre.size() );
+              s = pre[ s_id ];
+              skip_autapse = not autapses and s == t;
+              skip_multapse = not multapses and std::find( chosen.begin(), chosen.end(), s_id ) != chosen.end();
+            } while ( skip_autapse or skip_multapse );
+            if ( not multapses )
+              chosen.push_back( s_id );
+            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+        else
+        {
+          for ( int64_t s : pre )
+          {
+            if ( not autapses and s == t )
+              continue;
+            if ( rng.drand() >= p )
+              continue;
+            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+      }
+    }
+    catch ( const std::exception& e )
+    {
+      errors[ vp ] = e.what();
+    }
+  };
+  if ( n_vp_ == 1 )
+    worker( 0 );
+  else
+  {
+    std::vector< std::thread > th;
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      th.emplace_back( worker, vp );
+    for ( auto& x : th )
+      x.join();
+  }
+  for ( auto& e : errors )
+    if ( not e.empty() )
+      throw std::runtime_error( e );
+  for ( int vp = 0; vp < n_vp_; ++vp )
+  {
+    const ConnList& C = conns_[ vp ];
+    if ( C.size() > static_cast< size_t >( ( 1u << 27 ) - 2 ) )
+      throw std::runtime_error( "KernelException: Too many connections: at most 134217726 connections supported per "
+                                "virtual process and synapse model." );
+    for ( size_t i = start[ vp ]; i < C.size(); ++i )
+    {
+      const long ds = C.d[ i ];
+      if ( d_min_seen_ < 0 or ds < d_min_seen_ )
+        d_min_seen_ = ds;
+      if ( ds > d_max_seen_ )
+        d_max_seen_ = ds;
+    }
+  }
+  store_dirty_ = true;
+}
+
+void
+Kernel::connect_arrays( const int64_t* src, const int64_t* tgt, const double* w, const double* d_ms, int64_t n )
+{
+  // ConnectionManager::connect_arrays: each VP creates the connections of its targets in
+  // array order (connection_manager.cpp:624-760)
+  for ( int64_t i = 0; i < n; ++i )
+  {
+    if ( node( src[ i ] ).kind != 0 or node( tgt[ i ] ).kind != 0 )
+      throw std::runtime_error( "IllegalConnection: connect_arrays connects neurons only" );
+  }
+  for ( int vp = 0; vp < n_vp_; ++vp )
+  {
+    ConnList& C = conns_[ vp ];
+    for ( int64_t i = 0; i < n; ++i )
+    {
+      if ( vp_of( tgt[ i ] ) != vp )
+        continue;
+      const long ds = tb_.delay_ms_to_steps( d_ms ? d_ms[ i ] : 1.0 );
+      check_delay( ds );
+      if ( d_min_seen_ < 0 or ds < d_min_seen_ )
+        d_min_seen_ = ds;
+      d_max_seen_ = std::max( d_max_seen_, ds );
+      C.src.push_back( static_cast< uint32_t >( src[ i ] ) );
+      C.tgt.push_back( static_cast< uint32_t >( tgt[ i ] ) );
+      C.w.push_back( w ? w[ i ] : 1.0 );
+      C.d.push_back( static_cast< uint16_t >( ds ) );
+    }
+  }
+  store_dirty_ = true;
+}
+
+void
+Kernel::set_spike_counts( int64_t gen_first, int64_t n, int64_t t0_step, int64_t n_steps, const uint8_t* counts )
+{
+  for ( int64_t i = 0; i < n; ++i )
+    if ( node( gen_first + i ).kind != 1 )
+      throw std::runtime_error( "BadProperty: spike counts can only be set on spike_generator nodes" );
+  DenseBlock b;
+  b.t0 = t0_step;
+  b.n_steps = n_steps;
+  b.cols = n;
+  b.counts.assign( counts, counts + n * n_steps );
+  dense_.push_back( std::move( b ) );
+  for ( int64_t i = 0; i < n; ++i )
+  {
+    SpikeGen& G = gens_[ node( gen_first + i ).idx ];
+    G.dense_block = static_cast< int >( dense_.size() - 1 );
+    G.dense_col = i;
+  }
+  store_dirty_ = true;
+}
+
+int64_t
+Kernel::num_connections() const
+{
+  int64_t n = 0;
+  for ( const auto& c : conns_ )
+    n += static_cast< int64_t >( c.size() );
+  return n + static_cast< int64_t >( dev_conns_.size() );
+}
+
+void
+Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int32_t* vp ) const
+{
+  // neuron connections per VP in creation order, then device connections
+  int64_t k = 0;
+  for ( int v = 0; v < n_vp_; ++v )
+  {
+    const ConnList& C = conns_[ v ];
+    for ( size_t i = 0; i < C.size(); ++i, ++k )
+    {
+      src[ k ] = C.src[ i ];
+      tgt[ k ] = C.tgt[ i ];
+      w[ k ] = C.w[ i ];
+      d[ k ] = C.d[ i ];
+      if ( vp )
+        vp[ k ] = v;
+    }
+  }
+  for ( const auto& c : dev_conns_ )
+  {
+    src[ k ] = c.gen;
+    tgt[ k ] = c.tgt;
+    w[ k ] = c.w;
+    d[ k ] = c.d;
+    if ( vp )
+      vp[ k ] = vp_of( c.tgt );
+    ++k;
+  }
+}
+
+// Propagator P32 of IAFPropagatorExp::evaluate (libnestutil/iaf_propagator.cpp:42-74)
+static double
+iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
+{
+  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
+  const double gamma = beta / c_m;
+  const double inv_tau_syn = 1 / tau_syn;
+  const double inv_tau_m = 1 / tau_m;
+  const double inv_c_m = 1 / c_m;
+  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
+  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
+  const double expm1_h_tau = std::expm1( h * inv_beta );
+  const double P32 = gamma * exp_h_tau_syn * expm1_h_tau;
+  if ( std::isnormal( P32 ) and P32 > 0 )
+    return P32;
+  const double exp_h_tau = std::exp( -h * inv_tau_m );
+  return h * inv_c_m * exp_h_tau;
+}
+
+void
+Kernel::build_spec( BuildSpec& b )
+{
+  const uint32_t N = static_cast< uint32_t >( params_.size() );
+  const double h = tb_.resolution_ms();
+  b.n_vp = n_vp_;
+  b.min_delay = static_cast< int >( min_delay_ );
+  b.max_delay = static_cast< int >( max_delay_ );
+  b.n = N;
+  // ---- device order: model-major; within a model NEST's receive order (VP, node id)
+  std::vector< uint32_t > order( N );  // neuron idx by (vp, gid)
+  for ( uint32_t i = 0; i < N; ++i )
+    order[ i ] = i;
+  std::stable_sort( order.begin(), order.end(),
+    [ this ]( uint32_t a, uint32_t c ) { return vp_of( neuron_gid_[ a ] ) < vp_of( neuron_gid_[ c ] ); } );
+  std::vector< uint32_t > ord_of( N );
+  for ( uint32_t o = 0; o < N; ++o )
+    ord_of[ order[ o ] ] = o;
+  b.vp_begin.assign( n_vp_ + 1, 0 );
+  for ( uint32_t o = 0; o < N; ++o )
+    b.vp_begin[ vp_of( neuron_gid_[ order[ o ] ] ) + 1 ]++;
+  for ( int v = 0; v < n_vp_; ++v )
+    b.vp_begin[ v + 1 ] += b.vp_begin[ v ];
+  uint32_t counts[ N_MODELS ] = { 0, 0, 0 };
+  for ( uint32_t i = 0; i < N; ++i )
+    counts[ params_[ i ].model ]++;
+  b.model_begin[ 0 ] = 0;
+  for ( int m = 0; m < N_MODELS; ++m )
+    b.model_begin[ m + 1 ] = b.model_begin[ m ] + counts[ m ];
+  std::vector< uint32_t > dev_of( N );
+  {
+    uint32_t cur[ N_MODELS ];
+    for ( int m = 0; m < N_MODELS; ++m )
+      cur[ m ] = b.model_begin[ m ];
+    for ( uint32_t o = 0; o < N; ++o )
+    {
+      const uint32_t i = order[ o ];
+      dev_of[ i ] = cur[ params_[ i ].model ]++;
+    }
+  }
+  bool identity = true;
+  b.dev2ord.assign( N, 0 );
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    b.dev2ord[ dev_of[ i ] ] = ord_of[ i ];
+    identity = identity and dev_of[ i ] == ord_of[ i ];
+  }
+  if ( identity )
+    b.dev2ord.clear();
+  gid_to_dev_.assign( nodes_.size() + 1, -1 );
+  dev_to_gid_.assign( N, 0 );
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    gid_to_dev_[ neuron_gid_[ i ] ] = static_cast< int32_t >( dev_of[ i ] );
+    dev_to_gid_[ dev_of[ i ] ] = neuron_gid_[ i ];
+  }
+  // ---- drive: one spike_generator per neuron at most
+  std::vector< int64_t > gen_of( N, -1 );
+  std::vector< double > drive_w( N, 0 );
+  std::vector< long > drive_d( N, 0 );
+  for ( const auto& c : dev_conns_ )
+  {
+    const uint32_t i = node( c.tgt ).idx;
+    if ( gen_of[ i ] >= 0 )
+      throw std::runtime_error( "NotImplemented: at most one spike_generator per neuron on the device path" );
+    gen_of[ i ] = c.gen;
+    drive_w[ i ] = c.w;
+    drive_d[ i ] = c.d;
+  }
+  // dense blocks: user blocks plus one block for generators given by spike_times
+  std::vector< DenseBlock > blocks = dense_;
+  {
+    int64_t tmax = 0;
+    bool any = false;
+    for ( const auto& G : gens_ )
+      if ( G.dense_block < 0 and G.poisson_lambda == 0 and not G.stamps.empty() )
+      {
+        any = true;
+        tmax = std::max( tmax, G.stamps.back() );
+      }
+    if ( any )
+    {
+      DenseBlock sb;
+      sb.t0 = 0;
+      sb.n_steps = tmax;
+      sb.cols = static_cast< int64_t >( gens_.size() );
+      sb.counts.assign( static_cast< size_t >( sb.n_steps * sb.cols ), 0 );
+      for ( size_t g = 0; g < gens_.size(); ++g )
+      {
+        const SpikeGen& G = gens_[ g ];
+        if ( G.dense_block >= 0 or G.poisson_lambda != 0 )
+          continue;
+        for ( size_t k = 0; k < G.stamps.size(); ++k )
+        {
+          if ( k > 0 and G.stamps[ k ] == G.stamps[ k - 1 ] )
+            throw std::runtime_error( "NotImplemented: repeated spike times in one spike_generator; use spike_multiplicities" );
+          const int64_t m = G.mults.empty() ? 1 : G.mults[ k ];
+          if ( m < 0 or m > 255 )
+            throw std::runtime_error( "NotImplemented: spike multiplicities above 255" );
+          sb.counts[ static_cast< size_t >( ( G.stamps[ k ] - 1 ) * sb.cols + g ) ] = static_cast< uint8_t >( m );
+        }
+      }
+      blocks.push_back( std::move( sb ) );
+    }
+  }
+  // merge all dense blocks into one matrix [step][col]
+  std::vector< int64_t > block_col0( blocks.size(), 0 );
+  int64_t t0 = 0, t1 = 0, cols = 0;
+  for ( size_t k = 0; k < blocks.size(); ++k )
+  {
+    if ( k == 0 )
+    {
+      t0 = blocks[ k ].t0;
+      t1 = blocks[ k ].t0 + blocks[ k ].n_steps;
+    }
+    t0 = std::min( t0, blocks[ k ].t0 );
+    t1 = std::max( t1, blocks[ k ].t0 + blocks[ k ].n_steps );
+    block_col0[ k ] = cols;
+    cols += blocks[ k ].cols;
+  }
+  if ( blocks.size() == 1 )
+  {
+    b.drive_counts = blocks[ 0 ].counts;
+    b.drive_cols = cols;
+  }
+  else if ( blocks.size() > 1 )
+  {
+    b.drive_cols = cols;
+    b.drive_counts.assign( static_cast< size_t >( ( t1 - t0 ) * cols ), 0 );
+    for ( size_t k = 0; k < blocks.size(); ++k )
+      for ( int64_t s = 0; s < blocks[ k ].n_steps; ++s )
+        std::memcpy( &b.drive_counts[ static_cast< size_t >( ( blocks[ k ].t0 - t0 + s ) * cols + block_col0[ k ] ) ],
+          &blocks[ k ].counts[ static_cast< size_t >( s * blocks[ k ].cols ) ], static_cast< size_t >( blocks[ k ].cols ) );
+  }
+  const size_t user_blocks = dense_.size();
+  // ---- parameter sets
+  std::map< std::string, uint16_t > pset_ids;
+  std::map< std::pair< double, uint64_t >, int32_t > tables;
+  b.drive_idx.assign( N, 0xffffffffu );
+  for ( int m = 0; m < N_MODELS; ++m )
+  {
+    const uint32_t nm = b.model_begin[ m + 1 ] - b.model_begin[ m ];
+    b.pidx[ m ].assign( nm, 0 );
+    b.r[ m ].assign( nm, 0 );
+    const int nf = m == M_IAF_PSC_DELTA ? 1 : ( m == M_IAF_PSC_EXP ? 3 : 5 );
+    for ( int f = 0; f < 5; ++f )
+      b.st[ m ][ f ].assign( f < nf ? nm : 0, 0.0 );
+  }
+  int max_scap = 1;
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    const NeuronParams& P = params_[ i ];
+    const NeuronState& S = state_[ i ];
+    ParamSet ps;
+    std::memset( &ps, 0, sizeof ps );
+    ps.model = P.model;
+    const long R = tb_.ms_to_steps( P.t_ref );  // Time( Time::ms( t_ref ) ).get_steps()
+    ps.refractory_counts = static_cast< int32_t >( R );
+    if ( P.model == M_IAF_PSC_DELTA )
+    {
+      // iaf_psc_delta::pre_run_hook (iaf_psc_delta.cpp:234-270)
+      const double P33 = std::exp( -h / P.tau_m );
+      const double P30 = 1 / P.c_m * ( 1 - P33 ) * P.tau_m;
+      ps.p[ 0 ] = P30;
+      ps.p[ 1 ] = P33;
+      ps.p[ 2 ] = 0.0 + P.I_e;
+      ps.p[ 3 ] = P.V_th;
+      ps.p[ 4 ] = P.V_reset;
+      ps.p[ 5 ] = P.V_min;
+    }
+    else if ( P.model == M_IAF_PSC_EXP )
+    {
+      // iaf_psc_exp::pre_run_hook (iaf_psc_exp.cpp:244-286)
+      const double P11ex = std::exp( -h / P.tau_ex );
+      const double P11in = std::exp( -h / P.tau_in );
+      const double P22 = std::exp( -h / P.tau_m );
+      const double P21ex = iaf_propagator_exp( P.tau_ex, P.tau_m, P.c_m, h );
+      const double P21in = iaf_propagator_exp( P.tau_in, P.tau_m, P.c_m, h );
+      const double P20 = P.tau_m / P.c_m * ( 1.0 - P22 );
+      ps.p[ 0 ] = P22;
+      ps.p[ 1 ] = P21ex;
+      ps.p[ 2 ] = P21in;
+      ps.p[ 3 ] = P20;
+      ps.p[ 4 ] = P11ex;
+      ps.p[ 5 ] = P11in;
+      ps.p[ 6 ] = P.I_e + 0.0;
+      ps.p[ 7 ] = P.V_th;
+      ps.p[ 8 ] = P.V_reset;
+      ps.p[ 9 ] = ( 1. - P11ex ) * 0.0;
+    }
+    else
+    {
+      // aeif_cond_exp::pre_run_hook (aeif_cond_exp.cpp:420-435)
+      const double vals[] = { P.V_peak, P.V_reset, P.g_L, P.c_m, P.E_ex, P.E_in, P.E_L, P.Delta_T, P.tau_w, P.a, P.b,
+        P.V_th, P.tau_ex, P.tau_in, P.I_e, P.tol, P.Delta_T > 0. ? P.V_peak : P.V_th, h };
+      for ( int q = 0; q < 18; ++q )
+        ps.p[ q ] = vals[ q ];
+    }
+    max_scap = std::max( max_scap, static_cast< int >( ( min_delay_ + R ) / ( R + 1 ) ) );
+    // drive
+    if ( gen_of[ i ] >= 0 )
+    {
+      const SpikeGen& G = gens_[ node( gen_of[ i ] ).idx ];
+      ps.drive_w = drive_w[ i ];
+      ps.drive_d = static_cast< int32_t >( drive_d[ i ] );
+      if ( G.poisson_lambda > 0 )
+      {
+        ps.drive_mode = 2;
+        auto key = std::make_pair( G.poisson_lambda, G.poisson_seed );
+        auto it = tables.find( key );
+        if ( it == tables.end() )
+        {
+          it = tables.emplace( key, static_cast< int32_t >( b.ptables.size() ) ).first;
+          b.ptables.push_back( make_poisson_table( G.poisson_lambda, G.poisson_seed ) );
+        }
+        ps.drive_table = it->second;
+        ps.drive_t0 = G.poisson_t0;
+        ps.drive_steps = G.poisson_steps;
+        b.drive_idx[ dev_of[ i ] ] = static_cast< uint32_t >( gen_of[ i ] );
+      }
+      else if ( G.dense_block >= 0 or not G.stamps.empty() )
+      {
+        ps.drive_mode = 1;
+        ps.drive_t0 = blocks.size() > 1 ? t0 : blocks[ 0 ].t0;
+        ps.drive_steps = blocks.size() > 1 ? t1 - t0 : blocks[ 0 ].n_steps;
+        int64_t col;
+        if ( G.dense_block >= 0 )
+          col = block_col0[ G.dense_block ] + G.dense_col;
+        else
+          col = block_col0[ user_blocks ] + node( gen_of[ i ] ).idx;
+        b.drive_idx[ dev_of[ i ] ] = static_cast< uint32_t >( col );
+      }
+    }
+    std::string key( reinterpret_cast< const char* >( &ps ), sizeof ps );
+    auto it = pset_ids.find( key );
+    if ( it == pset_ids.end() )
+    {
+      if ( b.psets.size() >= 65535 )
+        throw std::runtime_error( "NotImplemented: more than 65535 distinct parameter sets" );
+      it = pset_ids.emplace( key, static_cast< uint16_t >( b.psets.size() ) ).first;
+      b.psets.push_back( ps );
+    }
+    const int m = P.model;
+    const uint32_t j = dev_of[ i ] - b.model_begin[ m ];
+    b.pidx[ m ][ j ] = it->second;
+    b.r[ m ][ j ] = S.r;
+    const double sv[] = { S.v0, S.v1, S.v2, S.v3, m == M_AEIF_COND_EXP ? h : 0.0 };
+    for ( int f = 0; f < 5; ++f )
+      if ( not b.st[ m ][ f ].empty() )
+        b.st[ m ][ f ][ j ] = sv[ f ];
+  }
+  b.s_cap = std::min( max_scap, 4 );
+  // ---- out-CSR by source device index: LSD passes, delay first, then source; both stable,
+  //      so each row is delay-sorted with creation order kept within a delay
+  int64_t E = 0;
+  for ( const auto& C : conns_ )
+    E += static_cast< int64_t >( C.size() );
+  const long D = max_delay_ - min_delay_ + 1;
+  int dbits = 0;
+  while ( ( 1l << dbits ) < D )
+    ++dbits;
+  b.tgt_bits = 32 - dbits;
+  if ( dbits > 0 and N >= ( 1u << b.tgt_bits ) )
+    throw std::runtime_error( "NotImplemented: too many neurons for the delay range in one instance" );
+  std::vector< uint32_t > s_dev( E ), word( E );
+  std::vector< double > wv( E );
+  std::vector< uint32_t > dkey( E );
+  {
+    int64_t k = 0;
+    for ( const auto& C : conns_ )
+      for ( size_t i = 0; i < C.size(); ++i, ++k )
+      {
+        s_dev[ k ] = dev_of[ node( C.src[ i ] ).idx ];
+        const uint32_t t = dev_of[ node( C.tgt[ i ] ).idx ];
+        const uint32_t doff = static_cast< uint32_t >( C.d[ i ] - min_delay_ );
+        dkey[ k ] = doff;
+        word[ k ] = dbits > 0 ? ( t | ( doff << b.tgt_bits ) ) : t;
+        wv[ k ] = C.w[ i ];
+      }
+  }
+  std::vector< int64_t > perm1( E );
+  {
+    std::vector< int64_t > cnt( D + 1, 0 );
+    for ( int64_t k = 0; k < E; ++k )
+      cnt[ dkey[ k ] + 1 ]++;
+    for ( long q = 0; q < D; ++q )
+      cnt[ q + 1 ] += cnt[ q ];
+    for ( int64_t k = 0; k < E; ++k )
+      perm1[ cnt[ dkey[ k ] ]++ ] = k;
+  }
+  b.row_ptr.assign( N + 1, 0 );
+  for ( int64_t k = 0; k < E; ++k )
+    b.row_ptr[ s_dev[ k ] + 1 ]++;
+  for ( uint32_t q = 0; q < N; ++q )
+    b.row_ptr[ q + 1 ] += b.row_ptr[ q ];
+  b.tgt_d.resize( E );
+  b.w.resize( E );
+  {
+    std::vector< uint64_t > cur( b.row_ptr.begin(), b.row_ptr.end() - 1 );
+    for ( int64_t q = 0; q < E; ++q )
+    {
+      const int64_t k = perm1[ q ];
+      const uint64_t pos = cur[ s_dev[ k ] ]++;
+      b.tgt_d[ pos ] = word[ k ];
+      b.w[ pos ] = wv[ k ];
+    }
+  }
+  b.in_ptr.assign( N + 1, 0 );
+  const uint32_t tmask = b.tgt_bits >= 32 ? 0xffffffffu : ( ( 1u << b.tgt_bits ) - 1u );
+  for ( int64_t q = 0; q < E; ++q )
+    b.in_ptr[ ( b.tgt_d[ q ] & tmask ) + 1 ]++;
+  for ( uint32_t q = 0; q < N; ++q )
+    b.in_ptr[ q + 1 ] += b.in_ptr[ q ];
+  b.log_cap = std::max< uint64_t >( 1u << 20, static_cast< uint64_t >( N ) * 16 );
+}
+
+void
+Kernel::prepare()
+{
+  // SimulationManager::prepare (simulation_manager.cpp:507-570)
+  if ( not simulated_ )
+  {
+    min_delay_ = d_min_seen_ < 0 ? 1 : d_min_seen_;
+    max_delay_ = std::max( d_max_seen_, 1L );
+  }
+  if ( params_.empty() )
+  {
+    prepared_ = true;
+    return;
+  }
+  if ( not engine_ )
+    engine_ = std::make_unique< Engine >( device_ );
+  if ( store_dirty_ or not engine_->built() )
+  {
+    if ( simulated_ and state_dirty_ )
+      throw std::runtime_error( "NotImplemented: parameter changes after Simulate are not supported on the device path." );
+    collect_log();
+    BuildSpec b;
+    build_spec( b );
+    engine_->build( b, simulated_ );
+    store_dirty_ = false;
+    state_dirty_ = false;
+  }
+  prepared_ = true;
+}
+
+void
+Kernel::prepare_for_batch()
+{
+  if ( not prepared_ )
+    prepare();
+}
+
+void
+Kernel::after_batch_run( int64_t )
+{
+  simulated_ = true;
+  collect_log();
+}
+
+void
+Kernel::run( double t_ms )
+{
+  if ( not prepared_ )
+    throw std::runtime_error( "KernelException: Run called without calling Prepare." );
+  if ( t_ms < 0 )
+    throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
+  const long steps = tb_.ms_to_steps( t_ms );
+  if ( tb_.ms_to_tics( t_ms ) % tb_.tics_per_step != 0 )
+    throw std::runtime_error( "BadParameter: The simulation time must be a multiple of the simulation resolution." );
+  simulated_ = true;
+  if ( steps == 0 or params_.empty() )
+    return;
+  if ( not runner_ )
+  {
+    if ( not own_runner_ )
+      own_runner_ = std::make_unique< Runner >( device_, slices_per_chunk_ );
+    runner_ = own_runner_.get();
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  runner_->run( { engine_.get() }, { steps } );
+  const auto t1 = std::chrono::steady_clock::now();
+  last_wall_ms_ = std::chrono::duration< double, std::milli >( t1 - t0 ).count();
+  last_kernel_ms_ = runner_->last_kernel_ms();
+  collect_log();
+}
+
+void
+Kernel::cleanup()
+{
+  prepared_ = false;
+}
+
+void
+Kernel::simulate( double t_ms )
+{
+  prepare();
+  run( t_ms );
+  cleanup();
+}
+
+void
+Kernel::recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::vector< int64_t >& steps )
+{
+  const NodeRec& r = node( rec_gid );
+  if ( r.kind != 2 )
+    throw std::runtime_error( "BadProperty: node is not a spike_recorder" );
+  collect_log();
+  const auto& rec = recorders_[ r.idx ];
+  std::vector< std::pair< int64_t, int64_t > > ev;
+  for ( size_t i = 0; i < spike_sender_.size(); ++i )
+    if ( std::binary_search( rec.begin(), rec.end(), spike_sender_[ i ] ) )
+      ev.emplace_back( spike_step_[ i ], spike_sender_[ i ] );
+  std::sort( ev.begin(), ev.end() );
+  senders.resize( ev.size() );
+  steps.resize( ev.size() );
+  for ( size_t i = 0; i < ev.size(); ++i )
+  {
+    steps[ i ] = ev[ i ].first;
+    senders[ i ] = ev[ i ].second;
+  }
+}
+
+}  // namespace nestgpu

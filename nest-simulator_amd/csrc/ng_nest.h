@@ -1,0 +1,334 @@
+// ng_nest.h -- host kernel with NEST's Create/Connect/Simulate surface.
+//
+// Mirrors the parts of nestkernel that decide WHAT is simulated: time base
+// (nest_time.h), node placement on virtual processes (node_manager.cpp:188-230,
+// vp_manager_impl.h:43-52), the per-VP random streams (random_manager.cpp:42-100),
+// the connection builders and their draw order (conn_builder.cpp), delay extrema
+// (connection_manager.cpp:485-530) and the Prepare/Run/Cleanup cycle
+// (simulation_manager.cpp:507-671). The simulation itself runs on the device engine.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ng_device.h"
+#include "ng_engine.h"
+#include "ng_json.h"
+
+namespace nestgpu
+{
+
+// nestkernel/nest_time.cpp:36-47,88-103; nest_time.h:306-311,508-522,539-542
+struct TimeBase
+{
+  double tics_per_ms = 1000.0;
+  long tics_per_step = 100;
+  double tics_per_step_inv = 1.0 / 100.0;
+  long tics_per_step_rnd = 99;
+  double ms_per_tic = 1.0 / 1000.0;
+  double ms_per_step = 100 / 1000.0;
+  double steps_per_ms = 1 / ( 100 / 1000.0 );
+
+  void set_resolution( double ms );
+  long
+  ms_to_tics( double ms ) const
+  {
+    return static_cast< long >( ms * tics_per_ms + 0.5 );
+  }
+  long
+  tics_to_steps( long tics ) const
+  {
+    return static_cast< long >( ( tics + tics_per_step_rnd ) * tics_per_step_inv );
+  }
+  long
+  ms_to_steps( double ms ) const
+  {
+    return tics_to_steps( ms_to_tics( ms ) );
+  }
+  long delay_ms_to_steps( double ms ) const;
+  double
+  resolution_ms() const
+  {
+    return ms_per_tic * tics_per_step;
+  }
+};
+
+// randutils seed_seq_fe<4, uint32_t> (O'Neill), the seeding NEST applies to every engine
+struct SeedSeqFe128
+{
+  typedef uint32_t result_type;
+  uint32_t mixer[ 4 ];
+  explicit SeedSeqFe128( const std::vector< uint32_t >& entropy );
+  template < typename It >
+  void
+  generate( It first, It last ) const
+  {
+    uint32_t hc = 0x8b51f9ddu;
+    int src = 0;
+    for ( It dst = first; dst != last; ++dst )
+    {
+      uint32_t v = mixer[ src ];
+      src = ( src + 1 ) & 3;
+      v ^= hc;
+      hc *= 0x58f38dedu;
+      v *= hc;
+      v ^= v >> 16;
+      *dst = v;
+    }
+  }
+  static constexpr size_t
+  size()
+  {
+    return 4;
+  }
+};
+
+// RandomGenerator< std::mt19937_64 > (random_generators.h:141-297)
+struct VpRng
+{
+  std::mt19937_64 engine;
+  std::uniform_int_distribution< unsigned long > ulong_dist;
+  std::uniform_real_distribution<> unit_dist { 0.0, 1.0 };
+  explicit VpRng( const std::vector< uint32_t >& seed );
+  double
+  drand()
+  {
+    return unit_dist( engine );
+  }
+  unsigned long
+  ulrand( unsigned long n )
+  {
+    std::uniform_int_distribution< unsigned long >::param_type p( 0, n - 1 );
+    return ulong_dist( engine, p );
+  }
+};
+
+// nest::Parameter expression trees used by syn_spec weights/delays (parameter.h)
+class Param
+{
+public:
+  enum Op
+  {
+    CONST,
+    UNIFORM,
+    UNIFORM_INT,
+    NORMAL,
+    SUM,
+    PRODUCT,
+    MAX,
+    MIN,
+    REDRAW
+  };
+  static std::unique_ptr< Param > from_json( const Json& j, int n_vp );
+  double value( VpRng& rng, int vp );
+  bool
+  is_const() const
+  {
+    return op_ == CONST;
+  }
+  double
+  const_value() const
+  {
+    return a_;
+  }
+
+private:
+  Op op_ = CONST;
+  double a_ = 0, b_ = 0;
+  std::unique_ptr< Param > l_, r_;
+  std::vector< std::normal_distribution<> > normals_;  // one per VP (parameter.cpp:95-117)
+};
+
+struct NeuronParams
+{
+  int32_t model;
+  double tau_m, c_m, t_ref, E_L, I_e, V_th, V_min, V_reset;  // iaf: V values relative to E_L
+  double tau_ex, tau_in;
+  double V_peak, g_L, E_ex, E_in, Delta_T, tau_w, a, b, tol;  // aeif: absolute
+  double delta, rho;
+  int32_t refr_input;
+};
+
+struct NeuronState
+{
+  double v0, v1, v2, v3, v4;  // delta: y3 | exp: V, i_ex, i_in | aeif: V, g_ex, g_in, w, h_int
+  int32_t r;
+};
+
+struct SpikeGen
+{
+  std::vector< int64_t > stamps;  // spike stamps in steps (emission step = stamp - 1)
+  std::vector< int64_t > mults;
+  // dense per-step counts: column in a kernel-wide block
+  int dense_block = -1;
+  int64_t dense_col = 0;
+  // seeded Poisson train (rate per step, seed); stream id = generator node id
+  double poisson_lambda = 0;
+  uint64_t poisson_seed = 0;
+  int64_t poisson_t0 = 0, poisson_steps = 0;
+};
+
+struct DenseBlock
+{
+  int64_t t0 = 0, n_steps = 0, cols = 0;
+  std::vector< uint8_t > counts;  // [step][col]
+};
+
+// connection in creation order of its VP (SoA storage per VP)
+struct ConnList
+{
+  std::vector< uint32_t > src, tgt;
+  std::vector< double > w;
+  std::vector< uint16_t > d;
+  size_t
+  size() const
+  {
+    return src.size();
+  }
+};
+
+class Kernel
+{
+public:
+  explicit Kernel( int device );
+  ~Kernel();
+
+  void reset();
+  void set_kernel_status( const Json& d );
+  std::string get_kernel_status_json() const;
+
+  int64_t create( const std::string& model, int64_t n, const Json& params );
+  void set_status( const std::vector< int64_t >& gids, const Json& params );
+  void set_values( const std::vector< int64_t >& gids, const std::string& key, const double* v );
+  void get_values( const std::vector< int64_t >& gids, const std::string& key, double* out );
+  std::string model_name( int64_t gid ) const;
+
+  void connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const Json& conn_spec,
+    const Json& syn_spec );
+  void connect_arrays( const int64_t* src, const int64_t* tgt, const double* w, const double* d_ms, int64_t n );
+
+  void set_spike_counts( int64_t gen_first, int64_t n, int64_t t0_step, int64_t n_steps, const uint8_t* counts );
+
+  int64_t num_connections() const;
+  void get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int32_t* vp ) const;
+
+  void prepare();
+  void run( double t_ms );
+  void cleanup();
+  void simulate( double t_ms );
+
+  // spike_recorder events (senders, stamp steps), sorted by (time, sender)
+  void recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::vector< int64_t >& steps );
+
+  // batched / served use
+  Engine& engine()
+  {
+    return *engine_;
+  }
+  bool prepared() const
+  {
+    return prepared_;
+  }
+  void prepare_for_batch();  // prepare without running (idempotent)
+  void after_batch_run( int64_t steps );
+  int64_t steps_for( double t_ms ) const
+  {
+    return tb_.ms_to_steps( t_ms );
+  }
+  const TimeBase& timebase() const
+  {
+    return tb_;
+  }
+  int64_t min_delay() const
+  {
+    return min_delay_;
+  }
+  double last_run_kernel_ms() const
+  {
+    return last_kernel_ms_;
+  }
+  double last_run_wall_ms() const
+  {
+    return last_wall_ms_;
+  }
+  uint64_t local_spike_counter() const
+  {
+    return spike_count_;
+  }
+  uint64_t num_neurons() const
+  {
+    return params_.size();
+  }
+  void set_runner( Runner* r )
+  {
+    runner_ = r;
+  }
+
+private:
+  struct NodeRec
+  {
+    uint8_t kind;  // 0 neuron, 1 spike_generator, 2 spike_recorder
+    uint32_t idx;
+  };
+  NodeRec& node( int64_t gid );
+  const NodeRec& node( int64_t gid ) const;
+  int vp_of( int64_t gid ) const
+  {
+    return static_cast< int >( gid % n_vp_ );
+  }
+  void init_rngs();
+  static NeuronParams default_params( int model );
+  void apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool creating );
+  void check_delay( long steps );
+  void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
+  void build_spec( BuildSpec& b );
+  void collect_log();
+
+  int device_;
+  TimeBase tb_;
+  int n_vp_ = 1;
+  uint32_t seed_ = 143202461;
+  std::vector< VpRng > rngs_;
+  bool rng_dirty_ = true;
+  std::vector< NodeRec > nodes_;  // index gid - 1
+  std::vector< NeuronParams > params_;
+  std::vector< NeuronState > state_;
+  std::vector< int64_t > neuron_gid_;
+  std::vector< SpikeGen > gens_;
+  std::vector< int64_t > gen_gid_;
+  std::vector< DenseBlock > dense_;
+  std::vector< std::vector< int64_t > > recorders_;  // recorded senders (sorted) per recorder
+  std::vector< int64_t > rec_gid_;
+  std::vector< ConnList > conns_;  // per VP
+  struct DevConn
+  {
+    int64_t gen, tgt;
+    double w;
+    long d;
+  };
+  std::vector< DevConn > dev_conns_;
+  long d_min_seen_ = -1, d_max_seen_ = 1;
+  long min_delay_ = 1, max_delay_ = 1;
+  bool simulated_ = false, prepared_ = false, store_dirty_ = true, state_dirty_ = false;
+  std::unique_ptr< Engine > engine_;
+  std::unique_ptr< Runner > own_runner_;
+  Runner* runner_ = nullptr;
+  std::vector< int32_t > gid_to_dev_;  // neuron gid -> device index (valid after prepare)
+  std::vector< int64_t > dev_to_gid_;
+  std::vector< int64_t > spike_sender_, spike_step_;  // all spikes so far
+  uint64_t spike_count_ = 0;
+  double last_kernel_ms_ = 0, last_wall_ms_ = 0;
+  int slices_per_chunk_ = 64;
+};
+
+// Seeded Poisson counts (the pre-generated drive) for streams [stream0, stream0+n) and
+// emission steps [step0, step0+n_steps); out is [step][stream]. Identical on the device.
+void poisson_counts( uint64_t seed, double lambda, uint64_t stream0, int64_t n, int64_t step0, int64_t n_steps,
+  uint8_t* out );
+PoissonTable make_poisson_table( double lambda, uint64_t seed );
+
+}  // namespace nestgpu

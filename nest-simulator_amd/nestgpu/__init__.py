@@ -1,0 +1,390 @@
+"""nestgpu -- PyNEST-shaped Python binding of the MI355X point-neuron simulation step.
+
+The calls mirror PyNEST's (pynest/nest/lib/hl_api_*.py): ResetKernel, SetKernelStatus,
+GetKernelStatus, Create, Connect, Simulate/Prepare/Run/Cleanup, GetConnections,
+NodeCollection.get/set, spike_recorder events and the nest.random / nest.math parameter
+objects. Every call goes through the C ABI declared in include/nestgpu.h
+(libnestgpu.so, built in place by `make -C nest-simulator_amd`). The library must be
+present: there is no CPU fallback.
+"""
+
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnestgpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libnestgpu.so is not built (%s); run `make -C nest-simulator_amd` or __graft_entry__.build()" % LIB_PATH)
+
+_lib = ctypes.CDLL(LIB_PATH)
+_vp = ctypes.c_void_p
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = {
+    "nestgpu_kernel_new": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "nestgpu_kernel_free": (None, [_vp]),
+    "nestgpu_last_error": (ctypes.c_char_p, [_vp]),
+    "nestgpu_reset_kernel": (ctypes.c_int, [_vp]),
+    "nestgpu_set_kernel_status": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "nestgpu_get_kernel_status": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
+    "nestgpu_create": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, _i64p]),
+    "nestgpu_set_status": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64, ctypes.c_char_p]),
+    "nestgpu_set_values": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64, ctypes.c_char_p, _f64p]),
+    "nestgpu_get_values": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64, ctypes.c_char_p, _f64p]),
+    "nestgpu_connect": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, ctypes.c_char_p,
+                                       ctypes.c_char_p]),
+    "nestgpu_connect_arrays": (ctypes.c_int, [_vp, _i64p, _i64p, _f64p, _f64p, ctypes.c_int64]),
+    "nestgpu_set_spike_counts": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_uint8)]),
+    "nestgpu_num_connections": (ctypes.c_int64, [_vp]),
+    "nestgpu_get_connections": (ctypes.c_int, [_vp, _i64p, _i64p, _f64p, _i64p, ctypes.POINTER(ctypes.c_int32)]),
+    "nestgpu_prepare": (ctypes.c_int, [_vp]),
+    "nestgpu_run": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "nestgpu_cleanup": (ctypes.c_int, [_vp]),
+    "nestgpu_simulate": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "nestgpu_recorder_events": (ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _f64p, ctypes.c_int64, _i64p]),
+    "nestgpu_batch_simulate": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
+    "nestgpu_poisson_counts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64,
+                                              ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]),
+    "nestgpu_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class NESTError(RuntimeError):
+    """Raised when a kernel call fails; the message is the kernel's error text."""
+
+
+def _i64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64).ravel())
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _lib.nestgpu_device_count(ctypes.byref(n))
+    return n.value
+
+
+def poisson_counts(seed, lam, stream0, n, step0, n_steps):
+    """Seeded Poisson counts [n_steps, n] (uint8) -- the drive of spike_generator(poisson_rate=...)."""
+    out = np.zeros((n_steps, n), dtype=np.uint8)
+    if _lib.nestgpu_poisson_counts(int(seed), float(lam), int(stream0), int(n), int(step0), int(n_steps),
+                                   _ptr(out, ctypes.c_uint8)) != 0:
+        raise NESTError(_lib.nestgpu_last_error(None).decode())
+    return out
+
+
+# ------------------------------------------------------------------ parameters
+class Parameter:
+    """nest.Parameter expression (pynest/nest/lib/hl_api_types.py) serialised as JSON."""
+
+    def __init__(self, spec):
+        self.spec = spec
+
+    @staticmethod
+    def _wrap(x):
+        return x.spec if isinstance(x, Parameter) else float(x)
+
+    def __add__(self, other):
+        return Parameter({"sum": [self.spec, Parameter._wrap(other)]})
+
+    __radd__ = __add__
+
+    def __mul__(self, other):
+        return Parameter({"product": [self.spec, Parameter._wrap(other)]})
+
+    __rmul__ = __mul__
+
+    def __sub__(self, other):
+        return self + (-1.0) * other if isinstance(other, Parameter) else self + (-float(other))
+
+    def __neg__(self):
+        return self * -1.0
+
+
+class _Random:
+    @staticmethod
+    def uniform(min=0.0, max=1.0):
+        return Parameter({"uniform": {"min": float(min), "max": float(max)}})
+
+    @staticmethod
+    def uniform_int(max):
+        return Parameter({"uniform_int": {"max": int(max)}})
+
+    @staticmethod
+    def normal(mean=0.0, std=1.0):
+        return Parameter({"normal": {"mean": float(mean), "std": float(std)}})
+
+
+class _Math:
+    @staticmethod
+    def max(p, value):
+        return Parameter({"max": [Parameter._wrap(p), float(value)]})
+
+    @staticmethod
+    def min(p, value):
+        return Parameter({"min": [Parameter._wrap(p), float(value)]})
+
+    @staticmethod
+    def redraw(p, min, max):
+        return Parameter({"redraw": [Parameter._wrap(p), float(min), float(max)]})
+
+
+random = _Random()
+math = _Math()
+
+
+def _to_json(d):
+    def conv(x):
+        if isinstance(x, Parameter):
+            return x.spec
+        if isinstance(x, np.ndarray):
+            return x.tolist()
+        if isinstance(x, (np.floating, np.integer)):
+            return x.item()
+        if isinstance(x, dict):
+            return {k: conv(v) for k, v in x.items()}
+        if isinstance(x, (list, tuple)):
+            return [conv(v) for v in x]
+        return x
+
+    return json.dumps(conv(d if d is not None else {}))
+
+
+# ------------------------------------------------------------------ nodes
+class NodeCollection:
+    """Ids of nodes of one kernel (pynest NodeCollection subset)."""
+
+    def __init__(self, kernel, ids):
+        self._k = kernel
+        self._ids = _i64(ids)
+
+    def __len__(self):
+        return len(self._ids)
+
+    def __iter__(self):
+        for i in self._ids:
+            yield NodeCollection(self._k, [i])
+
+    def __getitem__(self, key):
+        if isinstance(key, (int, np.integer)):
+            return NodeCollection(self._k, [self._ids[key]])
+        return NodeCollection(self._k, self._ids[key])
+
+    def __add__(self, other):
+        return NodeCollection(self._k, np.concatenate([self._ids, other._ids]))
+
+    def tolist(self):
+        return [int(i) for i in self._ids]
+
+    @property
+    def global_id(self):
+        return self._ids.copy()
+
+    def get(self, key, output=None):
+        if key == "events":
+            return self._k._events(self)
+        if isinstance(key, (list, tuple)):
+            return {k: self.get(k) for k in key}
+        out = np.zeros(len(self._ids))
+        self._k._chk(_lib.nestgpu_get_values(self._k._h, _ptr(self._ids, ctypes.c_int64), len(self._ids),
+                                             key.encode(), _ptr(out, ctypes.c_double)))
+        return out[0] if len(out) == 1 else out
+
+    def set(self, params=None, **kwargs):
+        params = dict(params or {}, **kwargs)
+        scalar = {}
+        for k, v in params.items():
+            if np.ndim(v) > 0 and k not in ("spike_times", "spike_multiplicities"):
+                vals = np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
+                if len(vals) != len(self._ids):
+                    raise NESTError("value array length does not match the NodeCollection")
+                self._k._chk(_lib.nestgpu_set_values(self._k._h, _ptr(self._ids, ctypes.c_int64), len(self._ids),
+                                                     k.encode(), _ptr(vals, ctypes.c_double)))
+            elif isinstance(v, Parameter):
+                raise NESTError("random node parameters: draw them in Python and pass an array")
+            else:
+                scalar[k] = v
+        if scalar:
+            self._k._chk(_lib.nestgpu_set_status(self._k._h, _ptr(self._ids, ctypes.c_int64), len(self._ids),
+                                                 _to_json(scalar).encode()))
+
+    def __repr__(self):
+        return "NodeCollection(ids=%s)" % (self._ids.tolist() if len(self._ids) < 8 else
+                                           "%d..%d (%d)" % (self._ids[0], self._ids[-1], len(self._ids)))
+
+
+# ------------------------------------------------------------------ kernel
+class Kernel:
+    """One simulation kernel (NEST has a single global kernel; here any number can coexist)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        if _lib.nestgpu_kernel_new(int(device), ctypes.byref(h)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        self._h = h
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.nestgpu_kernel_free(h)
+            self._h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise NESTError(_lib.nestgpu_last_error(self._h).decode())
+
+    def ResetKernel(self):
+        self._chk(_lib.nestgpu_reset_kernel(self._h))
+
+    def SetKernelStatus(self, params):
+        self._chk(_lib.nestgpu_set_kernel_status(self._h, _to_json(params).encode()))
+
+    def GetKernelStatus(self, key=None):
+        n = ctypes.c_size_t(0)
+        self._chk(_lib.nestgpu_get_kernel_status(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        self._chk(_lib.nestgpu_get_kernel_status(self._h, buf, n.value + 1, ctypes.byref(n)))
+        d = json.loads(buf.value.decode())
+        return d if key is None else d[key]
+
+    def Create(self, model, n=1, params=None):
+        first = ctypes.c_int64(0)
+        self._chk(_lib.nestgpu_create(self._h, model.encode(), int(n), _to_json(params or {}).encode(),
+                                      ctypes.byref(first)))
+        return NodeCollection(self, np.arange(first.value, first.value + n))
+
+    def Connect(self, pre, post, conn_spec=None, syn_spec=None):
+        pre_ids = pre._ids if isinstance(pre, NodeCollection) else _i64(pre)
+        post_ids = post._ids if isinstance(post, NodeCollection) else _i64(post)
+        if conn_spec is None:
+            conn_spec = "all_to_all"
+        if isinstance(syn_spec, str):
+            syn_spec = {"synapse_model": syn_spec}
+        self._chk(_lib.nestgpu_connect(self._h, _ptr(pre_ids, ctypes.c_int64), len(pre_ids),
+                                       _ptr(post_ids, ctypes.c_int64), len(post_ids),
+                                       _to_json(conn_spec).encode(), _to_json(syn_spec or {}).encode()))
+
+    def ConnectArrays(self, sources, targets, weights=None, delays=None):
+        s, t = _i64(sources), _i64(targets)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        d = None if delays is None else np.ascontiguousarray(delays, dtype=np.float64)
+        self._chk(_lib.nestgpu_connect_arrays(self._h, _ptr(s, ctypes.c_int64), _ptr(t, ctypes.c_int64),
+                                              None if w is None else _ptr(w, ctypes.c_double),
+                                              None if d is None else _ptr(d, ctypes.c_double), len(s)))
+
+    def SetSpikeCounts(self, generators, counts, first_step=0):
+        c = np.ascontiguousarray(counts, dtype=np.uint8)
+        ids = generators._ids
+        if c.ndim != 2 or c.shape[1] != len(ids) or np.any(np.diff(ids) != 1):
+            raise NESTError("counts must be [n_steps, n_generators] for contiguous generator ids")
+        self._chk(_lib.nestgpu_set_spike_counts(self._h, int(ids[0]), len(ids), int(first_step), c.shape[0],
+                                                _ptr(c, ctypes.c_uint8)))
+
+    def GetConnections(self):
+        n = _lib.nestgpu_num_connections(self._h)
+        s = np.zeros(n, np.int64)
+        t = np.zeros(n, np.int64)
+        w = np.zeros(n)
+        d = np.zeros(n, np.int64)
+        vp = np.zeros(n, np.int32)
+        self._chk(_lib.nestgpu_get_connections(self._h, _ptr(s, ctypes.c_int64), _ptr(t, ctypes.c_int64),
+                                               _ptr(w, ctypes.c_double), _ptr(d, ctypes.c_int64),
+                                               _ptr(vp, ctypes.c_int32)))
+        return {"source": s, "target": t, "weight": w, "delay_steps": d, "vp": vp}
+
+    def Prepare(self):
+        self._chk(_lib.nestgpu_prepare(self._h))
+
+    def Run(self, t):
+        self._chk(_lib.nestgpu_run(self._h, float(t)))
+
+    def Cleanup(self):
+        self._chk(_lib.nestgpu_cleanup(self._h))
+
+    def Simulate(self, t):
+        self._chk(_lib.nestgpu_simulate(self._h, float(t)))
+
+    def _events(self, rec):
+        rid = int(rec._ids[0])
+        n = ctypes.c_int64(0)
+        self._chk(_lib.nestgpu_recorder_events(self._h, rid, None, None, 0, ctypes.byref(n)))
+        s = np.zeros(n.value, np.int64)
+        tm = np.zeros(n.value)
+        self._chk(_lib.nestgpu_recorder_events(self._h, rid, _ptr(s, ctypes.c_int64), _ptr(tm, ctypes.c_double),
+                                               n.value, ctypes.byref(n)))
+        return {"senders": s, "times": tm}
+
+
+def batch_simulate(kernels, times):
+    """Advance several kernels on one device together (batched form)."""
+    arr = (_vp * len(kernels))(*[k._h for k in kernels])
+    t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (len(kernels),)))
+    if _lib.nestgpu_batch_simulate(arr, len(kernels), _ptr(t, ctypes.c_double)) != 0:
+        raise NESTError(_lib.nestgpu_last_error(None).decode())
+
+
+# ------------------------------------------------------------------ module-level kernel (PyNEST style)
+_default = None
+
+
+def _k():
+    global _default
+    if _default is None:
+        _default = Kernel(0)
+    return _default
+
+
+def ResetKernel():
+    _k().ResetKernel()
+
+
+def SetKernelStatus(params):
+    _k().SetKernelStatus(params)
+
+
+def GetKernelStatus(key=None):
+    return _k().GetKernelStatus(key)
+
+
+def Create(model, n=1, params=None):
+    return _k().Create(model, n, params)
+
+
+def Connect(pre, post, conn_spec=None, syn_spec=None):
+    _k().Connect(pre, post, conn_spec, syn_spec)
+
+
+def GetConnections():
+    return _k().GetConnections()
+
+
+def Simulate(t):
+    _k().Simulate(t)
+
+
+def Prepare():
+    _k().Prepare()
+
+
+def Run(t):
+    _k().Run(t)
+
+
+def Cleanup():
+    _k().Cleanup()

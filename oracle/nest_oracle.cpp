@@ -1,0 +1,1596 @@
+// nest_oracle.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// A single-process CPU restatement of NEST's per-min_delay-slice simulation step
+// for point-neuron networks (iaf_psc_delta, iaf_psc_exp, aeif_cond_exp,
+// static_synapse, spike_generator drive), written to be read side by side with
+// the reference sources it follows. It exists to CHECK the MI355X product path:
+// only the test suite, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// load it. Nothing in the product links, calls or falls back to it.
+//
+// Reference files followed (paths relative to the NEST source tree):
+//   time base ........ nestkernel/nest_time.h:306-311,508-522,539-542; nest_time.cpp:88-103;
+//                      libnestutil/numerics.cpp:41-44 (ld_round)
+//   RNG .............. nestkernel/random_manager.cpp:42-48,95; random_generators.h:141-297;
+//                      thirdparty/randutils.hpp:215-385 (seed_seq_fe128, restated below)
+//   parameters ....... nestkernel/parameter.h:131-286,468-600,878-990; parameter.cpp:95-117
+//   connect .......... nestkernel/conn_builder.cpp:379-437 (single_connect_), 960-1031 (one_to_one),
+//                      1170-1255 (all_to_all), 1448-1545 (fixed_indegree), 1881-1965 (bernoulli);
+//                      connector_model_impl.h:220-319; delay_checker.cpp:146-171
+//   placement ........ nestkernel/vp_manager_impl.h:43-52; node_manager.cpp:188-230
+//   slice loop ....... nestkernel/simulation_manager.cpp:614-671,842-1189,1192-1223
+//   send/gather ...... nestkernel/event_delivery_manager_impl.h:35-118;
+//                      event_delivery_manager.cpp:251-303,368-497
+//   deliver .......... event_delivery_manager.cpp:599-776; connector_base.h:399-428;
+//                      models/static_synapse.h:160-169; nestkernel/event.h:1006-1013
+//   ring buffers ..... nestkernel/ring_buffer.h:157-203,395-408
+//   models ........... models/iaf_psc_delta.cpp:69-343; models/iaf_psc_exp.cpp:71-363;
+//                      libnestutil/iaf_propagator.cpp:42-74; models/aeif_cond_exp.cpp:79-541
+//   drive ............ models/spike_generator.cpp:321-389
+//
+// aeif_cond_exp: NEST integrates it with GSL's rkf45 stepper, the y' error control and
+// gsl_odeiv_evolve_apply (aeif_cond_exp.cpp:386,395,463). GSL is not part of the
+// reference tree; its published algorithm (the Runge-Kutta-Fehlberg 4(5) tableau and the
+// documented step-size control rule) is restated in rkf45_evolve() below. Bit parity with
+// GSL itself is therefore unpinned; the restatement is pinned against the LSODAR trajectory
+// the reference's test suite holds and against closed-form special cases.
+//
+// Compile with -O2 -ffp-contract=off and no -ffast-math: expression order and rounding
+// follow the reference line by line.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle
+{
+
+// ------------------------------------------------------------------ time base
+// nestkernel/nest_time.cpp:36-47,88-103 and nest_time.h:306-311,508-522,539-542
+struct TimeBase
+{
+  double tics_per_ms = 1000.0;
+  long tics_per_step = 100;
+  double tics_per_step_inv = 1.0 / 100.0;
+  long tics_per_step_rnd = 99;
+  double ms_per_tic = 1.0 / 1000.0;
+  double ms_per_step = 100 / 1000.0;
+  double steps_per_ms = 1 / ( 100 / 1000.0 );
+
+  void
+  set_resolution( double ms )
+  {
+    tics_per_step = static_cast< long >( std::floor( tics_per_ms * ms + 0.5 ) );
+    tics_per_step_inv = 1. / static_cast< double >( tics_per_step );
+    tics_per_step_rnd = tics_per_step - 1;
+    ms_per_step = tics_per_step / tics_per_ms;
+    steps_per_ms = 1 / ms_per_step;
+  }
+  long
+  ms_to_tics( double ms ) const
+  {
+    return static_cast< long >( ms * tics_per_ms + 0.5 );
+  }
+  long
+  tics_to_steps( long tics ) const
+  {
+    return static_cast< long >( ( tics + tics_per_step_rnd ) * tics_per_step_inv );
+  }
+  long
+  ms_to_steps( double ms ) const
+  {
+    return tics_to_steps( ms_to_tics( ms ) );
+  }
+  bool
+  is_grid_ms( double ms ) const
+  {
+    return ms_to_tics( ms ) % tics_per_step == 0;
+  }
+  long
+  delay_ms_to_steps( double ms ) const
+  {
+    return static_cast< long >( std::floor( ms * steps_per_ms + 0.5 ) );
+  }
+  double
+  resolution_ms() const
+  {
+    return ms_per_tic * tics_per_step;
+  }
+};
+
+// ------------------------------------------------------------- seed sequence
+// O'Neill's seed_seq_fe<4, uint32_t> (randutils), restated from its published algorithm:
+// hash the entropy words into a 4-word mixer, cross-mix every pair, then emit output words
+// through a second multiplicative hash. mix_rounds is 1 for four words.
+struct SeedSeqFe128
+{
+  typedef uint32_t result_type;
+  uint32_t m[ 4 ];
+
+  explicit SeedSeqFe128( const std::vector< uint32_t >& in )
+  {
+    uint32_t hc = 0x43b0d7e5u;
+    auto hash = [ &hc ]( uint32_t v )
+    {
+      v ^= hc;
+      hc *= 0x931e8875u;
+      v *= hc;
+      v ^= v >> 16;
+      return v;
+    };
+    auto mix = []( uint32_t x, uint32_t y )
+    {
+      uint32_t r = 0xca01f9ddu * x - 0x4973f715u * y;
+      r ^= r >> 16;
+      return r;
+    };
+    size_t cur = 0;
+    for ( int i = 0; i < 4; ++i )
+      m[ i ] = hash( cur < in.size() ? in[ cur++ ] : 0u );
+    for ( int s = 0; s < 4; ++s )
+      for ( int d = 0; d < 4; ++d )
+        if ( s != d )
+          m[ d ] = mix( m[ d ], hash( m[ s ] ) );
+    for ( ; cur < in.size(); ++cur )
+      for ( int d = 0; d < 4; ++d )
+        m[ d ] = mix( m[ d ], hash( in[ cur ] ) );
+  }
+  template < typename It >
+  void
+  generate( It first, It last ) const
+  {
+    uint32_t hc = 0x8b51f9ddu;
+    int s = 0;
+    for ( It d = first; d != last; ++d )
+    {
+      uint32_t v = m[ s ];
+      s = ( s + 1 ) & 3;
+      v ^= hc;
+      hc *= 0x58f38dedu;
+      v *= hc;
+      v ^= v >> 16;
+      *d = v;
+    }
+  }
+  static constexpr size_t
+  size()
+  {
+    return 4;
+  }
+};
+
+// nestkernel/random_generators.h:141-297: engine + the three draws the builders use.
+struct Rng
+{
+  std::mt19937_64 eng;
+  std::uniform_int_distribution< unsigned long > ul;
+  std::uniform_real_distribution<> u01 { 0.0, 1.0 };
+  explicit Rng( const std::vector< uint32_t >& seed )
+  {
+    SeedSeqFe128 sseq( seed );
+    eng.seed( sseq );
+  }
+  double
+  drand()
+  {
+    return u01( eng );
+  }
+  unsigned long
+  ulrand( unsigned long n )
+  {
+    std::uniform_int_distribution< unsigned long >::param_type p( 0, n - 1 );
+    return ul( eng, p );
+  }
+};
+
+// ------------------------------------------------------------------ parameters
+// Prefix-encoded expression tree over doubles; op codes:
+//   0 const v | 1 uniform min max | 2 uniform_int max | 3 normal mean std
+//   4 sum A B | 5 product A B     | 6 max A v          | 7 min A v | 8 redraw A min max
+// nestkernel/parameter.h:131-286 (leaves), 468-600 (product/sum), 878-990 (max/redraw)
+struct Param
+{
+  int op = 0;
+  double a = 0, b = 0;
+  std::unique_ptr< Param > l, r;
+  std::vector< std::normal_distribution<> > normals;  // one per VP (parameter.cpp:95-117)
+
+  static std::unique_ptr< Param >
+  parse( const double*& p, const double* end, int n_vp )
+  {
+    if ( p >= end )
+      throw std::runtime_error( "truncated parameter spec" );
+    auto x = std::make_unique< Param >();
+    x->op = static_cast< int >( *p++ );
+    switch ( x->op )
+    {
+    case 0:
+      x->a = *p++;
+      break;
+    case 1:
+      x->a = *p++;
+      x->b = *p++;
+      if ( x->a >= x->b )
+        throw std::runtime_error( "nest::UniformParameter: min < max required." );
+      x->b -= x->a;  // range
+      break;
+    case 2:
+      x->a = *p++;
+      if ( x->a <= 0 )
+        throw std::runtime_error( "nest::UniformIntParameter: max > 0 required." );
+      break;
+    case 3:
+      x->a = *p++;
+      x->b = *p++;
+      if ( x->b < 0 )
+        throw std::runtime_error( "nest::NormalParameter: std >= 0 required." );
+      for ( int i = 0; i < n_vp; ++i )
+        x->normals.emplace_back( x->a, x->b );
+      break;
+    case 4:
+    case 5:
+      x->l = parse( p, end, n_vp );
+      x->r = parse( p, end, n_vp );
+      break;
+    case 6:
+    case 7:
+      x->l = parse( p, end, n_vp );
+      x->a = *p++;
+      break;
+    case 8:
+      x->l = parse( p, end, n_vp );
+      x->a = *p++;
+      x->b = *p++;
+      break;
+    default:
+      throw std::runtime_error( "unknown parameter op" );
+    }
+    return x;
+  }
+
+  double
+  value( Rng& rng, int vp )
+  {
+    switch ( op )
+    {
+    case 0:
+      return a;
+    case 1:
+      return a + rng.drand() * b;
+    case 2:
+      return static_cast< double >( rng.ulrand( static_cast< unsigned long >( a ) ) );
+    case 3:
+      if ( b == 0 )
+        return a;
+      return normals[ vp ]( rng.eng );
+    case 4:
+    {
+      const double x = l->value( rng, vp );
+      return x + r->value( rng, vp );
+    }
+    case 5:
+    {
+      const double x = l->value( rng, vp );
+      return x * r->value( rng, vp );
+    }
+    case 6:
+      return std::max( l->value( rng, vp ), a );
+    case 7:
+      return std::min( l->value( rng, vp ), a );
+    case 8:
+    {
+      // nestkernel/parameter.cpp RedrawParameter::value: redraw until inside [min, max]
+      for ( int i = 0; i < 1000; ++i )
+      {
+        const double v = l->value( rng, vp );
+        if ( a <= v and v <= b )
+          return v;
+      }
+      throw std::runtime_error( "Number of redraws exceeded limit." );
+    }
+    }
+    return 0;
+  }
+  bool
+  is_const() const
+  {
+    return op == 0;
+  }
+};
+
+// ------------------------------------------------------------------ models
+enum Model
+{
+  IAF_PSC_DELTA = 0,
+  IAF_PSC_EXP = 1,
+  AEIF_COND_EXP = 2
+};
+
+struct Neuron
+{
+  int model;
+  long gid;
+  // iaf_psc_delta (values relative to E_L where NEST stores them so)
+  double tau_m, c_m, t_ref, E_L, I_e, V_th, V_min, V_reset;
+  bool refr_input = false;
+  // iaf_psc_exp
+  double tau_ex, tau_in;
+  // aeif_cond_exp (absolute values)
+  double V_peak, g_L, E_ex, E_in, Delta_T, tau_w, a, b, tol;
+  // state
+  double y0 = 0, y3 = 0, refr_buf = 0;          // delta
+  double V = 0, i_ex = 0, i_in = 0, i0 = 0, i1 = 0;  // exp
+  double ay[ 4 ] = { 0, 0, 0, 0 };               // aeif V, g_ex, g_in, w
+  double I_stim = 0, hstep = 0, integ = 0;  // aeif: B_.step_, B_.IntegrationStep_
+  long r = 0;
+  // propagators (pre_run_hook)
+  double P33, P30, P11ex, P11in, P22, P21ex, P21in, P20, V_peak_eff;
+  long RefractoryCounts;
+  // ring buffers: ch 0..3 ([SYN_IN, SYN_EX, I0, I1] for exp; [spikes, currents] for delta;
+  // [exc, inh, currents] for aeif), slot-major
+  std::vector< double > rb;
+  bool initialized = false;
+};
+
+// IAFPropagator::evaluate_P32_ (libnestutil/iaf_propagator.cpp:42-74)
+static double
+iaf_propagator_p32( double tau_syn, double tau_m, double c_m, double h )
+{
+  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
+  const double gamma = beta / c_m;
+  const double inv_tau_syn = 1 / tau_syn;
+  const double inv_tau_m = 1 / tau_m;
+  const double inv_c_m = 1 / c_m;
+  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
+  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
+  const double expm1_h_tau = std::expm1( h * inv_beta );
+  const double P32 = gamma * exp_h_tau_syn * expm1_h_tau;
+  if ( std::isnormal( P32 ) and P32 > 0 )
+    return P32;
+  const double exp_h_tau = std::exp( -h * inv_tau_m );
+  return h * inv_c_m * exp_h_tau;
+}
+
+// ----------------------------------------------------- portable exp/log (fdlibm)
+// Sun fdlibm e_exp.c / e_log.c method (argument reduction by ln2, rational / polynomial
+// kernels). Used only in "portable" aeif mode so that a CPU run can be compared bit for bit
+// with device code that uses the same published method.
+static double
+fd_exp( double x )
+{
+  const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10,
+               invln2 = 1.44269504088896338700e+00;
+  const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03,
+               P3 = 6.61375632143793436117e-05, P4 = -1.65339022054652515390e-06,
+               P5 = 4.13813679705723846039e-08;
+  if ( x != x )
+    return x;
+  if ( x > 709.782712893383973096 )
+    return std::numeric_limits< double >::infinity();
+  if ( x < -745.13321910194110842 )
+    return 0.0;
+  const double kd = std::floor( x * invln2 + 0.5 );
+  const int k = static_cast< int >( kd );
+  const double hi = x - kd * ln2hi;
+  const double lo = kd * ln2lo;
+  const double rr = hi - lo;
+  const double t = rr * rr;
+  const double c = rr - t * ( P1 + t * ( P2 + t * ( P3 + t * ( P4 + t * P5 ) ) ) );
+  const double y = 1.0 - ( ( lo - ( rr * c ) / ( 2.0 - c ) ) - hi );
+  return std::ldexp( y, k );
+}
+
+static double
+fd_log( double x )
+{
+  const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+  const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01, Lg3 = 2.857142874366239149e-01,
+               Lg4 = 2.222219843214978396e-01, Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
+               Lg7 = 1.479819860511658591e-01;
+  if ( !( x > 0 ) )
+    return x == 0 ? -std::numeric_limits< double >::infinity() : std::numeric_limits< double >::quiet_NaN();
+  if ( x == std::numeric_limits< double >::infinity() )
+    return x;
+  int e;
+  double mnt = std::frexp( x, &e );
+  if ( mnt < 0.70710678118654752440 )
+  {
+    mnt = mnt * 2.0;
+    e -= 1;
+  }
+  const double f = mnt - 1.0;
+  const double kd = static_cast< double >( e );
+  const double s = f / ( 2.0 + f );
+  const double z = s * s;
+  const double w = z * z;
+  const double t1 = w * ( Lg2 + w * ( Lg4 + w * Lg6 ) );
+  const double t2 = z * ( Lg1 + w * ( Lg3 + w * ( Lg5 + w * Lg7 ) ) );
+  const double R = t2 + t1;
+  const double hfsq = 0.5 * f * f;
+  return kd * ln2hi - ( ( hfsq - ( s * ( hfsq + R ) + kd * ln2lo ) ) - f );
+}
+
+struct MathMode
+{
+  bool portable = false;
+  double
+  exp( double x ) const
+  {
+    return portable ? fd_exp( x ) : std::exp( x );
+  }
+  double
+  pow( double x, double y ) const
+  {
+    return portable ? fd_exp( y * fd_log( x ) ) : std::pow( x, y );
+  }
+};
+
+// aeif_cond_exp_dynamics (models/aeif_cond_exp.cpp:79-124)
+static void
+aeif_rhs( const Neuron& n, const double* y, double* f, const MathMode& mm )
+{
+  const bool is_refractory = n.r > 0;
+  const double V = is_refractory ? n.V_reset : std::min( y[ 0 ], n.V_peak );
+  const double g_ex = y[ 1 ];
+  const double g_in = y[ 2 ];
+  const double w = y[ 3 ];
+  const double I_syn_exc = g_ex * ( V - n.E_ex );
+  const double I_syn_inh = g_in * ( V - n.E_in );
+  const double I_spike = n.Delta_T == 0. ? 0. : ( n.g_L * n.Delta_T * mm.exp( ( V - n.V_th ) / n.Delta_T ) );
+  f[ 0 ] = is_refractory ? 0. : ( -n.g_L * ( V - n.E_L ) + I_spike - I_syn_exc - I_syn_inh - w + n.I_e + n.I_stim ) / n.c_m;
+  f[ 1 ] = -g_ex / n.tau_ex;
+  f[ 2 ] = -g_in / n.tau_in;
+  f[ 3 ] = ( n.a * ( V - n.E_L ) - w ) / n.tau_w;
+}
+
+// Runge-Kutta-Fehlberg 4(5): one step of size h from y (k1 = dydt_in given), the 5th order
+// solution propagates; yerr is the difference of the embedded 5th and 4th order solutions.
+static void
+rkf45_step( const Neuron& n, double* y, double h, const double* k1, double* yerr, double* dydt_out, const MathMode& mm )
+{
+  static const double b21 = 1.0 / 4.0;
+  static const double b31 = 3.0 / 32.0, b32 = 9.0 / 32.0;
+  static const double b41 = 1932.0 / 2197.0, b42 = -7200.0 / 2197.0, b43 = 7296.0 / 2197.0;
+  static const double b51 = 8341.0 / 4104.0, b52 = -32832.0 / 4104.0, b53 = 29440.0 / 4104.0, b54 = -845.0 / 4104.0;
+  static const double b61 = -6080.0 / 20520.0, b62 = 41040.0 / 20520.0, b63 = -28352.0 / 20520.0,
+                      b64 = 9295.0 / 20520.0, b65 = -5643.0 / 20520.0;
+  static const double c1 = 902880.0 / 7618050.0, c3 = 3953664.0 / 7618050.0, c4 = 3855735.0 / 7618050.0,
+                      c5 = -1371249.0 / 7618050.0, c6 = 277020.0 / 7618050.0;
+  static const double ec1 = 1.0 / 360.0, ec3 = -128.0 / 4275.0, ec4 = -2197.0 / 75240.0, ec5 = 1.0 / 50.0,
+                      ec6 = 2.0 / 55.0;
+  double k2[ 4 ], k3[ 4 ], k4[ 4 ], k5[ 4 ], k6[ 4 ], yt[ 4 ];
+  for ( int i = 0; i < 4; ++i )
+    yt[ i ] = y[ i ] + b21 * h * k1[ i ];
+  aeif_rhs( n, yt, k2, mm );
+  for ( int i = 0; i < 4; ++i )
+    yt[ i ] = y[ i ] + h * ( b31 * k1[ i ] + b32 * k2[ i ] );
+  aeif_rhs( n, yt, k3, mm );
+  for ( int i = 0; i < 4; ++i )
+    yt[ i ] = y[ i ] + h * ( b41 * k1[ i ] + b42 * k2[ i ] + b43 * k3[ i ] );
+  aeif_rhs( n, yt, k4, mm );
+  for ( int i = 0; i < 4; ++i )
+    yt[ i ] = y[ i ] + h * ( b51 * k1[ i ] + b52 * k2[ i ] + b53 * k3[ i ] + b54 * k4[ i ] );
+  aeif_rhs( n, yt, k5, mm );
+  for ( int i = 0; i < 4; ++i )
+    yt[ i ] = y[ i ] + h * ( b61 * k1[ i ] + b62 * k2[ i ] + b63 * k3[ i ] + b64 * k4[ i ] + b65 * k5[ i ] );
+  aeif_rhs( n, yt, k6, mm );
+  for ( int i = 0; i < 4; ++i )
+  {
+    const double di = c1 * k1[ i ] + c3 * k3[ i ] + c4 * k4[ i ] + c5 * k5[ i ] + c6 * k6[ i ];
+    y[ i ] = y[ i ] + h * di;
+  }
+  for ( int i = 0; i < 4; ++i )
+  {
+    const double ei = ec1 * k1[ i ] + ec3 * k3[ i ] + ec4 * k4[ i ] + ec5 * k5[ i ] + ec6 * k6[ i ];
+    yerr[ i ] = h * ei;
+  }
+  aeif_rhs( n, y, dydt_out, mm );
+}
+
+// Step-size control on the derivative ("yp" control: a_y = 0, a_dydt = 1): desired error
+// D_i = eps_abs + eps_rel * |h y'_i|; ratio r = max_i |yerr_i| / D_i; shrink when r > 1.1
+// by S r^(-1/q) (no more than 5x), grow when r < 0.5 by S r^(-1/(q+1)) (no more than 5x,
+// never below 1); S = 0.9 and q = 5, the stepper's order. Returns -1 dec, 0 nil, +1 inc.
+static int
+yp_hadjust( double tol, const double* yerr, const double* yp, double* h, const MathMode& mm )
+{
+  const double S = 0.9;
+  const double h_old = *h;
+  double rmax = std::numeric_limits< double >::min();
+  for ( int i = 0; i < 4; ++i )
+  {
+    const double D0 = tol * ( 0.0 * std::fabs( 0.0 ) + 1.0 * std::fabs( h_old * yp[ i ] ) ) + tol;
+    const double r = std::fabs( yerr[ i ] ) / std::fabs( D0 );
+    rmax = r > rmax ? r : rmax;  // GSL_MAX_DBL
+  }
+  if ( rmax > 1.1 )
+  {
+    double r = S / mm.pow( rmax, 1.0 / 5.0 );
+    if ( r < 0.2 )
+      r = 0.2;
+    *h = r * h_old;
+    return -1;
+  }
+  else if ( rmax < 0.5 )
+  {
+    double r = S / mm.pow( rmax, 1.0 / ( 5.0 + 1.0 ) );
+    if ( r > 5.0 )
+      r = 5.0;
+    if ( r < 1.0 )
+      r = 1.0;
+    *h = r * h_old;
+    return 1;
+  }
+  return 0;
+}
+
+// One call of the adaptive evolution: advance *t towards t1 by one accepted step.
+static void
+rkf45_evolve( const Neuron& n, double* t, double t1, double* h, double* y, const MathMode& mm )
+{
+  const double t0 = *t;
+  double h0 = *h;
+  const double dt = t1 - t0;
+  double y0[ 4 ], dydt_in[ 4 ], yerr[ 4 ], dydt_out[ 4 ];
+  std::memcpy( y0, y, sizeof y0 );
+  aeif_rhs( n, y, dydt_in, mm );
+  for ( ;; )
+  {
+    bool final_step = false;
+    if ( h0 > dt )
+    {
+      h0 = dt;
+      final_step = true;
+    }
+    rkf45_step( n, y, h0, dydt_in, yerr, dydt_out, mm );
+    *t = final_step ? t1 : t0 + h0;
+    const double h_old = h0;
+    const int st = yp_hadjust( n.tol, yerr, dydt_out, &h0, mm );
+    if ( st < 0 )
+    {
+      const double t_curr = *t;
+      const double t_next = ( *t ) + h0;
+      if ( std::fabs( h0 ) < std::fabs( h_old ) and t_next != t_curr )
+      {
+        std::memcpy( y, y0, sizeof y0 );
+        continue;
+      }
+      h0 = h_old;
+    }
+    break;
+  }
+  *h = h0;
+}
+
+// ------------------------------------------------------------------ kernel
+struct Conn
+{
+  long src, tgt;
+  double w;
+  long d;  // steps
+};
+
+struct Drive
+{
+  long gid0, n, n_steps;  // targets gid0..gid0+n-1, emission steps 0..n_steps-1
+  std::vector< uint8_t > counts;  // [step][i]
+  double w;
+  long d;  // steps
+};
+
+struct Kernel
+{
+  TimeBase tb;
+  int n_vp = 1;
+  uint32_t seed = 143202461;
+  std::vector< Rng > vp_rng;
+  long next_gid = 1;
+  std::vector< Neuron > neurons;
+  std::vector< long > gid2idx;  // -1 for devices
+  std::vector< std::vector< Conn > > conns;  // per VP, creation order
+  std::vector< Drive > drives;
+  long min_delay = 0, max_delay = 0;
+  long d_min_seen = std::numeric_limits< long >::max(), d_max_seen = 1;
+  bool simulated = false, prepared = false;
+  long clock = 0, slice = 0, from_step = 0, to_do = 0, L = 0;
+  int n_threads = 1;
+  MathMode mm;
+  std::string err;
+
+  // receive buffer (last gathered slice) and per-VP emission registers: (gid, lag)
+  std::vector< std::pair< long, long > > recv;
+  std::vector< std::vector< std::pair< long, long > > > reg;
+  // spike record: (gid, step)
+  std::vector< std::pair< long, long > > spikes;
+
+  // delivery store: per owner partition, CSR over gid of (tgt idx, d, w), creation order
+  struct Part
+  {
+    std::vector< long > ptr;
+    std::vector< int32_t > tgt;
+    std::vector< int32_t > d;
+    std::vector< double > w;
+  };
+  std::vector< Part > parts;
+  bool store_valid = false;
+  // drive lookup per neuron idx: (drive index, position) or -1
+  std::vector< std::pair< int, long > > drive_of;
+
+  explicit Kernel( int nvp, uint32_t sd, double res )
+    : n_vp( nvp )
+    , seed( sd )
+  {
+    if ( res > 0 )
+      tb.set_resolution( res );
+    for ( int vp = 0; vp < n_vp; ++vp )
+      vp_rng.emplace_back( std::vector< uint32_t > { seed, 0xb84c9baeu, static_cast< uint32_t >( vp ) } );
+    conns.resize( n_vp );
+    reg.resize( n_vp );
+    gid2idx.push_back( -1 );  // gid 0 unused
+  }
+
+  int
+  vp_of( long gid ) const
+  {
+    return static_cast< int >( gid % n_vp );
+  }
+
+  Neuron&
+  nrn( long gid )
+  {
+    if ( gid <= 0 or gid >= static_cast< long >( gid2idx.size() ) or gid2idx[ gid ] < 0 )
+      throw std::runtime_error( "unknown neuron gid" );
+    return neurons[ gid2idx[ gid ] ];
+  }
+
+  // Parameters_::set semantics (E_L first, then values relative to E_L)
+  static void
+  set_params( Neuron& n, const std::map< std::string, double >& d )
+  {
+    auto has = [ &d ]( const char* k ) { return d.count( k ) > 0; };
+    auto get = [ &d ]( const char* k ) { return d.at( k ); };
+    if ( n.model == AEIF_COND_EXP )
+    {
+      const char* keys[] = { "V_peak", "V_reset", "t_ref", "g_L", "C_m", "E_ex", "E_in", "E_L", "Delta_T", "tau_w",
+        "a", "b", "V_th", "tau_syn_ex", "tau_syn_in", "I_e", "gsl_error_tol" };
+      double* dst[] = { &n.V_peak, &n.V_reset, &n.t_ref, &n.g_L, &n.c_m, &n.E_ex, &n.E_in, &n.E_L, &n.Delta_T,
+        &n.tau_w, &n.a, &n.b, &n.V_th, &n.tau_ex, &n.tau_in, &n.I_e, &n.tol };
+      for ( size_t i = 0; i < sizeof( keys ) / sizeof( keys[ 0 ] ); ++i )
+        if ( has( keys[ i ] ) )
+          *dst[ i ] = get( keys[ i ] );
+      if ( has( "V_m" ) )
+        n.ay[ 0 ] = get( "V_m" );
+      if ( has( "g_ex" ) )
+        n.ay[ 1 ] = get( "g_ex" );
+      if ( has( "g_in" ) )
+        n.ay[ 2 ] = get( "g_in" );
+      if ( has( "w" ) )
+        n.ay[ 3 ] = get( "w" );
+      if ( n.V_reset >= n.V_peak )
+        throw std::runtime_error( "Ensure that: V_reset < V_peak ." );
+      if ( n.c_m <= 0 )
+        throw std::runtime_error( "Ensure that C_m > 0" );
+      if ( n.t_ref < 0 )
+        throw std::runtime_error( "Ensure that t_ref >= 0" );
+      if ( n.tau_ex <= 0 or n.tau_in <= 0 or n.tau_w <= 0 )
+        throw std::runtime_error( "All time constants must be strictly positive." );
+      if ( n.tol <= 0 )
+        throw std::runtime_error( "The gsl_error_tol must be strictly positive." );
+      return;
+    }
+    const double ELold = n.E_L;
+    if ( has( "E_L" ) )
+      n.E_L = get( "E_L" );
+    const double delta_EL = n.E_L - ELold;
+    if ( has( "V_reset" ) )
+      n.V_reset = get( "V_reset" ) - n.E_L;
+    else
+      n.V_reset -= delta_EL;
+    if ( has( "V_th" ) )
+      n.V_th = get( "V_th" ) - n.E_L;
+    else
+      n.V_th -= delta_EL;
+    if ( n.model == IAF_PSC_DELTA )
+    {
+      if ( has( "V_min" ) )
+        n.V_min = get( "V_min" ) - n.E_L;
+      else
+        n.V_min -= delta_EL;
+    }
+    if ( has( "I_e" ) )
+      n.I_e = get( "I_e" );
+    if ( has( "C_m" ) )
+      n.c_m = get( "C_m" );
+    if ( has( "tau_m" ) )
+      n.tau_m = get( "tau_m" );
+    if ( n.model == IAF_PSC_EXP )
+    {
+      if ( has( "tau_syn_ex" ) )
+        n.tau_ex = get( "tau_syn_ex" );
+      if ( has( "tau_syn_in" ) )
+        n.tau_in = get( "tau_syn_in" );
+    }
+    if ( has( "t_ref" ) )
+      n.t_ref = get( "t_ref" );
+    if ( n.V_reset >= n.V_th )
+      throw std::runtime_error( "Reset potential must be smaller than threshold." );
+    if ( n.c_m <= 0 )
+      throw std::runtime_error( "Capacitance must be > 0." );
+    if ( n.t_ref < 0 )
+      throw std::runtime_error( "Refractory time must not be negative." );
+    if ( n.tau_m <= 0 or ( n.model == IAF_PSC_EXP and ( n.tau_ex <= 0 or n.tau_in <= 0 ) ) )
+      throw std::runtime_error( "Membrane and synapse time constants must be strictly positive." );
+    if ( n.model == IAF_PSC_DELTA and has( "refractory_input" ) )
+      n.refr_input = get( "refractory_input" ) != 0;
+    // State_::set
+    if ( n.model == IAF_PSC_DELTA )
+    {
+      if ( has( "V_m" ) )
+        n.y3 = get( "V_m" ) - n.E_L;
+      else
+        n.y3 -= delta_EL;
+    }
+    else
+    {
+      if ( has( "V_m" ) )
+        n.V = get( "V_m" ) - n.E_L;
+      else
+        n.V -= delta_EL;
+      if ( has( "I_syn_ex" ) )
+        n.i_ex = get( "I_syn_ex" );
+      if ( has( "I_syn_in" ) )
+        n.i_in = get( "I_syn_in" );
+    }
+  }
+
+  static Neuron
+  default_neuron( int model )
+  {
+    Neuron n {};
+    n.model = model;
+    if ( model == IAF_PSC_DELTA )  // models/iaf_psc_delta.cpp:69-88
+    {
+      n.tau_m = 10.0;
+      n.c_m = 250.0;
+      n.t_ref = 2.0;
+      n.E_L = -70.0;
+      n.I_e = 0.0;
+      n.V_th = -55.0 - n.E_L;
+      n.V_min = -std::numeric_limits< double >::max();
+      n.V_reset = -70.0 - n.E_L;
+    }
+    else if ( model == IAF_PSC_EXP )  // models/iaf_psc_exp.cpp:71-94
+    {
+      n.tau_m = 10.0;
+      n.c_m = 250.0;
+      n.t_ref = 2.0;
+      n.E_L = -70.0;
+      n.I_e = 0.0;
+      n.V_th = -55.0 - n.E_L;
+      n.V_reset = -70.0 - n.E_L;
+      n.tau_ex = 2.0;
+      n.tau_in = 2.0;
+    }
+    else  // models/aeif_cond_exp.cpp:132-151
+    {
+      n.V_peak = 0.0;
+      n.V_reset = -60.0;
+      n.t_ref = 0.0;
+      n.g_L = 30.0;
+      n.c_m = 281.0;
+      n.E_ex = 0.0;
+      n.E_in = -85.0;
+      n.E_L = -70.6;
+      n.Delta_T = 2.0;
+      n.tau_w = 144.0;
+      n.a = 4.0;
+      n.b = 80.5;
+      n.V_th = -50.4;
+      n.tau_ex = 0.2;
+      n.tau_in = 2.0;
+      n.I_e = 0.0;
+      n.tol = 1e-6;
+      n.ay[ 0 ] = n.E_L;
+    }
+    return n;
+  }
+
+  long
+  create( int model, long n, const std::map< std::string, double >& params )
+  {
+    const long first = next_gid;
+    for ( long i = 0; i < n; ++i )
+    {
+      Neuron x = default_neuron( model );
+      x.gid = next_gid;
+      set_params( x, params );
+      gid2idx.push_back( static_cast< long >( neurons.size() ) );
+      neurons.push_back( x );
+      ++next_gid;
+    }
+    store_valid = false;
+    return first;
+  }
+
+  void
+  check_delay( long d_steps )
+  {
+    if ( d_steps < 1 )
+      throw std::runtime_error( "BadDelay: Delay must be greater than or equal to resolution" );
+    if ( simulated and ( d_steps < min_delay or d_steps > max_delay ) )
+      throw std::runtime_error( "BadDelay: Minimum and maximum delay cannot be changed after Simulate has been called." );
+    d_min_seen = std::min( d_min_seen, d_steps );
+    d_max_seen = std::max( d_max_seen, d_steps );
+  }
+
+  // BipartiteConnBuilder::single_connect_ (conn_builder.cpp:379-437): delay before weight
+  void
+  single_connect( long s, long t, int vp, Rng& rng, Param* wp, Param* dp )
+  {
+    double delay = 1.0, weight = 1.0;  // static_synapse defaults
+    if ( wp and dp )
+    {
+      delay = dp->value( rng, vp );
+      weight = wp->value( rng, vp );
+    }
+    else if ( dp )
+      delay = dp->value( rng, vp );
+    else if ( wp )
+      weight = wp->value( rng, vp );
+    const long ds = tb.delay_ms_to_steps( delay );
+    check_delay( ds );
+    conns[ vp ].push_back( Conn { s, t, weight, ds } );
+  }
+
+  void
+  connect( const long* pre, long n_pre, const long* post, long n_post, int rule, double rule_param, bool autapses,
+    bool multapses, const double* ws, int wn, const double* ds, int dn )
+  {
+    std::unique_ptr< Param > wp, dp;
+    if ( wn > 0 )
+    {
+      const double* p = ws;
+      wp = Param::parse( p, ws + wn, n_vp );
+    }
+    if ( dn > 0 )
+    {
+      const double* p = ds;
+      dp = Param::parse( p, ds + dn, n_vp );
+    }
+    for ( long i = 0; i < n_post; ++i )
+      nrn( post[ i ] );
+    for ( long i = 0; i < n_pre; ++i )
+      nrn( pre[ i ] );
+    if ( rule == 0 and n_pre != n_post )
+      throw std::runtime_error( "one_to_one connect requires equal number of sources and targets" );
+    if ( rule == 2 and n_pre == 0 )
+      throw std::runtime_error( "Source array must not be empty." );
+    if ( rule == 2 and rule_param < 0 )
+      throw std::runtime_error( "Indegree cannot be less than zero." );
+    if ( rule == 2 and not multapses and static_cast< long >( rule_param ) > n_pre )
+      throw std::runtime_error( "Indegree cannot be larger than population size." );
+    if ( rule == 3 and ( rule_param < 0 or rule_param > 1 ) )
+      throw std::runtime_error( "Connection probability 0 <= p <= 1 required." );
+    // threads run the VPs; the per-VP work is independent, so a sequential sweep is equivalent
+    for ( int vp = 0; vp < n_vp; ++vp )
+    {
+      Rng& rng = vp_rng[ vp ];
+      for ( long ti = 0; ti < n_post; ++ti )
+      {
+        const long t = post[ ti ];
+        if ( vp_of( t ) != vp )
+          continue;
+        if ( rule == 0 )
+        {
+          const long s = pre[ ti ];
+          if ( s == t and not autapses )
+            continue;
+          single_connect( s, t, vp, rng, wp.get(), dp.get() );
+        }
+        else if ( rule == 1 )
+        {
+          for ( long si = 0; si < n_pre; ++si )
+          {
+            if ( pre[ si ] == t and not autapses )
+              continue;
+            single_connect( pre[ si ], t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+        else if ( rule == 2 )
+        {
+          const long K = static_cast< long >( std::round( rule_param ) );
+          std::vector< long > chosen;
+          for ( long j = 0; j < K; ++j )
+          {
+            unsigned long s_id;
+            long s;
+            bool skip_aut, skip_mult;
+            do
+            {
+              s_id = rng.ulrand( static_cast< unsigned long >( n_pre ) );
+              s = pre[ s_id ];
+              skip_aut = not autapses and s == t;
+              skip_mult = not multapses and std::find( chosen.begin(), chosen.end(), static_cast< long >( s_id ) ) != chosen.end();
+            } while ( skip_aut or skip_mult );
+            if ( not multapses )
+              chosen.push_back( static_cast< long >( s_id ) );
+            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+        else if ( rule == 3 )
+        {
+          for ( long si = 0; si < n_pre; ++si )
+          {
+            const long s = pre[ si ];
+            if ( not autapses and s == t )
+              continue;
+            if ( rng.drand() >= rule_param )
+              continue;
+            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          }
+        }
+        else
+          throw std::runtime_error( "unknown rule" );
+      }
+    }
+    store_valid = false;
+  }
+
+  void
+  add_connections( const long* s, const long* t, const double* w, const long* d, long n )
+  {
+    for ( long i = 0; i < n; ++i )
+    {
+      nrn( s[ i ] );
+      nrn( t[ i ] );
+      check_delay( d[ i ] );
+      conns[ vp_of( t[ i ] ) ].push_back( Conn { s[ i ], t[ i ], w[ i ], d[ i ] } );
+    }
+    store_valid = false;
+  }
+
+  void
+  add_drive( long gid0, long n, long n_steps, const uint8_t* counts, double w, double delay_ms )
+  {
+    for ( long i = 0; i < n; ++i )
+      nrn( gid0 + i );
+    const long d = tb.delay_ms_to_steps( delay_ms );
+    check_delay( d );
+    Drive dr;
+    dr.gid0 = gid0;
+    dr.n = n;
+    dr.n_steps = n_steps;
+    dr.counts.assign( counts, counts + n * n_steps );
+    dr.w = w;
+    dr.d = d;
+    drives.push_back( std::move( dr ) );
+    next_gid += n;  // the generators occupy node ids like spike_generator nodes do
+    gid2idx.resize( next_gid, -1 );
+    store_valid = false;
+  }
+
+  // ---------------------------------------------------------------- prepare
+  void
+  prepare()
+  {
+    if ( not simulated )
+    {
+      // ConnectionManager::update_delay_extrema_ (connection_manager.cpp:485-530)
+      min_delay = d_min_seen == std::numeric_limits< long >::max() ? 1 : d_min_seen;
+      max_delay = std::max( d_max_seen, 1L );
+    }
+    L = min_delay + max_delay;
+    const double h = tb.resolution_ms();
+    for ( auto& n : neurons )
+    {
+      if ( not n.initialized )
+      {
+        n.rb.assign( static_cast< size_t >( L ) * 4, 0.0 );
+        n.hstep = h;
+        n.integ = h;
+        n.initialized = true;
+      }
+      n.RefractoryCounts = tb.ms_to_steps( n.t_ref );
+      if ( n.model == IAF_PSC_DELTA )
+      {
+        n.P33 = std::exp( -h / n.tau_m );
+        n.P30 = 1 / n.c_m * ( 1 - n.P33 ) * n.tau_m;
+      }
+      else if ( n.model == IAF_PSC_EXP )
+      {
+        n.P11ex = std::exp( -h / n.tau_ex );
+        n.P11in = std::exp( -h / n.tau_in );
+        n.P22 = std::exp( -h / n.tau_m );
+        n.P21ex = iaf_propagator_p32( n.tau_ex, n.tau_m, n.c_m, h );
+        n.P21in = iaf_propagator_p32( n.tau_in, n.tau_m, n.c_m, h );
+        n.P20 = n.tau_m / n.c_m * ( 1.0 - n.P22 );
+      }
+      else
+        n.V_peak_eff = n.Delta_T > 0. ? n.V_peak : n.V_th;
+    }
+    if ( not store_valid )
+      build_store();
+    prepared = true;
+  }
+
+  int
+  owner( long idx ) const
+  {
+    return static_cast< int >( idx % n_threads );
+  }
+
+  void
+  build_store()
+  {
+    const long G = next_gid;
+    parts.assign( n_threads, Part() );
+    for ( auto& p : parts )
+      p.ptr.assign( G + 1, 0 );
+    for ( int vp = 0; vp < n_vp; ++vp )
+      for ( const auto& c : conns[ vp ] )
+        parts[ owner( gid2idx[ c.tgt ] ) ].ptr[ c.src + 1 ]++;
+    for ( auto& p : parts )
+    {
+      for ( long g = 0; g < G; ++g )
+        p.ptr[ g + 1 ] += p.ptr[ g ];
+      p.tgt.resize( p.ptr[ G ] );
+      p.d.resize( p.ptr[ G ] );
+      p.w.resize( p.ptr[ G ] );
+    }
+    std::vector< std::vector< long > > cur( n_threads );
+    for ( int i = 0; i < n_threads; ++i )
+      cur[ i ].assign( parts[ i ].ptr.begin(), parts[ i ].ptr.end() - 1 );
+    // stable: VP order, then creation order within a VP; per target this is creation order
+    for ( int vp = 0; vp < n_vp; ++vp )
+      for ( const auto& c : conns[ vp ] )
+      {
+        const long ti = gid2idx[ c.tgt ];
+        const int o = owner( ti );
+        const long pos = cur[ o ][ c.src ]++;
+        parts[ o ].tgt[ pos ] = static_cast< int32_t >( ti );
+        parts[ o ].d[ pos ] = static_cast< int32_t >( c.d );
+        parts[ o ].w[ pos ] = c.w;
+      }
+    drive_of.assign( neurons.size(), std::make_pair( -1, 0L ) );
+    for ( size_t k = 0; k < drives.size(); ++k )
+      for ( long i = 0; i < drives[ k ].n; ++i )
+      {
+        const long idx = gid2idx[ drives[ k ].gid0 + i ];
+        if ( drive_of[ idx ].first >= 0 )
+          throw std::runtime_error( "at most one drive generator per neuron is supported" );
+        drive_of[ idx ] = std::make_pair( static_cast< int >( k ), i );
+      }
+    store_valid = true;
+  }
+
+  // ------------------------------------------------------------ neuron update
+  // handle(): add the weighted event to the target's ring buffer slot (absolute step mod L)
+  inline void
+  handle( Neuron& n, long abs_step, double w, double mult )
+  {
+    const size_t slot = static_cast< size_t >( abs_step % L ) * 4;
+    if ( n.model == IAF_PSC_DELTA )
+      n.rb[ slot + 0 ] += w * mult;
+    else if ( n.model == IAF_PSC_EXP )
+    {
+      const double s = w * mult;
+      n.rb[ slot + ( s > 0 ? 1 : 0 ) ] += s;
+    }
+    else
+    {
+      if ( w > 0.0 )
+        n.rb[ slot + 0 ] += w * mult;
+      else
+        n.rb[ slot + 1 ] += -w * mult;
+    }
+  }
+
+  // Node::update over lags [from, to); spikes are reported through mask bits
+  void
+  update_neuron( Neuron& n, long origin, long from, long to, uint64_t* mask, long& spikes_local )
+  {
+    const double h = tb.resolution_ms();
+    for ( long lag = from; lag < to; ++lag )
+    {
+      const size_t slot = static_cast< size_t >( ( origin + lag ) % L ) * 4;
+      bool spiked = false;
+      if ( n.model == IAF_PSC_DELTA )
+      {
+        // models/iaf_psc_delta.cpp:272-331
+        if ( n.r == 0 )
+        {
+          const double in = n.rb[ slot ];
+          n.rb[ slot ] = 0.0;
+          n.y3 = n.P30 * ( n.y0 + n.I_e ) + n.P33 * n.y3 + in;
+          if ( n.refr_input and n.refr_buf != 0.0 )
+          {
+            n.y3 += n.refr_buf;
+            n.refr_buf = 0.0;
+          }
+          n.y3 = ( n.y3 < n.V_min ? n.V_min : n.y3 );
+        }
+        else
+        {
+          const double in = n.rb[ slot ];
+          n.rb[ slot ] = 0.0;
+          if ( n.refr_input )
+            n.refr_buf += in * std::exp( -n.r * h / n.tau_m );
+          --n.r;
+        }
+        if ( n.y3 >= n.V_th )
+        {
+          n.r = n.RefractoryCounts;
+          n.y3 = n.V_reset;
+          spiked = true;
+        }
+        n.y0 = n.rb[ slot + 1 ];
+        n.rb[ slot + 1 ] = 0.0;
+      }
+      else if ( n.model == IAF_PSC_EXP )
+      {
+        // models/iaf_psc_exp.cpp:288-349 (SYN_IN = 0, SYN_EX = 1, I0 = 2, I1 = 3)
+        if ( n.r == 0 )
+          n.V = n.V * n.P22 + n.i_ex * n.P21ex + n.i_in * n.P21in + ( n.I_e + n.i0 ) * n.P20;
+        else
+          --n.r;
+        n.i_ex *= n.P11ex;
+        n.i_in *= n.P11in;
+        n.i_ex += ( 1. - n.P11ex ) * n.i1;
+        n.i_ex += n.rb[ slot + 1 ];
+        n.i_in += n.rb[ slot + 0 ];
+        if ( n.V >= n.V_th )
+        {
+          n.r = n.RefractoryCounts;
+          n.V = n.V_reset;
+          spiked = true;
+        }
+        n.i0 = n.rb[ slot + 2 ];
+        n.i1 = n.rb[ slot + 3 ];
+        n.rb[ slot + 0 ] = n.rb[ slot + 1 ] = n.rb[ slot + 2 ] = n.rb[ slot + 3 ] = 0.0;
+      }
+      else
+      {
+        // models/aeif_cond_exp.cpp:443-523
+        double t = 0.0;
+        while ( t < n.hstep )
+        {
+          rkf45_evolve( n, &t, n.hstep, &n.integ, n.ay, mm );
+          if ( n.ay[ 0 ] < -1e3 or n.ay[ 3 ] < -1e6 or n.ay[ 3 ] > 1e6 )
+            throw std::runtime_error( "NumericalInstability: aeif_cond_exp" );
+          if ( n.r > 0 )
+            n.ay[ 0 ] = n.V_reset;
+          else if ( n.ay[ 0 ] >= n.V_peak_eff )
+          {
+            n.ay[ 0 ] = n.V_reset;
+            n.ay[ 3 ] += n.b;
+            n.r = n.RefractoryCounts > 0 ? n.RefractoryCounts + 1 : 0;
+            if ( spiked )
+              throw std::runtime_error( "aeif_cond_exp: more than one spike in one step is not representable" );
+            spiked = true;
+          }
+        }
+        if ( n.r > 0 )
+          --n.r;
+        n.ay[ 1 ] += n.rb[ slot + 0 ];
+        n.ay[ 2 ] += n.rb[ slot + 1 ];
+        n.I_stim = n.rb[ slot + 2 ];
+        n.rb[ slot + 0 ] = n.rb[ slot + 1 ] = n.rb[ slot + 2 ] = 0.0;
+      }
+      if ( spiked )
+      {
+        mask[ lag >> 6 ] |= uint64_t( 1 ) << ( lag & 63 );
+        ++spikes_local;
+      }
+    }
+  }
+
+  // --------------------------------------------------------------- simulate
+  void
+  simulate( double t_ms )
+  {
+    prepare();
+    const long steps = tb.ms_to_steps( t_ms );
+    run_steps( steps );
+  }
+
+  void
+  run_steps( long steps )
+  {
+    if ( steps < 0 )
+      throw std::runtime_error( "negative simulation time" );
+    to_do += steps;
+    if ( to_do == 0 )
+      return;
+    simulated = true;
+    const long nw = ( min_delay + 63 ) / 64;
+    std::vector< uint64_t > masks( neurons.size() * nw, 0 );
+    long to_step = std::min( from_step + to_do, min_delay );
+    // neuron order per VP: ascending gid (node_manager.cpp:188-230)
+    std::vector< std::vector< long > > vp_nodes( n_vp );
+    for ( size_t i = 0; i < neurons.size(); ++i )
+      vp_nodes[ vp_of( neurons[ i ].gid ) ].push_back( static_cast< long >( i ) );
+    std::vector< std::string > errs( n_threads );
+    while ( to_do > 0 )
+    {
+      // (a) deliver the previous slice's spikes at the start of a slice
+      if ( slice > 0 and from_step == 0 )
+        deliver();
+      const long origin = clock;
+      // (b) update all nodes over [from, to); neurons first, then drive generators. Only the
+      //     neurons' own buffers are read, and drive lands at least min_delay ahead.
+#pragma omp parallel for num_threads( n_threads ) schedule( static )
+      for ( long i = 0; i < static_cast< long >( neurons.size() ); ++i )
+      {
+        long dummy = 0;
+        try
+        {
+          update_neuron( neurons[ i ], origin, from_step, to_step, &masks[ i * nw ], dummy );
+          const auto& dv = drive_of[ i ];
+          if ( dv.first >= 0 )
+          {
+            const Drive& dr = drives[ dv.first ];
+            for ( long lag = from_step; lag < to_step; ++lag )
+            {
+              const long e = origin + lag;  // emission step; spike time (e+1)*h
+              if ( e >= dr.n_steps )
+                break;
+              const uint8_t c = dr.counts[ static_cast< size_t >( e ) * dr.n + dv.second ];
+              if ( c )
+                handle( neurons[ i ], e + dr.d, dr.w, static_cast< double >( c ) );
+            }
+          }
+        }
+        catch ( const std::exception& ex )
+        {
+#ifdef _OPENMP
+          errs[ omp_get_thread_num() ] = ex.what();
+#else
+          errs[ 0 ] = ex.what();
+#endif
+        }
+      }
+      for ( auto& e : errs )
+        if ( not e.empty() )
+          throw std::runtime_error( e );
+      // emission registers: per VP, node order, then lag (event_delivery_manager_impl.h:103-118)
+      for ( int vp = 0; vp < n_vp; ++vp )
+        for ( long i : vp_nodes[ vp ] )
+        {
+          uint64_t* m = &masks[ i * nw ];
+          for ( long w = 0; w < nw; ++w )
+            while ( m[ w ] )
+            {
+              const int b = __builtin_ctzll( m[ w ] );
+              m[ w ] &= m[ w ] - 1;
+              const long lag = w * 64 + b;
+              reg[ vp ].emplace_back( neurons[ i ].gid, lag );
+              spikes.emplace_back( neurons[ i ].gid, origin + lag + 1 );
+            }
+        }
+      // (c) gather at the end of a full slice: thread-major collocation
+      if ( to_step == min_delay )
+      {
+        recv.clear();
+        for ( int vp = 0; vp < n_vp; ++vp )
+        {
+          recv.insert( recv.end(), reg[ vp ].begin(), reg[ vp ].end() );
+          reg[ vp ].clear();
+        }
+      }
+      // advance_time_ (simulation_manager.cpp:1192-1223)
+      to_do -= to_step - from_step;
+      if ( to_step == min_delay )
+      {
+        clock += min_delay;
+        ++slice;
+        from_step = 0;
+      }
+      else
+        from_step = to_step;
+      const long end_sim = from_step + to_do;
+      to_step = min_delay < end_sim ? min_delay : end_sim;
+    }
+  }
+
+  // deliver_events_ (event_delivery_manager.cpp:599-776): every owner walks the whole receive
+  // buffer in order and sends each spike along its run of local connections
+  void
+  deliver()
+  {
+#pragma omp parallel num_threads( n_threads )
+    {
+#ifdef _OPENMP
+      const int tid = omp_get_thread_num();
+#else
+      const int tid = 0;
+#endif
+      const Part& p = parts[ tid ];
+      for ( const auto& sp : recv )
+      {
+        const long s = sp.first;
+        const long lag = sp.second;
+        // stamp = clock + lag + 1 - min_delay; slot step = stamp + d - 1
+        const long base = clock + lag - min_delay;
+        for ( long k = p.ptr[ s ]; k < p.ptr[ s + 1 ]; ++k )
+          handle( neurons[ p.tgt[ k ] ], base + p.d[ k ], p.w[ k ], 1.0 );
+      }
+    }
+  }
+
+  double
+  get_state( long gid, const std::string& f )
+  {
+    Neuron& n = nrn( gid );
+    if ( n.model == IAF_PSC_DELTA )
+    {
+      if ( f == "V_m" )
+        return n.y3 + n.E_L;
+      if ( f == "refractory" )
+        return static_cast< double >( n.r );
+    }
+    else if ( n.model == IAF_PSC_EXP )
+    {
+      if ( f == "V_m" )
+        return n.V + n.E_L;
+      if ( f == "I_syn_ex" )
+        return n.i_ex;
+      if ( f == "I_syn_in" )
+        return n.i_in;
+      if ( f == "refractory" )
+        return static_cast< double >( n.r );
+    }
+    else
+    {
+      if ( f == "V_m" )
+        return n.ay[ 0 ];
+      if ( f == "g_ex" )
+        return n.ay[ 1 ];
+      if ( f == "g_in" )
+        return n.ay[ 2 ];
+      if ( f == "w" )
+        return n.ay[ 3 ];
+      if ( f == "refractory" )
+        return static_cast< double >( n.r );
+      if ( f == "IntegrationStep" )
+        return n.integ;
+    }
+    throw std::runtime_error( "unknown state field " + f );
+  }
+};
+
+}  // namespace oracle
+
+// ------------------------------------------------------------------ C API
+// Plain C entry points for the test suite (ctypes). Errors return nonzero and leave a
+// message readable through or_error().
+using oracle::Kernel;
+
+extern "C" {
+
+void*
+or_new( int n_vp, uint32_t seed, double resolution_ms )
+{
+  try
+  {
+    return new Kernel( n_vp, seed, resolution_ms );
+  }
+  catch ( ... )
+  {
+    return nullptr;
+  }
+}
+
+void
+or_free( void* k )
+{
+  delete static_cast< Kernel* >( k );
+}
+
+const char*
+or_error( void* k )
+{
+  return static_cast< Kernel* >( k )->err.c_str();
+}
+
+#define OR_TRY( ... )                                \
+  Kernel* K = static_cast< Kernel* >( k );           \
+  try                                                 \
+  {                                                   \
+    __VA_ARGS__;                                      \
+  }                                                   \
+  catch ( const std::exception& e )                   \
+  {                                                   \
+    K->err = e.what();                                \
+    return -1;                                        \
+  }                                                   \
+  return 0;
+
+int
+or_set_threads( void* k, int n )
+{
+  OR_TRY( if ( K->prepared ) throw std::runtime_error( "threads must be set before the first simulate" );
+          K->n_threads = n < 1 ? 1 : n; K->store_valid = false; )
+}
+
+int
+or_set_portable_math( void* k, int on )
+{
+  OR_TRY( K->mm.portable = on != 0; )
+}
+
+long
+or_create( void* k, int model, long n, int n_params, const char** keys, const double* vals )
+{
+  Kernel* K = static_cast< Kernel* >( k );
+  try
+  {
+    std::map< std::string, double > d;
+    for ( int i = 0; i < n_params; ++i )
+      d[ keys[ i ] ] = vals[ i ];
+    return K->create( model, n, d );
+  }
+  catch ( const std::exception& e )
+  {
+    K->err = e.what();
+    return -1;
+  }
+}
+
+int
+or_set_status( void* k, long gid, int n_params, const char** keys, const double* vals )
+{
+  OR_TRY( std::map< std::string, double > d; for ( int i = 0; i < n_params; ++i ) d[ keys[ i ] ] = vals[ i ];
+          Kernel::set_params( K->nrn( gid ), d ); )
+}
+
+int
+or_get_state( void* k, const long* gids, long n, const char* field, double* out )
+{
+  OR_TRY( for ( long i = 0; i < n; ++i ) out[ i ] = K->get_state( gids[ i ], field ); )
+}
+
+int
+or_connect( void* k,
+  const long* pre,
+  long n_pre,
+  const long* post,
+  long n_post,
+  int rule,
+  double rule_param,
+  int autapses,
+  int multapses,
+  const double* wspec,
+  int wn,
+  const double* dspec,
+  int dn )
+{
+  OR_TRY( K->connect( pre, n_pre, post, n_post, rule, rule_param, autapses != 0, multapses != 0, wspec, wn, dspec, dn ); )
+}
+
+int
+or_add_connections( void* k, const long* s, const long* t, const double* w, const long* d, long n )
+{
+  OR_TRY( K->add_connections( s, t, w, d, n ); )
+}
+
+long
+or_num_connections( void* k )
+{
+  Kernel* K = static_cast< Kernel* >( k );
+  long n = 0;
+  for ( const auto& v : K->conns )
+    n += static_cast< long >( v.size() );
+  return n;
+}
+
+// connections per VP in creation order, VPs ascending
+int
+or_get_connections( void* k, long* src, long* tgt, double* w, long* d )
+{
+  OR_TRY( long i = 0; for ( const auto& v
+                            : K->conns ) for ( const auto& c
+                                               : v ) {
+    src[ i ] = c.src;
+    tgt[ i ] = c.tgt;
+    w[ i ] = c.w;
+    d[ i ] = c.d;
+    ++i;
+  } )
+}
+
+int
+or_add_drive( void* k, long gid0, long n, long n_steps, const uint8_t* counts, double w, double delay_ms )
+{
+  OR_TRY( K->add_drive( gid0, n, n_steps, counts, w, delay_ms ); )
+}
+
+int
+or_simulate( void* k, double t_ms )
+{
+  OR_TRY( K->simulate( t_ms ); )
+}
+
+int
+or_run_steps( void* k, long steps )
+{
+  OR_TRY( K->prepare(); K->run_steps( steps ); )
+}
+
+long
+or_num_spikes( void* k )
+{
+  return static_cast< long >( static_cast< Kernel* >( k )->spikes.size() );
+}
+
+int
+or_get_spikes( void* k, long* gid, long* step )
+{
+  OR_TRY( for ( size_t i = 0; i < K->spikes.size(); ++i ) {
+    gid[ i ] = K->spikes[ i ].first;
+    step[ i ] = K->spikes[ i ].second;
+  } )
+}
+
+int
+or_clear_spikes( void* k )
+{
+  OR_TRY( K->spikes.clear(); )
+}
+
+// min_delay, max_delay, clock (steps), from_step, slice
+int
+or_kernel_status( void* k, long* out5 )
+{
+  OR_TRY( out5[ 0 ] = K->min_delay; out5[ 1 ] = K->max_delay; out5[ 2 ] = K->clock; out5[ 3 ] = K->from_step;
+          out5[ 4 ] = K->slice; )
+}
+
+// RNG probes for pinning the seeding against the reference's own randutils header
+int
+or_seed_seq_generate( const uint32_t* seed, int n_seed, uint32_t* out, int n_out )
+{
+  std::vector< uint32_t > s( seed, seed + n_seed );
+  oracle::SeedSeqFe128 ss( s );
+  ss.generate( out, out + n_out );
+  return 0;
+}
+
+int
+or_mt64_first( const uint32_t* seed, int n_seed, uint64_t* out, int n_out )
+{
+  oracle::Rng r( std::vector< uint32_t >( seed, seed + n_seed ) );
+  for ( int i = 0; i < n_out; ++i )
+    out[ i ] = r.eng();
+  return 0;
+}
+
+double
+or_fd_exp( double x )
+{
+  return oracle::fd_exp( x );
+}
+
+double
+or_fd_log( double x )
+{
+  return oracle::fd_log( x );
+}
+
+double
+or_iaf_propagator( double tau_syn, double tau_m, double c_m, double h )
+{
+  return oracle::iaf_propagator_p32( tau_syn, tau_m, c_m, h );
+}
+
+}  // extern "C"
