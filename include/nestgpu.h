@@ -29,11 +29,13 @@ int nestgpu_kernel_new( int device, nestgpu_kernel** out );
 void nestgpu_kernel_free( nestgpu_kernel* k );
 /* Message of the last failing call on k (or of the last failing call without a kernel when k is NULL). */
 const char* nestgpu_last_error( const nestgpu_kernel* k );
-/* nest::reset_kernel (nestkernel/nest.cpp:62). */
+/* nest::reset_kernel (nestkernel/nest.cpp:101; pynest/nestkernel_api.pyx:330 llapi_reset_kernel). */
 int nestgpu_reset_kernel( nestgpu_kernel* k );
 
-/* nest::set_kernel_status / get_kernel_status (nestkernel/nest.cpp:118-131):
- * keys resolution, rng_seed, total_num_virtual_procs, local_num_threads, rng_type. */
+/* nest::set_kernel_status / get_kernel_status (nestkernel/nest.cpp:149,157; nestkernel_api.pyx:547,568):
+ * keys resolution, rng_seed, total_num_virtual_procs, local_num_threads, rng_type, and the
+ * device layout keys targets_per_block, delivery_mode ("auto", "window", "global"),
+ * slices_per_chunk, kernel_timing. */
 int nestgpu_set_kernel_status( nestgpu_kernel* k, const char* json );
 /* Writes the status dictionary as JSON into buf (NUL-terminated); *len receives the full length. */
 int nestgpu_get_kernel_status( nestgpu_kernel* k, char* buf, size_t cap, size_t* len );
@@ -41,11 +43,12 @@ int nestgpu_get_kernel_status( nestgpu_kernel* k, char* buf, size_t cap, size_t*
 /* nest::create (nestkernel/nest.cpp:362): models iaf_psc_delta, iaf_psc_exp, aeif_cond_exp,
  * spike_generator, spike_recorder. *first_id receives the id of the first new node. */
 int nestgpu_create( nestgpu_kernel* k, const char* model, int64_t n, const char* params_json, int64_t* first_id );
-/* nest::set_node_status with one dictionary for every node (NodeCollection.set(dict)). */
+/* nest::set_nc_status (nestkernel/nest.cpp:210) with one dictionary for every node (NodeCollection.set(dict)). */
 int nestgpu_set_status( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* params_json );
 /* NodeCollection.set(key=array): one value per node. */
 int nestgpu_set_values( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* key, const double* values );
-/* NodeCollection.get(key): one value per node (state is read back from the device after Simulate). */
+/* nest::get_nc_status (nestkernel/nest.cpp:169), NodeCollection.get(key): one value per node
+ * (state is read back from the device after Simulate). */
 int nestgpu_get_values( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* key, double* out );
 
 /* nest::connect (nestkernel/nest.cpp:424): conn_spec rules one_to_one, all_to_all,
@@ -64,7 +67,8 @@ int nestgpu_connect_arrays( nestgpu_kernel* k, const int64_t* sources, const int
 int nestgpu_set_spike_counts( nestgpu_kernel* k, int64_t first_id, int64_t n, int64_t first_step, int64_t n_steps,
   const uint8_t* counts );
 
-/* GetConnections: per VP in creation order, then spike_generator connections. */
+/* nest::get_connections (nestkernel/nest.cpp:473), GetConnections: per VP in creation order,
+ * then spike_generator connections. */
 int64_t nestgpu_num_connections( nestgpu_kernel* k );
 int nestgpu_get_connections( nestgpu_kernel* k, int64_t* sources, int64_t* targets, double* weights,
   int64_t* delay_steps, int32_t* vps );

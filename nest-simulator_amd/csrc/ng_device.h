@@ -1,11 +1,13 @@
 // ng_device.h -- device-side layout of one network instance (shared by host and kernels).
 //
-// One network instance lives in HBM as structure-of-arrays state per neuron model,
-// slot-major ring buffers, an out-CSR synapse store sorted by delay, and the per-slice
-// delivery scratch. Neurons carry a device index that is model-major; within a model the
-// order is NEST's spike receive order (VP, then node id), so compacting spike flags by
-// order position reproduces the order in which NEST's deliver_events walks its receive
-// buffer (nestkernel/event_delivery_manager.cpp:470-497,599-776).
+// Neurons carry a device index equal to their position in NEST's spike receive order
+// (virtual process, then node id; nestkernel/event_delivery_manager.cpp:470-497), so a
+// block of consecutive device indices emits its spikes in receive order. Targets are
+// owned by blocks of `tb` consecutive device indices; one workgroup per block delivers
+// every synaptic event for its targets and then updates them. The synapse store is an
+// out-CSR by source whose rows are grouped by target block, with a per-row table of
+// absolute run starts, so the owner of block b reads exactly its run of every spiking
+// source's row: one table lookup and one contiguous run of 12-byte records.
 #pragma once
 
 #include <cstdint>
@@ -21,15 +23,6 @@ enum ModelId : int32_t
   N_MODELS = 3
 };
 
-// Number of ring-buffer channels per model on the device. Current input channels of the
-// reference models (I0/I1, currents_) carry only CurrentEvents, which this path does not
-// produce, so they are not stored; their zero contribution is folded into the parameters.
-__host__ __device__ constexpr int
-model_channels( int m )
-{
-  return m == M_IAF_PSC_DELTA ? 1 : 2;
-}
-
 // Per parameter-set constants, computed on the host in pre_run_hook order with glibc libm.
 // Field meaning per model:
 //   iaf_psc_delta: p[0]=P30 p[1]=P33 p[2]=y0+I_e (=0.0+I_e) p[3]=V_th p[4]=V_reset p[5]=V_min
@@ -38,22 +31,23 @@ model_channels( int m )
 //   aeif_cond_exp: p[0]=V_peak p[1]=V_reset p[2]=g_L p[3]=C_m p[4]=E_ex p[5]=E_in p[6]=E_L
 //                  p[7]=Delta_T p[8]=tau_w p[9]=a p[10]=b p[11]=V_th p[12]=tau_syn_ex
 //                  p[13]=tau_syn_in p[14]=I_e p[15]=gsl_error_tol p[16]=V_peak_eff p[17]=h
+// Current inputs (I0/I1, currents_) carry only CurrentEvents, which this path does not
+// produce; their zero contributions are folded into p[] so the arithmetic is unchanged.
 struct ParamSet
 {
   double p[ 20 ];
   int32_t refractory_counts;
   int32_t model;
-  // external drive from a spike_generator (one per neuron): weight and delay in steps
+  // external drive from a spike_generator (at most one per neuron)
   double drive_w;
-  int32_t drive_d;
-  int32_t drive_mode;  // 0 none, 1 dense counts, 2 seeded Poisson counts
-  int32_t drive_table;  // index into Poisson CDF tables (mode 2)
-  int32_t drive_col;    // column offset into the dense count matrix (mode 1), per-neuron via drive_idx
-  int64_t drive_t0;     // first emission step covered by the drive
-  int64_t drive_steps;  // number of emission steps covered
+  int32_t drive_d;      // steps
+  int32_t drive_mode;   // 0 none, 1 dense counts, 2 seeded Poisson counts
+  int32_t drive_table;  // Poisson table (mode 2)
+  int32_t pad;
+  int64_t drive_t0;     // first emission step covered
+  int64_t drive_steps;  // emission steps covered
 };
 
-// Poisson counts for seeded drive: inverse CDF against a host-computed table
 struct PoissonTable
 {
   double cdf[ 64 ];
@@ -62,77 +56,75 @@ struct PoissonTable
   uint64_t seed;
 };
 
-// One synaptic event staged for the deterministic per-target reduction.
-struct Rec
+// one static synapse: FP64 weight split in two words (records are 12 bytes, 4-byte aligned)
+// and local target | (delay - min_delay) << 16
+struct SynRec
 {
-  uint32_t k;   // spike index in receive order
-  uint32_t pr;  // (position in the source row << slot_bits) | relative slot
-  double w;
+  uint32_t w_lo, w_hi;
+  uint32_t word;
 };
+
+// ring-buffer slots staged in LDS per workgroup and slice (window delivery), in doubles
+constexpr uint32_t WIN_CAP = 12288;
+constexpr uint32_t MAX_TB = 512;      // targets per block = threads per workgroup
+constexpr uint32_t PMAX_WIN = 1024;   // blocks per instance, window delivery
+constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
 
 struct NetDev
 {
   // ---- sizes
-  uint32_t n;                        // neurons (device index space)
-  uint32_t model_begin[ N_MODELS + 1 ];
-  int32_t min_delay, max_delay, L;   // steps; ring buffer length L = min + max
-  int32_t nw;                        // spike-mask words per neuron (64 lags each)
-  int32_t tgt_bits, slot_bits;       // packing of tgt_d words and Rec::pr
+  uint32_t n;         // neurons
+  uint32_t tb;        // targets per block
+  uint32_t n_blocks;  // blocks of this instance
+  uint32_t blk0;      // first global block index of this instance in a launch
+  int32_t min_delay, max_delay, L;  // steps; ring length L = min + max
   int32_t n_vp;
-  int32_t small_max;                 // per-target event count handled inline by the update kernel
-  uint32_t cmax;                     // max 256-synapse chunks of any source row
-  int32_t s_cap;                     // record capacity per in-synapse per slice
-  // ---- state (per model, model-local index)
-  double* st[ N_MODELS ][ 5 ];
-  int32_t* r[ N_MODELS ];
-  uint16_t* pidx[ N_MODELS ];
+  int32_t uniform_model;  // model id when all neurons share one, else -1
+  uint32_t spk_cap;       // per-block spike list capacity (tb * min_delay)
+  int32_t nch;            // ring-buffer channels per slot: 1 (iaf_psc_delta only) or 2
+  int32_t win;            // 1: the max_delay slots a slice touches are staged in LDS
+  // ---- state by device index
+  double* v[ 5 ];     // delta: y3 | exp: V, I_ex, I_in | aeif: V, g_ex, g_in, w, IntegrationStep
+  int32_t* r;
+  const uint16_t* pidx;
+  const uint8_t* model;  // per neuron model id (used when uniform_model < 0)
   const ParamSet* psets;
-  double* rb[ N_MODELS ];            // [L][C][n_m]
-  // ---- spikes
-  uint64_t* mask;                    // [n][nw] by order position
-  const uint32_t* ord2dev;           // nullptr: identity
-  const uint32_t* dev2ord;           // nullptr: identity
-  const uint32_t* vp_begin;          // order-position start of each VP (n_vp+1)
-  uint32_t* spk;                     // spike list: (src dev index) ...
-  uint32_t* spk_lag;                 // ... and lag, receive order
-  uint32_t* spk_tmp;                 // 2 x capacity scratch for the split-slice reorder
-  uint32_t* ctr;                     // [0]=K (spikes pending delivery) [1]=n_big [2]=overflow
-                                     // [3]=parts in current slice [4]=flag scratch
-  uint32_t* scan_state;              // decoupled look-back: per block (flag<<30 | value), + ticket
-  // ---- synapse store (out-CSR by source device index, rows sorted by delay)
-  const uint64_t* row_ptr;
-  const uint32_t* tgt_d;             // target | (delay - min_delay) << tgt_bits
-  const double* wgt;
-  // ---- delivery scratch
-  uint32_t* cnt;                     // per target events this slice
-  const uint64_t* in_ptr;            // in-degree prefix (record base = in_ptr * s_cap)
-  Rec* rec;
-  Rec* rec_tmp;                      // merge scratch for very large targets (grid x max_cap)
-  uint64_t max_cap;                  // largest per-target record capacity
-  uint32_t* big;                     // targets with more than small_max events
+  double* rb;         // [block][slot][channel][tb], slot = absolute step % L
+  // ---- spikes: per block lists of (local << 16 | part << 8 | lag), two slice parities
+  uint32_t* blk_cnt;  // [2][n_blocks]
+  uint32_t* blk_spk;  // [2][n_blocks][spk_cap]
+  uint32_t* flat_src; // exact receive order of a slice split across Run calls
+  uint32_t* flat_lag;
+  uint32_t* flat_meta;  // [0..1] flat_n per parity, [2..3] flat_ready per parity
+  const uint32_t* vp_begin;  // device index start of each VP (n_vp + 1)
+  // ---- synapse store
+  const uint32_t* boff;      // n * (n_blocks + 1): start of each block's run in syn; the
+                             // last entry of a row is the row's end
+  const SynRec* syn;
   // ---- drive
-  const uint8_t* drive_counts;       // [step][cols]
+  const uint8_t* drive_counts;  // [step][cols]
   int64_t drive_cols;
-  const uint32_t* drive_idx;         // per neuron column, or 0xffffffff
+  const uint32_t* drive_idx;    // per neuron: column (mode 1) or stream id (mode 2); ~0 none
   const PoissonTable* ptables;
-  // ---- spike log
+  // ---- readout and flags
   uint32_t* log_src;
   int64_t* log_step;
   unsigned long long* log_n;
   uint64_t log_cap;
+  unsigned long long* counters;  // [0] error flags, [1] synaptic events delivered, [2] spikes
   // ---- time (host-written before each chunk)
-  int64_t clock;                     // slice origin (steps) at chunk start
-  int32_t from0;                     // from_step at chunk start
+  int64_t clock;   // slice origin at chunk start
+  int32_t from0;   // from_step at chunk start
+  int32_t part0;   // index of the run part that starts at from0 within its slice
   int32_t active;
-  uint32_t chunk_seq;                // tags the compaction scan state of each slice
-  int64_t todo;                      // steps to simulate in this chunk
+  int32_t log_enabled;  // a spike_recorder is connected: log every spike
+  int64_t todo;    // steps to simulate in this chunk
 };
 
-// Per-slice phase of one instance, derived on the device from the chunk header.
 struct SlicePhase
 {
   int64_t origin;
-  int32_t from, to;
+  int32_t from, to, part, parity;
   bool active, deliver, complete;
 };
 
@@ -142,7 +134,9 @@ slice_phase( const NetDev& d, int s )
   SlicePhase ph;
   ph.origin = d.clock + static_cast< int64_t >( s ) * d.min_delay;
   ph.from = s == 0 ? d.from0 : 0;
-  const int64_t before = s == 0 ? 0 : static_cast< int64_t >( d.min_delay - d.from0 ) + static_cast< int64_t >( s - 1 ) * d.min_delay;
+  ph.part = s == 0 ? d.part0 : 0;
+  const int64_t before =
+    s == 0 ? 0 : static_cast< int64_t >( d.min_delay - d.from0 ) + static_cast< int64_t >( s - 1 ) * d.min_delay;
   const int64_t rem = d.todo - before;
   ph.active = d.active and rem > 0;
   int64_t to = ph.from + rem;
@@ -151,7 +145,16 @@ slice_phase( const NetDev& d, int s )
   ph.to = static_cast< int32_t >( to );
   ph.deliver = ph.active and ph.from == 0 and ph.origin > 0;
   ph.complete = ph.active and ph.to == d.min_delay;
+  ph.parity = static_cast< int32_t >( ( ph.origin / d.min_delay ) & 1 );
   return ph;
 }
+
+enum ErrorFlag : unsigned long long
+{
+  ERR_AEIF_INSTABILITY = 1,
+  ERR_AEIF_DOUBLE_SPIKE = 2,
+  ERR_SPIKE_LIST = 4,
+  ERR_LOG = 8
+};
 
 }  // namespace nestgpu

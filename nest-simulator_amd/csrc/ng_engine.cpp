@@ -25,11 +25,18 @@ Engine::~Engine()
   if ( built_ )
   {
     hipSetDevice( device_ );
-    for ( void* p : store_allocs_ )
-      hipFree( p );
-    for ( void* p : state_allocs_ )
-      hipFree( p );
+    free_list( store_allocs_ );
+    free_list( state_allocs_ );
+    free_list( log_allocs_ );
   }
+}
+
+void
+Engine::free_list( std::vector< void* >& l )
+{
+  for ( void* p : l )
+    hipFree( p );
+  l.clear();
 }
 
 void*
@@ -56,180 +63,163 @@ Engine::upload( const std::vector< T >& v )
 }
 
 void
-Engine::free_store()
-{
-  for ( void* p : store_allocs_ )
-    hipFree( p );
-  store_allocs_.clear();
-}
-
-void
 Engine::build( const BuildSpec& b, bool keep_state )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  if ( keep_state and not built_ )
-    keep_state = false;
-  // ---- synapse store and delivery scratch (always rebuilt)
-  free_store();
+  if ( keep_state and ( not built_ or net_.n != b.n or net_.tb != b.tb or net_.min_delay != b.min_delay
+                        or net_.max_delay != b.max_delay or net_.nch != b.nch ) )
+    throw std::runtime_error( "device store rebuild changed the network shape after Simulate" );
+  if ( b.min_delay > 256 )
+    throw std::runtime_error( "NotImplemented: min_delay above 256 steps on the device path" );
+  if ( b.tb == 0 or b.tb > MAX_TB )
+    throw std::runtime_error( "targets_per_block must be in [1, " + std::to_string( MAX_TB ) + "]" );
+  if ( b.nch != 1 and b.nch != 2 )
+    throw std::runtime_error( "ring-buffer channels must be 1 or 2" );
+  const uint32_t P = b.n_blocks();
+  if ( P > ( b.win ? PMAX_WIN : PMAX_GLOB ) )
+    throw std::runtime_error( "NotImplemented: too many target blocks in one instance; raise targets_per_block" );
+  if ( b.win and static_cast< uint64_t >( b.max_delay ) * b.nch * b.tb > WIN_CAP )
+    throw std::runtime_error( "window delivery: max_delay * channels * targets_per_block exceeds the LDS window" );
+  if ( b.syn.size() >= ( uint64_t( 1 ) << 32 ) )
+    throw std::runtime_error( "NotImplemented: 2^32 or more synapses in one instance" );
+  free_list( store_allocs_ );
   auto S = [ this ]( void* p )
   {
     store_allocs_.push_back( p );
     return p;
   };
   net_.n = b.n;
-  for ( int m = 0; m <= N_MODELS; ++m )
-    net_.model_begin[ m ] = b.model_begin[ m ];
+  net_.tb = b.tb;
+  net_.n_blocks = P;
   net_.min_delay = b.min_delay;
   net_.max_delay = b.max_delay;
   net_.L = b.min_delay + b.max_delay;
-  net_.nw = ( b.min_delay + 63 ) / 64;
-  if ( net_.nw > 4 )
-    throw std::runtime_error( "min_delay above 256 steps is not supported on the device path" );
-  net_.tgt_bits = b.tgt_bits;
-  int sb = 1;
-  while ( ( 1 << sb ) < b.max_delay )
-    ++sb;
-  net_.slot_bits = sb;
   net_.n_vp = b.n_vp;
-  net_.small_max = 32;
-  net_.s_cap = b.s_cap;
-  uint64_t max_len = 0;
-  for ( uint32_t i = 0; i < b.n; ++i )
-    max_len = std::max( max_len, b.row_ptr[ i + 1 ] - b.row_ptr[ i ] );
-  if ( max_len >= ( uint64_t( 1 ) << ( 32 - sb ) ) )
-    throw std::runtime_error( "out-degree too large for the record packing" );
-  net_.cmax = static_cast< uint32_t >( std::max< uint64_t >( 1, ( max_len + 255 ) / 256 ) );
-  net_.row_ptr = static_cast< const uint64_t* >( S( upload( b.row_ptr ) ) );
-  net_.tgt_d = static_cast< const uint32_t* >( S( upload( b.tgt_d ) ) );
-  net_.wgt = static_cast< const double* >( S( upload( b.w ) ) );
-  std::vector< uint64_t > inp = b.in_ptr;
-  uint64_t max_cap = 0;
-  for ( uint32_t i = 0; i < b.n; ++i )
-    max_cap = std::max( max_cap, ( inp[ i + 1 ] - inp[ i ] ) * static_cast< uint64_t >( b.s_cap ) );
-  net_.max_cap = max_cap;
-  net_.in_ptr = static_cast< const uint64_t* >( S( upload( inp ) ) );
-  net_.cnt = static_cast< uint32_t* >( S( alloc( sizeof( uint32_t ) * std::max< uint32_t >( b.n, 1 ) ) ) );
-  const uint64_t n_rec = std::max< uint64_t >( 1, inp[ b.n ] * static_cast< uint64_t >( b.s_cap ) );
-  net_.rec = static_cast< Rec* >( S( alloc( sizeof( Rec ) * n_rec ) ) );
-  // merge scratch only needed when some target can exceed the LDS sort size
-  const uint64_t tmp_regions = max_cap > 2048 ? NG_GRID_BIG : 0;
-  net_.rec_tmp = static_cast< Rec* >( S( alloc( sizeof( Rec ) * std::max< uint64_t >( 1, tmp_regions * max_cap ) ) ) );
-  net_.big = static_cast< uint32_t* >( S( alloc( sizeof( uint32_t ) * std::max< uint32_t >( b.n, 1 ) ) ) );
-  // ---- drive (inputs; replaced on rebuild)
+  net_.uniform_model = b.uniform_model;
+  net_.spk_cap = b.tb * static_cast< uint32_t >( b.min_delay );
+  net_.nch = b.nch;
+  net_.win = b.win ? 1 : 0;
+  net_.log_enabled = b.log_enabled ? 1 : 0;
+  mask_ = 0;
+  for ( const auto& ps : b.psets )
+    mask_ |= 1 << ps.model;
+  // ---- store and inputs (replaced on every build)
+  net_.boff = static_cast< const uint32_t* >( S( upload( b.boff ) ) );
+  net_.syn = static_cast< const SynRec* >( S( upload( b.syn ) ) );
+  net_.psets = static_cast< const ParamSet* >( S( upload( b.psets ) ) );
+  net_.pidx = static_cast< const uint16_t* >( S( upload( b.pidx ) ) );
+  net_.model = static_cast< const uint8_t* >( S( upload( b.model ) ) );
+  net_.vp_begin = static_cast< const uint32_t* >( S( upload( b.vp_begin ) ) );
   net_.drive_counts = static_cast< const uint8_t* >( S( upload( b.drive_counts ) ) );
   net_.drive_cols = b.drive_cols;
   net_.drive_idx = static_cast< const uint32_t* >( S( upload( b.drive_idx ) ) );
   net_.ptables = static_cast< const PoissonTable* >( S( upload( b.ptables ) ) );
-  net_.psets = static_cast< const ParamSet* >( S( upload( b.psets ) ) );
-  for ( int m = 0; m < N_MODELS; ++m )
-    net_.pidx[ m ] = static_cast< uint16_t* >( S( upload( b.pidx[ m ] ) ) );
-  if ( not b.dev2ord.empty() )
-  {
-    std::vector< uint32_t > o2d( b.n );
-    for ( uint32_t i = 0; i < b.n; ++i )
-      o2d[ b.dev2ord[ i ] ] = i;
-    net_.dev2ord = static_cast< const uint32_t* >( S( upload( b.dev2ord ) ) );
-    net_.ord2dev = static_cast< const uint32_t* >( S( upload( o2d ) ) );
-  }
-  else
-  {
-    net_.dev2ord = nullptr;
-    net_.ord2dev = nullptr;
-  }
-  net_.vp_begin = static_cast< const uint32_t* >( S( upload( b.vp_begin ) ) );
-
   if ( not keep_state )
   {
-    for ( void* p : state_allocs_ )
-      hipFree( p );
-    state_allocs_.clear();
+    free_list( state_allocs_ );
     auto T = [ this ]( void* p )
     {
       state_allocs_.push_back( p );
       return p;
     };
-    for ( int m = 0; m < N_MODELS; ++m )
-    {
-      const uint32_t nm = b.model_begin[ m + 1 ] - b.model_begin[ m ];
-      for ( int f = 0; f < 5; ++f )
-        net_.st[ m ][ f ] = b.st[ m ][ f ].empty() ? nullptr : static_cast< double* >( T( upload( b.st[ m ][ f ] ) ) );
-      net_.r[ m ] = static_cast< int32_t* >( T( upload( b.r[ m ] ) ) );
-      net_.rb[ m ] = static_cast< double* >(
-        T( alloc( sizeof( double ) * static_cast< size_t >( net_.L ) * model_channels( m ) * std::max< uint32_t >( nm, 1 ) ) ) );
-    }
-    const uint64_t spk_cap = static_cast< uint64_t >( std::max< uint32_t >( b.n, 1 ) ) * b.min_delay;
-    net_.mask = static_cast< uint64_t* >( T( alloc( sizeof( uint64_t ) * std::max< uint32_t >( b.n, 1 ) * net_.nw ) ) );
-    net_.spk = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * spk_cap ) ) );
-    net_.spk_lag = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * spk_cap ) ) );
-    net_.spk_tmp = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 2 * spk_cap ) ) );
-    net_.ctr = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 16 ) ) );
-    net_.scan_state = static_cast< uint32_t* >( T( alloc( sizeof( uint64_t ) * ( ( b.n + 255 ) / 256 + 1 ) ) ) );
-    net_.log_cap = b.log_cap;
-    net_.log_src = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * b.log_cap ) ) );
-    net_.log_step = static_cast< int64_t* >( T( alloc( sizeof( int64_t ) * b.log_cap ) ) );
+    for ( int f = 0; f < 5; ++f )
+      net_.v[ f ] = static_cast< double* >( T( upload( b.v[ f ] ) ) );
+    net_.r = static_cast< int32_t* >( T( upload( b.r ) ) );
+    net_.rb = static_cast< double* >( T( alloc( sizeof( double ) * static_cast< size_t >( P ) * net_.L * b.nch * b.tb ) ) );
+    net_.blk_cnt = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 2 * std::max< uint32_t >( P, 1 ) ) ) );
+    net_.blk_spk = static_cast< uint32_t* >(
+      T( alloc( sizeof( uint32_t ) * 2 * static_cast< size_t >( std::max< uint32_t >( P, 1 ) ) * net_.spk_cap ) ) );
+    const size_t flat_cap = static_cast< size_t >( std::max< uint32_t >( b.n, 1 ) ) * b.min_delay;
+    net_.flat_src = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * flat_cap ) ) );
+    net_.flat_lag = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * flat_cap ) ) );
+    net_.flat_meta = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 4 ) ) );
+    net_.log_cap = 0;
     net_.log_n = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) ) ) );
+    net_.counters = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) * 4 ) ) );
     clock = 0;
     from_step = 0;
+    part = 0;
+  }
+  if ( b.log_cap != net_.log_cap )
+  {
+    // the caller drains the log before a rebuild, so the buffers can be replaced
+    free_list( log_allocs_ );
+    auto Lg = [ this ]( void* p )
+    {
+      log_allocs_.push_back( p );
+      return p;
+    };
+    net_.log_cap = b.log_cap;
+    net_.log_src = static_cast< uint32_t* >( Lg( alloc( sizeof( uint32_t ) * b.log_cap ) ) );
+    net_.log_step = static_cast< int64_t* >( Lg( alloc( sizeof( int64_t ) * b.log_cap ) ) );
+    unsigned long long z = 0;
+    hip_check( hipMemcpy( net_.log_n, &z, sizeof z, hipMemcpyHostToDevice ), "log reset" );
   }
   net_.clock = clock;
   net_.from0 = from_step;
+  net_.part0 = part;
   net_.todo = 0;
   net_.active = 0;
   built_ = true;
 }
 
 void
-Engine::set_state( int model, int field, uint32_t j, double v )
+Engine::set_state( int field, uint32_t dev, double v )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  hip_check( hipMemcpy( net_.st[ model ][ field ] + j, &v, sizeof v, hipMemcpyHostToDevice ), "set_state" );
+  hip_check( hipMemcpy( net_.v[ field ] + dev, &v, sizeof v, hipMemcpyHostToDevice ), "set_state" );
 }
 
 void
-Engine::set_r( int model, uint32_t j, int32_t v )
+Engine::get_state( int field, std::vector< double >& out )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  hip_check( hipMemcpy( net_.r[ model ] + j, &v, sizeof v, hipMemcpyHostToDevice ), "set_r" );
+  out.resize( net_.n );
+  if ( net_.n )
+    hip_check( hipMemcpy( out.data(), net_.v[ field ], net_.n * sizeof( double ), hipMemcpyDeviceToHost ), "get_state" );
 }
 
 void
-Engine::get_state( int model, int field, std::vector< double >& out )
+Engine::get_r( std::vector< int32_t >& out )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  const uint32_t nm = net_.model_begin[ model + 1 ] - net_.model_begin[ model ];
-  out.resize( nm );
-  if ( nm )
-    hip_check( hipMemcpy( out.data(), net_.st[ model ][ field ], nm * sizeof( double ), hipMemcpyDeviceToHost ),
-      "get_state" );
+  out.resize( net_.n );
+  if ( net_.n )
+    hip_check( hipMemcpy( out.data(), net_.r, net_.n * sizeof( int32_t ), hipMemcpyDeviceToHost ), "get_r" );
+}
+
+unsigned long long
+Engine::read_counter( int i ) const
+{
+  unsigned long long v = 0;
+  hip_check( hipMemcpy( &v, net_.counters + i, sizeof v, hipMemcpyDeviceToHost ), "read counter" );
+  return v;
 }
 
 void
-Engine::get_r( int model, std::vector< int32_t >& out )
+Engine::clear_error_flags()
 {
-  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  const uint32_t nm = net_.model_begin[ model + 1 ] - net_.model_begin[ model ];
-  out.resize( nm );
-  if ( nm )
-    hip_check( hipMemcpy( out.data(), net_.r[ model ], nm * sizeof( int32_t ), hipMemcpyDeviceToHost ), "get_r" );
+  unsigned long long z = 0;
+  hip_check( hipMemcpy( net_.counters, &z, sizeof z, hipMemcpyHostToDevice ), "clear flags" );
 }
 
 void
 Engine::drain_log()
 {
-  std::vector< uint32_t >& src = log_src;
-  std::vector< int64_t >& step = log_step;
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
   unsigned long long n = 0;
   hip_check( hipMemcpy( &n, net_.log_n, sizeof n, hipMemcpyDeviceToHost ), "log_n" );
   if ( n > net_.log_cap )
     throw std::runtime_error( "spike log overflow: more spikes in one chunk than the log holds" );
-  const size_t old = src.size();
-  src.resize( old + n );
-  step.resize( old + n );
+  if ( not net_.log_enabled )
+    return;
+  const size_t old = log_src.size();
+  log_src.resize( old + n );
+  log_step.resize( old + n );
   if ( n )
   {
-    hip_check( hipMemcpy( src.data() + old, net_.log_src, n * sizeof( uint32_t ), hipMemcpyDeviceToHost ), "log" );
-    hip_check( hipMemcpy( step.data() + old, net_.log_step, n * sizeof( int64_t ), hipMemcpyDeviceToHost ), "log" );
+    hip_check( hipMemcpy( log_src.data() + old, net_.log_src, n * sizeof( uint32_t ), hipMemcpyDeviceToHost ), "log" );
+    hip_check( hipMemcpy( log_step.data() + old, net_.log_step, n * sizeof( int64_t ), hipMemcpyDeviceToHost ), "log" );
   }
   unsigned long long z = 0;
   hip_check( hipMemcpy( net_.log_n, &z, sizeof z, hipMemcpyHostToDevice ), "log reset" );
@@ -242,13 +232,6 @@ Runner::Runner( int device, int slices_per_chunk )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
   hip_check( hipStreamCreateWithFlags( &stream_, hipStreamNonBlocking ), "hipStreamCreate" );
-  hipDeviceProp_t prop;
-  hip_check( hipGetDeviceProperties( &prop, device_ ), "hipGetDeviceProperties" );
-  const int cus = prop.multiProcessorCount;
-  grid_expand = cus * 8;
-  grid_big = NG_GRID_BIG;  // k_big indexes one merge-scratch region per workgroup
-  grid_update = cus * 8;
-  grid_compact = cus;  // one resident workgroup per CU: the look-back never waits on a non-resident tile
 }
 
 Runner::~Runner()
@@ -258,6 +241,8 @@ Runner::~Runner()
     hipGraphExecDestroy( exec_ );
   if ( d_nets_ )
     hipFree( d_nets_ );
+  if ( d_prefix_ )
+    hipFree( d_prefix_ );
   if ( h_nets_ )
     hipHostFree( h_nets_ );
   if ( stream_ )
@@ -277,33 +262,55 @@ Runner::ensure_capacity( size_t n )
   }
   if ( d_nets_ )
     hipFree( d_nets_ );
+  if ( d_prefix_ )
+    hipFree( d_prefix_ );
   if ( h_nets_ )
     hipHostFree( h_nets_ );
   hip_check( hipMalloc( &d_nets_, n * sizeof( NetDev ) ), "hipMalloc nets" );
+  hip_check( hipMalloc( &d_prefix_, ( n + 1 ) * sizeof( uint32_t ) ), "hipMalloc prefix" );
   hip_check( hipHostMalloc( &h_nets_, n * sizeof( NetDev ), hipHostMallocDefault ), "hipHostMalloc nets" );
   cap_ = n;
+  prefix_.clear();
 }
 
 void
-Runner::capture( int n_nets, const bool* has_model )
+Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
 {
-  if ( exec_ and exec_n_ == n_nets and std::equal( has_model, has_model + N_MODELS, exec_models_ ) )
+  if ( exec_ and exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win )
     return;
   if ( exec_ )
   {
     hipGraphExecDestroy( exec_ );
     exec_ = nullptr;
   }
-  LaunchCfg cfg { grid_expand, grid_big, grid_update, grid_compact };
   hipGraph_t g;
   hip_check( hipStreamBeginCapture( stream_, hipStreamCaptureModeThreadLocal ), "hipStreamBeginCapture" );
   for ( int s = 0; s < S_; ++s )
-    launch_slice( d_nets_, n_nets, s, has_model, cfg, stream_ );
+    launch_slice( d_nets_, d_prefix_, n_nets, total_blocks, mask, win, s, stream_ );
   hip_check( hipStreamEndCapture( stream_, &g ), "hipStreamEndCapture" );
   hip_check( hipGraphInstantiate( &exec_, g, nullptr, nullptr, 0 ), "hipGraphInstantiate" );
   hip_check( hipGraphDestroy( g ), "hipGraphDestroy" );
   exec_n_ = n_nets;
-  std::copy( has_model, has_model + N_MODELS, exec_models_ );
+  exec_total_ = total_blocks;
+  exec_mask_ = mask;
+  exec_win_ = win;
+}
+
+void
+Runner::launch_one( int n, uint32_t total, int mask, bool win, int s, hipEvent_t* ev )
+{
+  if ( ev )
+    hip_check( hipEventRecord( ev[ 0 ], stream_ ), "hipEventRecord" );
+  hip_check( launch_slice( d_nets_, d_prefix_, n, total, mask, win, s, stream_ ), "launch_slice" );
+  if ( ev )
+  {
+    hip_check( hipEventRecord( ev[ 1 ], stream_ ), "hipEventRecord" );
+    hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
+    float ms = 0;
+    hip_check( hipEventElapsedTime( &ms, ev[ 0 ], ev[ 1 ] ), "hipEventElapsedTime" );
+    kernel_ms[ K_SLICE ] += ms;
+    kernel_launches[ K_SLICE ] += 1;
+  }
 }
 
 void
@@ -314,43 +321,79 @@ Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >
   if ( n == 0 )
     return;
   ensure_capacity( n );
-  bool has_model[ N_MODELS ] = { false, false, false };
-  for ( auto* e : engines )
-    for ( int m = 0; m < N_MODELS; ++m )
-      has_model[ m ] = has_model[ m ] or e->has_model( m );
+  int mask = 0;
+  bool win = false;
+  std::vector< uint32_t > prefix( n + 1, 0 );
+  for ( int i = 0; i < n; ++i )
+  {
+    mask |= engines[ i ]->model_mask();
+    win = win or engines[ i ]->window();
+    prefix[ i + 1 ] = prefix[ i ] + engines[ i ]->net().n_blocks;
+  }
+  const uint32_t total = prefix[ n ];
+  if ( total == 0 )
+    return;
+  // the window kernel keeps the block-count prefix of every instance in a smaller LDS table
+  for ( int i = 0; win and i < n; ++i )
+    if ( engines[ i ]->net().n_blocks > PMAX_WIN )
+      throw std::runtime_error( "NotImplemented: an instance with more than " + std::to_string( PMAX_WIN )
+        + " target blocks cannot run beside window-delivery instances; raise its targets_per_block" );
+  if ( prefix != prefix_ )
+  {
+    hip_check( hipMemcpy( d_prefix_, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice ), "prefix" );
+    prefix_ = prefix;
+  }
   std::vector< int64_t > remaining = steps;
-  LaunchCfg cfg { grid_expand, grid_big, grid_update, grid_compact };
-  hipEvent_t e0, e1;
+  hipEvent_t e0, e1, ev[ 2 ];
   hip_check( hipEventCreate( &e0 ), "hipEventCreate" );
   hip_check( hipEventCreate( &e1 ), "hipEventCreate" );
+  hip_check( hipEventCreate( &ev[ 0 ] ), "hipEventCreate" );
+  hip_check( hipEventCreate( &ev[ 1 ] ), "hipEventCreate" );
   last_ms_ = 0;
   for ( ;; )
   {
-    bool any = false;
+    bool any = false, split = false;
     for ( int i = 0; i < n; ++i )
     {
       Engine& E = *engines[ i ];
       NetDev d = E.net();
       d.clock = E.clock;
       d.from0 = E.from_step;
-      d.chunk_seq = E.chunk_seq;
-      const int64_t span = static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S_ - 1 ) * d.min_delay;
+      d.part0 = E.part;
+      const int64_t span =
+        static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S_ - 1 ) * d.min_delay;
       d.todo = std::min( remaining[ i ], span );
       d.active = d.todo > 0 ? 1 : 0;
       any = any or d.active;
+      split = split or ( d.active and E.from_step > 0 );
+      if ( d.part0 > 255 )
+        throw std::runtime_error( "NotImplemented: a slice split into more than 256 Run parts" );
       h_nets_[ i ] = d;
     }
     if ( not any )
       break;
-    if ( use_graph )
-      capture( n, has_model );
+    if ( use_graph and not time_kernels and not split )
+      capture( n, total, mask, win );
     hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
     hip_check( hipEventRecord( e0, stream_ ), "hipEventRecord" );
-    if ( use_graph )
-      hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
-    else
+    if ( split or time_kernels or not use_graph )
+    {
       for ( int s = 0; s < S_; ++s )
-        hip_check( launch_slice( d_nets_, n, s, has_model, cfg, stream_ ), "launch_slice" );
+      {
+        launch_one( n, total, mask, win, s, time_kernels ? ev : nullptr );
+        if ( s == 0 and split )
+          for ( int i = 0; i < n; ++i )
+          {
+            // the slice that continued a previous part completes here: restore NEST's order
+            const NetDev& d = h_nets_[ i ];
+            const SlicePhase ph = slice_phase( d, 0 );
+            if ( ph.active and d.from0 > 0 and ph.complete )
+              hip_check( launch_flatten( d_nets_ + i, ph.parity, d.part0 + 1, stream_ ), "launch_flatten" );
+          }
+      }
+    }
+    else
+      hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
     hip_check( hipEventRecord( e1, stream_ ), "hipEventRecord" );
     hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
     float ms = 0;
@@ -362,36 +405,43 @@ Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >
       const NetDev& d = h_nets_[ i ];
       if ( not d.active )
         continue;
-      uint32_t ctr[ 8 ];
-      hip_check( hipMemcpy( ctr, E.net().ctr, sizeof ctr, hipMemcpyDeviceToHost ), "ctr" );
-      if ( ctr[ 2 ] )
+      const unsigned long long flags = E.read_counter( 0 );
+      if ( flags )
       {
+        E.clear_error_flags();
         std::string why;
-        if ( ctr[ 2 ] & 1u )
-          why += " event records exceeded their capacity (more spikes per source and slice than reserved);";
-        if ( ctr[ 2 ] & 2u )
+        if ( flags & ERR_AEIF_INSTABILITY )
           why += " NumericalInstability in aeif_cond_exp;";
-        if ( ctr[ 2 ] & 4u )
+        if ( flags & ERR_AEIF_DOUBLE_SPIKE )
           why += " aeif_cond_exp emitted two spikes within one step;";
-        if ( ctr[ 2 ] & 8u )
-          why += " spike compaction did not converge;";
+        if ( flags & ERR_SPIKE_LIST )
+          why += " spike list capacity exceeded;";
+        if ( flags & ERR_LOG )
+          why += " spike log capacity exceeded;";
         throw std::runtime_error( "device run failed:" + why );
       }
-      const int64_t total = E.from_step + d.todo;
-      E.clock += ( total / d.min_delay ) * d.min_delay;
-      E.from_step = static_cast< int32_t >( total % d.min_delay );
-      E.chunk_seq += 1;
+      const int64_t total_steps = E.from_step + d.todo;
+      const int64_t slices_done = total_steps / d.min_delay;
+      E.clock += slices_done * d.min_delay;
+      const int32_t nf = static_cast< int32_t >( total_steps % d.min_delay );
+      if ( nf == 0 )
+        E.part = 0;
+      else if ( slices_done == 0 )
+        E.part += 1;  // another part of the same slice
+      else
+        E.part = 1;
+      E.from_step = nf;
       remaining[ i ] -= d.todo;
       unsigned long long ln = 0;
       hip_check( hipMemcpy( &ln, E.net().log_n, sizeof ln, hipMemcpyDeviceToHost ), "log_n" );
-      if ( ln > E.net().log_cap )
-        throw std::runtime_error( "spike log overflow within one chunk" );
       if ( ln > E.net().log_cap / 2 )
         E.drain_log();
     }
   }
   hipEventDestroy( e0 );
   hipEventDestroy( e1 );
+  hipEventDestroy( ev[ 0 ] );
+  hipEventDestroy( ev[ 1 ] );
 }
 
 }  // namespace nestgpu

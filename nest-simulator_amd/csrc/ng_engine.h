@@ -13,45 +13,55 @@
 namespace nestgpu
 {
 
-// workgroups of k_big; each owns one merge-scratch region of the largest record capacity
-constexpr int NG_GRID_BIG = 512;
+// Enqueue the slice kernel for slice s of a chunk over all blocks of n_nets instances.
+// mask: bit m set if neuron model m occurs (selects a specialised kernel); win: some
+// instance uses window delivery (selects the kernel with the LDS ring-buffer window).
+hipError_t launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask,
+  bool win, int s, hipStream_t st );
+// Rebuild the exact receive order of a slice that was split across Run calls.
+hipError_t launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st );
 
-struct LaunchCfg
+enum KernelKind
 {
-  int grid_expand, grid_big, grid_update, grid_compact;
+  K_SLICE = 0,
+  K_FLATTEN,
+  N_KINDS
 };
-// Enqueues the kernels of slice s (of a chunk) for n_nets instances.
-hipError_t launch_slice( const NetDev* nets, int n_nets, int s, const bool* has_model, const LaunchCfg& cfg,
-  hipStream_t st );
 
-// Everything the engine needs to place one network instance in HBM. Produced by the
-// host kernel at Prepare (the counterpart of update_connection_infrastructure and
-// prepare_nodes, nestkernel/simulation_manager.cpp:507-570,750-833).
+// Everything the engine needs to place one network instance in HBM. Produced by the host
+// kernel at Prepare (the counterpart of update_connection_infrastructure and prepare_nodes,
+// nestkernel/simulation_manager.cpp:507-570,750-833).
 struct BuildSpec
 {
   int n_vp = 1;
   int min_delay = 1, max_delay = 1;
   uint32_t n = 0;
-  uint32_t model_begin[ N_MODELS + 1 ] = { 0, 0, 0, 0 };
-  std::vector< uint32_t > dev2ord;  // empty: identity
-  std::vector< uint32_t > vp_begin;  // order positions, n_vp + 1 entries
+  uint32_t tb = 256;
+  int nch = 2;       // ring-buffer channels per slot
+  bool win = false;  // window delivery (LDS-staged ring-buffer slots)
+  int uniform_model = -1;
+  std::vector< uint32_t > vp_begin;  // device indices, n_vp + 1 entries
   std::vector< ParamSet > psets;
-  std::vector< uint16_t > pidx[ N_MODELS ];
-  std::vector< double > st[ N_MODELS ][ 5 ];
-  std::vector< int32_t > r[ N_MODELS ];
-  // out-CSR by source device index; rows sorted by delay, creation order within a delay
-  std::vector< uint64_t > row_ptr;
-  std::vector< uint32_t > tgt_d;
-  std::vector< double > w;
-  int tgt_bits = 32;
-  std::vector< uint64_t > in_ptr;
+  std::vector< uint16_t > pidx;
+  std::vector< uint8_t > model;
+  std::vector< double > v[ 5 ];
+  std::vector< int32_t > r;
+  // out-CSR by source device index, each row grouped by target block (creation order
+  // within a block); boff[s * (n_blocks + 1) + b] = index in syn of block b's run of row s
+  std::vector< uint32_t > boff;
+  std::vector< SynRec > syn;
   // drive
   std::vector< uint8_t > drive_counts;
   int64_t drive_cols = 0;
-  std::vector< uint32_t > drive_idx;  // per device index; 0xffffffff = none
+  std::vector< uint32_t > drive_idx;
   std::vector< PoissonTable > ptables;
-  int s_cap = 1;
   uint64_t log_cap = 1 << 20;
+  bool log_enabled = false;
+  uint32_t
+  n_blocks() const
+  {
+    return n == 0 ? 0 : ( n + tb - 1 ) / tb;
+  }
 };
 
 class Engine
@@ -63,63 +73,78 @@ public:
   Engine& operator=( const Engine& ) = delete;
 
   // First call allocates everything; later calls (connections changed) replace the
-  // synapse store and delivery scratch and keep state and ring buffers.
+  // synapse store and inputs and keep state, ring buffers and pending spikes.
   void build( const BuildSpec& b, bool keep_state );
 
-  void set_state( int model, int field, uint32_t j, double v );
-  void set_r( int model, uint32_t j, int32_t v );
-  void get_state( int model, int field, std::vector< double >& out );
-  void get_r( int model, std::vector< int32_t >& out );
+  void set_state( int field, uint32_t dev, double v );
+  void get_state( int field, std::vector< double >& out );
+  void get_r( std::vector< int32_t >& out );
 
-  // Moves the device spike log (device index, stamp step) to log_src/log_step.
+  // Moves the device spike log (device index, stamp step) into log_src / log_step.
   void drain_log();
   std::vector< uint32_t > log_src;
   std::vector< int64_t > log_step;
 
-  NetDev& net()
+  NetDev&
+  net()
   {
     return net_;
   }
-  const NetDev& net() const
+  const NetDev&
+  net() const
   {
     return net_;
   }
-  bool has_model( int m ) const
+  int
+  model_mask() const
   {
-    return net_.model_begin[ m + 1 ] > net_.model_begin[ m ];
+    return mask_;
   }
-  int device() const
+  bool
+  window() const
+  {
+    return net_.win != 0;
+  }
+  int
+  device() const
   {
     return device_;
   }
-  bool built() const
+  bool
+  built() const
   {
     return built_;
   }
-  uint64_t device_bytes() const
+  uint64_t
+  device_bytes() const
   {
     return bytes_;
   }
-  // host bookkeeping of NEST's clock (SimulationManager::clock_, from_step_)
+  unsigned long long read_counter( int i ) const;
+  void clear_error_flags();
+
+  // host bookkeeping of NEST's clock (SimulationManager::clock_, from_step_) and of the
+  // run parts the current slice has been split into
   int64_t clock = 0;
   int32_t from_step = 0;
-  uint32_t chunk_seq = 1;
+  int32_t part = 0;
 
 private:
   void* alloc( size_t bytes );
   template < typename T >
   T* upload( const std::vector< T >& v );
-  void free_store();
+  void free_list( std::vector< void* >& l );
 
   int device_;
   bool built_ = false;
+  int mask_ = 0;
   NetDev net_ {};
-  std::vector< void* > state_allocs_, store_allocs_;
+  std::vector< void* > state_allocs_, store_allocs_, log_allocs_;
   uint64_t bytes_ = 0;
 };
 
 // Runs chunks of slices for a set of engines on one device and stream, replaying a
-// captured hipGraph per chunk shape.
+// captured hipGraph (one kernel node per slice) for each chunk.
 class Runner
 {
 public:
@@ -127,33 +152,44 @@ public:
   ~Runner();
   // Advance every engine by steps[i] (0 = idle). Throws on device-side error flags.
   void run( const std::vector< Engine* >& engines, const std::vector< int64_t >& steps );
-  double last_kernel_ms() const
+  double
+  last_kernel_ms() const
   {
     return last_ms_;
   }
-  int slices_per_chunk() const
+  int
+  slices_per_chunk() const
   {
     return S_;
   }
-  hipStream_t stream() const
+  hipStream_t
+  stream() const
   {
     return stream_;
   }
   bool use_graph = true;
-  int grid_expand = 0, grid_big = 0, grid_update = 0, grid_compact = 0;
+  // per-kernel timing: one slice at a time with HIP events around every launch
+  bool time_kernels = false;
+  double kernel_ms[ N_KINDS ] = { 0, 0 };
+  int64_t kernel_launches[ N_KINDS ] = { 0, 0 };
 
 private:
   void ensure_capacity( size_t n );
-  void capture( int n_nets, const bool* has_model );
+  void capture( int n_nets, uint32_t total_blocks, int mask, bool win );
+  void launch_one( int n, uint32_t total, int mask, bool win, int s, hipEvent_t* ev );
   int device_;
   int S_;
   hipStream_t stream_ = nullptr;
   NetDev* d_nets_ = nullptr;
   NetDev* h_nets_ = nullptr;  // pinned
+  uint32_t* d_prefix_ = nullptr;
   size_t cap_ = 0;
   hipGraphExec_t exec_ = nullptr;
   int exec_n_ = -1;
-  bool exec_models_[ N_MODELS ] = { false, false, false };
+  uint32_t exec_total_ = 0;
+  int exec_mask_ = 0;
+  bool exec_win_ = false;
+  std::vector< uint32_t > prefix_;
   double last_ms_ = 0;
 };
 

@@ -1,22 +1,29 @@
-// ng_kernels.hip -- gfx950 kernels for one min_delay slice of NEST's simulation step.
+// ng_kernels.hip -- gfx950 kernel for one min_delay slice of NEST's simulation step.
 //
-// Per slice s of a chunk, for every network instance in the launch:
-//   k_expand   deliver the previous slice's spikes: walk each spike's out-row (CSR by
-//              source, delay-sorted) and stage one record per synaptic event in the
-//              target's slot range (nestkernel/event_delivery_manager.cpp:599-776,
-//              connector_base.h:399-428)
-//   k_big      targets with many events: sort their records in LDS and add them to the
-//              ring buffers in receive order (ring_buffer.h:157-161,405-408)
-//   k_update_* neuron update over [from, to) (models/*.cpp update()), fused with the
-//              per-target reduction of small event lists and the spike_generator drive
-//   k_compact  ordered compaction of spike flags into the receive-order spike list
-//              (event_delivery_manager_impl.h:103-118, event_delivery_manager.cpp:470-497)
+// One workgroup owns a block of `tb` consecutive targets (device index = receive-order
+// position) and, for slice s of a chunk:
+//   1. delivers the previous slice's spikes to its targets (EventDeliveryManager::
+//      deliver_events_, nestkernel/event_delivery_manager.cpp:599-776; Connector::send,
+//      connector_base.h:399-428; RingBuffer::add_value, ring_buffer.h:157-161,405-408):
+//      it walks the receive-order spike list, reads its own run of each spiking source's
+//      out-row, and adds every weight into the target's ring-buffer cell;
+//   2. updates its neurons over [from, to) (Node::update of iaf_psc_delta, iaf_psc_exp,
+//      aeif_cond_exp) including the spike_generator drive of that slice;
+//   3. appends its spikes, in receive order, to its block list for the next slice.
 //
-// Determinism: FP64 additions into one ring-buffer cell happen in NEST's order -- by
-// delivery batch, drive after the batch of the same slice, receive order within a batch
-// and connector-run order within a spike. The device sorts each target's records by
-// (spike index, row position) before adding, so no result depends on atomic order.
-// Built with -ffp-contract=off: every multiply and add is rounded as in the reference.
+// Window delivery: the slots a slice can touch through recurrent synapses are the
+// max_delay slots starting at the slice origin (lag + delay - min_delay < max_delay), and
+// the update reads the first min_delay of them. In window mode the workgroup copies those
+// slots of its block into LDS, accumulates there and writes them back once, so the
+// read-modify-write of every synaptic event is an LDS operation. Global mode (short
+// slices with sparse events) updates the ring buffer in HBM directly.
+//
+// Determinism: NEST adds into one ring-buffer cell in the order (delivery batch, receive
+// order of the spike, position in the connector run). A workgroup takes events in that key
+// order in rounds of R; each round is bucketed by target (LDS counters give every event a
+// slot in its target's bucket), and the thread owning a target sorts its bucket by key and
+// applies it in that order. No result depends on thread timing or atomics order.
+// Built with -ffp-contract=off: every multiply and add rounds as in the reference.
 
 #include <hip/hip_runtime.h>
 
@@ -27,304 +34,86 @@
 namespace nestgpu
 {
 
-constexpr int BS = 256;
-constexpr int BIG_LDS = 2048;  // records sorted in LDS by one workgroup
+constexpr int BS = static_cast< int >( MAX_TB );  // threads per workgroup = max targets per block
+constexpr int SPT = 4;                            // spikes per thread per chunk
+constexpr int CH = SPT * BS;                      // spikes per chunk
+constexpr int RQ = 4;                             // events per thread per round
+constexpr int R = RQ * BS;                        // events per round
+constexpr int IDX_SHIFT = 21;                     // bucket key = round index << 21 | slot-channel
+constexpr uint32_t SC_MASK = ( 1u << IDX_SHIFT ) - 1u;
+constexpr uint32_t NONE = 0xffffffffu;
+static_assert( R <= ( 1 << ( 32 - IDX_SHIFT ) ), "round index must fit the bucket key" );
 
-__device__ inline uint64_t
-rec_key( const Rec& r )
+template < bool WIN >
+struct SliceLds
 {
-  return ( static_cast< uint64_t >( r.k ) << 32 ) | r.pr;
+  double win[ WIN ? WIN_CAP : 1 ];  // ring-buffer window [slot - origin][channel][tb]
+  double bw[ R ];                   // event values, bucketed by target
+  unsigned long long log_base;
+  uint32_t pref[ ( WIN ? PMAX_WIN : PMAX_GLOB ) + 1 ];  // prefix of per-block spike counts
+  uint32_t c_g0[ CH ];       // first synapse of each chunk spike's run
+  uint32_t c_off[ CH + 1 ];  // exclusive prefix of run lengths
+  uint32_t bk[ R ];          // event keys (round index << 21 | slot-channel), bucketed by target
+  uint32_t bcnt[ BS ];       // events per target this round
+  uint32_t bst[ BS + 1 ];    // bucket starts
+  uint32_t sh[ BS / 64 ];
+  uint32_t cnt_base;
+  uint8_t c_lag[ CH ];
+};
+
+// XCD-aware block mapping: workgroups are dispatched round-robin over 8 XCDs; give each
+// XCD a contiguous range of blocks so neighbouring blocks share its L2.
+__device__ inline uint32_t
+xcd_block( uint32_t w, uint32_t total )
+{
+  const uint32_t x = w & 7u;
+  uint32_t base = 0;
+  for ( uint32_t q = 0; q < x; ++q )
+    base += ( total - q + 7u ) / 8u;
+  return base + ( w >> 3 );
 }
 
-__device__ inline int
-model_of( const NetDev& d, uint32_t dev )
+__device__ inline uint32_t
+upper_bound_u32( const uint32_t* a, uint32_t n, uint32_t v )
 {
-  int m = 0;
-  while ( m + 1 < N_MODELS and dev >= d.model_begin[ m + 1 ] )
-    ++m;
-  return m;
-}
-
-// handle(SpikeEvent&) of the three models: channel choice and value added.
-__device__ inline void
-rb_add( const NetDev& d, int m, uint32_t j, int64_t abs_step, double w, double mult )
-{
-  const uint32_t nm = d.model_begin[ m + 1 ] - d.model_begin[ m ];
-  const int C = model_channels( m );
-  const int64_t slot = abs_step % d.L;
-  double* base = d.rb[ m ] + static_cast< size_t >( slot ) * C * nm;
-  if ( m == M_IAF_PSC_DELTA )
-    base[ j ] += w * mult;  // iaf_psc_delta::handle
-  else if ( m == M_IAF_PSC_EXP )
+  uint32_t lo = 0, hi = n;
+  while ( lo < hi )
   {
-    const double s = w * mult;  // iaf_psc_exp::handle: SYN_IN = 0, SYN_EX = 1
-    base[ ( s > 0 ? 1 : 0 ) * static_cast< size_t >( nm ) + j ] += s;
-  }
-  else
-  {
-    if ( w > 0.0 )  // aeif_cond_exp::handle
-      base[ j ] += w * mult;
+    const uint32_t mid = ( lo + hi ) >> 1;
+    if ( a[ mid ] <= v )
+      lo = mid + 1;
     else
-      base[ static_cast< size_t >( nm ) + j ] += -w * mult;
+      hi = mid;
   }
+  return lo;
 }
 
-// ------------------------------------------------------------------ expand
-__global__ void __launch_bounds__( BS ) k_expand( const NetDev* __restrict__ nets, int n_nets, int s )
+// block-wide exclusive scan of one value per thread; returns the total
+__device__ inline uint32_t
+block_scan( uint32_t v, uint32_t* excl, uint32_t* sh )
 {
-  for ( int i = 0; i < n_nets; ++i )
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  uint32_t x = v;
+  for ( int off = 1; off < 64; off <<= 1 )
   {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.deliver )
-      continue;
-    const uint32_t K = d.ctr[ 0 ];
-    const uint64_t items = static_cast< uint64_t >( K ) * d.cmax;
-    const uint32_t tmask = ( d.tgt_bits >= 32 ) ? 0xffffffffu : ( ( 1u << d.tgt_bits ) - 1u );
-    for ( uint64_t it = blockIdx.x; it < items; it += gridDim.x )
-    {
-      const uint32_t k = static_cast< uint32_t >( it / d.cmax );
-      const uint32_t c = static_cast< uint32_t >( it - static_cast< uint64_t >( k ) * d.cmax );
-      const uint32_t src = d.spk[ k ];
-      const uint32_t lag = d.spk_lag[ k ];
-      const uint64_t r0 = d.row_ptr[ src ];
-      const uint64_t len = d.row_ptr[ src + 1 ] - r0;
-      const uint64_t pos = static_cast< uint64_t >( c ) * BS + threadIdx.x;
-      if ( pos >= len )
-        continue;
-      const uint64_t g = r0 + pos;
-      const uint32_t word = d.tgt_d[ g ];
-      const double w = d.wgt[ g ];
-      const uint32_t t = word & tmask;
-      const uint32_t rel = lag + ( d.tgt_bits >= 32 ? 0u : ( word >> d.tgt_bits ) );
-      const uint32_t p = atomicAdd( &d.cnt[ t ], 1u );
-      if ( p == static_cast< uint32_t >( d.small_max ) )
-      {
-        const uint32_t b = atomicAdd( &d.ctr[ 1 ], 1u );
-        d.big[ b ] = t;
-      }
-      const uint64_t ib = d.in_ptr[ t ];
-      const uint64_t cap = ( d.in_ptr[ t + 1 ] - ib ) * static_cast< uint64_t >( d.s_cap );
-      if ( p < cap )
-      {
-        Rec r;
-        r.k = k;
-        r.pr = ( static_cast< uint32_t >( pos ) << d.slot_bits ) | rel;
-        r.w = w;
-        d.rec[ ib * d.s_cap + p ] = r;
-      }
-      else
-        atomicOr( &d.ctr[ 2 ], 1u );
-    }
+    const uint32_t u = __shfl_up( x, off, 64 );
+    if ( lane >= off )
+      x += u;
   }
-}
-
-// ------------------------------------------------------------------ big targets
-// Bitonic sort of up to BIG_LDS records in LDS; larger lists are merge-sorted in global
-// memory through rec_tmp (one region per workgroup, sized by the largest capacity).
-__device__ void
-wg_sort_lds( uint64_t* key, uint32_t* idx, int n_pow2 )
-{
-  for ( int kk = 2; kk <= n_pow2; kk <<= 1 )
-    for ( int jj = kk >> 1; jj > 0; jj >>= 1 )
-    {
-      for ( int i = threadIdx.x; i < n_pow2; i += blockDim.x )
-      {
-        const int ixj = i ^ jj;
-        if ( ixj > i )
-        {
-          const bool up = ( i & kk ) == 0;
-          const uint64_t a = key[ i ], b = key[ ixj ];
-          if ( ( a > b ) == up )
-          {
-            key[ i ] = b;
-            key[ ixj ] = a;
-            const uint32_t t = idx[ i ];
-            idx[ i ] = idx[ ixj ];
-            idx[ ixj ] = t;
-          }
-        }
-      }
-      __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__( BS ) k_big( const NetDev* __restrict__ nets, int n_nets, int s )
-{
-  __shared__ uint64_t key[ BIG_LDS ];
-  __shared__ uint32_t idx[ BIG_LDS ];
-  for ( int i = 0; i < n_nets; ++i )
+  if ( lane == 63 )
+    sh[ wid ] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for ( int q = 0; q < BS / 64; ++q )
   {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.deliver )
-      continue;
-    const uint32_t nb = d.ctr[ 1 ];
-    const uint32_t smask = ( 1u << d.slot_bits ) - 1u;
-    for ( uint32_t b = blockIdx.x; b < nb; b += gridDim.x )
-    {
-      const uint32_t t = d.big[ b ];
-      const uint64_t ib = d.in_ptr[ t ];
-      const uint64_t cap = ( d.in_ptr[ t + 1 ] - ib ) * static_cast< uint64_t >( d.s_cap );
-      uint64_t n = d.cnt[ t ];
-      if ( n > cap )
-        n = cap;  // overflow is flagged by k_expand; the run reports it
-      Rec* R = d.rec + ib * d.s_cap;
-      const int m = model_of( d, t );
-      const uint32_t j = t - d.model_begin[ m ];
-      const int C = model_channels( m );
-      const int n_cells = d.max_delay * C;
-      if ( n <= BIG_LDS )
-      {
-        int p2 = 1;
-        while ( p2 < static_cast< int >( n ) )
-          p2 <<= 1;
-        for ( int q = threadIdx.x; q < p2; q += blockDim.x )
-        {
-          key[ q ] = q < static_cast< int >( n ) ? rec_key( R[ q ] ) : ~0ull;
-          idx[ q ] = q;
-        }
-        __syncthreads();
-        wg_sort_lds( key, idx, p2 );
-        // each cell's adds are done by one thread, in key order
-        for ( int cell = threadIdx.x; cell < n_cells; cell += blockDim.x )
-        {
-          for ( uint64_t q = 0; q < n; ++q )
-          {
-            const Rec r = R[ idx[ q ] ];
-            const int rel = static_cast< int >( r.pr & smask );
-            int ch = 0;
-            if ( m == M_IAF_PSC_EXP )
-              ch = r.w > 0 ? 1 : 0;
-            else if ( m == M_AEIF_COND_EXP )
-              ch = r.w > 0.0 ? 0 : 1;
-            if ( rel * C + ch != cell )
-              continue;
-            rb_add( d, m, j, ph.origin + rel, r.w, 1.0 );
-          }
-        }
-        __syncthreads();
-      }
-      else
-      {
-        // merge sort in global memory: LDS-sorted runs of BIG_LDS, then pairwise merges
-        Rec* A = R;
-        Rec* B = d.rec_tmp + static_cast< uint64_t >( blockIdx.x ) * d.max_cap;
-        for ( uint64_t r0 = 0; r0 < n; r0 += BIG_LDS )
-        {
-          const int len = static_cast< int >( n - r0 < BIG_LDS ? n - r0 : BIG_LDS );
-          int p2 = 1;
-          while ( p2 < len )
-            p2 <<= 1;
-          for ( int q = threadIdx.x; q < p2; q += blockDim.x )
-          {
-            key[ q ] = q < len ? rec_key( A[ r0 + q ] ) : ~0ull;
-            idx[ q ] = q;
-          }
-          __syncthreads();
-          wg_sort_lds( key, idx, p2 );
-          for ( int q = threadIdx.x; q < len; q += blockDim.x )
-            B[ r0 + q ] = A[ r0 + idx[ q ] ];
-          __syncthreads();
-        }
-        // B holds sorted runs; merge runs of width w into the other buffer
-        Rec* src = B;
-        Rec* dst = A;
-        for ( uint64_t w = BIG_LDS; w < n; w <<= 1 )
-        {
-          for ( uint64_t o = threadIdx.x; o < n; o += blockDim.x )
-          {
-            // output position o belongs to the pair starting at lo
-            const uint64_t lo = ( o / ( 2 * w ) ) * ( 2 * w );
-            const uint64_t mid = lo + w < n ? lo + w : n;
-            const uint64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
-            const uint64_t rank = o - lo;
-            // co-rank: number a taken from [lo, mid) among the first rank outputs
-            uint64_t a_lo = rank > ( hi - mid ) ? rank - ( hi - mid ) : 0;
-            uint64_t a_hi = rank < ( mid - lo ) ? rank : ( mid - lo );
-            while ( a_lo < a_hi )
-            {
-              const uint64_t a = ( a_lo + a_hi ) / 2;
-              const uint64_t bb = rank - a;
-              // take more from A-run if A[a] < B[bb-1]
-              if ( bb > 0 and a < mid - lo and rec_key( src[ lo + a ] ) < rec_key( src[ mid + bb - 1 ] ) )
-                a_lo = a + 1;
-              else
-                a_hi = a;
-            }
-            const uint64_t a = a_lo, bb = rank - a;
-            bool take_a;
-            if ( a >= mid - lo )
-              take_a = false;
-            else if ( bb >= hi - mid )
-              take_a = true;
-            else
-              take_a = rec_key( src[ lo + a ] ) < rec_key( src[ mid + bb ] );
-            dst[ o ] = take_a ? src[ lo + a ] : src[ mid + bb ];
-          }
-          __syncthreads();
-          Rec* tmp = src;
-          src = dst;
-          dst = tmp;
-        }
-        for ( int cell = threadIdx.x; cell < n_cells; cell += blockDim.x )
-        {
-          for ( uint64_t q = 0; q < n; ++q )
-          {
-            const Rec r = src[ q ];
-            const int rel = static_cast< int >( r.pr & smask );
-            int ch = 0;
-            if ( m == M_IAF_PSC_EXP )
-              ch = r.w > 0 ? 1 : 0;
-            else if ( m == M_AEIF_COND_EXP )
-              ch = r.w > 0.0 ? 0 : 1;
-            if ( rel * C + ch != cell )
-              continue;
-            rb_add( d, m, j, ph.origin + rel, r.w, 1.0 );
-          }
-        }
-        __syncthreads();
-      }
-      if ( threadIdx.x == 0 )
-        d.cnt[ t ] = 0;
-      __syncthreads();
-    }
+    if ( q < wid )
+      pre += sh[ q ];
+    tot += sh[ q ];
   }
-}
-
-// Small per-target lists: the update thread adds its records in key order by selection.
-__device__ inline void
-reduce_small( const NetDev& d, int m, uint32_t dev, uint32_t j, int64_t origin )
-{
-  const uint32_t c = d.cnt[ dev ];
-  if ( c == 0 )
-    return;
-  d.cnt[ dev ] = 0;
-  if ( c > static_cast< uint32_t >( d.small_max ) )
-    return;  // handled by k_big (cannot happen: k_big resets the count)
-  const uint64_t ib = d.in_ptr[ dev ];
-  const uint64_t cap = ( d.in_ptr[ dev + 1 ] - ib ) * static_cast< uint64_t >( d.s_cap );
-  const uint32_t n = c < cap ? c : static_cast< uint32_t >( cap );
-  const Rec* R = d.rec + ib * d.s_cap;
-  const uint32_t smask = ( 1u << d.slot_bits ) - 1u;
-  uint64_t last = 0;
-  for ( uint32_t it = 0; it < n; ++it )
-  {
-    uint64_t best = ~0ull;
-    uint32_t bi = 0;
-    for ( uint32_t q = 0; q < n; ++q )
-    {
-      const uint64_t kq = rec_key( R[ q ] );
-      if ( ( it == 0 or kq > last ) and kq < best )
-      {
-        best = kq;
-        bi = q;
-      }
-    }
-    last = best;
-    const Rec r = R[ bi ];
-    rb_add( d, m, j, origin + static_cast< int64_t >( r.pr & smask ), r.w, 1.0 );
-  }
+  *excl = pre + x - v;
+  __syncthreads();
+  return tot;
 }
 
 // Poisson count for seeded drive: splitmix64 of (seed, stream, step), inverse CDF
@@ -342,17 +131,47 @@ poisson_count( const PoissonTable& T, uint64_t stream, int64_t step )
   return k;
 }
 
+// Ring-buffer cells of one block seen from one slice: slot_rel = absolute step - origin.
+// Window mode keeps slots [0, max_delay) in LDS; the rest (only the drive reaches them)
+// and everything in global mode live in the block's ring in HBM.
+template < bool WIN >
+struct Cells
+{
+  double* lds;
+  double* rbb;  // block's ring: [slot][channel][tb]
+  uint32_t nct;  // doubles per slot (channels * tb)
+  uint32_t tb;
+  uint32_t s0;  // ring slot of the slice origin
+  uint32_t L;
+  uint32_t W;  // window slots
+  bool win;
+
+  __device__ double*
+  at( uint32_t slot_rel, int ch, uint32_t local ) const
+  {
+    if ( WIN and win and slot_rel < W )
+      return lds + slot_rel * nct + ch * tb + local;
+    uint32_t slot = s0 + slot_rel;
+    if ( slot >= L )
+      slot -= L;
+    return rbb + static_cast< size_t >( slot ) * nct + ch * tb + local;
+  }
+};
+
+// spike_generator drive of this part: handle() of the target with weight * multiplicity
+template < bool WIN >
 __device__ inline void
-drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t j, int64_t origin, int from, int to )
+drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
+  int64_t origin, int from, int to )
 {
   if ( ps.drive_mode == 0 )
     return;
   const uint32_t col = d.drive_idx[ dev ];
-  if ( col == 0xffffffffu )
+  if ( col == NONE )
     return;
   for ( int lag = from; lag < to; ++lag )
   {
-    const int64_t e = origin + lag;  // emission step: spike stamp e+1
+    const int64_t e = origin + lag;  // emission step; spike stamp e + 1
     if ( e < ps.drive_t0 or e >= ps.drive_t0 + ps.drive_steps )
       continue;
     uint32_t c;
@@ -360,47 +179,342 @@ drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t j
       c = d.drive_counts[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
     else
       c = poisson_count( d.ptables[ ps.drive_table ], col, e );
-    if ( c )
-      rb_add( d, m, j, e + ps.drive_d, ps.drive_w, static_cast< double >( c ) );
+    if ( c == 0 )
+      continue;
+    const double w = ps.drive_w;
+    const double mult = static_cast< double >( c );
+    const uint32_t at = static_cast< uint32_t >( lag + ps.drive_d );
+    if ( m == M_IAF_PSC_DELTA )
+      *C.at( at, 0, local ) += w * mult;
+    else if ( m == M_IAF_PSC_EXP )
+    {
+      const double sv = w * mult;
+      *C.at( at, sv > 0 ? 1 : 0, local ) += sv;
+    }
+    else
+    {
+      if ( w > 0.0 )
+        *C.at( at, 0, local ) += w * mult;
+      else
+        *C.at( at, 1, local ) += -w * mult;
+    }
   }
 }
 
-__device__ inline void
-write_mask( const NetDev& d, uint32_t dev, const uint64_t* bits )
-{
-  const uint32_t o = d.dev2ord ? d.dev2ord[ dev ] : dev;
-  for ( int w = 0; w < d.nw; ++w )
-    d.mask[ static_cast< uint64_t >( o ) * d.nw + w ] = bits[ w ];
-}
+// Phase timers of the slice kernel, compiled in only with -DNG_PROFILE: per phase, the sum
+// over workgroups of wall-clock ticks (read back with nestgpu_profile_read).
+#ifdef NG_PROFILE
+__device__ unsigned long long g_phase_ticks[ 8 ];
+#define NG_PHASE( i )                                                               \
+  do                                                                                \
+  {                                                                                 \
+    __syncthreads();                                                                \
+    if ( threadIdx.x == 0 )                                                         \
+    {                                                                               \
+      const unsigned long long now_ = wall_clock64();                               \
+      atomicAdd( &g_phase_ticks[ i ], now_ - t_phase_ );                            \
+      t_phase_ = now_;                                                              \
+    }                                                                               \
+  } while ( 0 )
+#else
+#define NG_PHASE( i ) \
+  do                  \
+  {                   \
+  } while ( 0 )
+#endif
 
-// ------------------------------------------------------------------ iaf_psc_delta
-// models/iaf_psc_delta.cpp:272-331 (refractory_input = false)
-__global__ void __launch_bounds__( BS ) k_update_iaf_psc_delta( const NetDev* __restrict__ nets, int n_nets, int s )
+template < int MASK, bool WIN >
+__global__ void __launch_bounds__( BS )
+  k_slice( const NetDev* __restrict__ nets, const uint32_t* __restrict__ blk_prefix, int n_nets, int s )
 {
-  constexpr int m = M_IAF_PSC_DELTA;
-  for ( int i = 0; i < n_nets; ++i )
+#ifdef NG_PROFILE
+  unsigned long long t_phase_ = wall_clock64();
+#endif
+  __shared__ SliceLds< WIN > S;
+
+  // ---- which instance and block
+  const uint32_t gb = xcd_block( blockIdx.x, gridDim.x );
+  int lo = 0, hi = n_nets;
+  while ( hi - lo > 1 )
   {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.active )
-      continue;
-    const uint32_t b0 = d.model_begin[ m ];
-    const uint32_t nm = d.model_begin[ m + 1 ] - b0;
-    for ( uint32_t j = blockIdx.x * BS + threadIdx.x; j < nm; j += gridDim.x * BS )
+    const int mid = ( lo + hi ) >> 1;
+    if ( blk_prefix[ mid ] <= gb )
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const NetDev& d = nets[ lo ];
+  const uint32_t b = gb - blk_prefix[ lo ];
+  const SlicePhase ph = slice_phase( d, s );
+  if ( not ph.active or b >= d.n_blocks )
+    return;
+  const uint32_t P = d.n_blocks;
+  const uint32_t tb = d.tb;
+  const int par = ph.parity;
+  const int ppar = par ^ 1;
+  const uint32_t t = threadIdx.x;
+
+  Cells< WIN > C;
+  C.lds = S.win;
+  C.nct = static_cast< uint32_t >( d.nch ) * tb;
+  C.tb = tb;
+  C.L = static_cast< uint32_t >( d.L );
+  C.s0 = static_cast< uint32_t >( ph.origin % d.L );
+  C.W = static_cast< uint32_t >( d.max_delay );
+  C.win = WIN and d.win != 0;
+  C.rbb = d.rb + static_cast< size_t >( b ) * C.L * C.nct;
+
+  // ================================================================ 0. stage the window
+  const uint32_t n_win = C.W * C.nct;
+  const uint32_t before_wrap = ( C.L - C.s0 ) * C.nct;
+  if ( WIN and C.win )
+  {
+    for ( uint32_t i = t; i < n_win; i += BS )
+      S.win[ i ] = C.rbb[ i < before_wrap ? C.s0 * C.nct + i : i - before_wrap ];
+  }
+
+  // ================================================================ 1. delivery
+  if ( ph.deliver )
+  {
+    S.bcnt[ t ] = 0;
+    const bool flat = d.flat_meta[ 2 + ppar ] != 0;
+    uint32_t K;
+    if ( flat )
+      K = d.flat_meta[ ppar ];
+    else
     {
-      const uint32_t dev = b0 + j;
-      if ( ph.deliver )
-        reduce_small( d, m, dev, j, ph.origin );
-      const ParamSet& ps = d.psets[ d.pidx[ m ][ j ] ];
+      // prefix of the per-block spike counts of the delivered slice
+      const uint32_t* cnt = d.blk_cnt + static_cast< size_t >( ppar ) * P;
+      uint32_t carry = 0;
+      for ( uint32_t p0 = 0; p0 < P; p0 += BS )
+      {
+        const uint32_t v = p0 + t < P ? cnt[ p0 + t ] : 0;
+        uint32_t ex;
+        const uint32_t tot = block_scan( v, &ex, S.sh );
+        if ( p0 + t < P )
+          S.pref[ p0 + t ] = carry + ex;
+        carry += tot;
+      }
+      if ( t == 0 )
+        S.pref[ P ] = carry;
+      K = carry;
+    }
+    __syncthreads();
+    NG_PHASE( 0 );
+    unsigned long long events = 0;
+    for ( uint32_t k0 = 0; k0 < K; k0 += CH )
+    {
+      // each thread resolves SPT consecutive spikes of the chunk: (source, lag), then its
+      // run [g0, g1) in the source's row; the loads of one step are independent
+      uint32_t src[ SPT ], lag[ SPT ];
+      const uint32_t kb = k0 + t * SPT;
+      if ( flat )
+      {
+#pragma unroll
+        for ( int q = 0; q < SPT; ++q )
+          if ( kb + q < K )
+          {
+            src[ q ] = d.flat_src[ kb + q ];
+            lag[ q ] = d.flat_lag[ kb + q ];
+          }
+      }
+      else if ( kb < K )
+      {
+        uint32_t blk = upper_bound_u32( S.pref, P + 1, kb ) - 1;
+#pragma unroll
+        for ( int q = 0; q < SPT; ++q )
+        {
+          const uint32_t k = kb + q;
+          if ( k < K )
+          {
+            while ( S.pref[ blk + 1 ] <= k )
+              ++blk;
+            const uint32_t e =
+              d.blk_spk[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - S.pref[ blk ] ) ];
+            src[ q ] = blk * tb + ( e >> 16 );
+            lag[ q ] = e & 0xffu;
+          }
+        }
+      }
+      uint32_t lens[ SPT ];
+      uint32_t mine = 0;
+#pragma unroll
+      for ( int q = 0; q < SPT; ++q )
+      {
+        lens[ q ] = 0;
+        const uint32_t j = t * SPT + q;
+        if ( kb + q < K )
+        {
+          const uint32_t* bo = d.boff + static_cast< size_t >( src[ q ] ) * ( P + 1 ) + b;
+          const uint32_t g0 = bo[ 0 ];
+          lens[ q ] = bo[ 1 ] - g0;
+          S.c_g0[ j ] = g0;
+          S.c_lag[ j ] = static_cast< uint8_t >( lag[ q ] );
+        }
+        mine += lens[ q ];
+      }
+      uint32_t ex;
+      const uint32_t E = block_scan( mine, &ex, S.sh );
+#pragma unroll
+      for ( int q = 0; q < SPT; ++q )
+      {
+        S.c_off[ t * SPT + q ] = ex;
+        ex += lens[ q ];
+      }
+      if ( t == 0 )
+        S.c_off[ CH ] = E;
+      __syncthreads();
+      NG_PHASE( 1 );
+      events += E;
+      // rounds of R events; the synapse records of round r + 1 are loaded while round r
+      // is applied
+      SynRec rec[ RQ ];
+      uint32_t rlag[ RQ ];
+      auto load_round = [ & ]( uint32_t e0 )
+      {
+        const uint32_t eb = e0 + t * RQ;
+        uint32_t j = eb < E ? upper_bound_u32( S.c_off, CH + 1, eb ) - 1 : 0;
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+          if ( eb + q < E )
+          {
+            while ( S.c_off[ j + 1 ] <= eb + q )
+              ++j;
+            rec[ q ] = d.syn[ S.c_g0[ j ] + ( eb + q - S.c_off[ j ] ) ];
+            rlag[ q ] = S.c_lag[ j ];
+          }
+      };
+      load_round( 0 );
+      for ( uint32_t e0 = 0; e0 < E; e0 += R )
+      {
+        // stage: thread t holds events e0 + t*RQ .. + RQ - 1 (key order = round index) and
+        // counts them into its target's bucket
+        uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ];
+        double val[ RQ ];
+        const uint32_t eb = e0 + t * RQ;
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+        {
+          loc[ q ] = NONE;
+          if ( eb + q < E )
+          {
+            const double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
+            const uint32_t local = rec[ q ].word & 0xffffu;
+            const uint32_t slot_rel = rlag[ q ] + ( rec[ q ].word >> 16 );
+            const int m = d.uniform_model >= 0 ? d.uniform_model : d.model[ b * tb + local ];
+            uint32_t ch = 0;
+            double v = w;  // static_synapse: weight * multiplicity 1
+            if ( m == M_IAF_PSC_EXP )
+              ch = w > 0 ? 1u : 0u;
+            else if ( m == M_AEIF_COND_EXP and not( w > 0.0 ) )
+            {
+              ch = 1;
+              v = -w;
+            }
+            loc[ q ] = local;
+            val[ q ] = v;
+            key[ q ] = ( static_cast< uint32_t >( t * RQ + q ) << IDX_SHIFT ) | ( slot_rel * d.nch + ch );
+            dst[ q ] = atomicAdd( &S.bcnt[ local ], 1u );  // an unordered place in the bucket
+          }
+        }
+        __syncthreads();
+        NG_PHASE( 2 );
+        const uint32_t mycnt = t < tb ? S.bcnt[ t ] : 0u;
+        uint32_t mybase;
+        const uint32_t tot = block_scan( mycnt, &mybase, S.sh );
+        if ( t < tb )
+          S.bst[ t ] = mybase;
+        if ( t == 0 )
+          S.bst[ tb ] = tot;
+        __syncthreads();
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+          if ( loc[ q ] != NONE )
+            S.bk[ S.bst[ loc[ q ] ] + dst[ q ] ] = key[ q ];
+        __syncthreads();
+        // sorted place of each event in its bucket: the number of smaller keys there
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+          if ( loc[ q ] != NONE )
+          {
+            const uint32_t lo = S.bst[ loc[ q ] ], hi = S.bst[ loc[ q ] + 1 ];
+            uint32_t r = 0;
+            for ( uint32_t i = lo; i < hi; ++i )
+              r += S.bk[ i ] < key[ q ] ? 1u : 0u;
+            dst[ q ] = lo + r;
+          }
+        __syncthreads();
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+          if ( loc[ q ] != NONE )
+          {
+            S.bk[ dst[ q ] ] = key[ q ] & SC_MASK;
+            S.bw[ dst[ q ] ] = val[ q ];
+          }
+        __syncthreads();
+        if ( e0 + R < E )
+          load_round( e0 + R );
+        if ( t < tb and mycnt )
+        {
+          // owner of target t adds its bucket into the cells in key order; consecutive
+          // events of one cell accumulate in a register
+          const uint32_t* kk = S.bk + mybase;
+          const double* ww = S.bw + mybase;
+          uint32_t a = 0;
+          if ( WIN and C.win )
+            while ( a < mycnt )
+            {
+              const uint32_t sc = kk[ a ];
+              double v = S.win[ sc * tb + t ];
+              for ( ; a < mycnt and kk[ a ] == sc; ++a )
+                v = v + ww[ a ];
+              S.win[ sc * tb + t ] = v;
+            }
+          else
+            while ( a < mycnt )
+            {
+              const uint32_t sc = kk[ a ];
+              const uint32_t ch = d.nch == 2 ? ( sc & 1u ) : 0u;
+              double* cell = C.at( d.nch == 2 ? ( sc >> 1 ) : sc, static_cast< int >( ch ), t );
+              double v = *cell;
+              for ( ; a < mycnt and kk[ a ] == sc; ++a )
+                v = v + ww[ a ];
+              *cell = v;
+            }
+        }
+        NG_PHASE( 3 );
+        if ( t < tb )
+          S.bcnt[ t ] = 0;
+        __syncthreads();
+      }
+    }
+    if ( t == 0 and events )
+      atomicAdd( &d.counters[ 1 ], events );
+  }
+  __syncthreads();  // window staged and delivered
+  NG_PHASE( 4 );
+
+  // ================================================================ 2. update
+  const uint32_t local = t;
+  const uint32_t dev = b * tb + local;
+  uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
+  if ( local < tb and dev < d.n )
+  {
+    const ParamSet& ps = d.psets[ d.pidx[ dev ] ];
+    const int m = ( MASK == 1 ) ? M_IAF_PSC_DELTA
+      : ( MASK == 2 )           ? M_IAF_PSC_EXP
+      : ( MASK == 4 )           ? M_AEIF_COND_EXP
+                                : ps.model;
+    if ( ( MASK & 1 ) and m == M_IAF_PSC_DELTA )
+    {
+      // models/iaf_psc_delta.cpp:272-331 (refractory_input = false)
       const double P30 = ps.p[ 0 ], P33 = ps.p[ 1 ], y0I = ps.p[ 2 ], V_th = ps.p[ 3 ], V_reset = ps.p[ 4 ],
                    V_min = ps.p[ 5 ];
-      double y3 = d.st[ m ][ 0 ][ j ];
-      int32_t r = d.r[ m ][ j ];
-      uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
+      double y3 = d.v[ 0 ][ dev ];
+      int32_t r = d.r[ dev ];
       for ( int lag = ph.from; lag < ph.to; ++lag )
       {
-        const int64_t slot = ( ph.origin + lag ) % d.L;
-        double* cell = d.rb[ m ] + static_cast< size_t >( slot ) * nm + j;
+        double* cell = C.at( lag, 0, local );
         const double in = *cell;
         *cell = 0.0;
         if ( r == 0 )
@@ -417,45 +531,24 @@ __global__ void __launch_bounds__( BS ) k_update_iaf_psc_delta( const NetDev* __
           bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
         }
       }
-      drive_adds( d, ps, m, dev, j, ph.origin, ph.from, ph.to );
-      d.st[ m ][ 0 ][ j ] = y3;
-      d.r[ m ][ j ] = r;
-      write_mask( d, dev, bits );
+      d.v[ 0 ][ dev ] = y3;
+      d.r[ dev ] = r;
     }
-  }
-}
-
-// ------------------------------------------------------------------ iaf_psc_exp
-// models/iaf_psc_exp.cpp:288-349 (delta = 0: deterministic threshold)
-__global__ void __launch_bounds__( BS ) k_update_iaf_psc_exp( const NetDev* __restrict__ nets, int n_nets, int s )
-{
-  constexpr int m = M_IAF_PSC_EXP;
-  for ( int i = 0; i < n_nets; ++i )
-  {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.active )
-      continue;
-    const uint32_t b0 = d.model_begin[ m ];
-    const uint32_t nm = d.model_begin[ m + 1 ] - b0;
-    for ( uint32_t j = blockIdx.x * BS + threadIdx.x; j < nm; j += gridDim.x * BS )
+    if ( ( MASK & 2 ) and m == M_IAF_PSC_EXP )
     {
-      const uint32_t dev = b0 + j;
-      if ( ph.deliver )
-        reduce_small( d, m, dev, j, ph.origin );
-      const ParamSet& ps = d.psets[ d.pidx[ m ][ j ] ];
+      // models/iaf_psc_exp.cpp:288-349 (delta = 0: deterministic threshold)
       const double P22 = ps.p[ 0 ], P21ex = ps.p[ 1 ], P21in = ps.p[ 2 ], P20 = ps.p[ 3 ], P11ex = ps.p[ 4 ],
                    P11in = ps.p[ 5 ], IeI0 = ps.p[ 6 ], Theta = ps.p[ 7 ], V_reset = ps.p[ 8 ], exI1 = ps.p[ 9 ];
-      double V = d.st[ m ][ 0 ][ j ];
-      double iex = d.st[ m ][ 1 ][ j ];
-      double iin = d.st[ m ][ 2 ][ j ];
-      int32_t r = d.r[ m ][ j ];
-      uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
+      double V = d.v[ 0 ][ dev ];
+      double iex = d.v[ 1 ][ dev ];
+      double iin = d.v[ 2 ][ dev ];
+      int32_t r = d.r[ dev ];
       for ( int lag = ph.from; lag < ph.to; ++lag )
       {
-        const int64_t slot = ( ph.origin + lag ) % d.L;
-        double* cin = d.rb[ m ] + static_cast< size_t >( slot ) * 2 * nm + j;
-        double* cex = cin + nm;
+        double* cin = C.at( lag, 0, local );
+        double* cex = C.at( lag, 1, local );
+        const double in_in = *cin;
+        const double in_ex = *cex;
         if ( r == 0 )
           V = V * P22 + iex * P21ex + iin * P21in + IeI0 * P20;
         else
@@ -463,304 +556,241 @@ __global__ void __launch_bounds__( BS ) k_update_iaf_psc_exp( const NetDev* __re
         iex *= P11ex;
         iin *= P11in;
         iex += exI1;
-        const double wex = *cex;
-        const double win = *cin;
-        iex += wex;
-        iin += win;
+        iex += in_ex;
+        iin += in_in;
         if ( V >= Theta )
         {
           r = ps.refractory_counts;
           V = V_reset;
           bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
         }
-        *cex = 0.0;
         *cin = 0.0;
+        *cex = 0.0;
       }
-      drive_adds( d, ps, m, dev, j, ph.origin, ph.from, ph.to );
-      d.st[ m ][ 0 ][ j ] = V;
-      d.st[ m ][ 1 ][ j ] = iex;
-      d.st[ m ][ 2 ][ j ] = iin;
-      d.r[ m ][ j ] = r;
-      write_mask( d, dev, bits );
+      d.v[ 0 ][ dev ] = V;
+      d.v[ 1 ][ dev ] = iex;
+      d.v[ 2 ][ dev ] = iin;
+      d.r[ dev ] = r;
     }
-  }
-}
-
-// ------------------------------------------------------------------ aeif_cond_exp
-// models/aeif_cond_exp.cpp:443-523
-__global__ void __launch_bounds__( BS ) k_update_aeif_cond_exp( const NetDev* __restrict__ nets, int n_nets, int s )
-{
-  constexpr int m = M_AEIF_COND_EXP;
-  for ( int i = 0; i < n_nets; ++i )
-  {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.active )
-      continue;
-    const uint32_t b0 = d.model_begin[ m ];
-    const uint32_t nm = d.model_begin[ m + 1 ] - b0;
-    for ( uint32_t j = blockIdx.x * BS + threadIdx.x; j < nm; j += gridDim.x * BS )
+    if ( ( MASK & 4 ) and m == M_AEIF_COND_EXP )
     {
-      const uint32_t dev = b0 + j;
-      if ( ph.deliver )
-        reduce_small( d, m, dev, j, ph.origin );
-      const ParamSet& ps = d.psets[ d.pidx[ m ][ j ] ];
-      AeifPar P;
-      P.V_peak = ps.p[ 0 ];
-      P.V_reset = ps.p[ 1 ];
-      P.g_L = ps.p[ 2 ];
-      P.C_m = ps.p[ 3 ];
-      P.E_ex = ps.p[ 4 ];
-      P.E_in = ps.p[ 5 ];
-      P.E_L = ps.p[ 6 ];
-      P.Delta_T = ps.p[ 7 ];
-      P.tau_w = ps.p[ 8 ];
-      P.a = ps.p[ 9 ];
-      P.b = ps.p[ 10 ];
-      P.V_th = ps.p[ 11 ];
-      P.tau_ex = ps.p[ 12 ];
-      P.tau_in = ps.p[ 13 ];
-      P.I_e = ps.p[ 14 ];
-      P.tol = ps.p[ 15 ];
+      // models/aeif_cond_exp.cpp:443-523
+      AeifPar A;
+      A.V_peak = ps.p[ 0 ];
+      A.V_reset = ps.p[ 1 ];
+      A.g_L = ps.p[ 2 ];
+      A.C_m = ps.p[ 3 ];
+      A.E_ex = ps.p[ 4 ];
+      A.E_in = ps.p[ 5 ];
+      A.E_L = ps.p[ 6 ];
+      A.Delta_T = ps.p[ 7 ];
+      A.tau_w = ps.p[ 8 ];
+      A.a = ps.p[ 9 ];
+      A.b = ps.p[ 10 ];
+      A.V_th = ps.p[ 11 ];
+      A.tau_ex = ps.p[ 12 ];
+      A.tau_in = ps.p[ 13 ];
+      A.I_e = ps.p[ 14 ];
+      A.tol = ps.p[ 15 ];
       const double V_peak_eff = ps.p[ 16 ];
       const double hstep = ps.p[ 17 ];
-      double y[ 4 ] = { d.st[ m ][ 0 ][ j ], d.st[ m ][ 1 ][ j ], d.st[ m ][ 2 ][ j ], d.st[ m ][ 3 ][ j ] };
-      double hint = d.st[ m ][ 4 ][ j ];
-      int32_t r = d.r[ m ][ j ];
-      uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
+      double y[ 4 ] = { d.v[ 0 ][ dev ], d.v[ 1 ][ dev ], d.v[ 2 ][ dev ], d.v[ 3 ][ dev ] };
+      double hint = d.v[ 4 ][ dev ];
+      int32_t r = d.r[ dev ];
       for ( int lag = ph.from; lag < ph.to; ++lag )
       {
-        double t = 0.0;
-        int guard = 0;
-        while ( t < hstep and guard < 1000000 )
+        double tt = 0.0;
+        for ( int guard = 0; tt < hstep and guard < 1000000; ++guard )
         {
-          ++guard;
-          aeif_evolve( P, r > 0, &t, hstep, &hint, y );
+          aeif_evolve( A, r > 0, &tt, hstep, &hint, y );
           if ( y[ 0 ] < -1e3 or y[ 3 ] < -1e6 or y[ 3 ] > 1e6 )
           {
-            atomicOr( &d.ctr[ 2 ], 2u );  // NumericalInstability
-            t = hstep;
+            atomicOr( &d.counters[ 0 ], ERR_AEIF_INSTABILITY );
+            tt = hstep;
             break;
           }
           if ( r > 0 )
-            y[ 0 ] = P.V_reset;
+            y[ 0 ] = A.V_reset;
           else if ( y[ 0 ] >= V_peak_eff )
           {
-            y[ 0 ] = P.V_reset;
-            y[ 3 ] += P.b;
+            y[ 0 ] = A.V_reset;
+            y[ 3 ] += A.b;
             r = ps.refractory_counts > 0 ? ps.refractory_counts + 1 : 0;
             if ( bits[ lag >> 6 ] & ( 1ull << ( lag & 63 ) ) )
-              atomicOr( &d.ctr[ 2 ], 4u );  // second spike within one step
+              atomicOr( &d.counters[ 0 ], ERR_AEIF_DOUBLE_SPIKE );
             bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
           }
         }
         if ( r > 0 )
           --r;
-        const int64_t slot = ( ph.origin + lag ) % d.L;
-        double* cex = d.rb[ m ] + static_cast< size_t >( slot ) * 2 * nm + j;
-        double* cinh = cex + nm;
+        double* cex = C.at( lag, 0, local );
+        double* cinh = C.at( lag, 1, local );
         y[ 1 ] += *cex;
         y[ 2 ] += *cinh;
         *cex = 0.0;
         *cinh = 0.0;
       }
-      drive_adds( d, ps, m, dev, j, ph.origin, ph.from, ph.to );
       for ( int q = 0; q < 4; ++q )
-        d.st[ m ][ q ][ j ] = y[ q ];
-      d.st[ m ][ 4 ][ j ] = hint;
-      d.r[ m ][ j ] = r;
-      write_mask( d, dev, bits );
+        d.v[ q ][ dev ] = y[ q ];
+      d.v[ 4 ][ dev ] = hint;
+      d.r[ dev ] = r;
     }
+    drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, ph.to );
   }
-}
+  NG_PHASE( 5 );
 
-// ------------------------------------------------------------------ compaction
-// Decoupled look-back scan over order positions (one tile of BS per workgroup step).
-// Tile state: (epoch << 32) | (flag << 30) | value, flag 1 = aggregate, 2 = inclusive.
-__global__ void __launch_bounds__( BS ) k_compact( const NetDev* __restrict__ nets, int n_nets, int s )
-{
-  __shared__ uint32_t warp_sums[ BS / 64 ];
-  __shared__ uint32_t s_excl;
-  __shared__ uint32_t s_base;
-  for ( int i = 0; i < n_nets; ++i )
+  // ================================================================ 3. spike lists
+  const uint32_t nsp = __popcll( bits[ 0 ] ) + __popcll( bits[ 1 ] ) + __popcll( bits[ 2 ] ) + __popcll( bits[ 3 ] );
+  uint32_t ex;
+  const uint32_t tot = block_scan( nsp, &ex, S.sh );  // also orders the update before the write-back
+  if ( WIN and C.win )
   {
-    const NetDev& d = nets[ i ];
-    const SlicePhase ph = slice_phase( d, s );
-    if ( not ph.active )
-      continue;
-    const uint32_t base = ph.deliver ? 0u : d.ctr[ 0 ];
-    const uint32_t epoch = d.chunk_seq * 1024u + static_cast< uint32_t >( s );
-    const uint32_t n_tiles = ( d.n + BS - 1 ) / BS;
-    uint64_t* state = reinterpret_cast< uint64_t* >( d.scan_state );
-    for ( uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x )
+    for ( uint32_t i = t; i < n_win; i += BS )
+      C.rbb[ i < before_wrap ? C.s0 * C.nct + i : i - before_wrap ] = S.win[ i ];
+  }
+  uint32_t* cnt = d.blk_cnt + static_cast< size_t >( par ) * P + b;
+  if ( t == 0 )
+  {
+    const uint32_t base = ph.from == 0 ? 0u : *cnt;  // a new slice starts an empty list
+    S.cnt_base = base;
+    if ( base + tot > d.spk_cap )
+      atomicOr( &d.counters[ 0 ], ERR_SPIKE_LIST );
+    *cnt = base + tot < d.spk_cap ? base + tot : d.spk_cap;
+    if ( ph.from == 0 and b == 0 )
+      d.flat_meta[ 2 + par ] = 0;
+    S.log_base = 0ull;
+    if ( tot )
     {
-      const uint32_t o = tile * BS + threadIdx.x;
-      uint32_t cnt = 0;
-      uint64_t words[ 4 ] = { 0, 0, 0, 0 };
-      if ( o < d.n )
-        for ( int w = 0; w < d.nw; ++w )
-        {
-          words[ w ] = d.mask[ static_cast< uint64_t >( o ) * d.nw + w ];
-          cnt += __popcll( words[ w ] );
-        }
-      // block inclusive scan of cnt
-      const int lane = threadIdx.x & 63;
-      const int wid = threadIdx.x >> 6;
-      uint32_t v = cnt;
-      for ( int off = 1; off < 64; off <<= 1 )
-      {
-        const uint32_t u = __shfl_up( v, off, 64 );
-        if ( lane >= off )
-          v += u;
-      }
-      if ( lane == 63 )
-        warp_sums[ wid ] = v;
-      __syncthreads();
-      uint32_t wpre = 0, total = 0;
-      for ( int q = 0; q < BS / 64; ++q )
-      {
-        if ( q < wid )
-          wpre += warp_sums[ q ];
-        total += warp_sums[ q ];
-      }
-      const uint32_t local_excl = wpre + v - cnt;
-      if ( threadIdx.x == 0 )
-      {
-        const uint64_t ep = static_cast< uint64_t >( epoch ) << 32;
-        if ( tile == 0 )
-        {
-          __hip_atomic_store( &state[ tile ], ep | ( 2ull << 30 ) | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
-          s_excl = 0;
-        }
-        else
-        {
-          __hip_atomic_store( &state[ tile ], ep | ( 1ull << 30 ) | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
-          uint32_t excl = 0;
-          int64_t jt = static_cast< int64_t >( tile ) - 1;
-          long spins = 0;
-          while ( jt >= 0 )
-          {
-            const uint64_t st = __hip_atomic_load( &state[ jt ], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT );
-            const uint32_t f = static_cast< uint32_t >( ( st >> 30 ) & 3u );
-            if ( ( st >> 32 ) != epoch or f == 0 )
-            {
-              if ( ++spins > ( 1l << 26 ) )
-              {
-                atomicOr( &d.ctr[ 2 ], 8u );  // look-back did not converge
-                break;
-              }
-              continue;
-            }
-            excl += static_cast< uint32_t >( st & 0x3fffffffu );
-            if ( f == 2 )
-              break;
-            --jt;
-          }
-          __hip_atomic_store(
-            &state[ tile ], ep | ( 2ull << 30 ) | ( excl + total ), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
-          s_excl = excl;
-        }
-        s_base = base;
-      }
-      __syncthreads();
-      uint32_t off = s_base + s_excl + local_excl;
-      if ( o < d.n and cnt )
-      {
-        const uint32_t dev = d.ord2dev ? d.ord2dev[ o ] : o;
-        const unsigned long long lb = *d.log_n;
-        for ( int w = 0; w < d.nw; ++w )
-        {
-          uint64_t x = words[ w ];
-          while ( x )
-          {
-            const int bpos = __ffsll( static_cast< long long >( x ) ) - 1;
-            x &= x - 1;
-            const uint32_t lag = w * 64 + bpos;
-            d.spk[ off ] = dev;
-            d.spk_lag[ off ] = lag;
-            const unsigned long long li = lb + ( off - s_base );
-            if ( li < d.log_cap )
-            {
-              d.log_src[ li ] = dev;
-              d.log_step[ li ] = ph.origin + lag + 1;
-            }
-            ++off;
-          }
-        }
-      }
-      if ( o < d.n and cnt )
-        for ( int w = 0; w < d.nw; ++w )
-          d.mask[ static_cast< uint64_t >( o ) * d.nw + w ] = 0;
-      if ( tile == n_tiles - 1 and threadIdx.x == 0 )
-      {
-        d.ctr[ 0 ] = s_base + s_excl + total;
-        d.ctr[ 4 ] = s_excl + total;  // spikes this part (for the log counter)
-      }
-      __syncthreads();
+      atomicAdd( &d.counters[ 2 ], static_cast< unsigned long long >( tot ) );
+      if ( d.log_enabled )
+        S.log_base = atomicAdd( d.log_n, static_cast< unsigned long long >( tot ) );
     }
   }
+  __syncthreads();
+  const bool log_on = d.log_enabled != 0;
+  if ( nsp )
+  {
+    uint32_t o = S.cnt_base + ex;
+    unsigned long long li = S.log_base + ex;
+    uint32_t* lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+    for ( int w = 0; w < 4; ++w )
+    {
+      unsigned long long x = bits[ w ];
+      while ( x )
+      {
+        const uint32_t lag = w * 64 + ( __ffsll( static_cast< long long >( x ) ) - 1 );
+        x &= x - 1;
+        if ( o < d.spk_cap )
+          lst[ o ] = ( local << 16 ) | ( static_cast< uint32_t >( ph.part ) << 8 ) | lag;
+        ++o;
+        if ( log_on )
+        {
+          if ( li < d.log_cap )
+          {
+            d.log_src[ li ] = dev;
+            d.log_step[ li ] = ph.origin + lag + 1;
+          }
+          else
+            atomicOr( &d.counters[ 0 ], ERR_LOG );
+          ++li;
+        }
+      }
+    }
+  }
+  NG_PHASE( 6 );
 }
 
-// Advances the spike-log counter and clears the big-target counter after compaction.
+#ifdef NG_PROFILE
+extern "C" int
+nestgpu_profile_read( unsigned long long* out8, int reset )
+{
+  if ( hipMemcpyFromSymbol( out8, HIP_SYMBOL( g_phase_ticks ), sizeof( unsigned long long ) * 8 ) != hipSuccess )
+    return 1;
+  if ( reset )
+  {
+    unsigned long long z[ 8 ] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+    if ( hipMemcpyToSymbol( HIP_SYMBOL( g_phase_ticks ), z, sizeof z ) != hipSuccess )
+      return 1;
+  }
+  return 0;
+}
+#endif
+
+// A slice split across Run calls: NEST's per-thread registers hold spikes part by part,
+// so its receive order is (VP, part, node, lag). Rebuild that order once the slice is
+// complete (rare path, one thread).
 __global__ void
-k_finish( const NetDev* __restrict__ nets, int n_nets, int s )
+k_flatten( NetDev* net, int par, int n_parts )
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if ( i >= n_nets )
+  if ( blockIdx.x != 0 or threadIdx.x != 0 )
     return;
-  const NetDev& d = nets[ i ];
-  const SlicePhase ph = slice_phase( d, s );
-  if ( not ph.active )
-    return;
-  *d.log_n += d.ctr[ 4 ];
-  d.ctr[ 4 ] = 0;
-  d.ctr[ 1 ] = 0;
-  if ( ph.deliver )
-    d.ctr[ 3 ] = 0;
-  d.ctr[ 3 ] += 1;  // parts in the current slice
-  if ( ph.complete and d.n_vp > 1 and d.ctr[ 3 ] > 1 )
+  const NetDev& d = *net;
+  const uint32_t P = d.n_blocks;
+  uint32_t o = 0;
+  for ( int vp = 0; vp < d.n_vp; ++vp )
   {
-    // A slice split across Run calls: NEST's registers hold each VP's spikes part by
-    // part, so the receive order is (VP, part, node, lag). Restore it with a stable
-    // counting pass by VP over the parts appended in order (rare; single thread).
-    const uint32_t K = d.ctr[ 0 ];
-    uint32_t* tsrc = d.spk_tmp;
-    uint32_t* tlag = d.spk_tmp + K;
-    for ( uint32_t q = 0; q < K; ++q )
-    {
-      tsrc[ q ] = d.spk[ q ];
-      tlag[ q ] = d.spk_lag[ q ];
-    }
-    uint32_t o = 0;
-    for ( int vp = 0; vp < d.n_vp; ++vp )
-      for ( uint32_t q = 0; q < K; ++q )
+    const uint32_t lo = d.vp_begin[ vp ], hi = d.vp_begin[ vp + 1 ];
+    if ( lo >= hi )
+      continue;
+    for ( int p = 0; p < n_parts; ++p )
+      for ( uint32_t b = lo / d.tb; b <= ( hi - 1 ) / d.tb and b < P; ++b )
       {
-        const uint32_t dev = tsrc[ q ];
-        const uint32_t ord = d.dev2ord ? d.dev2ord[ dev ] : dev;
-        if ( ord >= d.vp_begin[ vp ] and ord < d.vp_begin[ vp + 1 ] )
+        const uint32_t n = d.blk_cnt[ static_cast< size_t >( par ) * P + b ];
+        const uint32_t* lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+        for ( uint32_t q = 0; q < n and q < d.spk_cap; ++q )
         {
-          d.spk[ o ] = dev;
-          d.spk_lag[ o ] = tlag[ q ];
+          const uint32_t e = lst[ q ];
+          const uint32_t dev = b * d.tb + ( e >> 16 );
+          if ( static_cast< int >( ( e >> 8 ) & 0xffu ) != p or dev < lo or dev >= hi )
+            continue;
+          d.flat_src[ o ] = dev;
+          d.flat_lag[ o ] = e & 0xffu;
           ++o;
         }
       }
   }
+  d.flat_meta[ par ] = o;
+  d.flat_meta[ 2 + par ] = 1;
 }
 
 // ------------------------------------------------------------------ launch wrappers
-hipError_t
-launch_slice( const NetDev* nets, int n_nets, int s, const bool* has_model, const LaunchCfg& cfg, hipStream_t st )
+template < bool WIN >
+static void
+launch_mask( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, int s,
+  hipStream_t st )
 {
-  k_expand<<< cfg.grid_expand, BS, 0, st >>>( nets, n_nets, s );
-  k_big<<< cfg.grid_big, BS, 0, st >>>( nets, n_nets, s );
-  if ( has_model[ M_IAF_PSC_DELTA ] )
-    k_update_iaf_psc_delta<<< cfg.grid_update, BS, 0, st >>>( nets, n_nets, s );
-  if ( has_model[ M_IAF_PSC_EXP ] )
-    k_update_iaf_psc_exp<<< cfg.grid_update, BS, 0, st >>>( nets, n_nets, s );
-  if ( has_model[ M_AEIF_COND_EXP ] )
-    k_update_aeif_cond_exp<<< cfg.grid_update, BS, 0, st >>>( nets, n_nets, s );
-  k_compact<<< cfg.grid_compact, BS, 0, st >>>( nets, n_nets, s );
-  k_finish<<< ( n_nets + 63 ) / 64, 64, 0, st >>>( nets, n_nets, s );
+  switch ( mask )
+  {
+  case 1:
+    k_slice< 1, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    break;
+  case 2:
+    k_slice< 2, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    break;
+  case 4:
+    k_slice< 4, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    break;
+  default:
+    k_slice< 7, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    break;
+  }
+}
+
+hipError_t
+launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, bool win,
+  int s, hipStream_t st )
+{
+  if ( win )
+    launch_mask< true >( nets, blk_prefix, n_nets, total_blocks, mask, s, st );
+  else
+    launch_mask< false >( nets, blk_prefix, n_nets, total_blocks, mask, s, st );
+  return hipGetLastError();
+}
+
+hipError_t
+launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st )
+{
+  k_flatten<<< 1, 64, 0, st >>>( net, par, n_parts );
   return hipGetLastError();
 }
 

@@ -283,6 +283,8 @@ Kernel::reset()
   spike_sender_.clear();
   spike_step_.clear();
   spike_count_ = 0;
+  targets_per_block_ = 0;
+  delivery_mode_ = 0;
 }
 
 void
@@ -335,6 +337,34 @@ Kernel::set_kernel_status( const Json& d )
       slices_per_chunk_ = std::max( 1, static_cast< int >( kv.second.number() ) );
       own_runner_.reset();
     }
+    else if ( k == "targets_per_block" )
+    {
+      const int v = static_cast< int >( kv.second.number() );
+      if ( v < 0 or v > static_cast< int >( MAX_TB ) )
+        throw std::runtime_error(
+          "BadProperty: targets_per_block must be in [0, " + std::to_string( MAX_TB ) + "] (0 = automatic)" );
+      if ( simulated_ )
+        throw std::runtime_error( "KernelException: targets_per_block cannot change after Simulate" );
+      targets_per_block_ = v;
+      store_dirty_ = true;
+    }
+    else if ( k == "delivery_mode" )
+    {
+      const std::string v = kv.second.string();
+      const int m = v == "auto" ? 0 : v == "window" ? 1 : v == "global" ? 2 : -1;
+      if ( m < 0 )
+        throw std::runtime_error( "BadProperty: delivery_mode must be 'auto', 'window' or 'global'" );
+      if ( simulated_ and m != delivery_mode_ )
+        throw std::runtime_error( "KernelException: delivery_mode cannot change after Simulate" );
+      delivery_mode_ = m;
+      store_dirty_ = true;
+    }
+    else if ( k == "kernel_timing" )
+    {
+      time_kernels_ = kv.second.number() != 0;
+      if ( own_runner_ )
+        own_runner_->time_kernels = time_kernels_;
+    }
     else if ( k == "rng_type" )
     {
       if ( kv.second.string() != "mt19937_64" )
@@ -360,8 +390,25 @@ Kernel::get_kernel_status_json() const
     << ", \"max_delay\": " << max_delay_ * tb_.ms_per_step << ", \"min_delay_steps\": " << min_delay_
     << ", \"max_delay_steps\": " << max_delay_ << ", \"num_connections\": " << num_connections()
     << ", \"network_size\": " << nodes_.size() << ", \"biological_time\": " << ( clk + fr ) * tb_.ms_per_step
-    << ", \"clock_steps\": " << clk << ", \"from_step\": " << fr << ", \"local_spike_counter\": " << spike_count_
-    << ", \"time_simulate_kernel_ms\": " << last_kernel_ms_ << ", \"time_simulate_ms\": " << last_wall_ms_
+    << ", \"clock_steps\": " << clk << ", \"from_step\": " << fr << ", \"local_spike_counter\": " << local_spike_counter()
+    << ", \"time_simulate_kernel_ms\": " << last_kernel_ms_ << ", \"time_last_run_ms\": " << last_wall_ms_
+    << ", \"time_simulate\": " << time_simulate_ms_ * 1e-3
+    // delivery, update and spike gathering run fused in one slice kernel: its time is
+    // reported as time_update and included in time_deliver_spike_data's place
+    << ", \"time_update\": " << time_kernel_ms_[ K_SLICE ] * 1e-3
+    << ", \"time_gather_spike_data\": " << time_kernel_ms_[ K_FLATTEN ] * 1e-3
+    << ", \"time_deliver_spike_data\": 0, \"fused_deliver_update\": true"
+    << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? engine_->net().tb : 0 )
+    << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
+    << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
+    << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
+  static const char* kn[] = { "slice", "flatten" };
+  for ( int q = 0; q < N_KINDS; ++q )
+    o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << time_kernel_ms_[ q ];
+  o << "}, \"kernel_launches\": {";
+  for ( int q = 0; q < N_KINDS; ++q )
+    o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << kernel_launches_[ q ];
+  o << "}, \"synaptic_events\": " << synaptic_events()
     << ", \"device_bytes\": " << ( engine_ ? engine_->device_bytes() : 0 ) << "}";
   return o.str();
 }
@@ -755,8 +802,7 @@ Kernel::set_values( const std::vector< int64_t >& gids, const std::string& key, 
     if ( key == "V_m" and live )
     {
       const int dev = gid_to_dev_[ gids[ i ] ];
-      const uint32_t j = static_cast< uint32_t >( dev ) - engine_->net().model_begin[ p.model ];
-      engine_->set_state( p.model, 0, j, p.model == M_AEIF_COND_EXP ? v[ i ] : v[ i ] - p.E_L );
+      engine_->set_state( 0, static_cast< uint32_t >( dev ), p.model == M_AEIF_COND_EXP ? v[ i ] : v[ i ] - p.E_L );
       continue;
     }
     Json d;
@@ -776,8 +822,8 @@ void
 Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, double* out )
 {
   const bool live = engine_ and engine_->built() and simulated_;
-  std::vector< double > cache[ N_MODELS ][ 5 ];
-  std::vector< int32_t > rcache[ N_MODELS ];
+  std::vector< double > cache[ 5 ];
+  std::vector< int32_t > rcache;
   for ( size_t i = 0; i < gids.size(); ++i )
   {
     const NodeRec& r = node( gids[ i ] );
@@ -805,19 +851,18 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       int32_t rr;
       if ( live )
       {
-        const int dev = gid_to_dev_[ gids[ i ] ];
-        const uint32_t j = static_cast< uint32_t >( dev ) - engine_->net().model_begin[ p.model ];
+        const uint32_t dev = static_cast< uint32_t >( gid_to_dev_[ gids[ i ] ] );
         if ( field >= 0 )
         {
-          if ( cache[ p.model ][ field ].empty() )
-            engine_->get_state( p.model, field, cache[ p.model ][ field ] );
-          v = cache[ p.model ][ field ][ j ];
+          if ( cache[ field ].empty() )
+            engine_->get_state( field, cache[ field ] );
+          v = cache[ field ][ dev ];
         }
         else
         {
-          if ( rcache[ p.model ].empty() )
-            engine_->get_r( p.model, rcache[ p.model ] );
-          rr = rcache[ p.model ][ j ];
+          if ( rcache.empty() )
+            engine_->get_r( rcache );
+          rr = rcache[ dev ];
         }
       }
       else
@@ -852,6 +897,26 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       out[ i ] = p.tau_in;
     else if ( key == "global_id" )
       out[ i ] = static_cast< double >( gids[ i ] );
+    else if ( key == "V_min" and p.model == M_IAF_PSC_DELTA )
+      out[ i ] = p.V_min + p.E_L;
+    else if ( aeif and key == "V_peak" )
+      out[ i ] = p.V_peak;
+    else if ( aeif and key == "g_L" )
+      out[ i ] = p.g_L;
+    else if ( aeif and key == "E_ex" )
+      out[ i ] = p.E_ex;
+    else if ( aeif and key == "E_in" )
+      out[ i ] = p.E_in;
+    else if ( aeif and key == "Delta_T" )
+      out[ i ] = p.Delta_T;
+    else if ( aeif and key == "tau_w" )
+      out[ i ] = p.tau_w;
+    else if ( aeif and key == "a" )
+      out[ i ] = p.a;
+    else if ( aeif and key == "b" )
+      out[ i ] = p.b;
+    else if ( aeif and key == "gsl_error_tol" )
+      out[ i ] = p.tol;
     else
       throw std::runtime_error( "BadProperty: unknown property '" + key + "'" );
   }
@@ -1219,341 +1284,6 @@ Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int3
   }
 }
 
-// Propagator P32 of IAFPropagatorExp::evaluate (libnestutil/iaf_propagator.cpp:42-74)
-static double
-iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
-{
-  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
-  const double gamma = beta / c_m;
-  const double inv_tau_syn = 1 / tau_syn;
-  const double inv_tau_m = 1 / tau_m;
-  const double inv_c_m = 1 / c_m;
-  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
-  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
-  const double expm1_h_tau = std::expm1( h * inv_beta );
-  const double P32 = gamma * exp_h_tau_syn * expm1_h_tau;
-  if ( std::isnormal( P32 ) and P32 > 0 )
-    return P32;
-  const double exp_h_tau = std::exp( -h * inv_tau_m );
-  return h * inv_c_m * exp_h_tau;
-}
-
-void
-Kernel::build_spec( BuildSpec& b )
-{
-  const uint32_t N = static_cast< uint32_t >( params_.size() );
-  const double h = tb_.resolution_ms();
-  b.n_vp = n_vp_;
-  b.min_delay = static_cast< int >( min_delay_ );
-  b.max_delay = static_cast< int >( max_delay_ );
-  b.n = N;
-  // ---- device order: model-major; within a model NEST's receive order (VP, node id)
-  std::vector< uint32_t > order( N );  // neuron idx by (vp, gid)
-  for ( uint32_t i = 0; i < N; ++i )
-    order[ i ] = i;
-  std::stable_sort( order.begin(), order.end(),
-    [ this ]( uint32_t a, uint32_t c ) { return vp_of( neuron_gid_[ a ] ) < vp_of( neuron_gid_[ c ] ); } );
-  std::vector< uint32_t > ord_of( N );
-  for ( uint32_t o = 0; o < N; ++o )
-    ord_of[ order[ o ] ] = o;
-  b.vp_begin.assign( n_vp_ + 1, 0 );
-  for ( uint32_t o = 0; o < N; ++o )
-    b.vp_begin[ vp_of( neuron_gid_[ order[ o ] ] ) + 1 ]++;
-  for ( int v = 0; v < n_vp_; ++v )
-    b.vp_begin[ v + 1 ] += b.vp_begin[ v ];
-  uint32_t counts[ N_MODELS ] = { 0, 0, 0 };
-  for ( uint32_t i = 0; i < N; ++i )
-    counts[ params_[ i ].model ]++;
-  b.model_begin[ 0 ] = 0;
-  for ( int m = 0; m < N_MODELS; ++m )
-    b.model_begin[ m + 1 ] = b.model_begin[ m ] + counts[ m ];
-  std::vector< uint32_t > dev_of( N );
-  {
-    uint32_t cur[ N_MODELS ];
-    for ( int m = 0; m < N_MODELS; ++m )
-      cur[ m ] = b.model_begin[ m ];
-    for ( uint32_t o = 0; o < N; ++o )
-    {
-      const uint32_t i = order[ o ];
-      dev_of[ i ] = cur[ params_[ i ].model ]++;
-    }
-  }
-  bool identity = true;
-  b.dev2ord.assign( N, 0 );
-  for ( uint32_t i = 0; i < N; ++i )
-  {
-    b.dev2ord[ dev_of[ i ] ] = ord_of[ i ];
-    identity = identity and dev_of[ i ] == ord_of[ i ];
-  }
-  if ( identity )
-    b.dev2ord.clear();
-  gid_to_dev_.assign( nodes_.size() + 1, -1 );
-  dev_to_gid_.assign( N, 0 );
-  for ( uint32_t i = 0; i < N; ++i )
-  {
-    gid_to_dev_[ neuron_gid_[ i ] ] = static_cast< int32_t >( dev_of[ i ] );
-    dev_to_gid_[ dev_of[ i ] ] = neuron_gid_[ i ];
-  }
-  // ---- drive: one spike_generator per neuron at most
-  std::vector< int64_t > gen_of( N, -1 );
-  std::vector< double > drive_w( N, 0 );
-  std::vector< long > drive_d( N, 0 );
-  for ( const auto& c : dev_conns_ )
-  {
-    const uint32_t i = node( c.tgt ).idx;
-    if ( gen_of[ i ] >= 0 )
-      throw std::runtime_error( "NotImplemented: at most one spike_generator per neuron on the device path" );
-    gen_of[ i ] = c.gen;
-    drive_w[ i ] = c.w;
-    drive_d[ i ] = c.d;
-  }
-  // dense blocks: user blocks plus one block for generators given by spike_times
-  std::vector< DenseBlock > blocks = dense_;
-  {
-    int64_t tmax = 0;
-    bool any = false;
-    for ( const auto& G : gens_ )
-      if ( G.dense_block < 0 and G.poisson_lambda == 0 and not G.stamps.empty() )
-      {
-        any = true;
-        tmax = std::max( tmax, G.stamps.back() );
-      }
-    if ( any )
-    {
-      DenseBlock sb;
-      sb.t0 = 0;
-      sb.n_steps = tmax;
-      sb.cols = static_cast< int64_t >( gens_.size() );
-      sb.counts.assign( static_cast< size_t >( sb.n_steps * sb.cols ), 0 );
-      for ( size_t g = 0; g < gens_.size(); ++g )
-      {
-        const SpikeGen& G = gens_[ g ];
-        if ( G.dense_block >= 0 or G.poisson_lambda != 0 )
-          continue;
-        for ( size_t k = 0; k < G.stamps.size(); ++k )
-        {
-          if ( k > 0 and G.stamps[ k ] == G.stamps[ k - 1 ] )
-            throw std::runtime_error( "NotImplemented: repeated spike times in one spike_generator; use spike_multiplicities" );
-          const int64_t m = G.mults.empty() ? 1 : G.mults[ k ];
-          if ( m < 0 or m > 255 )
-            throw std::runtime_error( "NotImplemented: spike multiplicities above 255" );
-          sb.counts[ static_cast< size_t >( ( G.stamps[ k ] - 1 ) * sb.cols + g ) ] = static_cast< uint8_t >( m );
-        }
-      }
-      blocks.push_back( std::move( sb ) );
-    }
-  }
-  // merge all dense blocks into one matrix [step][col]
-  std::vector< int64_t > block_col0( blocks.size(), 0 );
-  int64_t t0 = 0, t1 = 0, cols = 0;
-  for ( size_t k = 0; k < blocks.size(); ++k )
-  {
-    if ( k == 0 )
-    {
-      t0 = blocks[ k ].t0;
-      t1 = blocks[ k ].t0 + blocks[ k ].n_steps;
-    }
-    t0 = std::min( t0, blocks[ k ].t0 );
-    t1 = std::max( t1, blocks[ k ].t0 + blocks[ k ].n_steps );
-    block_col0[ k ] = cols;
-    cols += blocks[ k ].cols;
-  }
-  if ( blocks.size() == 1 )
-  {
-    b.drive_counts = blocks[ 0 ].counts;
-    b.drive_cols = cols;
-  }
-  else if ( blocks.size() > 1 )
-  {
-    b.drive_cols = cols;
-    b.drive_counts.assign( static_cast< size_t >( ( t1 - t0 ) * cols ), 0 );
-    for ( size_t k = 0; k < blocks.size(); ++k )
-      for ( int64_t s = 0; s < blocks[ k ].n_steps; ++s )
-        std::memcpy( &b.drive_counts[ static_cast< size_t >( ( blocks[ k ].t0 - t0 + s ) * cols + block_col0[ k ] ) ],
-          &blocks[ k ].counts[ static_cast< size_t >( s * blocks[ k ].cols ) ], static_cast< size_t >( blocks[ k ].cols ) );
-  }
-  const size_t user_blocks = dense_.size();
-  // ---- parameter sets
-  std::map< std::string, uint16_t > pset_ids;
-  std::map< std::pair< double, uint64_t >, int32_t > tables;
-  b.drive_idx.assign( N, 0xffffffffu );
-  for ( int m = 0; m < N_MODELS; ++m )
-  {
-    const uint32_t nm = b.model_begin[ m + 1 ] - b.model_begin[ m ];
-    b.pidx[ m ].assign( nm, 0 );
-    b.r[ m ].assign( nm, 0 );
-    const int nf = m == M_IAF_PSC_DELTA ? 1 : ( m == M_IAF_PSC_EXP ? 3 : 5 );
-    for ( int f = 0; f < 5; ++f )
-      b.st[ m ][ f ].assign( f < nf ? nm : 0, 0.0 );
-  }
-  int max_scap = 1;
-  for ( uint32_t i = 0; i < N; ++i )
-  {
-    const NeuronParams& P = params_[ i ];
-    const NeuronState& S = state_[ i ];
-    ParamSet ps;
-    std::memset( &ps, 0, sizeof ps );
-    ps.model = P.model;
-    const long R = tb_.ms_to_steps( P.t_ref );  // Time( Time::ms( t_ref ) ).get_steps()
-    ps.refractory_counts = static_cast< int32_t >( R );
-    if ( P.model == M_IAF_PSC_DELTA )
-    {
-      // iaf_psc_delta::pre_run_hook (iaf_psc_delta.cpp:234-270)
-      const double P33 = std::exp( -h / P.tau_m );
-      const double P30 = 1 / P.c_m * ( 1 - P33 ) * P.tau_m;
-      ps.p[ 0 ] = P30;
-      ps.p[ 1 ] = P33;
-      ps.p[ 2 ] = 0.0 + P.I_e;
-      ps.p[ 3 ] = P.V_th;
-      ps.p[ 4 ] = P.V_reset;
-      ps.p[ 5 ] = P.V_min;
-    }
-    else if ( P.model == M_IAF_PSC_EXP )
-    {
-      // iaf_psc_exp::pre_run_hook (iaf_psc_exp.cpp:244-286)
-      const double P11ex = std::exp( -h / P.tau_ex );
-      const double P11in = std::exp( -h / P.tau_in );
-      const double P22 = std::exp( -h / P.tau_m );
-      const double P21ex = iaf_propagator_exp( P.tau_ex, P.tau_m, P.c_m, h );
-      const double P21in = iaf_propagator_exp( P.tau_in, P.tau_m, P.c_m, h );
-      const double P20 = P.tau_m / P.c_m * ( 1.0 - P22 );
-      ps.p[ 0 ] = P22;
-      ps.p[ 1 ] = P21ex;
-      ps.p[ 2 ] = P21in;
-      ps.p[ 3 ] = P20;
-      ps.p[ 4 ] = P11ex;
-      ps.p[ 5 ] = P11in;
-      ps.p[ 6 ] = P.I_e + 0.0;
-      ps.p[ 7 ] = P.V_th;
-      ps.p[ 8 ] = P.V_reset;
-      ps.p[ 9 ] = ( 1. - P11ex ) * 0.0;
-    }
-    else
-    {
-      // aeif_cond_exp::pre_run_hook (aeif_cond_exp.cpp:420-435)
-      const double vals[] = { P.V_peak, P.V_reset, P.g_L, P.c_m, P.E_ex, P.E_in, P.E_L, P.Delta_T, P.tau_w, P.a, P.b,
-        P.V_th, P.tau_ex, P.tau_in, P.I_e, P.tol, P.Delta_T > 0. ? P.V_peak : P.V_th, h };
-      for ( int q = 0; q < 18; ++q )
-        ps.p[ q ] = vals[ q ];
-    }
-    max_scap = std::max( max_scap, static_cast< int >( ( min_delay_ + R ) / ( R + 1 ) ) );
-    // drive
-    if ( gen_of[ i ] >= 0 )
-    {
-      const SpikeGen& G = gens_[ node( gen_of[ i ] ).idx ];
-      ps.drive_w = drive_w[ i ];
-      ps.drive_d = static_cast< int32_t >( drive_d[ i ] );
-      if ( G.poisson_lambda > 0 )
-      {
-        ps.drive_mode = 2;
-        auto key = std::make_pair( G.poisson_lambda, G.poisson_seed );
-        auto it = tables.find( key );
-        if ( it == tables.end() )
-        {
-          it = tables.emplace( key, static_cast< int32_t >( b.ptables.size() ) ).first;
-          b.ptables.push_back( make_poisson_table( G.poisson_lambda, G.poisson_seed ) );
-        }
-        ps.drive_table = it->second;
-        ps.drive_t0 = G.poisson_t0;
-        ps.drive_steps = G.poisson_steps;
-        b.drive_idx[ dev_of[ i ] ] = static_cast< uint32_t >( gen_of[ i ] );
-      }
-      else if ( G.dense_block >= 0 or not G.stamps.empty() )
-      {
-        ps.drive_mode = 1;
-        ps.drive_t0 = blocks.size() > 1 ? t0 : blocks[ 0 ].t0;
-        ps.drive_steps = blocks.size() > 1 ? t1 - t0 : blocks[ 0 ].n_steps;
-        int64_t col;
-        if ( G.dense_block >= 0 )
-          col = block_col0[ G.dense_block ] + G.dense_col;
-        else
-          col = block_col0[ user_blocks ] + node( gen_of[ i ] ).idx;
-        b.drive_idx[ dev_of[ i ] ] = static_cast< uint32_t >( col );
-      }
-    }
-    std::string key( reinterpret_cast< const char* >( &ps ), sizeof ps );
-    auto it = pset_ids.find( key );
-    if ( it == pset_ids.end() )
-    {
-      if ( b.psets.size() >= 65535 )
-        throw std::runtime_error( "NotImplemented: more than 65535 distinct parameter sets" );
-      it = pset_ids.emplace( key, static_cast< uint16_t >( b.psets.size() ) ).first;
-      b.psets.push_back( ps );
-    }
-    const int m = P.model;
-    const uint32_t j = dev_of[ i ] - b.model_begin[ m ];
-    b.pidx[ m ][ j ] = it->second;
-    b.r[ m ][ j ] = S.r;
-    const double sv[] = { S.v0, S.v1, S.v2, S.v3, m == M_AEIF_COND_EXP ? h : 0.0 };
-    for ( int f = 0; f < 5; ++f )
-      if ( not b.st[ m ][ f ].empty() )
-        b.st[ m ][ f ][ j ] = sv[ f ];
-  }
-  b.s_cap = std::min( max_scap, 4 );
-  // ---- out-CSR by source device index: LSD passes, delay first, then source; both stable,
-  //      so each row is delay-sorted with creation order kept within a delay
-  int64_t E = 0;
-  for ( const auto& C : conns_ )
-    E += static_cast< int64_t >( C.size() );
-  const long D = max_delay_ - min_delay_ + 1;
-  int dbits = 0;
-  while ( ( 1l << dbits ) < D )
-    ++dbits;
-  b.tgt_bits = 32 - dbits;
-  if ( dbits > 0 and N >= ( 1u << b.tgt_bits ) )
-    throw std::runtime_error( "NotImplemented: too many neurons for the delay range in one instance" );
-  std::vector< uint32_t > s_dev( E ), word( E );
-  std::vector< double > wv( E );
-  std::vector< uint32_t > dkey( E );
-  {
-    int64_t k = 0;
-    for ( const auto& C : conns_ )
-      for ( size_t i = 0; i < C.size(); ++i, ++k )
-      {
-        s_dev[ k ] = dev_of[ node( C.src[ i ] ).idx ];
-        const uint32_t t = dev_of[ node( C.tgt[ i ] ).idx ];
-        const uint32_t doff = static_cast< uint32_t >( C.d[ i ] - min_delay_ );
-        dkey[ k ] = doff;
-        word[ k ] = dbits > 0 ? ( t | ( doff << b.tgt_bits ) ) : t;
-        wv[ k ] = C.w[ i ];
-      }
-  }
-  std::vector< int64_t > perm1( E );
-  {
-    std::vector< int64_t > cnt( D + 1, 0 );
-    for ( int64_t k = 0; k < E; ++k )
-      cnt[ dkey[ k ] + 1 ]++;
-    for ( long q = 0; q < D; ++q )
-      cnt[ q + 1 ] += cnt[ q ];
-    for ( int64_t k = 0; k < E; ++k )
-      perm1[ cnt[ dkey[ k ] ]++ ] = k;
-  }
-  b.row_ptr.assign( N + 1, 0 );
-  for ( int64_t k = 0; k < E; ++k )
-    b.row_ptr[ s_dev[ k ] + 1 ]++;
-  for ( uint32_t q = 0; q < N; ++q )
-    b.row_ptr[ q + 1 ] += b.row_ptr[ q ];
-  b.tgt_d.resize( E );
-  b.w.resize( E );
-  {
-    std::vector< uint64_t > cur( b.row_ptr.begin(), b.row_ptr.end() - 1 );
-    for ( int64_t q = 0; q < E; ++q )
-    {
-      const int64_t k = perm1[ q ];
-      const uint64_t pos = cur[ s_dev[ k ] ]++;
-      b.tgt_d[ pos ] = word[ k ];
-      b.w[ pos ] = wv[ k ];
-    }
-  }
-  b.in_ptr.assign( N + 1, 0 );
-  const uint32_t tmask = b.tgt_bits >= 32 ? 0xffffffffu : ( ( 1u << b.tgt_bits ) - 1u );
-  for ( int64_t q = 0; q < E; ++q )
-    b.in_ptr[ ( b.tgt_d[ q ] & tmask ) + 1 ]++;
-  for ( uint32_t q = 0; q < N; ++q )
-    b.in_ptr[ q + 1 ] += b.in_ptr[ q ];
-  b.log_cap = std::max< uint64_t >( 1u << 20, static_cast< uint64_t >( N ) * 16 );
-}
-
 void
 Kernel::prepare()
 {
@@ -1617,11 +1347,25 @@ Kernel::run( double t_ms )
       own_runner_ = std::make_unique< Runner >( device_, slices_per_chunk_ );
     runner_ = own_runner_.get();
   }
+  runner_->time_kernels = time_kernels_;
+  double km0[ N_KINDS ];
+  int64_t kl0[ N_KINDS ];
+  for ( int q = 0; q < N_KINDS; ++q )
+  {
+    km0[ q ] = runner_->kernel_ms[ q ];
+    kl0[ q ] = runner_->kernel_launches[ q ];
+  }
   const auto t0 = std::chrono::steady_clock::now();
   runner_->run( { engine_.get() }, { steps } );
   const auto t1 = std::chrono::steady_clock::now();
   last_wall_ms_ = std::chrono::duration< double, std::milli >( t1 - t0 ).count();
+  time_simulate_ms_ += last_wall_ms_;
   last_kernel_ms_ = runner_->last_kernel_ms();
+  for ( int q = 0; q < N_KINDS; ++q )
+  {
+    time_kernel_ms_[ q ] += runner_->kernel_ms[ q ] - km0[ q ];
+    kernel_launches_[ q ] += runner_->kernel_launches[ q ] - kl0[ q ];
+  }
   collect_log();
 }
 
@@ -1637,6 +1381,14 @@ Kernel::simulate( double t_ms )
   prepare();
   run( t_ms );
   cleanup();
+}
+
+uint64_t
+Kernel::synaptic_events() const
+{
+  if ( not engine_ or not engine_->built() )
+    return 0;
+  return engine_->read_counter( 1 );
 }
 
 void

@@ -255,9 +255,10 @@ public:
   {
     return last_wall_ms_;
   }
+  uint64_t synaptic_events() const;
   uint64_t local_spike_counter() const
   {
-    return spike_count_;
+    return engine_ and engine_->built() ? engine_->read_counter( 2 ) : 0;
   }
   uint64_t num_neurons() const
   {
@@ -286,6 +287,7 @@ private:
   void check_delay( long steps );
   void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
   void build_spec( BuildSpec& b );
+  void choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const;
   void collect_log();
 
   int device_;
@@ -323,6 +325,12 @@ private:
   uint64_t spike_count_ = 0;
   double last_kernel_ms_ = 0, last_wall_ms_ = 0;
   int slices_per_chunk_ = 64;
+  bool time_kernels_ = false;
+  double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
+  int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
+  int targets_per_block_ = 0;  // 0: chosen from the network size
+  int delivery_mode_ = 0;      // 0 automatic, 1 window (LDS ring-buffer slots), 2 global
+  double time_simulate_ms_ = 0;
 };
 
 // Seeded Poisson counts (the pre-generated drive) for streams [stream0, stream0+n) and

@@ -1316,6 +1316,7 @@ struct Kernel
       const int tid = 0;
 #endif
       const Part& p = parts[ tid ];
+      long delivered = 0;
       for ( const auto& sp : recv )
       {
         const long s = sp.first;
@@ -1324,9 +1325,13 @@ struct Kernel
         const long base = clock + lag - min_delay;
         for ( long k = p.ptr[ s ]; k < p.ptr[ s + 1 ]; ++k )
           handle( neurons[ p.tgt[ k ] ], base + p.d[ k ], p.w[ k ], 1.0 );
+        delivered += p.ptr[ s + 1 ] - p.ptr[ s ];
       }
+#pragma omp atomic
+      events_delivered += delivered;
     }
   }
+  long events_delivered = 0;
 
   double
   get_state( long gid, const std::string& f )
@@ -1540,6 +1545,12 @@ or_get_spikes( void* k, long* gid, long* step )
     gid[ i ] = K->spikes[ i ].first;
     step[ i ] = K->spikes[ i ].second;
   } )
+}
+
+long
+or_synaptic_events( void* k )
+{
+  return static_cast< Kernel* >( k )->events_delivered;
 }
 
 int

@@ -39,6 +39,7 @@ def lib():
         L.or_create.restype = ctypes.c_long
         L.or_num_connections.restype = ctypes.c_long
         L.or_num_spikes.restype = ctypes.c_long
+        L.or_synaptic_events.restype = ctypes.c_long
         L.or_fd_exp.restype = ctypes.c_double
         L.or_fd_exp.argtypes = [ctypes.c_double]
         L.or_fd_log.restype = ctypes.c_double
@@ -191,6 +192,9 @@ class Oracle:
         s = np.zeros(n, np.int64)
         self._chk(self.L.or_get_spikes(ctypes.c_void_p(self.k), _p(g, ctypes.c_long), _p(s, ctypes.c_long)))
         return g, s
+
+    def synaptic_events(self):
+        return int(self.L.or_synaptic_events(ctypes.c_void_p(self.k)))
 
     def status(self):
         out = np.zeros(5, np.int64)

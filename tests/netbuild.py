@@ -40,9 +40,11 @@ def _ids(pops_ids, sl):
     return pops_ids[p][a:b]
 
 
-def build_gpu(desc, device=0, kernel=None):
+def build_gpu(desc, device=0, kernel=None, layout=None):
+    """layout: extra kernel status for the device layout (delivery_mode, targets_per_block)."""
     k = kernel or nestgpu.Kernel(device)
     st = {"total_num_virtual_procs": desc.get("n_vp", 1), "rng_seed": desc.get("seed", 143202461)}
+    st.update(layout or {})
     if "resolution" in desc:
         st["resolution"] = desc["resolution"]
     k.SetKernelStatus(st)

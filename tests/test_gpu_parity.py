@@ -12,9 +12,15 @@ import netbuild as nb
 pytestmark = pytest.mark.gpu
 
 
-def _compare(desc, T=None, steps=None, check_vm=True, portable_math=False, fields=("V_m",)):
+# device layouts: automatic choice, LDS window with many small blocks, global ring buffer
+LAYOUTS = {"auto": {}, "win64": {"delivery_mode": "window", "targets_per_block": 64},
+           "glob32": {"delivery_mode": "global", "targets_per_block": 32}}
+layouts = pytest.mark.parametrize("layout", list(LAYOUTS))
+
+
+def _compare(desc, T=None, steps=None, check_vm=True, portable_math=False, fields=("V_m",), layout="auto"):
     T = desc["T"] if T is None else T
-    k, pops, sr = nb.build_gpu(desc)
+    k, pops, sr = nb.build_gpu(desc, layout=LAYOUTS[layout])
     o, opops = nb.build_oracle(desc, portable_math=portable_math)
     if steps is None:
         k.Simulate(T)
@@ -23,6 +29,8 @@ def _compare(desc, T=None, steps=None, check_vm=True, portable_math=False, field
         for t in steps:
             k.Simulate(t)
             o.simulate(t)
+    print("layout=%s window=%s tb=%d" % (layout, k.GetKernelStatus("window_delivery"),
+                                         k.GetKernelStatus("targets_per_block")), flush=True)
     ev = sr.get("events")
     og, os_ = nb.sorted_spikes_oracle(o)
     res = desc.get("resolution", 0.1)
@@ -39,34 +47,41 @@ def _compare(desc, T=None, steps=None, check_vm=True, portable_math=False, field
     return k, o, ev
 
 
-def test_brunel_delta_small():
-    _compare(nb.brunel_delta(N=500, K=50, T=200.0))
+@layouts
+def test_brunel_delta_small(layout):
+    _compare(nb.brunel_delta(N=500, K=50, T=200.0), layout=layout)
 
 
-def test_brunel_delta_multi_vp():
-    _compare(nb.brunel_delta(N=600, K=60, T=150.0, n_vp=4, seed=77))
+@layouts
+def test_brunel_delta_multi_vp(layout):
+    _compare(nb.brunel_delta(N=600, K=60, T=150.0, n_vp=4, seed=77), layout=layout)
 
 
-def test_exp_random_small():
-    _compare(nb.exp_random(N=2000, KE=100, KI=25, T=200.0), fields=("V_m", "I_syn_ex", "I_syn_in"))
+@layouts
+def test_exp_random_small(layout):
+    _compare(nb.exp_random(N=2000, KE=100, KI=25, T=200.0), fields=("V_m", "I_syn_ex", "I_syn_in"), layout=layout)
 
 
-def test_exp_random_multi_vp():
-    _compare(nb.exp_random(N=1500, KE=80, KI=20, T=100.0, n_vp=3, seed=5), fields=("V_m", "I_syn_ex", "I_syn_in"))
+@layouts
+def test_exp_random_multi_vp(layout):
+    _compare(nb.exp_random(N=1500, KE=80, KI=20, T=100.0, n_vp=3, seed=5), fields=("V_m", "I_syn_ex", "I_syn_in"),
+             layout=layout)
 
 
-def test_split_simulation_partial_slices():
+@layouts
+def test_split_simulation_partial_slices(layout):
     # T not a multiple of min_delay, split into calls that end mid-slice, several VPs:
     # receive order then depends on the run parts (test_split_simulation / stepped simulation)
     desc = nb.brunel_delta(N=400, K=40, T=100.0, n_vp=3, seed=9)
-    _compare(desc, steps=[10.3, 0.4, 21.1, 30.0, 1.2, 37.0])
+    _compare(desc, steps=[10.3, 0.4, 21.1, 30.0, 1.2, 37.0], layout=layout)
 
 
-def test_split_equals_single_run():
+@layouts
+def test_split_equals_single_run(layout):
     desc = nb.exp_random(N=800, KE=60, KI=15, T=60.0, seed=21)
-    k1, pops1, sr1 = nb.build_gpu(desc)
+    k1, pops1, sr1 = nb.build_gpu(desc, layout=LAYOUTS[layout])
     k1.Simulate(60.0)
-    k2, pops2, sr2 = nb.build_gpu(desc)
+    k2, pops2, sr2 = nb.build_gpu(desc, layout=LAYOUTS[layout])
     for t in (7.3, 12.7, 40.0):
         k2.Simulate(t)
     e1, e2 = sr1.get("events"), sr2.get("events")
@@ -75,7 +90,8 @@ def test_split_equals_single_run():
     assert np.array_equal(pops1[0].get("V_m"), pops2[0].get("V_m"))
 
 
-def test_bernoulli_min_delay_one_step():
+@pytest.mark.parametrize("layout", ["auto", "glob32", "win64"])
+def test_bernoulli_min_delay_one_step(layout):
     # C3-like: pairwise_bernoulli, normal delays clipped at dt -> min_delay = 1 step
     params = {"tau_m": 10.0, "tau_syn_ex": 0.5, "tau_syn_in": 0.5, "C_m": 250.0, "V_th": -55.0,
               "V_reset": -70.0, "t_ref": 2.0, "E_L": -70.0}
@@ -98,6 +114,6 @@ def test_bernoulli_min_delay_one_step():
         drive=[(0, rng.poisson(1.6, size=(steps, N1)).astype(np.uint8), 87.8, 0.1),
                (1, rng.poisson(1.5, size=(steps, N2)).astype(np.uint8), 87.8, 0.1)],
         T=T)
-    k, o, ev = _compare(desc, fields=("V_m", "I_syn_ex", "I_syn_in"))
+    k, o, ev = _compare(desc, fields=("V_m", "I_syn_ex", "I_syn_in"), layout=layout)
     assert k.GetKernelStatus("min_delay_steps") == 1
     assert len(ev["senders"]) > 0

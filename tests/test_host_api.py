@@ -1,0 +1,169 @@
+"""Host side of the device path, no GPU: the C ABI exports, Create/Connect/SetStatus
+semantics and error behaviour of the kernel, and connectivity identical to the oracle's.
+
+Nothing here calls Prepare/Run (those touch the device).
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import netbuild as nb
+import nestgpu
+from _oracle import ROOT, Oracle
+
+HEADER = os.path.join(ROOT, "include", "nestgpu.h")
+
+
+def _declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(nestgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared_symbols()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(nestgpu.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def _conns_gpu(k):
+    c = k.GetConnections()
+    return c["source"], c["target"], c["weight"], c["delay_steps"]
+
+
+CASES = [
+    # (n_vp, seed, pre_n, post_n, rule, param, weight, delay, autapses, multapses, same_pop)
+    (1, 1, 50, 40, "fixed_indegree", 7, ("normal", 1.0, 0.3), ("uniform", 0.5, 2.0), True, True, False),
+    (3, 11, 60, 60, "fixed_indegree", 9, ("sum", 2.0, ("normal", 0.0, 1.0)), 1.5, False, False, True),
+    (4, 5, 30, 45, "pairwise_bernoulli", 0.2, ("max", ("normal", 0.5, 1.0), 0.0), ("redraw", ("normal", 2.0, 1.0), 0.1, 4.0), True, True, False),
+    (2, 77, 25, 25, "pairwise_bernoulli", 0.3, 1.0, 1.0, False, True, True),
+    (5, 3, 20, 20, "one_to_one", 0, ("uniform", -1.0, 1.0), ("sum", 1.0, ("product", ("uniform_int", 21), 0.1)), True, True, False),
+    (3, 8, 12, 17, "all_to_all", 0, ("normal", -2.0, 0.5), 2.0, True, True, False),
+    (2, 9, 15, 15, "all_to_all", 0, 0.5, 1.0, False, True, True),
+    (6, 123, 40, 30, "fixed_indegree", 25, ("min", ("normal", 0.0, 2.0), 1.0), ("uniform", 0.1, 3.0), True, False, False),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[4] + "-vp%d" % c[0] for c in CASES])
+def test_connectivity_identical_to_oracle(case):
+    n_vp, seed, npre, npost, rule, rp, w, d, aut, mult, same = case
+    pops = [("iaf_psc_delta", npre, {})] if same else [("iaf_psc_delta", npre, {}), ("iaf_psc_exp", npost, {})]
+    post = (0, 0, npre) if same else (1, 0, npost)
+    desc = dict(n_vp=n_vp, seed=seed, pops=pops, conns=[((0, 0, npre), post, rule, rp, w, d, aut, mult)])
+    k, kp, sr = nb.build_gpu(desc)
+    o, op = nb.build_oracle(desc)
+    s1, t1, w1, d1 = _conns_gpu(k)
+    # drop the recorder connections (devices carry no synapses in the oracle)
+    keep = t1 != sr.global_id
+    s1, t1, w1, d1 = s1[keep], t1[keep], w1[keep], d1[keep]
+    s2, t2, w2, d2 = o.connections()
+    assert len(s1) == len(s2)
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+    assert np.array_equal(w1, w2)
+    assert np.array_equal(d1, d2)
+    if not aut:
+        assert not np.any(s1 == t1)
+    if not mult:
+        pairs = s1 * 100000 + t1
+        assert len(np.unique(pairs)) == len(pairs)
+
+
+def test_connectivity_independent_of_call_granularity_per_vp():
+    # the per-VP streams are consumed in connect order: two Connect calls equal one oracle build
+    desc = dict(n_vp=3, seed=4, pops=[("iaf_psc_delta", 30, {})],
+                conns=[((0, 0, 20), (0, 0, 30), "fixed_indegree", 3, ("normal", 1.0, 1.0), 1.0),
+                       ((0, 20, 30), (0, 0, 30), "pairwise_bernoulli", 0.4, -2.0, ("uniform", 1.0, 2.0))])
+    k, kp, sr = nb.build_gpu(desc)
+    o, op = nb.build_oracle(desc)
+    s1, t1, w1, d1 = _conns_gpu(k)
+    keep = t1 != sr.global_id
+    s2, t2, w2, d2 = o.connections()
+    assert np.array_equal(s1[keep], s2) and np.array_equal(w1[keep], w2) and np.array_equal(d1[keep], d2)
+
+
+def test_get_set_status_roundtrip_and_relative_voltages():
+    k = nestgpu.Kernel(0)
+    n = k.Create("iaf_psc_delta", 3, {"E_L": -65.0, "V_th": -50.0})
+    # absolute V_m is kept when E_L changes without V_m (models/iaf_psc_delta.cpp:177-187)
+    assert np.all(n.get("V_m") == -70.0)
+    n.set(V_m=np.array([-60.0, -61.0, -62.0]))
+    np.testing.assert_array_equal(n.get("V_m"), [-60.0, -61.0, -62.0])
+    # voltages are stored relative to E_L but an E_L change alone keeps their absolute
+    # values (models/iaf_psc_delta.cpp:109-142)
+    np.testing.assert_array_equal(n.get("V_th"), -50.0)
+    n.set({"E_L": -75.0})
+    np.testing.assert_allclose(n.get("V_th"), -50.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(n.get("V_reset"), -70.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(n.get("V_m"), [-60.0, -61.0, -62.0], rtol=0, atol=1e-12)
+    n.set({"E_L": -60.0, "V_th": -40.0})
+    np.testing.assert_allclose(n.get("V_th"), -40.0, rtol=0, atol=1e-12)
+    e = k.Create("aeif_cond_exp", 2, {"a": 2.0})
+    np.testing.assert_array_equal(e.get("a"), [2.0, 2.0])
+    np.testing.assert_array_equal(e.get("w"), [0.0, 0.0])
+
+
+@pytest.mark.parametrize("bad, msg", [
+    (lambda k: k.Create("no_such_model", 1), "UnknownModelName"),
+    (lambda k: k.Create("iaf_psc_delta", 1, {"no_such_param": 1.0}), "UnaccessedDictionaryEntry"),
+    (lambda k: k.Create("iaf_psc_delta", 1, {"C_m": -1.0}), "Capacitance"),
+    (lambda k: k.Create("iaf_psc_exp", 1, {"V_reset": -40.0, "V_th": -50.0}), "Reset potential"),
+    (lambda k: k.Create("aeif_cond_exp", 1, {"V_peak": -60.0}), "V_peak"),
+    (lambda k: k.SetKernelStatus({"rng_seed": 0}), "RNG seed"),
+    (lambda k: k.SetKernelStatus({"no_such_key": 1}), "UnaccessedDictionaryEntry"),
+])
+def test_error_paths(bad, msg):
+    k = nestgpu.Kernel(0)
+    with pytest.raises(nestgpu.NESTError, match=msg):
+        bad(k)
+
+
+def test_bad_delay_rejected():
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_delta", 2)
+    with pytest.raises(nestgpu.NESTError, match="BadDelay"):
+        k.Connect(a, a, "all_to_all", {"delay": 0.04})  # rounds to 0 steps at 0.1 ms
+
+
+def test_resolution_fixed_once_nodes_exist():
+    k = nestgpu.Kernel(0)
+    k.Create("iaf_psc_delta", 1)
+    with pytest.raises(nestgpu.NESTError, match="resolution"):
+        k.SetKernelStatus({"resolution": 0.01})
+
+
+def test_reset_kernel_clears_network():
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_delta", 5)
+    k.Connect(a, a, "all_to_all")
+    assert len(k.GetConnections()["source"]) == 25
+    k.ResetKernel()
+    assert len(k.GetConnections()["source"]) == 0
+    b = k.Create("iaf_psc_exp", 2)
+    assert b.tolist() == [1, 2]
+
+
+def test_poisson_counts_statistics_and_determinism():
+    lam = 0.8
+    c1 = nestgpu.poisson_counts(11, lam, 100, 500, 0, 400)
+    c2 = nestgpu.poisson_counts(11, lam, 100, 500, 0, 400)
+    assert c1.shape == (400, 500)
+    assert np.array_equal(c1, c2)
+    # stream/step addressing: a sub-window equals the same window of the full matrix
+    sub = nestgpu.poisson_counts(11, lam, 250, 50, 120, 30)
+    assert np.array_equal(sub, c1[120:150, 150:200])
+    m, v = c1.mean(), c1.var()
+    se = np.sqrt(lam / c1.size)
+    assert abs(m - lam) < 5 * se
+    assert abs(v - lam) < 0.05 * lam
+    # frequency of k spikes follows the Poisson pmf
+    from scipy import stats
+    ks = np.arange(6)
+    obs = np.array([(c1 == x).sum() for x in ks])
+    exp = stats.poisson(lam).pmf(ks) * c1.size
+    assert np.all(np.abs(obs - exp) < 6 * np.sqrt(exp) + 5)
+    assert not np.array_equal(c1, nestgpu.poisson_counts(12, lam, 100, 500, 0, 400))
