@@ -21,8 +21,9 @@
 // Determinism: NEST adds into one ring-buffer cell in the order (delivery batch, receive
 // order of the spike, position in the connector run). A workgroup takes events in that key
 // order in rounds of R; each round is bucketed by target (LDS counters give every event a
-// slot in its target's bucket), and the thread owning a target sorts its bucket by key and
-// applies it in that order. No result depends on thread timing or atomics order.
+// place in its target's bucket), every event finds its rank in the bucket ordered by
+// (cell, key), and the first event of each cell's run adds the run in key order. No result
+// depends on thread timing or atomics order.
 // Built with -ffp-contract=off: every multiply and add rounds as in the reference.
 
 #include <hip/hip_runtime.h>
@@ -39,10 +40,11 @@ constexpr int SPT = 4;                            // spikes per thread per chunk
 constexpr int CH = SPT * BS;                      // spikes per chunk
 constexpr int RQ = 4;                             // events per thread per round
 constexpr int R = RQ * BS;                        // events per round
-constexpr int IDX_SHIFT = 21;                     // bucket key = round index << 21 | slot-channel
-constexpr uint32_t SC_MASK = ( 1u << IDX_SHIFT ) - 1u;
+constexpr int IDX_BITS = 11;  // bucket key = slot-channel << 11 | round index: sorted keys group
+                              // a target's events by cell, in key order within each cell
 constexpr uint32_t NONE = 0xffffffffu;
-static_assert( R <= ( 1 << ( 32 - IDX_SHIFT ) ), "round index must fit the bucket key" );
+constexpr int LB = 8;  // lags of ring-buffer input loaded together in the update
+static_assert( R <= ( 1 << IDX_BITS ), "round index must fit the bucket key" );
 
 template < bool WIN >
 struct SliceLds
@@ -53,7 +55,7 @@ struct SliceLds
   uint32_t pref[ ( WIN ? PMAX_WIN : PMAX_GLOB ) + 1 ];  // prefix of per-block spike counts
   uint32_t c_g0[ CH ];       // first synapse of each chunk spike's run
   uint32_t c_off[ CH + 1 ];  // exclusive prefix of run lengths
-  uint32_t bk[ R ];          // event keys (round index << 21 | slot-channel), bucketed by target
+  uint32_t bk[ R ];          // event keys, then slot-channel of each event, bucketed by target
   uint32_t bcnt[ BS ];       // events per target this round
   uint32_t bst[ BS + 1 ];    // bucket starts
   uint32_t sh[ BS / 64 ];
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__( BS )
             }
             loc[ q ] = local;
             val[ q ] = v;
-            key[ q ] = ( static_cast< uint32_t >( t * RQ + q ) << IDX_SHIFT ) | ( slot_rel * d.nch + ch );
+            key[ q ] = ( ( slot_rel * d.nch + ch ) << IDX_BITS ) | static_cast< uint32_t >( t * RQ + q );
             dst[ q ] = atomicAdd( &S.bcnt[ local ], 1u );  // an unordered place in the bucket
           }
         }
@@ -421,7 +423,7 @@ __global__ void __launch_bounds__( BS )
         NG_PHASE( 2 );
         const uint32_t mycnt = t < tb ? S.bcnt[ t ] : 0u;
         uint32_t mybase;
-        const uint32_t tot = block_scan( mycnt, &mybase, S.sh );
+        const uint32_t tot = block_scan( mycnt, &mybase, S.sh );  // bucket starts
         if ( t < tb )
           S.bst[ t ] = mybase;
         if ( t == 0 )
@@ -448,39 +450,74 @@ __global__ void __launch_bounds__( BS )
         for ( int q = 0; q < RQ; ++q )
           if ( loc[ q ] != NONE )
           {
-            S.bk[ dst[ q ] ] = key[ q ] & SC_MASK;
+            S.bk[ dst[ q ] ] = key[ q ] >> IDX_BITS;
             S.bw[ dst[ q ] ] = val[ q ];
           }
         __syncthreads();
         if ( e0 + R < E )
           load_round( e0 + R );
-        if ( t < tb and mycnt )
+        // the first event of each run of one cell in a sorted bucket adds the whole run in
+        // key order; runs of different cells proceed in parallel (independent loads first)
+        uint32_t ci[ RQ ];
+        bool lead[ RQ ];
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
         {
-          // owner of target t adds its bucket into the cells in key order; consecutive
-          // events of one cell accumulate in a register
-          const uint32_t* kk = S.bk + mybase;
-          const double* ww = S.bw + mybase;
-          uint32_t a = 0;
-          if ( WIN and C.win )
-            while ( a < mycnt )
+          lead[ q ] = false;
+          if ( loc[ q ] != NONE )
+          {
+            const uint32_t sc = key[ q ] >> IDX_BITS;
+            lead[ q ] = dst[ q ] == S.bst[ loc[ q ] ] or S.bk[ dst[ q ] - 1 ] != sc;
+            if ( WIN and C.win )
+              ci[ q ] = sc * tb + loc[ q ];
+            else
             {
-              const uint32_t sc = kk[ a ];
-              double v = S.win[ sc * tb + t ];
-              for ( ; a < mycnt and kk[ a ] == sc; ++a )
-                v = v + ww[ a ];
-              S.win[ sc * tb + t ] = v;
+              uint32_t slot = C.s0 + ( d.nch == 2 ? ( sc >> 1 ) : sc );
+              if ( slot >= C.L )
+                slot -= C.L;
+              ci[ q ] = slot * C.nct + ( d.nch == 2 ? ( sc & 1u ) : 0u ) * tb + loc[ q ];
             }
-          else
-            while ( a < mycnt )
-            {
-              const uint32_t sc = kk[ a ];
-              const uint32_t ch = d.nch == 2 ? ( sc & 1u ) : 0u;
-              double* cell = C.at( d.nch == 2 ? ( sc >> 1 ) : sc, static_cast< int >( ch ), t );
-              double v = *cell;
-              for ( ; a < mycnt and kk[ a ] == sc; ++a )
-                v = v + ww[ a ];
-              *cell = v;
-            }
+          }
+        }
+        double cv[ RQ ];
+        if ( WIN and C.win )
+        {
+#pragma unroll
+          for ( int q = 0; q < RQ; ++q )
+            if ( lead[ q ] )
+              cv[ q ] = S.win[ ci[ q ] ];
+        }
+        else
+        {
+#pragma unroll
+          for ( int q = 0; q < RQ; ++q )
+            if ( lead[ q ] )
+              cv[ q ] = C.rbb[ ci[ q ] ];
+        }
+#pragma unroll
+        for ( int q = 0; q < RQ; ++q )
+          if ( lead[ q ] )
+          {
+            const uint32_t sc = key[ q ] >> IDX_BITS;
+            const uint32_t hi = S.bst[ loc[ q ] + 1 ];
+            double v = cv[ q ];
+            for ( uint32_t i = dst[ q ]; i < hi and S.bk[ i ] == sc; ++i )
+              v = v + S.bw[ i ];
+            cv[ q ] = v;
+          }
+        if ( WIN and C.win )
+        {
+#pragma unroll
+          for ( int q = 0; q < RQ; ++q )
+            if ( lead[ q ] )
+              S.win[ ci[ q ] ] = cv[ q ];
+        }
+        else
+        {
+#pragma unroll
+          for ( int q = 0; q < RQ; ++q )
+            if ( lead[ q ] )
+              C.rbb[ ci[ q ] ] = cv[ q ];
         }
         NG_PHASE( 3 );
         if ( t < tb )
@@ -512,24 +549,35 @@ __global__ void __launch_bounds__( BS )
                    V_min = ps.p[ 5 ];
       double y3 = d.v[ 0 ][ dev ];
       int32_t r = d.r[ dev ];
-      for ( int lag = ph.from; lag < ph.to; ++lag )
+      // ring-buffer cells are read LB lags at a time (independent loads), then consumed in
+      // lag order and cleared
+      for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
       {
-        double* cell = C.at( lag, 0, local );
-        const double in = *cell;
-        *cell = 0.0;
-        if ( r == 0 )
-        {
-          y3 = P30 * y0I + P33 * y3 + in;
-          y3 = ( y3 < V_min ? V_min : y3 );
-        }
-        else
-          --r;
-        if ( y3 >= V_th )
-        {
-          r = ps.refractory_counts;
-          y3 = V_reset;
-          bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
-        }
+        double win_[ LB ];
+#pragma unroll
+        for ( int q = 0; q < LB; ++q )
+          if ( l0 + q < ph.to )
+            win_[ q ] = *C.at( l0 + q, 0, local );
+#pragma unroll
+        for ( int q = 0; q < LB; ++q )
+          if ( l0 + q < ph.to )
+          {
+            const int lag = l0 + q;
+            *C.at( lag, 0, local ) = 0.0;
+            if ( r == 0 )
+            {
+              y3 = P30 * y0I + P33 * y3 + win_[ q ];
+              y3 = ( y3 < V_min ? V_min : y3 );
+            }
+            else
+              --r;
+            if ( y3 >= V_th )
+            {
+              r = ps.refractory_counts;
+              y3 = V_reset;
+              bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
+            }
+          }
       }
       d.v[ 0 ][ dev ] = y3;
       d.r[ dev ] = r;
@@ -543,29 +591,39 @@ __global__ void __launch_bounds__( BS )
       double iex = d.v[ 1 ][ dev ];
       double iin = d.v[ 2 ][ dev ];
       int32_t r = d.r[ dev ];
-      for ( int lag = ph.from; lag < ph.to; ++lag )
+      for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
       {
-        double* cin = C.at( lag, 0, local );
-        double* cex = C.at( lag, 1, local );
-        const double in_in = *cin;
-        const double in_ex = *cex;
-        if ( r == 0 )
-          V = V * P22 + iex * P21ex + iin * P21in + IeI0 * P20;
-        else
-          --r;
-        iex *= P11ex;
-        iin *= P11in;
-        iex += exI1;
-        iex += in_ex;
-        iin += in_in;
-        if ( V >= Theta )
-        {
-          r = ps.refractory_counts;
-          V = V_reset;
-          bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
-        }
-        *cin = 0.0;
-        *cex = 0.0;
+        double w_in[ LB ], w_ex[ LB ];
+#pragma unroll
+        for ( int q = 0; q < LB; ++q )
+          if ( l0 + q < ph.to )
+          {
+            w_in[ q ] = *C.at( l0 + q, 0, local );
+            w_ex[ q ] = *C.at( l0 + q, 1, local );
+          }
+#pragma unroll
+        for ( int q = 0; q < LB; ++q )
+          if ( l0 + q < ph.to )
+          {
+            const int lag = l0 + q;
+            if ( r == 0 )
+              V = V * P22 + iex * P21ex + iin * P21in + IeI0 * P20;
+            else
+              --r;
+            iex *= P11ex;
+            iin *= P11in;
+            iex += exI1;
+            iex += w_ex[ q ];
+            iin += w_in[ q ];
+            if ( V >= Theta )
+            {
+              r = ps.refractory_counts;
+              V = V_reset;
+              bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
+            }
+            *C.at( lag, 0, local ) = 0.0;
+            *C.at( lag, 1, local ) = 0.0;
+          }
       }
       d.v[ 0 ][ dev ] = V;
       d.v[ 1 ][ dev ] = iex;
