@@ -50,6 +50,160 @@ def build_c2(k, nestgpu, N, KE, KI, n_vp, seed, drive_seed, rate_hz=8000.0, t_st
     return pop, sg
 
 
+C3_POPS = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]  # E/I alternating
+
+
+def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9):
+    """8-population layered circuit: 77,169 iaf_psc_exp, pairwise_bernoulli with an 8x8
+    probability matrix drawn from the seed (uniform in [0, 0.14]), weights normal(87.8, 10 %)
+    and x -4 from inhibitory populations, delays normal (E 1.5 ms, I 0.75 ms, sd 50 %)
+    clipped at dt, background Poisson 8 Hz x K_ext with K_ext uniform in [1500, 2500] per
+    population. Returns (populations, total neurons, expected drive events per step)."""
+    rng = np.random.default_rng(drive_seed)
+    k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": seed})
+    pops = [k.Create("iaf_psc_exp", n, NEURON) for n in C3_POPS]
+    for p in pops:
+        p.set(V_m=rng.uniform(-70.0, -55.0, len(p)))
+    prob = rng.uniform(0.0, 0.14, size=(8, 8))
+    for j, pre in enumerate(pops):
+        exc = j % 2 == 0
+        w = nestgpu.random.normal(87.8, 8.78) if exc else -4.0 * nestgpu.random.normal(87.8, 8.78)
+        d = nestgpu.math.max(nestgpu.random.normal(1.5, 0.75) if exc else nestgpu.random.normal(0.75, 0.375), 0.1)
+        for i, post in enumerate(pops):
+            k.Connect(pre, post, {"rule": "pairwise_bernoulli", "p": float(prob[i, j])}, {"weight": w, "delay": d})
+    k_ext = rng.integers(1500, 2501, size=8)
+    lam = 0.0
+    for i, post in enumerate(pops):
+        rate = 8.0 * float(k_ext[i])
+        sg = k.Create("spike_generator", len(post), {"poisson_rate": rate, "poisson_seed": drive_seed + i,
+                                                     "poisson_stop": t_stop})
+        k.Connect(sg, post, "one_to_one", {"weight": 87.8, "delay": 0.1})
+        lam += len(post) * (1.0 - np.exp(-rate * 0.1e-3))
+    return pops, sum(C3_POPS), lam
+
+
+AEIF_C4 = {"a": 4.0, "b": 80.5, "tau_w": 144.0, "Delta_T": 2.0, "g_L": 30.0, "gsl_error_tol": 1e-3}
+
+
+def c4_specs(n=256, seed=4242, n_min=1000, n_max=50000):
+    """Batched sweep: Brunel-style instances, model iaf_psc_exp or aeif_cond_exp 50/50, N
+    log-uniform in [n_min, n_max], in-degree 10 % of N capped at 2000 (80 % E), eta uniform
+    [1, 3], g uniform [3, 7], delay 1.5 ms, one seed per instance."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        model = "aeif_cond_exp" if rng.random() < 0.5 else "iaf_psc_exp"
+        N = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        specs.append(dict(model=model, N=N, K=min(int(0.1 * N), 2000), eta=float(rng.uniform(1.0, 3.0)),
+                          g=float(rng.uniform(3.0, 7.0)), seed=int(rng.integers(1, 2 ** 31)),
+                          drive_seed=int(rng.integers(1, 2 ** 31))))
+    return specs
+
+
+def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True):
+    """One batched-sweep instance. iaf_psc_exp: J = 87.8 pA x min(1, sqrt(250 / K)), drive at
+    eta x nu_th x C_E (nu_th from the PSC amplitude); aeif_cond_exp: J = 6 nS x sqrt(0.8 / K),
+    drive eta x 10 kHz of 6 nS. Inhibitory weights -g J. Returns the population."""
+    k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": sp["seed"],
+                       "keep_host_connections": keep_host_connections})
+    N, K = sp["N"], sp["K"]
+    KE = int(0.8 * K)
+    KI = K - KE
+    NE = int(0.8 * N)
+    rng = np.random.default_rng(sp["drive_seed"])
+    if sp["model"] == "iaf_psc_exp":
+        pop = k.Create("iaf_psc_exp", N, NEURON)
+        pop.set(V_m=rng.uniform(-70.0, -55.0, N))
+        J = 87.8 * min(1.0, np.sqrt(250.0 / K))
+        w_drive = J
+        rate_hz = sp["eta"] * 15.0 * 250.0 / (J * 0.5 * 10.0) * 1000.0
+    else:
+        pop = k.Create("aeif_cond_exp", N, AEIF_C4)
+        pop.set(V_m=rng.uniform(-70.6, -55.0, N))
+        J = 6.0 * np.sqrt(0.8 / K)
+        w_drive = 6.0
+        rate_hz = sp["eta"] * 10000.0
+    k.Connect(pop[:NE], pop, {"rule": "fixed_indegree", "indegree": KE}, {"weight": J, "delay": 1.5})
+    k.Connect(pop[NE:], pop, {"rule": "fixed_indegree", "indegree": KI}, {"weight": -sp["g"] * J, "delay": 1.5})
+    sg = k.Create("spike_generator", N, {"poisson_rate": rate_hz, "poisson_seed": sp["drive_seed"]})
+    k.Connect(sg, pop, "one_to_one", {"weight": w_drive, "delay": 1.5})
+    return pop
+
+
+def run_batched(args, nestgpu):
+    """Batched form: every instance advances in the same slice launches (one grid over all
+    blocks of all instances, replayed from a hipGraph). value = aggregate recurrent synaptic
+    events/s of the batch; the same instances then continue one by one on the plain path."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    specs = c4_specs(args.instances, args.seed)
+    t_build = time.perf_counter()
+    kernels = [nestgpu.Kernel(0) for _ in specs]
+    workers = max(1, min(16, (os.cpu_count() or 4) // 2))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False), zip(kernels, specs)))
+    n_neurons = sum(sp["N"] for sp in specs)
+    n_syn = sum(k.GetKernelStatus("num_connections") for k in kernels) - n_neurons
+    print("built %d instances: %d neurons, %d synapses in %.1f s" % (len(specs), n_neurons, n_syn,
+                                                                    time.perf_counter() - t_build),
+          file=sys.stderr, flush=True)
+    chosen = None
+    if args.autotune:
+        t_at = time.perf_counter()
+        chosen = nestgpu.batch_autotune(kernels, probe_ms=args.probe_ms)
+        print("autotune: %.1f s, targets_per_block per size class %s" % (
+            time.perf_counter() - t_at, sorted(set(zip([sp["N"].bit_length() for sp in specs], chosen)))),
+            file=sys.stderr, flush=True)
+    with ThreadPoolExecutor(4) as ex:
+        list(ex.map(lambda k: k.Prepare(), kernels))
+    build_s = time.perf_counter() - t_build
+    print("prepared in %.1f s total" % build_s, file=sys.stderr, flush=True)
+
+    def events():
+        return sum(k.GetKernelStatus("synaptic_events") for k in kernels)
+
+    for i in range(args.warmup):
+        nestgpu.batch_simulate(kernels, args.t_step)
+        print("batched warmup %d done" % i, file=sys.stderr, flush=True)
+    ev0 = events()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        nestgpu.batch_simulate(kernels, args.t_step)
+    wall = time.perf_counter() - t0
+    ev_b = events() - ev0
+    value = ev_b / wall
+    rtf = wall / (args.steps * args.t_step * 1e-3)
+    print("batched: %d x %g ms, %.3f s wall, RTF %.3f, %.3g events/s" % (args.steps, args.t_step, wall, rtf, value),
+          file=sys.stderr, flush=True)
+    # the same instances, continued one at a time through the plain path
+    plain_wall = 0.0
+    ev0 = events()
+    for k in kernels:
+        t0 = time.perf_counter()
+        k.Run(args.plain_ms)
+        plain_wall += time.perf_counter() - t0
+    ev_p = events() - ev0
+    plain_value = ev_p / plain_wall
+    print("plain one by one: %g ms each, %.3f s wall, %.3g events/s; batched / plain = %.2f" % (
+        args.plain_ms, plain_wall, plain_value, value / plain_value), file=sys.stderr, flush=True)
+    models = {m: sum(1 for sp in specs if sp["model"] == m) for m in ("iaf_psc_exp", "aeif_cond_exp")}
+    line = {
+        "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic", "rtf": rtf,
+        "config": {"workload": "batched sweep: %d Brunel-style instances (iaf_psc_exp %d, aeif_cond_exp %d), N "
+                               "log-uniform [1k, 50k], in-degree min(0.1 N, 2000), delay 1.5 ms, one step = %g ms "
+                               "of every instance" % (len(specs), models["iaf_psc_exp"], models["aeif_cond_exp"],
+                                                       args.t_step),
+                   "n_instances": len(specs), "n_neurons": n_neurons, "n_synapses": n_syn, "build_s": build_s,
+                   "targets_per_block": sorted(set(chosen)) if chosen else "library default"},
+        "batched_vs_plain": value / plain_value,
+        "plain_path": {"value": plain_value, "wall_s": plain_wall, "ms_per_instance": args.plain_ms},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(N, KE, KI, n_vp, seed, drive_seed, t_sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import nestgpu
@@ -77,6 +231,20 @@ def cpu_baseline(N, KE, KI, n_vp, seed, drive_seed, t_sample, threads):
                       "%d recurrent events in %.2f s wall, RTF %.1f" % (t_sample, ev, wall, wall / (t_sample * 1e-3))}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_slice_c2_hbm_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per slice launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,10 +257,18 @@ def main():
     ap.add_argument("--n-vp", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20241)
     ap.add_argument("--drive-seed", type=int, default=777)
-    ap.add_argument("--roofline-ms", type=float, default=200.0)
+    ap.add_argument("--roofline-ms", type=float, default=128.0,
+                    help="biological ms of the per-launch timed pass (a multiple of 64 slices avoids a short chunk)")
     ap.add_argument("--cpu-sample-ms", type=float, default=30.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: 100k-neuron exponential-synapse random network (default); c3: 77k-neuron "
+                         "8-population circuit with min_delay one step; c4: batched sweep of many instances")
+    ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
+    ap.add_argument("--plain-ms", type=float, default=100.0, help="c4: biological ms per instance on the plain path")
+    ap.add_argument("--autotune", action="store_true", help="c4: pick targets_per_block per size class first")
+    ap.add_argument("--probe-ms", type=float, default=30.0, help="c4: autotuner probe length")
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     args = ap.parse_args()
@@ -107,15 +283,25 @@ def main():
     if nestgpu.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "c4":
+        if args.seed == 20241:
+            args.seed = 4242
+        run_batched(args, nestgpu)
+        return
 
     t_build = time.perf_counter()
     k = nestgpu.Kernel(0)
     k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block})
-    pop, sg = build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
-    n_syn = k.GetKernelStatus("num_connections") - args.n
+    if args.workload == "c2":
+        pop, sg = build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
+        n_neurons = args.n
+        drive_nz_per_step = args.n * (1.0 - np.exp(-8000.0 * 0.1e-3))
+    else:
+        pops, n_neurons, drive_nz_per_step = build_c3(k, nestgpu, args.n_vp, args.seed, args.drive_seed)
+    n_syn = k.GetKernelStatus("num_connections") - n_neurons
     k.Prepare()
     build_s = time.perf_counter() - t_build
-    print("built: %d neurons, %d synapses in %.1f s" % (args.n, n_syn, build_s), file=sys.stderr, flush=True)
+    print("built: %d neurons, %d synapses in %.1f s" % (n_neurons, n_syn, build_s), file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         k.Run(args.t_step)
@@ -132,7 +318,7 @@ def main():
     bio_s = args.steps * args.t_step * 1e-3
     value = events / wall
     rtf = wall / bio_s
-    rate = spikes / args.n / bio_s
+    rate = spikes / n_neurons / bio_s
     print("timed: %d steps, %.3f s wall, RTF %.4f, %.3g events/s, rate %.2f Hz" % (args.steps, wall, rtf, value, rate),
           file=sys.stderr, flush=True)
 
@@ -153,39 +339,44 @@ def main():
     #   32 B per neuron-step (two ring-buffer channels read and cleared)
     #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
     #   16 B per nonzero drive count (ring-buffer RMW; counts are generated in the kernel)
-    lam = 8000.0 * 0.1e-3
-    nz = args.n * steps_r * (1.0 - np.exp(-lam))
+    nz = drive_nz_per_step * steps_r
     launches = kl["slice"]
-    alg_total = 28.0 * events_r + 32.0 * args.n * steps_r + 56.0 * args.n * slices_r + 16.0 * nz
+    alg_total = 28.0 * events_r + 32.0 * n_neurons * steps_r + 56.0 * n_neurons * slices_r + 16.0 * nz
     alg_per_launch = alg_total / launches
     avg_us = 1e3 * kms["slice"] / launches
     achieved = alg_per_launch / (avg_us * 1e-6) / 1e9
-    dominant = "k_slice (fused deliver + update, iaf_psc_exp)"
+    dominant = "k_slice (fused deliver + update + gather, iaf_psc_exp)"
     per_kernel = {n: {"avg_us": 1e3 * ms / kl[n], "launches": kl[n], "total_ms": ms}
                   for n, ms in kms.items() if kl.get(n)}
     print("slice kernel: %.1f us avg over %d launches, %.3g events in roofline pass" % (avg_us, launches, events_r),
           file=sys.stderr, flush=True)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "c2":
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed, args.cpu_sample_ms, threads)
         print("cpu baseline: %.3g events/s on %d threads" % (cpu["value"], threads), file=sys.stderr, flush=True)
 
+    if args.workload == "c2":
+        workload_text = ("exponential-synapse random network: %d iaf_psc_exp, fixed_indegree %d E + %d I (%d synapses), "
+                         "delays 1.0-3.0 ms, seeded 8 kHz Poisson drive, dt 0.1 ms, one step = Run(%g ms)"
+                         % (args.n, args.ke, args.ki, n_syn, args.t_step))
+    else:
+        workload_text = ("8-population layered circuit: 77169 iaf_psc_exp, pairwise_bernoulli (%d synapses), normal "
+                         "delays clipped at dt, Poisson background 8 Hz x K_ext, dt 0.1 ms, one step = Run(%g ms)"
+                         % (n_syn, args.t_step))
+    default_layout = args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workload == "c2"
     line = {
         "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "rtf": rtf,
-        "config": {"workload": "exponential-synapse random network: %d iaf_psc_exp, fixed_indegree %d E + %d I "
-                               "(%d synapses), delays 1.0-3.0 ms, seeded 8 kHz Poisson drive, dt 0.1 ms, one "
-                               "step = Run(%g ms)" % (args.n, args.ke, args.ki, n_syn, args.t_step),
-                   "n_neurons": args.n, "n_synapses": n_syn, "n_vp": args.n_vp, "t_step_ms": args.t_step,
-                   "mean_rate_hz": rate, "build_s": build_s,
+        "config": {"workload": workload_text, "n_neurons": n_neurons, "n_synapses": n_syn, "n_vp": args.n_vp,
+                   "t_step_ms": args.t_step, "min_delay_steps": min_delay, "mean_rate_hz": rate, "build_s": build_s,
                    "delivery": "window" if st2["window_delivery"] else "global",
                    "targets_per_block": st2["targets_per_block"]},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic() if default_layout else None,
                      "avg_launch_us": avg_us, "alg_bytes_per_launch": alg_per_launch,
                      "events_per_launch": events_r / launches, "kernels": per_kernel},
         "cpu_baseline": cpu,

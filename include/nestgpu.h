@@ -89,6 +89,35 @@ int nestgpu_recorder_events( nestgpu_kernel* k, int64_t recorder_id, int64_t* se
  * one kernel per process). t_ms[i] is the simulation time of kernel i. */
 int nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms );
 
+/* Layout autotuner of the batched form (no counterpart in the reference): groups the kernels
+ * (none simulated yet) into size classes by neuron count (one per octave), times probe runs of
+ * probe_ms on up to 8 instances of each class for every candidate targets_per_block, and sets
+ * the fastest on every instance of the class (global delivery). Probe runs are discarded: every
+ * kernel returns to its state before the first Prepare. chosen[i] (may be NULL) receives the
+ * value picked for kernel i. */
+int nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand,
+  int* chosen );
+
+/* Served form (counterpart of nest-server, pynest/nest/server/hl_api_server.py, which runs
+ * one kernel per request and serialises calls): a request loop on one device. Admitted
+ * networks stay resident in HBM across their Simulate calls; all running requests advance
+ * together in the same slice launches, and new requests join at chunk boundaries as soon as
+ * the HBM budget (bytes) has room. Finished requests release their device memory; their
+ * spikes and final state stay readable through the kernel handle. */
+typedef struct nestgpu_server nestgpu_server;
+int nestgpu_server_new( int device, uint64_t hbm_budget_bytes, int slices_per_chunk, nestgpu_server** out );
+void nestgpu_server_free( nestgpu_server* s );
+/* k: built on the host (Create/Connect), not prepared or simulated, owned by the caller and
+ * untouched by it until the request is done; t_ms: its n_calls Simulate durations. */
+int nestgpu_server_submit( nestgpu_server* s, nestgpu_kernel* k, const double* t_ms, int n_calls, int64_t* id );
+/* Waits up to timeout_s (negative: forever). state: 0 queued, 1 admitted, 2 running, 3 done,
+ * -1 failed. times[4] (may be NULL): submit, admit, start of the first run, done, in seconds
+ * since the server started. events: recurrent synaptic events of the request. */
+int nestgpu_server_wait( nestgpu_server* s, int64_t id, double timeout_s, int* state, double* times,
+  uint64_t* events );
+/* Server statistics as JSON (same buffer convention as nestgpu_get_kernel_status). */
+int nestgpu_server_stats( nestgpu_server* s, char* buf, size_t cap, size_t* len );
+
 /* Seeded Poisson drive counts, the input the spike_generator option poisson_rate produces:
  * out[s * n + i] for streams stream0 + i and emission steps step0 + s, lambda per step. */
 int nestgpu_poisson_counts( uint64_t seed, double lambda, uint64_t stream0, int64_t n, int64_t step0, int64_t n_steps,

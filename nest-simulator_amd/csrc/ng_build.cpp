@@ -39,9 +39,11 @@ void
 Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const
 {
   // Window delivery stages max_delay slots x channels x tb doubles in LDS per workgroup and
-  // slice; it pays when a slice carries many events per cell, i.e. for longer slices.
+  // slice. Its LDS footprint allows one workgroup per CU, and on the MI355X the global ring
+  // with three workgroups per CU measured faster on every network tried, so automatic
+  // selection takes global delivery; window delivery is available on request.
   const uint64_t per_target = static_cast< uint64_t >( max_delay ) * nch;
-  const bool want_win = delivery_mode_ == 1 or ( delivery_mode_ == 0 and min_delay_ >= 4 );
+  const bool want_win = delivery_mode_ == 1;
   win = false;
   if ( targets_per_block_ > 0 )
   {

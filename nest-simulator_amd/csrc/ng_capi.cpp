@@ -2,12 +2,15 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <chrono>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/nestgpu.h"
 #include "ng_nest.h"
+#include "ng_server.h"
 
 struct nestgpu_kernel
 {
@@ -262,6 +265,202 @@ nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms )
     runner->run( engines, steps );
     for ( int i = 0; i < n; ++i )
       ks[ i ]->k->after_batch_run( steps[ i ] );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand, int* chosen )
+{
+  try
+  {
+    if ( n <= 0 )
+      return 0;
+    if ( n_cand <= 0 or candidates == nullptr )
+      throw std::runtime_error( "batch_autotune needs at least one candidate targets_per_block" );
+    const int dev = ks[ 0 ]->device;
+    // size classes: one per octave of the neuron count
+    std::map< int, std::vector< int > > classes;
+    for ( int i = 0; i < n; ++i )
+    {
+      if ( ks[ i ]->device != dev )
+        throw std::runtime_error( "batched kernels must share one device" );
+      const uint64_t nn = std::max< uint64_t >( 1, ks[ i ]->k->num_neurons() );
+      int c = 0;
+      while ( ( uint64_t( 2 ) << c ) <= nn )
+        ++c;
+      classes[ c ].push_back( i );
+    }
+    nestgpu::Runner runner( dev, 64 );
+    for ( auto& kv : classes )
+    {
+      const std::vector< int >& members = kv.second;
+      std::vector< int > sample;
+      const size_t ns = std::min< size_t >( 8, members.size() );
+      for ( size_t q = 0; q < ns; ++q )
+        sample.push_back( members[ q * members.size() / ns ] );
+      int best = -1;
+      double best_ms = 0;
+      for ( int c = 0; c < n_cand; ++c )
+      {
+        const int tb = candidates[ c ];
+        bool ok = tb >= 1 and tb <= static_cast< int >( nestgpu::MAX_TB );
+        for ( int i : sample )
+          ok = ok and ( ks[ i ]->k->num_neurons() + tb - 1 ) / tb <= nestgpu::PMAX_GLOB;
+        if ( not ok )
+          continue;
+        std::vector< nestgpu::Engine* > engines;
+        std::vector< int64_t > steps;
+        std::vector< bool > keep;
+        for ( int i : sample )
+        {
+          keep.push_back( ks[ i ]->k->keep_host_connections() );
+          ks[ i ]->k->set_keep_host_connections( true );  // probes rebuild the device copy
+          ks[ i ]->k->reset_device();
+          ks[ i ]->k->set_layout( tb, 2 );
+          ks[ i ]->k->prepare_for_batch();
+          engines.push_back( &ks[ i ]->k->engine() );
+          steps.push_back( ks[ i ]->k->steps_for( probe_ms ) );
+        }
+        runner.run( engines, steps );  // warm-up: graph capture, caches
+        const auto t0 = std::chrono::steady_clock::now();
+        runner.run( engines, steps );
+        const double ms = std::chrono::duration< double, std::milli >( std::chrono::steady_clock::now() - t0 ).count();
+        if ( best < 0 or ms < best_ms )
+        {
+          best = tb;
+          best_ms = ms;
+        }
+        for ( size_t q = 0; q < sample.size(); ++q )
+        {
+          ks[ sample[ q ] ]->k->reset_device();
+          ks[ sample[ q ] ]->k->set_keep_host_connections( keep[ q ] );
+        }
+      }
+      if ( best < 0 )
+        throw std::runtime_error( "batch_autotune: no candidate fits the instances of one size class" );
+      for ( int i : members )
+      {
+        ks[ i ]->k->reset_device();
+        ks[ i ]->k->set_layout( best, 2 );
+        if ( chosen )
+          chosen[ i ] = best;
+      }
+    }
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+struct nestgpu_server
+{
+  std::unique_ptr< nestgpu::Server > s;
+};
+
+int
+nestgpu_server_new( int device, uint64_t hbm_budget_bytes, int slices_per_chunk, nestgpu_server** out )
+{
+  try
+  {
+    if ( out == nullptr )
+      throw std::runtime_error( "out is NULL" );
+    auto* h = new nestgpu_server;
+    h->s = std::make_unique< nestgpu::Server >( device, hbm_budget_bytes, slices_per_chunk );
+    *out = h;
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+void
+nestgpu_server_free( nestgpu_server* s )
+{
+  delete s;
+}
+
+int
+nestgpu_server_submit( nestgpu_server* s, nestgpu_kernel* k, const double* t_ms, int n_calls, int64_t* id )
+{
+  try
+  {
+    if ( s == nullptr or k == nullptr or id == nullptr or ( n_calls > 0 and t_ms == nullptr ) )
+      throw std::runtime_error( "null argument" );
+    std::vector< int64_t > steps;
+    for ( int i = 0; i < n_calls; ++i )
+    {
+      if ( t_ms[ i ] < 0 )
+        throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
+      steps.push_back( k->k->steps_for( t_ms[ i ] ) );
+    }
+    *id = s->s->submit( k->k.get(), steps );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_wait( nestgpu_server* s, int64_t id, double timeout_s, int* state, double* times, uint64_t* events )
+{
+  try
+  {
+    if ( s == nullptr )
+      throw std::runtime_error( "null server" );
+    const nestgpu::RequestInfo r = s->s->wait( id, timeout_s );
+    if ( state )
+      *state = r.state;
+    if ( times )
+    {
+      times[ 0 ] = r.t_submit;
+      times[ 1 ] = r.t_admit;
+      times[ 2 ] = r.t_start;
+      times[ 3 ] = r.t_done;
+    }
+    if ( events )
+      *events = r.events;
+    if ( r.state == nestgpu::REQ_FAILED )
+      g_err = r.error;
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_stats( nestgpu_server* s, char* buf, size_t cap, size_t* len )
+{
+  try
+  {
+    if ( s == nullptr )
+      throw std::runtime_error( "null server" );
+    const std::string j = s->s->stats_json();
+    if ( len )
+      *len = j.size();
+    if ( buf and cap > 0 )
+    {
+      const size_t n = std::min( cap - 1, j.size() );
+      std::memcpy( buf, j.data(), n );
+      buf[ n ] = 0;
+    }
     return 0;
   }
   catch ( const std::exception& e )

@@ -232,6 +232,11 @@ Runner::Runner( int device, int slices_per_chunk )
 {
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
   hip_check( hipStreamCreateWithFlags( &stream_, hipStreamNonBlocking ), "hipStreamCreate" );
+  for ( int q = 0; q < 2; ++q )
+  {
+    hip_check( hipEventCreate( &ev_chunk_[ q ] ), "hipEventCreate" );
+    hip_check( hipEventCreate( &ev_kernel_[ q ] ), "hipEventCreate" );
+  }
 }
 
 Runner::~Runner()
@@ -245,6 +250,17 @@ Runner::~Runner()
     hipFree( d_prefix_ );
   if ( h_nets_ )
     hipHostFree( h_nets_ );
+  if ( d_status_ )
+    hipFree( d_status_ );
+  if ( h_status_ )
+    hipHostFree( h_status_ );
+  for ( int q = 0; q < 2; ++q )
+  {
+    if ( ev_chunk_[ q ] )
+      hipEventDestroy( ev_chunk_[ q ] );
+    if ( ev_kernel_[ q ] )
+      hipEventDestroy( ev_kernel_[ q ] );
+  }
   if ( stream_ )
     hipStreamDestroy( stream_ );
 }
@@ -266,7 +282,14 @@ Runner::ensure_capacity( size_t n )
     hipFree( d_prefix_ );
   if ( h_nets_ )
     hipHostFree( h_nets_ );
+  if ( d_status_ )
+    hipFree( d_status_ );
+  if ( h_status_ )
+    hipHostFree( h_status_ );
   hip_check( hipMalloc( &d_nets_, n * sizeof( NetDev ) ), "hipMalloc nets" );
+  hip_check( hipMalloc( &d_status_, 2 * n * sizeof( unsigned long long ) ), "hipMalloc status" );
+  hip_check( hipHostMalloc( &h_status_, 2 * n * sizeof( unsigned long long ), hipHostMallocDefault ),
+    "hipHostMalloc status" );
   hip_check( hipMalloc( &d_prefix_, ( n + 1 ) * sizeof( uint32_t ) ), "hipMalloc prefix" );
   hip_check( hipHostMalloc( &h_nets_, n * sizeof( NetDev ), hipHostMallocDefault ), "hipHostMalloc nets" );
   cap_ = n;
@@ -316,10 +339,20 @@ Runner::launch_one( int n, uint32_t total, int mask, bool win, int s, hipEvent_t
 void
 Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >& steps )
 {
+  std::vector< int64_t > remaining = steps;
+  last_ms_ = 0;
+  while ( run_chunk( engines, remaining ) )
+  {
+  }
+}
+
+bool
+Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >& remaining )
+{
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
   const int n = static_cast< int >( engines.size() );
   if ( n == 0 )
-    return;
+    return false;
   ensure_capacity( n );
   int mask = 0;
   bool win = false;
@@ -332,7 +365,7 @@ Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >
   }
   const uint32_t total = prefix[ n ];
   if ( total == 0 )
-    return;
+    return false;
   // the window kernel keeps the block-count prefix of every instance in a smaller LDS table
   for ( int i = 0; win and i < n; ++i )
     if ( engines[ i ]->net().n_blocks > PMAX_WIN )
@@ -343,105 +376,100 @@ Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >
     hip_check( hipMemcpy( d_prefix_, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice ), "prefix" );
     prefix_ = prefix;
   }
-  std::vector< int64_t > remaining = steps;
-  hipEvent_t e0, e1, ev[ 2 ];
-  hip_check( hipEventCreate( &e0 ), "hipEventCreate" );
-  hip_check( hipEventCreate( &e1 ), "hipEventCreate" );
-  hip_check( hipEventCreate( &ev[ 0 ] ), "hipEventCreate" );
-  hip_check( hipEventCreate( &ev[ 1 ] ), "hipEventCreate" );
-  last_ms_ = 0;
-  for ( ;; )
+  bool any = false, split = false;
+  for ( int i = 0; i < n; ++i )
   {
-    bool any = false, split = false;
-    for ( int i = 0; i < n; ++i )
+    Engine& E = *engines[ i ];
+    NetDev d = E.net();
+    d.clock = E.clock;
+    d.from0 = E.from_step;
+    d.part0 = E.part;
+    const int64_t span =
+      static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S_ - 1 ) * d.min_delay;
+    d.todo = std::min( remaining[ i ], span );
+    d.active = d.todo > 0 ? 1 : 0;
+    any = any or d.active;
+    split = split or ( d.active and E.from_step > 0 );
+    if ( d.part0 > 255 )
+      throw std::runtime_error( "NotImplemented: a slice split into more than 256 Run parts" );
+    h_nets_[ i ] = d;
+  }
+  if ( not any )
+    return false;
+  if ( use_graph and not time_kernels and not split )
+    capture( n, total, mask, win );
+  hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
+  hip_check( hipEventRecord( ev_chunk_[ 0 ], stream_ ), "hipEventRecord" );
+  if ( split or time_kernels or not use_graph )
+  {
+    for ( int s = 0; s < S_; ++s )
     {
-      Engine& E = *engines[ i ];
-      NetDev d = E.net();
-      d.clock = E.clock;
-      d.from0 = E.from_step;
-      d.part0 = E.part;
-      const int64_t span =
-        static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S_ - 1 ) * d.min_delay;
-      d.todo = std::min( remaining[ i ], span );
-      d.active = d.todo > 0 ? 1 : 0;
-      any = any or d.active;
-      split = split or ( d.active and E.from_step > 0 );
-      if ( d.part0 > 255 )
-        throw std::runtime_error( "NotImplemented: a slice split into more than 256 Run parts" );
-      h_nets_[ i ] = d;
-    }
-    if ( not any )
-      break;
-    if ( use_graph and not time_kernels and not split )
-      capture( n, total, mask, win );
-    hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
-    hip_check( hipEventRecord( e0, stream_ ), "hipEventRecord" );
-    if ( split or time_kernels or not use_graph )
-    {
-      for ( int s = 0; s < S_; ++s )
-      {
-        launch_one( n, total, mask, win, s, time_kernels ? ev : nullptr );
-        if ( s == 0 and split )
-          for ( int i = 0; i < n; ++i )
-          {
-            // the slice that continued a previous part completes here: restore NEST's order
-            const NetDev& d = h_nets_[ i ];
-            const SlicePhase ph = slice_phase( d, 0 );
-            if ( ph.active and d.from0 > 0 and ph.complete )
-              hip_check( launch_flatten( d_nets_ + i, ph.parity, d.part0 + 1, stream_ ), "launch_flatten" );
-          }
-      }
-    }
-    else
-      hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
-    hip_check( hipEventRecord( e1, stream_ ), "hipEventRecord" );
-    hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
-    float ms = 0;
-    hipEventElapsedTime( &ms, e0, e1 );
-    last_ms_ += ms;
-    for ( int i = 0; i < n; ++i )
-    {
-      Engine& E = *engines[ i ];
-      const NetDev& d = h_nets_[ i ];
-      if ( not d.active )
-        continue;
-      const unsigned long long flags = E.read_counter( 0 );
-      if ( flags )
-      {
-        E.clear_error_flags();
-        std::string why;
-        if ( flags & ERR_AEIF_INSTABILITY )
-          why += " NumericalInstability in aeif_cond_exp;";
-        if ( flags & ERR_AEIF_DOUBLE_SPIKE )
-          why += " aeif_cond_exp emitted two spikes within one step;";
-        if ( flags & ERR_SPIKE_LIST )
-          why += " spike list capacity exceeded;";
-        if ( flags & ERR_LOG )
-          why += " spike log capacity exceeded;";
-        throw std::runtime_error( "device run failed:" + why );
-      }
-      const int64_t total_steps = E.from_step + d.todo;
-      const int64_t slices_done = total_steps / d.min_delay;
-      E.clock += slices_done * d.min_delay;
-      const int32_t nf = static_cast< int32_t >( total_steps % d.min_delay );
-      if ( nf == 0 )
-        E.part = 0;
-      else if ( slices_done == 0 )
-        E.part += 1;  // another part of the same slice
-      else
-        E.part = 1;
-      E.from_step = nf;
-      remaining[ i ] -= d.todo;
-      unsigned long long ln = 0;
-      hip_check( hipMemcpy( &ln, E.net().log_n, sizeof ln, hipMemcpyDeviceToHost ), "log_n" );
-      if ( ln > E.net().log_cap / 2 )
-        E.drain_log();
+      // launches where no instance has work are skipped (their workgroups would exit at once)
+      bool active = false;
+      for ( int i = 0; i < n and not active; ++i )
+        active = slice_phase( h_nets_[ i ], s ).active;
+      if ( not active )
+        break;
+      launch_one( n, total, mask, win, s, time_kernels ? ev_kernel_ : nullptr );
+      if ( s == 0 and split )
+        for ( int i = 0; i < n; ++i )
+        {
+          // the slice that continued a previous part completes here: restore NEST's order
+          const NetDev& d = h_nets_[ i ];
+          const SlicePhase ph = slice_phase( d, 0 );
+          if ( ph.active and d.from0 > 0 and ph.complete )
+            hip_check( launch_flatten( d_nets_ + i, ph.parity, d.part0 + 1, stream_ ), "launch_flatten" );
+        }
     }
   }
-  hipEventDestroy( e0 );
-  hipEventDestroy( e1 );
-  hipEventDestroy( ev[ 0 ] );
-  hipEventDestroy( ev[ 1 ] );
+  else
+    hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
+  hip_check( hipEventRecord( ev_chunk_[ 1 ], stream_ ), "hipEventRecord" );
+  // error flags and spike-log fill of every instance in one read-back
+  hip_check( launch_status( d_nets_, n, d_status_, stream_ ), "launch_status" );
+  hip_check( hipMemcpyAsync( h_status_, d_status_, 2 * n * sizeof( unsigned long long ), hipMemcpyDeviceToHost, stream_ ),
+    "status D2H" );
+  hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
+  float ms = 0;
+  hipEventElapsedTime( &ms, ev_chunk_[ 0 ], ev_chunk_[ 1 ] );
+  last_ms_ += ms;
+  for ( int i = 0; i < n; ++i )
+  {
+    Engine& E = *engines[ i ];
+    const NetDev& d = h_nets_[ i ];
+    if ( not d.active )
+      continue;
+    const unsigned long long flags = h_status_[ 2 * i ];
+    if ( flags )
+    {
+      E.clear_error_flags();
+      std::string why;
+      if ( flags & ERR_AEIF_INSTABILITY )
+        why += " NumericalInstability in aeif_cond_exp;";
+      if ( flags & ERR_AEIF_DOUBLE_SPIKE )
+        why += " aeif_cond_exp emitted two spikes within one step;";
+      if ( flags & ERR_SPIKE_LIST )
+        why += " spike list capacity exceeded;";
+      if ( flags & ERR_LOG )
+        why += " spike log capacity exceeded;";
+      throw std::runtime_error( "device run failed:" + why );
+    }
+    const int64_t total_steps = E.from_step + d.todo;
+    const int64_t slices_done = total_steps / d.min_delay;
+    E.clock += slices_done * d.min_delay;
+    const int32_t nf = static_cast< int32_t >( total_steps % d.min_delay );
+    if ( nf == 0 )
+      E.part = 0;
+    else if ( slices_done == 0 )
+      E.part += 1;  // another part of the same slice
+    else
+      E.part = 1;
+    E.from_step = nf;
+    remaining[ i ] -= d.todo;
+    if ( h_status_[ 2 * i + 1 ] > E.net().log_cap / 2 )
+      E.drain_log();
+  }
+  return true;
 }
 
 }  // namespace nestgpu

@@ -20,6 +20,8 @@ hipError_t launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_n
   bool win, int s, hipStream_t st );
 // Rebuild the exact receive order of a slice that was split across Run calls.
 hipError_t launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st );
+// out[2 i] = error flags, out[2 i + 1] = spike-log fill of instance i
+hipError_t launch_status( const NetDev* nets, int n_nets, unsigned long long* out, hipStream_t st );
 
 enum KernelKind
 {
@@ -152,6 +154,9 @@ public:
   ~Runner();
   // Advance every engine by steps[i] (0 = idle). Throws on device-side error flags.
   void run( const std::vector< Engine* >& engines, const std::vector< int64_t >& steps );
+  // One chunk (at most slices_per_chunk slices of each engine); remaining[i] is reduced by
+  // the steps done. Returns false when no engine had work.
+  bool run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >& remaining );
   double
   last_kernel_ms() const
   {
@@ -183,6 +188,10 @@ private:
   NetDev* d_nets_ = nullptr;
   NetDev* h_nets_ = nullptr;  // pinned
   uint32_t* d_prefix_ = nullptr;
+  unsigned long long* d_status_ = nullptr;
+  unsigned long long* h_status_ = nullptr;  // pinned
+  hipEvent_t ev_chunk_[ 2 ] = { nullptr, nullptr };
+  hipEvent_t ev_kernel_[ 2 ] = { nullptr, nullptr };
   size_t cap_ = 0;
   hipGraphExec_t exec_ = nullptr;
   int exec_n_ = -1;

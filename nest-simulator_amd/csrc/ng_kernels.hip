@@ -845,6 +845,25 @@ launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32
   return hipGetLastError();
 }
 
+__global__ void
+k_status( const NetDev* nets, int n_nets, unsigned long long* out )
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if ( i < n_nets )
+  {
+    out[ 2 * i ] = nets[ i ].counters[ 0 ];
+    out[ 2 * i + 1 ] = *nets[ i ].log_n;
+  }
+}
+
+hipError_t
+launch_status( const NetDev* nets, int n_nets, unsigned long long* out, hipStream_t st )
+{
+  if ( n_nets > 0 )
+    k_status<<< ( n_nets + 255 ) / 256, 256, 0, st >>>( nets, n_nets, out );
+  return hipGetLastError();
+}
+
 hipError_t
 launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st )
 {

@@ -285,6 +285,10 @@ Kernel::reset()
   spike_count_ = 0;
   targets_per_block_ = 0;
   delivery_mode_ = 0;
+  keep_host_connections_ = true;
+  host_connections_freed_ = false;
+  released_ = false;
+  released_events_ = released_spikes_ = 0;
 }
 
 void
@@ -359,6 +363,8 @@ Kernel::set_kernel_status( const Json& d )
       delivery_mode_ = m;
       store_dirty_ = true;
     }
+    else if ( k == "keep_host_connections" )
+      keep_host_connections_ = kv.second.number() != 0;
     else if ( k == "kernel_timing" )
     {
       time_kernels_ = kv.second.number() != 0;
@@ -409,7 +415,8 @@ Kernel::get_kernel_status_json() const
   for ( int q = 0; q < N_KINDS; ++q )
     o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << kernel_launches_[ q ];
   o << "}, \"synaptic_events\": " << synaptic_events()
-    << ", \"device_bytes\": " << ( engine_ ? engine_->device_bytes() : 0 ) << "}";
+    << ", \"device_bytes\": " << ( engine_ ? engine_->device_bytes() : 0 )
+    << ", \"device_bytes_estimate\": " << estimate_device_bytes() << "}";
   return o.str();
 }
 
@@ -962,6 +969,8 @@ void
 Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const Json& conn_spec,
   const Json& syn_spec )
 {
+  if ( host_connections_freed_ )
+    throw std::runtime_error( "KernelException: the host connection table was released; Connect is not possible" );
   if ( rng_dirty_ )
     init_rngs();
   std::string rule = "all_to_all";
@@ -1196,6 +1205,8 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
 void
 Kernel::connect_arrays( const int64_t* src, const int64_t* tgt, const double* w, const double* d_ms, int64_t n )
 {
+  if ( host_connections_freed_ )
+    throw std::runtime_error( "KernelException: the host connection table was released; Connect is not possible" );
   // ConnectionManager::connect_arrays: each VP creates the connections of its targets in
   // array order (connection_manager.cpp:624-760)
   for ( int64_t i = 0; i < n; ++i )
@@ -1248,7 +1259,7 @@ Kernel::set_spike_counts( int64_t gen_first, int64_t n, int64_t t0_step, int64_t
 int64_t
 Kernel::num_connections() const
 {
-  int64_t n = 0;
+  int64_t n = host_connections_freed_ ? freed_connections_ : 0;
   for ( const auto& c : conns_ )
     n += static_cast< int64_t >( c.size() );
   return n + static_cast< int64_t >( dev_conns_.size() );
@@ -1257,6 +1268,8 @@ Kernel::num_connections() const
 void
 Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int32_t* vp ) const
 {
+  if ( host_connections_freed_ )
+    throw std::runtime_error( "KernelException: the host connection table was released (keep_host_connections false)" );
   // neuron connections per VP in creation order, then device connections
   int64_t k = 0;
   for ( int v = 0; v < n_vp_; ++v )
@@ -1288,6 +1301,8 @@ void
 Kernel::prepare()
 {
   // SimulationManager::prepare (simulation_manager.cpp:507-570)
+  if ( released_ )
+    throw std::runtime_error( "KernelException: this network was released from the device after its last run" );
   if ( not simulated_ )
   {
     min_delay_ = d_min_seen_ < 0 ? 1 : d_min_seen_;
@@ -1305,13 +1320,118 @@ Kernel::prepare()
     if ( simulated_ and state_dirty_ )
       throw std::runtime_error( "NotImplemented: parameter changes after Simulate are not supported on the device path." );
     collect_log();
+    if ( host_connections_freed_ )
+      throw std::runtime_error( "KernelException: the host connection table was released; cannot rebuild the device copy" );
     BuildSpec b;
     build_spec( b );
     engine_->build( b, simulated_ );
     store_dirty_ = false;
     state_dirty_ = false;
+    if ( not keep_host_connections_ )
+    {
+      freed_connections_ = 0;
+      for ( auto& C : conns_ )
+      {
+        freed_connections_ += static_cast< int64_t >( C.size() );
+        C = ConnList();
+      }
+      host_connections_freed_ = true;
+    }
   }
   prepared_ = true;
+}
+
+void
+Kernel::reset_device()
+{
+  if ( host_connections_freed_ )
+    throw std::runtime_error( "KernelException: the host connection table was released; cannot rebuild the device copy" );
+  engine_.reset();
+  prepared_ = false;
+  simulated_ = false;
+  store_dirty_ = true;
+  spike_sender_.clear();
+  spike_step_.clear();
+  spike_count_ = 0;
+  last_kernel_ms_ = last_wall_ms_ = time_simulate_ms_ = 0;
+  for ( int q = 0; q < N_KINDS; ++q )
+  {
+    time_kernel_ms_[ q ] = 0;
+    kernel_launches_[ q ] = 0;
+  }
+}
+
+uint64_t
+Kernel::estimate_device_bytes() const
+{
+  const uint64_t N = params_.size();
+  if ( N == 0 )
+    return 0;
+  int64_t E = 0;
+  for ( const auto& C : conns_ )
+    E += static_cast< int64_t >( C.size() );
+  E += host_connections_freed_ ? freed_connections_ : 0;
+  const long dmin = simulated_ ? min_delay_ : ( d_min_seen_ < 0 ? 1 : d_min_seen_ );
+  const long dmax = simulated_ ? max_delay_ : std::max( d_max_seen_, 1L );
+  bool all_delta = true;
+  for ( const auto& Pm : params_ )
+    all_delta = all_delta and Pm.model == M_IAF_PSC_DELTA;
+  const int nch = all_delta ? 1 : 2;
+  uint32_t tb = 0;
+  bool win = false;
+  choose_layout( static_cast< uint32_t >( N ), static_cast< int >( dmax ), nch, tb, win );
+  const uint64_t P = ( N + tb - 1 ) / tb;
+  const uint64_t L = dmin + dmax;
+  uint64_t b = static_cast< uint64_t >( E ) * sizeof( SynRec ) + N * ( P + 1 ) * 4  // store
+    + P * L * nch * tb * 8                                                          // ring buffers
+    + N * ( 5 * 8 + 4 + 2 + 1 + 4 )                                                 // state, indices
+    + 2 * P * tb * dmin * 4 + 2 * N * dmin * 4                                     // spike lists
+    + ( std::any_of( recorders_.begin(), recorders_.end(), []( const auto& r ) { return not r.empty(); } )
+          ? std::max< uint64_t >( 1u << 20, N * 16 ) * 12
+          : 0 );
+  for ( const auto& D : dense_ )
+    b += D.counts.size();
+  return b + ( 1u << 20 );
+}
+
+void
+Kernel::release_device()
+{
+  if ( engine_ and engine_->built() )
+  {
+    collect_log();
+    std::vector< double > f[ 5 ];
+    for ( int q = 0; q < 5; ++q )
+      engine_->get_state( q, f[ q ] );
+    std::vector< int32_t > r;
+    engine_->get_r( r );
+    for ( size_t i = 0; i < state_.size(); ++i )
+    {
+      const uint32_t dev = static_cast< uint32_t >( gid_to_dev_[ neuron_gid_[ i ] ] );
+      NeuronState& S = state_[ i ];
+      S.v0 = f[ 0 ][ dev ];
+      S.v1 = f[ 1 ][ dev ];
+      S.v2 = f[ 2 ][ dev ];
+      S.v3 = f[ 3 ][ dev ];
+      S.v4 = f[ 4 ][ dev ];
+      S.r = r[ dev ];
+    }
+    released_events_ = engine_->read_counter( 1 );
+    released_spikes_ = engine_->read_counter( 2 );
+  }
+  engine_.reset();
+  prepared_ = false;
+  released_ = true;
+}
+
+void
+Kernel::set_layout( int targets_per_block, int delivery_mode )
+{
+  if ( simulated_ )
+    throw std::runtime_error( "KernelException: the device layout cannot change after Simulate" );
+  targets_per_block_ = targets_per_block;
+  delivery_mode_ = delivery_mode;
+  store_dirty_ = true;
 }
 
 void
@@ -1386,6 +1506,8 @@ Kernel::simulate( double t_ms )
 uint64_t
 Kernel::synaptic_events() const
 {
+  if ( released_ )
+    return released_events_;
   if ( not engine_ or not engine_->built() )
     return 0;
   return engine_->read_counter( 1 );

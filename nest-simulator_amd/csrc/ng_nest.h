@@ -233,7 +233,40 @@ public:
   {
     return prepared_;
   }
+  bool simulated() const
+  {
+    return simulated_;
+  }
+  int device() const
+  {
+    return device_;
+  }
   void prepare_for_batch();  // prepare without running (idempotent)
+  // Discard the device copy and every simulation result, back to the state before the
+  // first Prepare (used by the layout autotuner's probe runs).
+  void reset_device();
+  void set_layout( int targets_per_block, int delivery_mode );
+  // Device memory Prepare will allocate, estimated from the host tables (serving admission).
+  uint64_t estimate_device_bytes() const;
+  // Copy the device state into the host tables, collect the spike log and free the device
+  // copy; the kernel then answers status queries but cannot simulate further.
+  void release_device();
+  bool released() const
+  {
+    return released_;
+  }
+  bool keep_host_connections() const
+  {
+    return keep_host_connections_;
+  }
+  void set_keep_host_connections( bool keep )
+  {
+    keep_host_connections_ = keep;
+  }
+  int targets_per_block() const
+  {
+    return engine_ and engine_->built() ? static_cast< int >( engine_->net().tb ) : targets_per_block_;
+  }
   void after_batch_run( int64_t steps );
   int64_t steps_for( double t_ms ) const
   {
@@ -258,6 +291,8 @@ public:
   uint64_t synaptic_events() const;
   uint64_t local_spike_counter() const
   {
+    if ( released_ )
+      return released_spikes_;
     return engine_ and engine_->built() ? engine_->read_counter( 2 ) : 0;
   }
   uint64_t num_neurons() const
@@ -330,6 +365,11 @@ private:
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
   int targets_per_block_ = 0;  // 0: chosen from the network size
   int delivery_mode_ = 0;      // 0 automatic, 1 window (LDS ring-buffer slots), 2 global
+  bool keep_host_connections_ = true;  // false: the host copy is freed once the device holds it
+  bool host_connections_freed_ = false;
+  bool released_ = false;
+  uint64_t released_events_ = 0, released_spikes_ = 0;
+  int64_t freed_connections_ = 0;
   double time_simulate_ms_ = 0;
 };
 

@@ -51,6 +51,14 @@ _SIGS = {
     "nestgpu_simulate": (ctypes.c_int, [_vp, ctypes.c_double]),
     "nestgpu_recorder_events": (ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _f64p, ctypes.c_int64, _i64p]),
     "nestgpu_batch_simulate": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
+    "nestgpu_batch_autotune": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nestgpu_server_new": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "nestgpu_server_free": (None, [_vp]),
+    "nestgpu_server_submit": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, _i64p]),
+    "nestgpu_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _f64p,
+                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "nestgpu_server_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "nestgpu_poisson_counts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]),
     "nestgpu_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -337,6 +345,69 @@ def batch_simulate(kernels, times):
     t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (len(kernels),)))
     if _lib.nestgpu_batch_simulate(arr, len(kernels), _ptr(t, ctypes.c_double)) != 0:
         raise NESTError(_lib.nestgpu_last_error(None).decode())
+
+
+def batch_autotune(kernels, probe_ms=20.0, candidates=(64, 128, 256, 512)):
+    """Pick targets_per_block per size class of not yet simulated kernels by timing probe
+    runs (discarded). Returns the value chosen for each kernel."""
+    arr = (_vp * len(kernels))(*[k._h for k in kernels])
+    cand = (ctypes.c_int * len(candidates))(*[int(c) for c in candidates])
+    out = (ctypes.c_int * len(kernels))()
+    if _lib.nestgpu_batch_autotune(arr, len(kernels), float(probe_ms), cand, len(candidates), out) != 0:
+        raise NESTError(_lib.nestgpu_last_error(None).decode())
+    return list(out)
+
+
+class Server:
+    """Served form: continuous batching of simulate requests on one device within an HBM
+    budget. submit() takes a kernel built on the host (not yet simulated) and its Simulate
+    durations; the kernel must not be used until wait() reports it done."""
+
+    QUEUED, ADMITTED, RUNNING, DONE, FAILED = 0, 1, 2, 3, -1
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, slices_per_chunk=16):
+        h = _vp()
+        if _lib.nestgpu_server_new(int(device), int(hbm_budget_bytes), int(slices_per_chunk), ctypes.byref(h)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        self._h = h
+        self._kernels = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.nestgpu_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, kernel, times):
+        t = np.ascontiguousarray(np.atleast_1d(np.asarray(times, dtype=np.float64)))
+        rid = ctypes.c_int64(0)
+        if _lib.nestgpu_server_submit(self._h, kernel._h, _ptr(t, ctypes.c_double), len(t), ctypes.byref(rid)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        self._kernels[rid.value] = kernel
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Returns dict(state, t_submit, t_admit, t_start, t_done, events); raises if failed."""
+        st = ctypes.c_int(0)
+        tm = np.zeros(4)
+        ev = ctypes.c_uint64(0)
+        if _lib.nestgpu_server_wait(self._h, int(rid), float(timeout), ctypes.byref(st), _ptr(tm, ctypes.c_double),
+                                    ctypes.byref(ev)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        if st.value == self.FAILED:
+            raise NESTError("request %d failed: %s" % (rid, _lib.nestgpu_last_error(None).decode()))
+        return {"state": st.value, "t_submit": tm[0], "t_admit": tm[1], "t_start": tm[2], "t_done": tm[3],
+                "events": ev.value}
+
+    def stats(self):
+        n = ctypes.c_size_t(0)
+        _lib.nestgpu_server_stats(self._h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        if _lib.nestgpu_server_stats(self._h, buf, n.value + 1, ctypes.byref(n)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        return json.loads(buf.value.decode())
 
 
 # ------------------------------------------------------------------ module-level kernel (PyNEST style)

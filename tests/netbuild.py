@@ -139,3 +139,45 @@ def exp_random(N=2000, frac_e=0.8, KE=100, KI=25, T=200.0, seed=3, n_vp=1, drive
                ((0, NE, N), (0, 0, N), "fixed_indegree", KI, ("product", ("normal", 87.8, 8.8), -4.0), delay)],
         drive=[(0, rng.poisson(lam, size=(steps, N)).astype(np.uint8), 87.8, dmin)],
         T=T)
+
+
+AEIF_C4 = {"a": 4.0, "b": 80.5, "tau_w": 144.0, "Delta_T": 2.0, "g_L": 30.0, "gsl_error_tol": 1e-3}
+
+
+def aeif_random(N=1000, KE=80, KI=20, wE=2.0, g=5.0, lam=2.0, wD=6.0, T=100.0, seed=3, n_vp=2, drive_seed=1):
+    """Brunel-style aeif_cond_exp network (batched-sweep neuron parameters), conductances in nS."""
+    NE = int(0.8 * N)
+    rng = np.random.default_rng(drive_seed)
+    steps = int(round(T / 0.1))
+    return dict(
+        n_vp=n_vp, seed=seed,
+        pops=[("aeif_cond_exp", N, dict(AEIF_C4))],
+        v0={0: rng.uniform(-70.6, -55.0, N)},
+        conns=[((0, 0, NE), (0, 0, N), "fixed_indegree", KE, wE, 1.5),
+               ((0, NE, N), (0, 0, N), "fixed_indegree", KI, -g * wE, 1.5)],
+        drive=[(0, rng.poisson(lam, size=(steps, N)).astype(np.uint8), wD, 1.5)],
+        T=T)
+
+
+def mixed_models(T=100.0, seed=12, n_vp=3, drive_seed=5):
+    """iaf_psc_delta, iaf_psc_exp and aeif_cond_exp populations connected to each other."""
+    rng = np.random.default_rng(drive_seed)
+    steps = int(round(T / 0.1))
+    nd, ne, na = 300, 400, 200
+    delta = {"C_m": 1.0, "tau_m": 20.0, "V_th": 20.0, "V_reset": 10.0, "E_L": 0.0, "t_ref": 2.0}
+    exp = {"tau_m": 10.0, "tau_syn_ex": 0.5, "tau_syn_in": 0.5, "C_m": 250.0, "V_th": -55.0, "V_reset": -70.0,
+           "t_ref": 2.0, "E_L": -70.0}
+    return dict(
+        n_vp=n_vp, seed=seed,
+        pops=[("iaf_psc_delta", nd, delta), ("iaf_psc_exp", ne, exp), ("aeif_cond_exp", na, dict(AEIF_C4))],
+        v0={0: rng.uniform(0, 20, nd), 1: rng.uniform(-70, -55, ne), 2: rng.uniform(-70.6, -55, na)},
+        conns=[((0, 0, nd), (0, 0, nd), "fixed_indegree", 30, ("normal", 0.15, 0.05), 1.5),
+               ((0, 0, nd), (1, 0, ne), "fixed_indegree", 20, 60.0, ("uniform", 1.0, 2.0)),
+               ((1, 0, ne), (2, 0, na), "pairwise_bernoulli", 0.05, ("normal", 1.0, 0.3), 1.2),
+               ((2, 0, na), (0, 0, nd), "fixed_indegree", 10, -0.3, 1.0),
+               ((2, 0, na), (1, 0, ne), "fixed_indegree", 10, -150.0, 1.0),
+               ((1, 0, ne), (1, 0, ne), "fixed_indegree", 40, ("normal", 87.8, 8.8), ("uniform", 1.0, 3.0))],
+        drive=[(0, rng.poisson(2.0, size=(steps, nd)).astype(np.uint8), 0.1, 1.0),
+               (1, rng.poisson(0.8, size=(steps, ne)).astype(np.uint8), 87.8, 1.0),
+               (2, rng.poisson(2.0, size=(steps, na)).astype(np.uint8), 6.0, 1.0)],
+        T=T)

@@ -1,0 +1,83 @@
+"""Served form: requests admitted by HBM budget and batched continuously must give exactly
+the results each network has when run alone on the plain path."""
+
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import nestgpu
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(sp):
+    k = nestgpu.Kernel(0)
+    pop = bench.build_instance(k, nestgpu, sp, n_vp=2)
+    sr = k.Create("spike_recorder")
+    k.Connect(pop, sr)
+    return k, pop, sr
+
+
+def _result(pop, sr):
+    ev = sr.get("events")
+    return ev["senders"], ev["times"], np.atleast_1d(pop.get("V_m"))
+
+
+def test_served_equals_alone():
+    specs = bench.c4_specs(8, seed=99, n_min=500, n_max=4000)
+    rng = np.random.default_rng(3)
+    calls = [list(np.round(rng.uniform(5.0, 40.0, size=rng.integers(1, 4)), 1)) for _ in specs]
+    built = [_build(sp) for sp in specs]
+    # a budget for about half of the networks at a time: the rest queue for memory
+    est = [k.GetKernelStatus("device_bytes_estimate") for k, _, _ in built]
+    srv = nestgpu.Server(0, hbm_budget_bytes=max(max(est), sum(est) // 2), slices_per_chunk=8)
+    ids = []
+    for i, ((k, pop, sr), c) in enumerate(zip(built, calls)):
+        ids.append(srv.submit(k, c))
+        time.sleep(0.01 * (i % 3))  # staggered arrivals: some join while others run
+    infos = [srv.wait(r, timeout=120.0) for r in ids]
+    stats = srv.stats()
+    print("server", stats, flush=True)
+    for i, (sp, c) in enumerate(zip(specs, calls)):
+        assert infos[i]["state"] == nestgpu.Server.DONE
+        k, pop, sr = _build(sp)
+        for t in c:
+            k.Simulate(t)
+        a = _result(built[i][1], built[i][2])
+        b = _result(pop, sr)
+        print(sp["model"], sp["N"], c, "spikes", len(a[0]), "latency %.3f s" % (infos[i]["t_done"] - infos[i]["t_submit"]),
+              flush=True)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), "request %d differs from its run alone" % i
+        assert infos[i]["events"] == k.GetKernelStatus("synaptic_events")
+    assert stats["done"] == len(specs)
+    assert stats["max_concurrent"] >= 2
+    srv.close()
+
+
+def test_budget_forces_queueing():
+    specs = bench.c4_specs(4, seed=7, n_min=2000, n_max=3000)
+    built = [_build(sp) for sp in specs]
+    # room for one network at a time: requests run strictly one after another
+    est = [k.GetKernelStatus("device_bytes_estimate") for k, _, _ in built]
+    two_smallest = sorted(est)[0] + sorted(est)[1]
+    assert two_smallest > max(est)
+    srv = nestgpu.Server(0, hbm_budget_bytes=max(est), slices_per_chunk=8)
+    ids = [srv.submit(k, [20.0]) for k, _, _ in built]
+    infos = [srv.wait(r, timeout=120.0) for r in ids]
+    for a, b in zip(infos, infos[1:]):
+        assert b["t_admit"] >= a["t_done"] - 1e-3
+    srv.close()
+
+
+def test_oversized_request_rejected():
+    k, pop, sr = _build(bench.c4_specs(1, seed=1, n_min=5000, n_max=5001)[0])
+    srv = nestgpu.Server(0, hbm_budget_bytes=1 << 20)
+    with pytest.raises(nestgpu.NESTError, match="budget"):
+        srv.submit(k, [10.0])
+    srv.close()
