@@ -204,6 +204,107 @@ def run_batched(args, nestgpu):
     print(json.dumps(line), flush=True)
 
 
+def c5_requests(n, seed, qps):
+    """Served workload: batched-sweep-style networks with N log-uniform in [2k, 100k], T
+    uniform in [100, 2000] ms issued as 1-4 Simulate calls, arriving as a Poisson stream."""
+    rng = np.random.default_rng(seed)
+    specs = c4_specs(n, seed + 1, n_min=2000, n_max=100000)
+    reqs = []
+    t = 0.0
+    for sp in specs:
+        t += rng.exponential(1.0 / qps)
+        T = round(float(rng.uniform(100.0, 2000.0)), 1)
+        nc = int(rng.integers(1, 5))
+        cuts = np.sort(np.round(rng.uniform(0.0, T, size=nc - 1), 1))
+        calls = [float(x) for x in np.diff(np.concatenate([[0.0], cuts, [T]])) if x > 0]
+        reqs.append(dict(spec=sp, arrival=t, calls=calls))
+    return reqs
+
+
+def run_served(args, nestgpu):
+    """Served form: requests arrive open-loop at a fixed rate; a builder pool constructs each
+    network on the host and submits it; the server admits by HBM budget and batches all
+    running requests in the same slice launches. Reported: p50/p99 end-to-end latency and
+    aggregate recurrent events/s, next to the same requests run one at a time on the plain
+    path (measured service times, queued in arrival order)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    reqs = c5_requests(args.requests, args.seed, args.qps)
+    srv = nestgpu.Server(0, hbm_budget_bytes=int(args.hbm_budget_gib * (1 << 30)), slices_per_chunk=16)
+    lock = threading.Lock()
+    results = [None] * len(reqs)
+    t_origin = time.perf_counter()
+
+    def handle(i):
+        r = reqs[i]
+        k = nestgpu.Kernel(0)
+        build_instance(k, nestgpu, r["spec"], keep_host_connections=False)
+        rid = srv.submit(k, r["calls"])
+        info = srv.wait(rid)
+        done = time.perf_counter() - t_origin
+        with lock:
+            results[i] = dict(arrival=r["arrival"], done=done, events=info["events"], server=info)
+        del k
+
+    with ThreadPoolExecutor(args.builders) as ex:
+        futs = []
+        for i, r in enumerate(reqs):
+            delay = r["arrival"] - (time.perf_counter() - t_origin)
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(ex.submit(handle, i))
+        for f in futs:
+            f.result()
+    stats = srv.stats()
+    srv.close()
+    lat = np.array([x["done"] - x["arrival"] for x in results])
+    makespan = max(x["done"] for x in results) - min(x["arrival"] for x in results)
+    events = sum(x["events"] for x in results)
+    value = events / makespan
+    print("served: %d requests, makespan %.1f s, p50 %.2f s, p99 %.2f s, %.3g events/s, max concurrent %d" % (
+        len(reqs), makespan, np.percentile(lat, 50), np.percentile(lat, 99), value, stats["max_concurrent"]),
+        file=sys.stderr, flush=True)
+    # serial plain path: each request alone (host build, then its Simulate calls)
+    service = []
+    for r in reqs:
+        t0 = time.perf_counter()
+        k = nestgpu.Kernel(0)
+        build_instance(k, nestgpu, r["spec"], keep_host_connections=False)
+        for c in r["calls"]:
+            k.Simulate(c)
+        service.append(time.perf_counter() - t0)
+        del k
+    done = 0.0
+    lat_serial = []
+    for r, sv in zip(reqs, service):
+        done = max(done, r["arrival"]) + sv
+        lat_serial.append(done - r["arrival"])
+    lat_serial = np.array(lat_serial)
+    serial_makespan = done - reqs[0]["arrival"]
+    serial_value = events / serial_makespan
+    print("serial plain path: makespan %.1f s, p50 %.2f s, p99 %.2f s, %.3g events/s" % (
+        serial_makespan, np.percentile(lat_serial, 50), np.percentile(lat_serial, 99), serial_value),
+        file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": len(reqs),
+        "warmup": 0, "ms_per_step": 1e3 * makespan / len(reqs), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "config": {"workload": "served: %d requests at %g QPS (Poisson arrivals), batched-sweep-style networks "
+                               "N log-uniform [2k, 100k], T uniform [100, 2000] ms in 1-4 Simulate calls, HBM "
+                               "budget %g GiB, %d host builders" % (len(reqs), args.qps, args.hbm_budget_gib,
+                                                                   args.builders),
+                   "server": stats},
+        "serial_plain_path": {"value": serial_value, "makespan_s": serial_makespan,
+                              "latency_s": {"p50": float(np.percentile(lat_serial, 50)),
+                                            "p99": float(np.percentile(lat_serial, 99))}},
+        "served_vs_serial": value / serial_value,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(N, KE, KI, n_vp, seed, drive_seed, t_sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import nestgpu
@@ -262,13 +363,17 @@ def main():
     ap.add_argument("--cpu-sample-ms", type=float, default=30.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 100k-neuron exponential-synapse random network (default); c3: 77k-neuron "
                          "8-population circuit with min_delay one step; c4: batched sweep of many instances")
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
     ap.add_argument("--plain-ms", type=float, default=100.0, help="c4: biological ms per instance on the plain path")
     ap.add_argument("--autotune", action="store_true", help="c4: pick targets_per_block per size class first")
     ap.add_argument("--probe-ms", type=float, default=30.0, help="c4: autotuner probe length")
+    ap.add_argument("--requests", type=int, default=32, help="c5: number of requests")
+    ap.add_argument("--qps", type=float, default=1.0, help="c5: request arrival rate")
+    ap.add_argument("--builders", type=int, default=6, help="c5: host threads building request networks")
+    ap.add_argument("--hbm-budget-gib", type=float, default=128.0, help="c5: device memory budget of the server")
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     args = ap.parse_args()
@@ -283,10 +388,10 @@ def main():
     if nestgpu.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    if args.workload == "c4":
+    if args.workload in ("c4", "c5"):
         if args.seed == 20241:
             args.seed = 4242
-        run_batched(args, nestgpu)
+        (run_batched if args.workload == "c4" else run_served)(args, nestgpu)
         return
 
     t_build = time.perf_counter()

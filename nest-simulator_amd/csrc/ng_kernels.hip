@@ -55,8 +55,9 @@ struct SliceLds
   uint32_t pref[ ( WIN ? PMAX_WIN : PMAX_GLOB ) + 1 ];  // prefix of per-block spike counts
   uint32_t c_g0[ CH ];       // first synapse of each chunk spike's run
   uint32_t c_off[ CH + 1 ];  // exclusive prefix of run lengths
-  uint32_t bk[ R ];          // event keys, then slot-channel of each event, bucketed by target
-  uint32_t bcnt[ BS ];       // events per target this round
+  uint32_t bk[ R ];          // event keys (slot-channel << 11 | round index), bucketed by target
+  uint16_t perm[ R ];        // bucket position of the event at each sorted position
+  uint32_t bcnt[ 2 ][ BS ];  // events per target, alternating between rounds
   uint32_t bst[ BS + 1 ];    // bucket starts
   uint32_t sh[ BS / 64 ];
   uint32_t cnt_base;
@@ -160,7 +161,8 @@ struct Cells
   }
 };
 
-// spike_generator drive of this part: handle() of the target with weight * multiplicity
+// spike_generator drive of this part: handle() of the target with weight * multiplicity.
+// Each lag adds into a different cell, so the cells of LB lags are read together.
 template < bool WIN >
 __device__ inline void
 drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
@@ -171,35 +173,60 @@ drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t l
   const uint32_t col = d.drive_idx[ dev ];
   if ( col == NONE )
     return;
-  for ( int lag = from; lag < to; ++lag )
+  for ( int l0 = from; l0 < to; l0 += LB )
   {
-    const int64_t e = origin + lag;  // emission step; spike stamp e + 1
-    if ( e < ps.drive_t0 or e >= ps.drive_t0 + ps.drive_steps )
-      continue;
-    uint32_t c;
-    if ( ps.drive_mode == 1 )
-      c = d.drive_counts[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
-    else
-      c = poisson_count( d.ptables[ ps.drive_table ], col, e );
-    if ( c == 0 )
-      continue;
-    const double w = ps.drive_w;
-    const double mult = static_cast< double >( c );
-    const uint32_t at = static_cast< uint32_t >( lag + ps.drive_d );
-    if ( m == M_IAF_PSC_DELTA )
-      *C.at( at, 0, local ) += w * mult;
-    else if ( m == M_IAF_PSC_EXP )
+    double* cp[ LB ];
+    double add[ LB ];
+    bool has[ LB ];
+#pragma unroll
+    for ( int q = 0; q < LB; ++q )
     {
-      const double sv = w * mult;
-      *C.at( at, sv > 0 ? 1 : 0, local ) += sv;
-    }
-    else
-    {
-      if ( w > 0.0 )
-        *C.at( at, 0, local ) += w * mult;
+      has[ q ] = false;
+      const int lag = l0 + q;
+      const int64_t e = origin + lag;  // emission step; spike stamp e + 1
+      if ( lag >= to or e < ps.drive_t0 or e >= ps.drive_t0 + ps.drive_steps )
+        continue;
+      uint32_t c;
+      if ( ps.drive_mode == 1 )
+        c = d.drive_counts[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
       else
-        *C.at( at, 1, local ) += -w * mult;
+        c = poisson_count( d.ptables[ ps.drive_table ], col, e );
+      if ( c == 0 )
+        continue;
+      const double w = ps.drive_w;
+      const double mult = static_cast< double >( c );
+      const uint32_t at = static_cast< uint32_t >( lag + ps.drive_d );
+      has[ q ] = true;
+      if ( m == M_IAF_PSC_DELTA )
+      {
+        cp[ q ] = C.at( at, 0, local );
+        add[ q ] = w * mult;
+      }
+      else if ( m == M_IAF_PSC_EXP )
+      {
+        add[ q ] = w * mult;
+        cp[ q ] = C.at( at, add[ q ] > 0 ? 1 : 0, local );
+      }
+      else if ( w > 0.0 )
+      {
+        cp[ q ] = C.at( at, 0, local );
+        add[ q ] = w * mult;
+      }
+      else
+      {
+        cp[ q ] = C.at( at, 1, local );
+        add[ q ] = -w * mult;
+      }
     }
+    double old[ LB ];
+#pragma unroll
+    for ( int q = 0; q < LB; ++q )
+      if ( has[ q ] )
+        old[ q ] = *cp[ q ];
+#pragma unroll
+    for ( int q = 0; q < LB; ++q )
+      if ( has[ q ] )
+        *cp[ q ] = old[ q ] + add[ q ];
   }
 }
 
@@ -278,7 +305,9 @@ __global__ void __launch_bounds__( BS )
   // ================================================================ 1. delivery
   if ( ph.deliver )
   {
-    S.bcnt[ t ] = 0;
+    S.bcnt[ 0 ][ t ] = 0;
+    S.bcnt[ 1 ][ t ] = 0;
+    int cur = 0;  // counter buffer of the current round
     const bool flat = d.flat_meta[ 2 + ppar ] != 0;
     uint32_t K;
     if ( flat )
@@ -387,12 +416,12 @@ __global__ void __launch_bounds__( BS )
           }
       };
       load_round( 0 );
-      for ( uint32_t e0 = 0; e0 < E; e0 += R )
+      for ( uint32_t e0 = 0; e0 < E; e0 += R, cur ^= 1 )
       {
         // stage: thread t holds events e0 + t*RQ .. + RQ - 1 (key order = round index) and
-        // counts them into its target's bucket
-        uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ];
-        double val[ RQ ];
+        // takes an unordered place in its target's bucket
+        uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ], ci[ RQ ];
+        double val[ RQ ], cv[ RQ ];
         const uint32_t eb = e0 + t * RQ;
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
@@ -416,25 +445,71 @@ __global__ void __launch_bounds__( BS )
             loc[ q ] = local;
             val[ q ] = v;
             key[ q ] = ( ( slot_rel * d.nch + ch ) << IDX_BITS ) | static_cast< uint32_t >( t * RQ + q );
-            dst[ q ] = atomicAdd( &S.bcnt[ local ], 1u );  // an unordered place in the bucket
+            dst[ q ] = atomicAdd( &S.bcnt[ cur ][ local ], 1u );
+            if ( WIN and C.win )
+              ci[ q ] = ( slot_rel * d.nch + ch ) * tb + local;
+            else
+            {
+              uint32_t slot = C.s0 + slot_rel;
+              if ( slot >= C.L )
+                slot -= C.L;
+              ci[ q ] = slot * C.nct + ch * tb + local;
+            }
           }
         }
         __syncthreads();
         NG_PHASE( 2 );
-        const uint32_t mycnt = t < tb ? S.bcnt[ t ] : 0u;
+        // every thread has finished the previous round (its cell stores included): the cells
+        // of this round can be read now, while the buckets are formed
+        if ( not( WIN and C.win ) )
+        {
+#pragma unroll
+          for ( int q = 0; q < RQ; ++q )
+            if ( loc[ q ] != NONE )
+              cv[ q ] = C.rbb[ ci[ q ] ];
+        }
+        // bucket starts: block scan of the per-target counts
+        const uint32_t mycnt = t < tb ? S.bcnt[ cur ][ t ] : 0u;
         uint32_t mybase;
-        const uint32_t tot = block_scan( mycnt, &mybase, S.sh );  // bucket starts
-        if ( t < tb )
-          S.bst[ t ] = mybase;
-        if ( t == 0 )
-          S.bst[ tb ] = tot;
-        __syncthreads();
+        {
+          const int lane = t & 63;
+          const int wid = t >> 6;
+          uint32_t x = mycnt;
+          for ( int off = 1; off < 64; off <<= 1 )
+          {
+            const uint32_t u = __shfl_up( x, off, 64 );
+            if ( lane >= off )
+              x += u;
+          }
+          if ( lane == 63 )
+            S.sh[ wid ] = x;
+          S.bcnt[ cur ^ 1 ][ t ] = 0;  // counters of the next round
+          __syncthreads();
+          uint32_t pre = 0, tot = 0;
+          for ( int q = 0; q < BS / 64; ++q )
+          {
+            if ( q < wid )
+              pre += S.sh[ q ];
+            tot += S.sh[ q ];
+          }
+          mybase = pre + x - mycnt;
+          if ( t < tb )
+            S.bst[ t ] = mybase;
+          if ( t == 0 )
+            S.bst[ tb ] = tot;
+          __syncthreads();
+        }
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
           if ( loc[ q ] != NONE )
-            S.bk[ S.bst[ loc[ q ] ] + dst[ q ] ] = key[ q ];
+          {
+            dst[ q ] += S.bst[ loc[ q ] ];
+            S.bk[ dst[ q ] ] = key[ q ];
+            S.bw[ dst[ q ] ] = val[ q ];
+          }
         __syncthreads();
-        // sorted place of each event in its bucket: the number of smaller keys there
+        // sorted place of each event in its bucket by (cell, key): the number of smaller keys
+        uint32_t srt[ RQ ];
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
           if ( loc[ q ] != NONE )
@@ -443,86 +518,52 @@ __global__ void __launch_bounds__( BS )
             uint32_t r = 0;
             for ( uint32_t i = lo; i < hi; ++i )
               r += S.bk[ i ] < key[ q ] ? 1u : 0u;
-            dst[ q ] = lo + r;
-          }
-        __syncthreads();
-#pragma unroll
-        for ( int q = 0; q < RQ; ++q )
-          if ( loc[ q ] != NONE )
-          {
-            S.bk[ dst[ q ] ] = key[ q ] >> IDX_BITS;
-            S.bw[ dst[ q ] ] = val[ q ];
+            srt[ q ] = lo + r;
+            S.perm[ srt[ q ] ] = static_cast< uint16_t >( dst[ q ] );
           }
         __syncthreads();
         if ( e0 + R < E )
           load_round( e0 + R );
-        // the first event of each run of one cell in a sorted bucket adds the whole run in
-        // key order; runs of different cells proceed in parallel (independent loads first)
-        uint32_t ci[ RQ ];
-        bool lead[ RQ ];
+        // the first event of each cell's run adds the whole run in key order; runs of
+        // different cells proceed in parallel
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
         {
-          lead[ q ] = false;
-          if ( loc[ q ] != NONE )
+          if ( loc[ q ] == NONE )
+            continue;
+          const uint32_t sc = key[ q ] >> IDX_BITS;
+          const uint32_t lo = S.bst[ loc[ q ] ], hi = S.bst[ loc[ q ] + 1 ];
+          if ( srt[ q ] > lo and ( S.bk[ S.perm[ srt[ q ] - 1 ] ] >> IDX_BITS ) == sc )
           {
-            const uint32_t sc = key[ q ] >> IDX_BITS;
-            lead[ q ] = dst[ q ] == S.bst[ loc[ q ] ] or S.bk[ dst[ q ] - 1 ] != sc;
-            if ( WIN and C.win )
-              ci[ q ] = sc * tb + loc[ q ];
-            else
-            {
-              uint32_t slot = C.s0 + ( d.nch == 2 ? ( sc >> 1 ) : sc );
-              if ( slot >= C.L )
-                slot -= C.L;
-              ci[ q ] = slot * C.nct + ( d.nch == 2 ? ( sc & 1u ) : 0u ) * tb + loc[ q ];
-            }
+            loc[ q ] = NONE;  // not the first of its run
+            continue;
           }
+          double v = ( WIN and C.win ) ? S.win[ ci[ q ] ] : cv[ q ];
+          v = v + val[ q ];
+          for ( uint32_t i = srt[ q ] + 1; i < hi; ++i )
+          {
+            const uint32_t pi = S.perm[ i ];
+            if ( ( S.bk[ pi ] >> IDX_BITS ) != sc )
+              break;
+            v = v + S.bw[ pi ];
+          }
+          cv[ q ] = v;
         }
-        double cv[ RQ ];
         if ( WIN and C.win )
         {
 #pragma unroll
           for ( int q = 0; q < RQ; ++q )
-            if ( lead[ q ] )
-              cv[ q ] = S.win[ ci[ q ] ];
-        }
-        else
-        {
-#pragma unroll
-          for ( int q = 0; q < RQ; ++q )
-            if ( lead[ q ] )
-              cv[ q ] = C.rbb[ ci[ q ] ];
-        }
-#pragma unroll
-        for ( int q = 0; q < RQ; ++q )
-          if ( lead[ q ] )
-          {
-            const uint32_t sc = key[ q ] >> IDX_BITS;
-            const uint32_t hi = S.bst[ loc[ q ] + 1 ];
-            double v = cv[ q ];
-            for ( uint32_t i = dst[ q ]; i < hi and S.bk[ i ] == sc; ++i )
-              v = v + S.bw[ i ];
-            cv[ q ] = v;
-          }
-        if ( WIN and C.win )
-        {
-#pragma unroll
-          for ( int q = 0; q < RQ; ++q )
-            if ( lead[ q ] )
+            if ( loc[ q ] != NONE )
               S.win[ ci[ q ] ] = cv[ q ];
         }
         else
         {
 #pragma unroll
           for ( int q = 0; q < RQ; ++q )
-            if ( lead[ q ] )
+            if ( loc[ q ] != NONE )
               C.rbb[ ci[ q ] ] = cv[ q ];
         }
         NG_PHASE( 3 );
-        if ( t < tb )
-          S.bcnt[ t ] = 0;
-        __syncthreads();
       }
     }
     if ( t == 0 and events )
