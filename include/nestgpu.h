@@ -91,7 +91,7 @@ int nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms );
 
 /* Layout autotuner of the batched form (no counterpart in the reference): groups the kernels
  * (none simulated yet) into size classes by neuron count (one per octave), times probe runs of
- * probe_ms on up to 8 instances of each class for every candidate targets_per_block, and sets
+ * probe_ms on up to 4 instances of each class for every candidate targets_per_block, and sets
  * the fastest on every instance of the class (global delivery). Probe runs are discarded: every
  * kernel returns to its state before the first Prepare. chosen[i] (may be NULL) receives the
  * value picked for kernel i. */

@@ -301,7 +301,7 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
     {
       const std::vector< int >& members = kv.second;
       std::vector< int > sample;
-      const size_t ns = std::min< size_t >( 8, members.size() );
+      const size_t ns = std::min< size_t >( 4, members.size() );
       for ( size_t q = 0; q < ns; ++q )
         sample.push_back( members[ q * members.size() / ns ] );
       int best = -1;
