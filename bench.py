@@ -370,6 +370,9 @@ def main():
     ap.add_argument("--plain-ms", type=float, default=100.0, help="c4: biological ms per instance on the plain path")
     ap.add_argument("--autotune", action="store_true", help="c4: pick targets_per_block per size class first")
     ap.add_argument("--probe-ms", type=float, default=30.0, help="c4: autotuner probe length")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 512 MiB three times (scale for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--requests", type=int, default=32, help="c5: number of requests")
     ap.add_argument("--qps", type=float, default=1.0, help="c5: request arrival rate")
     ap.add_argument("--builders", type=int, default=6, help="c5: host threads building request networks")
@@ -377,6 +380,19 @@ def main():
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 64 * 1024 * 1024  # float64 elements: 512 MiB
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copy bytes per launch: read %d, written %d" % (n * 8, n * 8), flush=True)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the step "

@@ -36,32 +36,43 @@ namespace nestgpu
 {
 
 constexpr int BS = static_cast< int >( MAX_TB );  // threads per workgroup = max targets per block
-constexpr int SPT = 4;                            // spikes per thread per chunk
-constexpr int CH = SPT * BS;                      // spikes per chunk
-constexpr int RQ = 4;                             // events per thread per round
-constexpr int R = RQ * BS;                        // events per round
 constexpr int IDX_BITS = 11;  // bucket key = slot-channel << 11 | round index: sorted keys group
                               // a target's events by cell, in key order within each cell
+
+// Tile shape per kernel variant. The window kernel halves chunk and round so that, with its
+// ring-buffer window, two workgroups fit in the 160 KB of LDS of a CU.
+template < bool WIN >
+struct Tile
+{
+  static constexpr int SPT = WIN ? 2 : 4;  // spikes per thread per chunk
+  static constexpr int CH = SPT * BS;      // spikes per chunk
+  static constexpr int RQ = WIN ? 2 : 4;   // events per thread per round
+  static constexpr int R = RQ * BS;        // events per round
+  static constexpr uint32_t WINDOW = WIN ? WIN_CAP : 1;
+  static constexpr uint32_t PMAX = WIN ? PMAX_WIN : PMAX_GLOB;
+};
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr int LB = 8;  // lags of ring-buffer input loaded together in the update
-static_assert( R <= ( 1 << IDX_BITS ), "round index must fit the bucket key" );
+static_assert( Tile< false >::R <= ( 1 << IDX_BITS ) and Tile< true >::R <= ( 1 << IDX_BITS ),
+  "round index must fit the bucket key" );
 
 template < bool WIN >
 struct SliceLds
 {
-  double win[ WIN ? WIN_CAP : 1 ];  // ring-buffer window [slot - origin][channel][tb]
-  double bw[ R ];                   // event values, bucketed by target
+  using T = Tile< WIN >;
+  double win[ T::WINDOW ];  // ring-buffer window [slot - origin][channel][tb]
+  double bw[ T::R ];        // event values, bucketed by target
   unsigned long long log_base;
-  uint32_t pref[ ( WIN ? PMAX_WIN : PMAX_GLOB ) + 1 ];  // prefix of per-block spike counts
-  uint32_t c_g0[ CH ];       // first synapse of each chunk spike's run
-  uint32_t c_off[ CH + 1 ];  // exclusive prefix of run lengths
-  uint32_t bk[ R ];          // event keys (slot-channel << 11 | round index), bucketed by target
-  uint16_t perm[ R ];        // bucket position of the event at each sorted position
-  uint32_t bcnt[ 2 ][ BS ];  // events per target, alternating between rounds
-  uint32_t bst[ BS + 1 ];    // bucket starts
+  uint32_t pref[ T::PMAX + 1 ];  // prefix of per-block spike counts
+  uint32_t c_g0[ T::CH ];        // first synapse of each chunk spike's run
+  uint32_t c_off[ T::CH + 1 ];   // exclusive prefix of run lengths
+  uint32_t bk[ T::R ];           // event keys (slot-channel << 11 | round index), bucketed by target
+  uint16_t perm[ T::R ];         // bucket position of the event at each sorted position
+  uint32_t bcnt[ 2 ][ BS ];      // events per target, alternating between rounds
+  uint32_t bst[ BS + 1 ];        // bucket starts
   uint32_t sh[ BS / 64 ];
   uint32_t cnt_base;
-  uint8_t c_lag[ CH ];
+  uint8_t c_lag[ T::CH ];
 };
 
 // XCD-aware block mapping: workgroups are dispatched round-robin over 8 XCDs; give each
@@ -260,6 +271,10 @@ __global__ void __launch_bounds__( BS )
   unsigned long long t_phase_ = wall_clock64();
 #endif
   __shared__ SliceLds< WIN > S;
+  constexpr int SPT = Tile< WIN >::SPT;
+  constexpr int CH = Tile< WIN >::CH;
+  constexpr int RQ = Tile< WIN >::RQ;
+  constexpr int R = Tile< WIN >::R;
 
   // ---- which instance and block
   const uint32_t gb = xcd_block( blockIdx.x, gridDim.x );
@@ -458,7 +473,7 @@ __global__ void __launch_bounds__( BS )
           }
         }
         __syncthreads();
-        NG_PHASE( 2 );
+        NG_PHASE( 2 );  // stage
         // every thread has finished the previous round (its cell stores included): the cells
         // of this round can be read now, while the buckets are formed
         if ( not( WIN and C.win ) )
@@ -499,6 +514,7 @@ __global__ void __launch_bounds__( BS )
             S.bst[ tb ] = tot;
           __syncthreads();
         }
+        NG_PHASE( 3 );  // bucket starts
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
           if ( loc[ q ] != NONE )
@@ -516,12 +532,20 @@ __global__ void __launch_bounds__( BS )
           {
             const uint32_t lo = S.bst[ loc[ q ] ], hi = S.bst[ loc[ q ] + 1 ];
             uint32_t r = 0;
-            for ( uint32_t i = lo; i < hi; ++i )
+            uint32_t i = lo;
+            for ( ; i + 4 <= hi; i += 4 )
+            {
+              const uint32_t k0 = S.bk[ i ], k1 = S.bk[ i + 1 ], k2 = S.bk[ i + 2 ], k3 = S.bk[ i + 3 ];
+              r += ( k0 < key[ q ] ? 1u : 0u ) + ( k1 < key[ q ] ? 1u : 0u ) + ( k2 < key[ q ] ? 1u : 0u )
+                + ( k3 < key[ q ] ? 1u : 0u );
+            }
+            for ( ; i < hi; ++i )
               r += S.bk[ i ] < key[ q ] ? 1u : 0u;
             srt[ q ] = lo + r;
             S.perm[ srt[ q ] ] = static_cast< uint16_t >( dst[ q ] );
           }
         __syncthreads();
+        NG_PHASE( 4 );  // place and rank
         if ( e0 + R < E )
           load_round( e0 + R );
         // the first event of each cell's run adds the whole run in key order; runs of
@@ -563,14 +587,13 @@ __global__ void __launch_bounds__( BS )
             if ( loc[ q ] != NONE )
               C.rbb[ ci[ q ] ] = cv[ q ];
         }
-        NG_PHASE( 3 );
+        NG_PHASE( 5 );  // apply
       }
     }
     if ( t == 0 and events )
       atomicAdd( &d.counters[ 1 ], events );
   }
   __syncthreads();  // window staged and delivered
-  NG_PHASE( 4 );
 
   // ================================================================ 2. update
   const uint32_t local = t;
@@ -736,7 +759,7 @@ __global__ void __launch_bounds__( BS )
     }
     drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, ph.to );
   }
-  NG_PHASE( 5 );
+  NG_PHASE( 6 );
 
   // ================================================================ 3. spike lists
   const uint32_t nsp = __popcll( bits[ 0 ] ) + __popcll( bits[ 1 ] ) + __popcll( bits[ 2 ] ) + __popcll( bits[ 3 ] );
@@ -796,7 +819,7 @@ __global__ void __launch_bounds__( BS )
       }
     }
   }
-  NG_PHASE( 6 );
+  NG_PHASE( 7 );
 }
 
 #ifdef NG_PROFILE

@@ -1379,7 +1379,15 @@ Kernel::estimate_device_bytes() const
   const int nch = all_delta ? 1 : 2;
   uint32_t tb = 0;
   bool win = false;
-  choose_layout( static_cast< uint32_t >( N ), static_cast< int >( dmax ), nch, tb, win );
+  try
+  {
+    choose_layout( static_cast< uint32_t >( N ), static_cast< int >( dmax ), nch, tb, win );
+  }
+  catch ( const std::exception& )
+  {
+    // a forced window layout that does not fit fails at Prepare; estimate the global one
+    tb = std::max< uint32_t >( 32, std::min< uint32_t >( MAX_TB, static_cast< uint32_t >( ( N + 511 ) / 512 ) ) );
+  }
   const uint64_t P = ( N + tb - 1 ) / tb;
   const uint64_t L = dmin + dmax;
   uint64_t b = static_cast< uint64_t >( E ) * sizeof( SynRec ) + N * ( P + 1 ) * 4  // store
