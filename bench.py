@@ -379,6 +379,7 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=128.0, help="c5: device memory budget of the server")
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
+    ap.add_argument("--workgroups-per-cu", type=int, default=0, help="0: as many as fit")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -412,7 +413,8 @@ def main():
 
     t_build = time.perf_counter()
     k = nestgpu.Kernel(0)
-    k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block})
+    k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block,
+                       "workgroups_per_cu": args.workgroups_per_cu})
     if args.workload == "c2":
         pop, sg = build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
         n_neurons = args.n
@@ -486,7 +488,8 @@ def main():
         workload_text = ("8-population layered circuit: 77169 iaf_psc_exp, pairwise_bernoulli (%d synapses), normal "
                          "delays clipped at dt, Poisson background 8 Hz x K_ext, dt 0.1 ms, one step = Run(%g ms)"
                          % (n_syn, args.t_step))
-    default_layout = args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workload == "c2"
+    default_layout = (args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workgroups_per_cu == 0
+                      and args.workload == "c2")
     line = {
         "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",

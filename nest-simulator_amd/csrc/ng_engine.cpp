@@ -299,7 +299,8 @@ Runner::ensure_capacity( size_t n )
 void
 Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
 {
-  if ( exec_ and exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win )
+  if ( exec_ and exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win
+       and exec_wg_ == wg_per_cu )
     return;
   if ( exec_ )
   {
@@ -309,7 +310,8 @@ Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
   hipGraph_t g;
   hip_check( hipStreamBeginCapture( stream_, hipStreamCaptureModeThreadLocal ), "hipStreamBeginCapture" );
   for ( int s = 0; s < S_; ++s )
-    launch_slice( d_nets_, d_prefix_, n_nets, total_blocks, mask, win, s, stream_ );
+    launch_slice( d_nets_, d_prefix_, n_nets, total_blocks, mask, win, s, slice_lds_for_occupancy( wg_per_cu, win ),
+      stream_ );
   hip_check( hipStreamEndCapture( stream_, &g ), "hipStreamEndCapture" );
   hip_check( hipGraphInstantiate( &exec_, g, nullptr, nullptr, 0 ), "hipGraphInstantiate" );
   hip_check( hipGraphDestroy( g ), "hipGraphDestroy" );
@@ -317,6 +319,7 @@ Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
   exec_total_ = total_blocks;
   exec_mask_ = mask;
   exec_win_ = win;
+  exec_wg_ = wg_per_cu;
 }
 
 void
@@ -324,7 +327,8 @@ Runner::launch_one( int n, uint32_t total, int mask, bool win, int s, hipEvent_t
 {
   if ( ev )
     hip_check( hipEventRecord( ev[ 0 ], stream_ ), "hipEventRecord" );
-  hip_check( launch_slice( d_nets_, d_prefix_, n, total, mask, win, s, stream_ ), "launch_slice" );
+  hip_check( launch_slice( d_nets_, d_prefix_, n, total, mask, win, s, slice_lds_for_occupancy( wg_per_cu, win ), stream_ ),
+    "launch_slice" );
   if ( ev )
   {
     hip_check( hipEventRecord( ev[ 1 ], stream_ ), "hipEventRecord" );

@@ -17,7 +17,9 @@ namespace nestgpu
 // mask: bit m set if neuron model m occurs (selects a specialised kernel); win: some
 // instance uses window delivery (selects the kernel with the LDS ring-buffer window).
 hipError_t launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask,
-  bool win, int s, hipStream_t st );
+  bool win, int s, size_t dyn_lds, hipStream_t st );
+// Dynamic LDS bytes that cap the slice kernel at wg_per_cu workgroups per CU (0: no cap).
+size_t slice_lds_for_occupancy( int wg_per_cu, bool win );
 // Rebuild the exact receive order of a slice that was split across Run calls.
 hipError_t launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st );
 // out[2 i] = error flags, out[2 i + 1] = spike-log fill of instance i
@@ -173,6 +175,7 @@ public:
     return stream_;
   }
   bool use_graph = true;
+  int wg_per_cu = 0;  // resident slice workgroups per CU (0: as many as fit)
   // per-kernel timing: one slice at a time with HIP events around every launch
   bool time_kernels = false;
   double kernel_ms[ N_KINDS ] = { 0, 0 };
@@ -198,6 +201,7 @@ private:
   uint32_t exec_total_ = 0;
   int exec_mask_ = 0;
   bool exec_win_ = false;
+  int exec_wg_ = 0;
   std::vector< uint32_t > prefix_;
   double last_ms_ = 0;
 };

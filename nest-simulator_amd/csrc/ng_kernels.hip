@@ -879,33 +879,50 @@ k_flatten( NetDev* net, int par, int n_parts )
 template < bool WIN >
 static void
 launch_mask( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, int s,
-  hipStream_t st )
+  size_t dyn_lds, hipStream_t st )
 {
   switch ( mask )
   {
   case 1:
-    k_slice< 1, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 1, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   case 2:
-    k_slice< 2, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 2, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   case 4:
-    k_slice< 4, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 4, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   default:
-    k_slice< 7, WIN ><<< total_blocks, BS, 0, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 7, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   }
 }
 
+// Dynamic LDS that limits the slice kernel to wg_per_cu resident workgroups per CU (0: no
+// limit): fewer resident blocks keep the ring buffers they update within the XCD's L2.
+size_t
+slice_lds_for_occupancy( int wg_per_cu, bool win )
+{
+  if ( wg_per_cu <= 0 )
+    return 0;
+  hipFuncAttributes a;
+  const hipError_t e = win ? hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< 7, true > ) )
+                           : hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< 7, false > ) );
+  if ( e != hipSuccess )
+    return 0;
+  const size_t lds_per_cu = 160 * 1024;
+  const size_t want = lds_per_cu / static_cast< size_t >( wg_per_cu + 1 ) + 256;  // more than 1/(k+1)
+  return want > a.sharedSizeBytes ? want - a.sharedSizeBytes : 0;
+}
+
 hipError_t
 launch_slice( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, bool win,
-  int s, hipStream_t st )
+  int s, size_t dyn_lds, hipStream_t st )
 {
   if ( win )
-    launch_mask< true >( nets, blk_prefix, n_nets, total_blocks, mask, s, st );
+    launch_mask< true >( nets, blk_prefix, n_nets, total_blocks, mask, s, dyn_lds, st );
   else
-    launch_mask< false >( nets, blk_prefix, n_nets, total_blocks, mask, s, st );
+    launch_mask< false >( nets, blk_prefix, n_nets, total_blocks, mask, s, dyn_lds, st );
   return hipGetLastError();
 }
 

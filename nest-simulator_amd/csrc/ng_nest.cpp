@@ -285,6 +285,7 @@ Kernel::reset()
   spike_count_ = 0;
   targets_per_block_ = 0;
   delivery_mode_ = 0;
+  wg_per_cu_ = 0;
   keep_host_connections_ = true;
   host_connections_freed_ = false;
   released_ = false;
@@ -363,6 +364,13 @@ Kernel::set_kernel_status( const Json& d )
       delivery_mode_ = m;
       store_dirty_ = true;
     }
+    else if ( k == "workgroups_per_cu" )
+    {
+      const int v = static_cast< int >( kv.second.number() );
+      if ( v < 0 or v > 8 )
+        throw std::runtime_error( "BadProperty: workgroups_per_cu must be in [0, 8] (0 = as many as fit)" );
+      wg_per_cu_ = v;
+    }
     else if ( k == "keep_host_connections" )
       keep_host_connections_ = kv.second.number() != 0;
     else if ( k == "kernel_timing" )
@@ -407,7 +415,7 @@ Kernel::get_kernel_status_json() const
     << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? engine_->net().tb : 0 )
     << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
-    << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
+    << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
   static const char* kn[] = { "slice", "flatten" };
   for ( int q = 0; q < N_KINDS; ++q )
     o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << time_kernel_ms_[ q ];
@@ -1476,6 +1484,7 @@ Kernel::run( double t_ms )
     runner_ = own_runner_.get();
   }
   runner_->time_kernels = time_kernels_;
+  runner_->wg_per_cu = wg_per_cu_;
   double km0[ N_KINDS ];
   int64_t kl0[ N_KINDS ];
   for ( int q = 0; q < N_KINDS; ++q )

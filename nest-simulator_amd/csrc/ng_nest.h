@@ -360,6 +360,7 @@ private:
   uint64_t spike_count_ = 0;
   double last_kernel_ms_ = 0, last_wall_ms_ = 0;
   int slices_per_chunk_ = 64;
+  int wg_per_cu_ = 0;
   bool time_kernels_ = false;
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };

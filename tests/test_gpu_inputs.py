@@ -163,3 +163,51 @@ def test_full_size_layout_invariance_and_repeatability():
         assert other[3] == ref[3], layout
         assert np.array_equal(other[0], ref[0]) and np.array_equal(other[1], ref[1]), layout
         assert np.array_equal(other[2], ref[2]), layout
+
+
+def test_brunel_full_size_full_run():
+    # 5,000 iaf_psc_delta (4,000 E / 1,000 I), in-degree 400 E + 100 I, J = 0.1 mV, g = 5,
+    # delay 1.5 ms, Poisson drive eta = 2 as pre-generated counts, T = 1000 ms: the whole spike
+    # train and every V_m identical to the oracle
+    desc = nb.brunel_delta(N=5000, K=500, T=1000.0, seed=143, n_vp=4, drive_seed=2024)
+    k, pops, sr = nb.build_gpu(desc)
+    o, opops = nb.build_oracle(desc, threads=8)
+    k.Simulate(1000.0)
+    o.simulate(1000.0)
+    ev = sr.get("events")
+    og, os_ = nb.sorted_spikes_oracle(o)
+    print("Brunel full run: gpu %d spikes, oracle %d" % (len(ev["senders"]), len(og)), flush=True)
+    assert len(og) > 10000
+    assert np.array_equal(ev["senders"], og)
+    assert np.allclose(ev["times"], os_ * 0.1, rtol=0, atol=1e-9)
+    assert np.array_equal(np.atleast_1d(pops[0].get("V_m")), o.get(opops[0], "V_m"))
+
+
+def test_exponential_network_full_size_against_oracle():
+    # the benchmark network itself (100,000 iaf_psc_exp, 1.25e8 synapses, 8 kHz seeded drive)
+    # for 200 ms: every spike and every V_m identical to the oracle
+    T = 200.0
+    k = nestgpu.Kernel(0)
+    pop, sg = bench.build_c2(k, nestgpu, 100000, 1000, 250, 8, 20241, 777)
+    sr = k.Create("spike_recorder")
+    k.Connect(pop, sr)
+    k.Simulate(T)
+    ev = sr.get("events")
+    print("gpu: %d spikes" % len(ev["senders"]), flush=True)
+    NE = 80000
+    o = Oracle(n_vp=8, seed=20241, threads=16)
+    g = o.create("iaf_psc_exp", 100000, bench.NEURON)
+    rng = np.random.default_rng(777)
+    o.set_values(g, "V_m", rng.uniform(-70.0, -55.0, 100000))
+    delay = ("sum", ("product", ("uniform_int", 21), 0.1), 1.0)
+    o.connect(g[:NE], g, "fixed_indegree", 1000, weight=("normal", 87.8, 8.8), delay=delay)
+    o.connect(g[NE:], g, "fixed_indegree", 250, weight=("product", ("normal", 87.8, 8.8), -4.0), delay=delay)
+    counts = nestgpu.poisson_counts(777, 8000.0 * 0.1e-3, sg.tolist()[0], 100000, 0, int(T / 0.1))
+    o.add_drive(g[0], counts, 87.8, 1.0)
+    o.simulate(T)
+    og, os_ = nb.sorted_spikes_oracle(o)
+    print("oracle: %d spikes" % len(og), flush=True)
+    assert len(og) > 100000
+    assert np.array_equal(ev["senders"], og)
+    assert np.allclose(ev["times"], os_ * 0.1, rtol=0, atol=1e-9)
+    assert np.array_equal(np.atleast_1d(pop.get("V_m")), o.get(g, "V_m"))
