@@ -97,6 +97,16 @@ Kernel::build_spec( BuildSpec& b )
   b.nch = all_delta ? 1 : 2;
   choose_layout( N, b.max_delay, b.nch, b.tb, b.win );
   const uint32_t P = b.n_blocks();
+  // Global delivery can run in slot-range passes (delivery_passes), each touching a smaller
+  // part of the ring. Finding each spike's split points costs dependent loads that measured
+  // more than the smaller working set saves on the MI355X, so one pass is the default.
+  b.np = 1;
+  b.ws = b.max_delay;
+  if ( not b.win and delivery_passes_ > 1 )
+  {
+    b.np = std::max( 1, std::min( delivery_passes_, b.max_delay ) );
+    b.ws = ( b.max_delay + b.np - 1 ) / b.np;
+  }
   // ---- device index = position in NEST's receive order (VP, then node id)
   std::vector< uint32_t > order( N );
   for ( uint32_t i = 0; i < N; ++i )
@@ -409,6 +419,15 @@ Kernel::build_spec( BuildSpec& b )
         rec.w_lo = static_cast< uint32_t >( wbits );
         rec.w_hi = static_cast< uint32_t >( wbits >> 32 );
         rec.word = ( t - blk * tbs ) | ( static_cast< uint32_t >( conns_[ v ].d[ i ] - min_delay_ ) << 16 );
+      }
+      // each (source, block) run ordered by delay, creation order among equal delays: a cell
+      // (target, slot) still receives a spike's weights in creation order
+      for ( uint32_t s = s_lo; s < s_hi; ++s )
+      {
+        const uint32_t* row = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
+        for ( uint32_t q = 0; q < P; ++q )
+          std::stable_sort( b.syn.begin() + row[ q ], b.syn.begin() + row[ q + 1 ],
+            []( const SynRec& x, const SynRec& y ) { return ( x.word >> 16 ) < ( y.word >> 16 ); } );
       }
     } );
   // spikes are logged only while a spike_recorder observes some neuron

@@ -57,7 +57,8 @@ struct PoissonTable
 };
 
 // one static synapse: FP64 weight split in two words (records are 12 bytes, 4-byte aligned)
-// and local target | (delay - min_delay) << 16
+// and local target | (delay - min_delay) << 16. Within a (source, block) run, records are
+// ordered by delay, ties in creation order.
 struct SynRec
 {
   uint32_t w_lo, w_hi;
@@ -65,11 +66,12 @@ struct SynRec
 };
 
 // ring-buffer slots staged in LDS per workgroup and slice (window delivery), in doubles;
-// sized so that two window workgroups fit in a CU's LDS
-constexpr uint32_t WIN_CAP = 5888;
+// the rest of the window kernel's LDS and this fill one CU's 160 KB
+constexpr uint32_t WIN_CAP = 12416;
 constexpr uint32_t MAX_TB = 512;      // targets per block = threads per workgroup
 constexpr uint32_t PMAX_WIN = 1024;   // blocks per instance, window delivery
 constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
+constexpr int MAX_PASSES = 3;         // slot-range passes of global delivery
 
 struct NetDev
 {
@@ -84,6 +86,8 @@ struct NetDev
   uint32_t spk_cap;       // per-block spike list capacity (tb * min_delay)
   int32_t nch;            // ring-buffer channels per slot: 1 (iaf_psc_delta only) or 2
   int32_t win;            // 1: the max_delay slots a slice touches are staged in LDS
+  int32_t np;             // global delivery: passes over slot ranges per chunk (1..MAX_PASSES)
+  int32_t ws;             // slots per pass (np * ws >= max_delay)
   // ---- state by device index
   double* v[ 5 ];     // delta: y3 | exp: V, I_ex, I_in | aeif: V, g_ex, g_in, w, IntegrationStep
   int32_t* r;

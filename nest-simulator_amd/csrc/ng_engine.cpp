@@ -99,6 +99,10 @@ Engine::build( const BuildSpec& b, bool keep_state )
   net_.spk_cap = b.tb * static_cast< uint32_t >( b.min_delay );
   net_.nch = b.nch;
   net_.win = b.win ? 1 : 0;
+  if ( b.np < 1 or b.np > MAX_PASSES or static_cast< int64_t >( b.np ) * b.ws < b.max_delay or ( b.win and b.np != 1 ) )
+    throw std::runtime_error( "invalid delivery passes" );
+  net_.np = b.np;
+  net_.ws = b.ws;
   net_.log_enabled = b.log_enabled ? 1 : 0;
   mask_ = 0;
   for ( const auto& ps : b.psets )

@@ -35,19 +35,19 @@
 namespace nestgpu
 {
 
-constexpr int BS = static_cast< int >( MAX_TB );  // threads per workgroup = max targets per block
 constexpr int IDX_BITS = 11;  // bucket key = slot-channel << 11 | round index: sorted keys group
                               // a target's events by cell, in key order within each cell
 
-// Tile shape per kernel variant. The window kernel halves chunk and round so that, with its
-// ring-buffer window, two workgroups fit in the 160 KB of LDS of a CU.
+// Tile shape per kernel variant. The window kernel runs one 1024-thread workgroup per CU:
+// its ring-buffer window takes most of the CU's 160 KB of LDS, and 16 waves hide latency.
 template < bool WIN >
 struct Tile
 {
-  static constexpr int SPT = WIN ? 2 : 4;  // spikes per thread per chunk
-  static constexpr int CH = SPT * BS;      // spikes per chunk
-  static constexpr int RQ = WIN ? 2 : 4;   // events per thread per round
-  static constexpr int R = RQ * BS;        // events per round
+  static constexpr int BS = WIN ? 1024 : static_cast< int >( MAX_TB );  // threads per workgroup
+  static constexpr int SPT = WIN ? 2 : 4;      // spikes per thread per chunk
+  static constexpr int CH = SPT * BS;          // spikes per chunk
+  static constexpr int RQ = WIN ? 2 : 4;       // events per thread per round
+  static constexpr int R = RQ * BS;            // events per round
   static constexpr uint32_t WINDOW = WIN ? WIN_CAP : 1;
   static constexpr uint32_t PMAX = WIN ? PMAX_WIN : PMAX_GLOB;
 };
@@ -68,9 +68,9 @@ struct SliceLds
   uint32_t c_off[ T::CH + 1 ];   // exclusive prefix of run lengths
   uint32_t bk[ T::R ];           // event keys (slot-channel << 11 | round index), bucketed by target
   uint16_t perm[ T::R ];         // bucket position of the event at each sorted position
-  uint32_t bcnt[ 2 ][ BS ];      // events per target, alternating between rounds
-  uint32_t bst[ BS + 1 ];        // bucket starts
-  uint32_t sh[ BS / 64 ];
+  uint32_t bcnt[ 2 ][ T::BS ];   // events per target, alternating between rounds
+  uint32_t bst[ T::BS + 1 ];     // bucket starts
+  uint32_t sh[ T::BS / 64 ];
   uint32_t cnt_base;
   uint8_t c_lag[ T::CH ];
 };
@@ -102,7 +102,26 @@ upper_bound_u32( const uint32_t* a, uint32_t n, uint32_t v )
   return lo;
 }
 
+// number of records of a delay-ordered run [g0, g0 + len) whose delay offset is below t
+__device__ inline uint32_t
+run_split( const SynRec* __restrict__ syn, uint32_t g0, uint32_t len, int t )
+{
+  if ( t <= 0 )
+    return 0;
+  uint32_t lo = 0, hi = len;
+  while ( lo < hi )
+  {
+    const uint32_t mid = ( lo + hi ) >> 1;
+    if ( static_cast< int >( syn[ g0 + mid ].word >> 16 ) < t )
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // block-wide exclusive scan of one value per thread; returns the total
+template < int BS >
 __device__ inline uint32_t
 block_scan( uint32_t v, uint32_t* excl, uint32_t* sh )
 {
@@ -264,13 +283,14 @@ __device__ unsigned long long g_phase_ticks[ 8 ];
 #endif
 
 template < int MASK, bool WIN >
-__global__ void __launch_bounds__( BS )
+__global__ void __launch_bounds__( Tile< WIN >::BS )
   k_slice( const NetDev* __restrict__ nets, const uint32_t* __restrict__ blk_prefix, int n_nets, int s )
 {
 #ifdef NG_PROFILE
   unsigned long long t_phase_ = wall_clock64();
 #endif
   __shared__ SliceLds< WIN > S;
+  constexpr int BS = Tile< WIN >::BS;
   constexpr int SPT = Tile< WIN >::SPT;
   constexpr int CH = Tile< WIN >::CH;
   constexpr int RQ = Tile< WIN >::RQ;
@@ -336,7 +356,7 @@ __global__ void __launch_bounds__( BS )
       {
         const uint32_t v = p0 + t < P ? cnt[ p0 + t ] : 0;
         uint32_t ex;
-        const uint32_t tot = block_scan( v, &ex, S.sh );
+        const uint32_t tot = block_scan< BS >( v, &ex, S.sh );
         if ( p0 + t < P )
           S.pref[ p0 + t ] = carry + ex;
         carry += tot;
@@ -382,25 +402,50 @@ __global__ void __launch_bounds__( BS )
           }
         }
       }
+      uint32_t g0v[ SPT ], lenv[ SPT ], s1v[ SPT ], s2v[ SPT ];
+#pragma unroll
+      for ( int q = 0; q < SPT; ++q )
+      {
+        lenv[ q ] = 0;
+        if ( kb + q < K )
+        {
+          const uint32_t* bo = d.boff + static_cast< size_t >( src[ q ] ) * ( P + 1 ) + b;
+          g0v[ q ] = bo[ 0 ];
+          lenv[ q ] = bo[ 1 ] - g0v[ q ];
+          S.c_lag[ t * SPT + q ] = static_cast< uint8_t >( lag[ q ] );
+        }
+      }
+      // Slot-range passes: pass p takes the events landing in slots [p ws, (p + 1) ws), i.e.
+      // delay offsets [p ws - lag, (p + 1) ws - lag): a contiguous part of the delay-ordered
+      // run, split at s1 (and s2 for a third pass).
+#pragma unroll
+      for ( int q = 0; q < SPT; ++q )
+      {
+        s1v[ q ] = lenv[ q ];
+        s2v[ q ] = lenv[ q ];
+        if ( d.np > 1 and lenv[ q ] > 0 )
+        {
+          s1v[ q ] = run_split( d.syn, g0v[ q ], lenv[ q ], d.ws - static_cast< int >( lag[ q ] ) );
+          if ( d.np > 2 )
+            s2v[ q ] = run_split( d.syn, g0v[ q ], lenv[ q ], 2 * d.ws - static_cast< int >( lag[ q ] ) );
+        }
+      }
+      for ( int pass = 0; pass < d.np; ++pass )
+      {
       uint32_t lens[ SPT ];
       uint32_t mine = 0;
 #pragma unroll
       for ( int q = 0; q < SPT; ++q )
       {
-        lens[ q ] = 0;
-        const uint32_t j = t * SPT + q;
-        if ( kb + q < K )
-        {
-          const uint32_t* bo = d.boff + static_cast< size_t >( src[ q ] ) * ( P + 1 ) + b;
-          const uint32_t g0 = bo[ 0 ];
-          lens[ q ] = bo[ 1 ] - g0;
-          S.c_g0[ j ] = g0;
-          S.c_lag[ j ] = static_cast< uint8_t >( lag[ q ] );
-        }
+        const uint32_t lo = pass == 0 ? 0u : ( pass == 1 ? s1v[ q ] : s2v[ q ] );
+        const uint32_t hi = pass == d.np - 1 ? lenv[ q ] : ( pass == 0 ? s1v[ q ] : s2v[ q ] );
+        lens[ q ] = hi - lo;
+        if ( lens[ q ] )
+          S.c_g0[ t * SPT + q ] = g0v[ q ] + lo;
         mine += lens[ q ];
       }
       uint32_t ex;
-      const uint32_t E = block_scan( mine, &ex, S.sh );
+      const uint32_t E = block_scan< BS >( mine, &ex, S.sh );
 #pragma unroll
       for ( int q = 0; q < SPT; ++q )
       {
@@ -589,6 +634,7 @@ __global__ void __launch_bounds__( BS )
         }
         NG_PHASE( 5 );  // apply
       }
+      }  // passes
     }
     if ( t == 0 and events )
       atomicAdd( &d.counters[ 1 ], events );
@@ -764,7 +810,7 @@ __global__ void __launch_bounds__( BS )
   // ================================================================ 3. spike lists
   const uint32_t nsp = __popcll( bits[ 0 ] ) + __popcll( bits[ 1 ] ) + __popcll( bits[ 2 ] ) + __popcll( bits[ 3 ] );
   uint32_t ex;
-  const uint32_t tot = block_scan( nsp, &ex, S.sh );  // also orders the update before the write-back
+  const uint32_t tot = block_scan< BS >( nsp, &ex, S.sh );  // also orders the update before the write-back
   if ( WIN and C.win )
   {
     for ( uint32_t i = t; i < n_win; i += BS )
@@ -884,16 +930,16 @@ launch_mask( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_
   switch ( mask )
   {
   case 1:
-    k_slice< 1, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 1, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   case 2:
-    k_slice< 2, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 2, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   case 4:
-    k_slice< 4, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 4, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   default:
-    k_slice< 7, WIN ><<< total_blocks, BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    k_slice< 7, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
     break;
   }
 }

@@ -286,6 +286,7 @@ Kernel::reset()
   targets_per_block_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
+  delivery_passes_ = 0;
   keep_host_connections_ = true;
   host_connections_freed_ = false;
   released_ = false;
@@ -364,6 +365,17 @@ Kernel::set_kernel_status( const Json& d )
       delivery_mode_ = m;
       store_dirty_ = true;
     }
+    else if ( k == "delivery_passes" )
+    {
+      const int v = static_cast< int >( kv.second.number() );
+      if ( v < 0 or v > MAX_PASSES )
+        throw std::runtime_error(
+          "BadProperty: delivery_passes must be in [0, " + std::to_string( MAX_PASSES ) + "] (0 or 1 = one pass)" );
+      if ( simulated_ and v != delivery_passes_ )
+        throw std::runtime_error( "KernelException: delivery_passes cannot change after Simulate" );
+      delivery_passes_ = v;
+      store_dirty_ = true;
+    }
     else if ( k == "workgroups_per_cu" )
     {
       const int v = static_cast< int >( kv.second.number() );
@@ -415,6 +427,7 @@ Kernel::get_kernel_status_json() const
     << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? engine_->net().tb : 0 )
     << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
+    << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )
     << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
   static const char* kn[] = { "slice", "flatten" };
   for ( int q = 0; q < N_KINDS; ++q )

@@ -158,6 +158,7 @@ def test_full_size_layout_invariance_and_repeatability():
     print("full size: %d spikes, %d events" % (len(ref[0]), ref[3]), flush=True)
     assert len(ref[0]) > 10000
     for layout in ({}, {"delivery_mode": "global", "targets_per_block": 160},
+                   {"delivery_mode": "global", "delivery_passes": 3},
                    {"delivery_mode": "window", "targets_per_block": 0}):
         other = _c2_run(layout)
         assert other[3] == ref[3], layout
