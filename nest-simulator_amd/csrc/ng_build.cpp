@@ -38,48 +38,36 @@ iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
 void
 Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const
 {
-  // Window delivery stages max_delay slots x channels x tb doubles in LDS per workgroup and
-  // slice. Its LDS footprint allows one workgroup per CU, and on the MI355X the global ring
-  // with three workgroups per CU measured faster on every network tried, so automatic
-  // selection takes global delivery; window delivery is available on request.
-  const uint64_t per_target = static_cast< uint64_t >( max_delay ) * nch;
-  const bool want_win = delivery_mode_ == 1;
-  win = false;
+  // Block size: about two blocks per compute unit of an MI355X (256 CUs), multiple of 32,
+  // few enough blocks for the per-instance limit. Window delivery stages as many of the
+  // max_delay slots a slice can reach as fit the LDS window at that block size.
+  const uint32_t pmax = delivery_mode_ == 1 ? PMAX_WIN : PMAX_GLOB;
   if ( targets_per_block_ > 0 )
-  {
     tb = static_cast< uint32_t >( targets_per_block_ );
-    const uint32_t P = ( n + tb - 1 ) / tb;
-    win = want_win and per_target * tb <= WIN_CAP and P <= PMAX_WIN;
-  }
-  else if ( want_win and per_target * 16 <= WIN_CAP )
+  else
   {
-    // largest block the window holds, then equal blocks (fewer blocks mean fewer run
-    // lookups per spike; the last block is not left nearly empty)
-    const uint32_t tb_max = static_cast< uint32_t >( std::min< uint64_t >( MAX_TB, WIN_CAP / per_target ) );
-    const uint32_t P = ( n + tb_max - 1 ) / tb_max;
-    if ( P <= PMAX_WIN )
-    {
-      tb = ( n + P - 1 ) / P;
-      const uint32_t r8 = ( ( tb + 7 ) / 8 ) * 8;
-      if ( r8 <= tb_max )
-        tb = r8;
-      win = true;
-    }
-  }
-  if ( not win and targets_per_block_ == 0 )
-  {
-    // about two blocks per compute unit of an MI355X (256 CUs), multiple of 32, and few
-    // enough blocks for the per-instance limit
     tb = ( n + 511 ) / 512;
     tb = ( ( tb + 31 ) / 32 ) * 32;
     tb = std::max< uint32_t >( tb, 32 );
-    tb = std::max< uint32_t >( tb, ( n + PMAX_GLOB - 1 ) / PMAX_GLOB );
+    tb = std::max< uint32_t >( tb, ( n + pmax - 1 ) / pmax );
     tb = std::min< uint32_t >( tb, MAX_TB );
   }
+  const uint32_t P = ( n + tb - 1 ) / tb;
+  // auto: window delivery wherever it fits (it keeps the densest slots' read-modify-writes in
+  // LDS), global delivery for slot-range passes and for instances beyond the window's limits
+  const bool fits = window_slots( max_delay, nch, tb ) >= 1 and P <= PMAX_WIN;
+  win = fits and ( delivery_mode_ == 1 or ( delivery_mode_ == 0 and delivery_passes_ <= 1 ) );
   if ( delivery_mode_ == 1 and not win )
-    throw std::runtime_error( "BadProperty: delivery_mode 'window' does not fit: max_delay * channels * "
-                              "targets_per_block must be <= "
+    throw std::runtime_error( "BadProperty: delivery_mode 'window' does not fit: channels * targets_per_block must "
+                              "be <= "
       + std::to_string( WIN_CAP ) + " and the instance at most " + std::to_string( PMAX_WIN ) + " blocks" );
+}
+
+int
+Kernel::window_slots( int max_delay, int nch, uint32_t tb )
+{
+  return static_cast< int >(
+    std::min< uint64_t >( max_delay, WIN_CAP / ( static_cast< uint64_t >( nch ) * std::max< uint32_t >( tb, 1 ) ) ) );
 }
 
 void
@@ -97,6 +85,33 @@ Kernel::build_spec( BuildSpec& b )
   b.nch = all_delta ? 1 : 2;
   choose_layout( N, b.max_delay, b.nch, b.tb, b.win );
   const uint32_t P = b.n_blocks();
+  if ( b.win )
+  {
+    // The window covers the wn consecutive slots that expect the most events: with the
+    // emission lag uniform over the slice, a synapse of delay d feeds slots
+    // [d - min_delay, d - 1] equally.
+    b.wn = window_slots( b.max_delay, b.nch, b.tb );
+    std::vector< double > per_delay( b.max_delay + 1, 0.0 ), density( b.max_delay, 0.0 );
+    for ( const auto& C : conns_ )
+      for ( uint16_t dd : C.d )
+        per_delay[ dd ] += 1.0;
+    for ( int dd = b.min_delay; dd <= b.max_delay; ++dd )
+      for ( int l = 0; l < b.min_delay; ++l )
+        density[ l + dd - b.min_delay ] += per_delay[ dd ];
+    double run = 0, best = -1;
+    b.wa = 0;
+    for ( int s = 0; s < b.max_delay; ++s )
+    {
+      run += density[ s ];
+      if ( s >= b.wn )
+        run -= density[ s - b.wn ];
+      if ( s >= b.wn - 1 and run > best )
+      {
+        best = run;
+        b.wa = s - b.wn + 1;
+      }
+    }
+  }
   // Global delivery can run in slot-range passes (delivery_passes), each touching a smaller
   // part of the ring. Finding each spike's split points costs dependent loads that measured
   // more than the smaller working set saves on the MI355X, so one pass is the default.

@@ -66,8 +66,8 @@ struct SynRec
 };
 
 // ring-buffer slots staged in LDS per workgroup and slice (window delivery), in doubles;
-// the rest of the window kernel's LDS and this fill one CU's 160 KB
-constexpr uint32_t WIN_CAP = 12416;
+// sized so that two window workgroups fit in a CU's LDS
+constexpr uint32_t WIN_CAP = 5888;
 constexpr uint32_t MAX_TB = 512;      // targets per block = threads per workgroup
 constexpr uint32_t PMAX_WIN = 1024;   // blocks per instance, window delivery
 constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
@@ -85,7 +85,8 @@ struct NetDev
   int32_t uniform_model;  // model id when all neurons share one, else -1
   uint32_t spk_cap;       // per-block spike list capacity (tb * min_delay)
   int32_t nch;            // ring-buffer channels per slot: 1 (iaf_psc_delta only) or 2
-  int32_t win;            // 1: the max_delay slots a slice touches are staged in LDS
+  int32_t win;            // 1: ring slots [wa, wa + wn) of the slice are staged in LDS
+  int32_t wa, wn;         // window: first slot relative to the slice origin, slots
   int32_t np;             // global delivery: passes over slot ranges per chunk (1..MAX_PASSES)
   int32_t ws;             // slots per pass (np * ws >= max_delay)
   // ---- state by device index

@@ -78,8 +78,10 @@ Engine::build( const BuildSpec& b, bool keep_state )
   const uint32_t P = b.n_blocks();
   if ( P > ( b.win ? PMAX_WIN : PMAX_GLOB ) )
     throw std::runtime_error( "NotImplemented: too many target blocks in one instance; raise targets_per_block" );
-  if ( b.win and static_cast< uint64_t >( b.max_delay ) * b.nch * b.tb > WIN_CAP )
-    throw std::runtime_error( "window delivery: max_delay * channels * targets_per_block exceeds the LDS window" );
+  if ( b.win
+       and ( b.wn < 1 or b.wa < 0 or b.wa + b.wn > b.max_delay
+             or static_cast< uint64_t >( b.wn ) * b.nch * b.tb > WIN_CAP ) )
+    throw std::runtime_error( "window delivery: the window does not fit the LDS or the ring" );
   if ( b.syn.size() >= ( uint64_t( 1 ) << 32 ) )
     throw std::runtime_error( "NotImplemented: 2^32 or more synapses in one instance" );
   free_list( store_allocs_ );
@@ -99,6 +101,8 @@ Engine::build( const BuildSpec& b, bool keep_state )
   net_.spk_cap = b.tb * static_cast< uint32_t >( b.min_delay );
   net_.nch = b.nch;
   net_.win = b.win ? 1 : 0;
+  net_.wa = b.win ? b.wa : 0;
+  net_.wn = b.win ? b.wn : 0;
   if ( b.np < 1 or b.np > MAX_PASSES or static_cast< int64_t >( b.np ) * b.ws < b.max_delay or ( b.win and b.np != 1 ) )
     throw std::runtime_error( "invalid delivery passes" );
   net_.np = b.np;

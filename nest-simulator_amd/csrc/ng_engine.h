@@ -43,6 +43,7 @@ struct BuildSpec
   uint32_t tb = 256;
   int nch = 2;       // ring-buffer channels per slot
   bool win = false;  // window delivery (LDS-staged ring-buffer slots)
+  int wa = 0, wn = 0;  // window: first slot relative to the slice origin, slots
   int np = 1, ws = 1;  // global delivery: slot-range passes and slots per pass
   int uniform_model = -1;
   std::vector< uint32_t > vp_begin;  // device indices, n_vp + 1 entries

@@ -38,12 +38,12 @@ namespace nestgpu
 constexpr int IDX_BITS = 11;  // bucket key = slot-channel << 11 | round index: sorted keys group
                               // a target's events by cell, in key order within each cell
 
-// Tile shape per kernel variant. The window kernel runs one 1024-thread workgroup per CU:
-// its ring-buffer window takes most of the CU's 160 KB of LDS, and 16 waves hide latency.
+// Tile shape per kernel variant. The window kernel halves chunk and round so that, with its
+// ring-buffer window, two workgroups fit in the 160 KB of LDS of a CU.
 template < bool WIN >
 struct Tile
 {
-  static constexpr int BS = WIN ? 1024 : static_cast< int >( MAX_TB );  // threads per workgroup
+  static constexpr int BS = static_cast< int >( MAX_TB );  // threads per workgroup
   static constexpr int SPT = WIN ? 2 : 4;      // spikes per thread per chunk
   static constexpr int CH = SPT * BS;          // spikes per chunk
   static constexpr int RQ = WIN ? 2 : 4;       // events per thread per round
@@ -165,8 +165,8 @@ poisson_count( const PoissonTable& T, uint64_t stream, int64_t step )
 }
 
 // Ring-buffer cells of one block seen from one slice: slot_rel = absolute step - origin.
-// Window mode keeps slots [0, max_delay) in LDS; the rest (only the drive reaches them)
-// and everything in global mode live in the block's ring in HBM.
+// Window mode keeps slots [wa, wa + wn) in LDS (the slots most events of a slice land in);
+// all other cells, and every cell in global mode, live in the block's ring in HBM.
 template < bool WIN >
 struct Cells
 {
@@ -176,14 +176,19 @@ struct Cells
   uint32_t tb;
   uint32_t s0;  // ring slot of the slice origin
   uint32_t L;
-  uint32_t W;  // window slots
+  uint32_t wa, wn;  // window: first slot (relative to the origin) and slots
   bool win;
 
+  __device__ bool
+  in_window( uint32_t slot_rel ) const
+  {
+    return WIN and win and slot_rel - wa < wn;
+  }
   __device__ double*
   at( uint32_t slot_rel, int ch, uint32_t local ) const
   {
-    if ( WIN and win and slot_rel < W )
-      return lds + slot_rel * nct + ch * tb + local;
+    if ( in_window( slot_rel ) )
+      return lds + ( slot_rel - wa ) * nct + ch * tb + local;
     uint32_t slot = s0 + slot_rel;
     if ( slot >= L )
       slot -= L;
@@ -324,17 +329,21 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   C.tb = tb;
   C.L = static_cast< uint32_t >( d.L );
   C.s0 = static_cast< uint32_t >( ph.origin % d.L );
-  C.W = static_cast< uint32_t >( d.max_delay );
+  C.wa = static_cast< uint32_t >( d.wa );
+  C.wn = static_cast< uint32_t >( d.wn );
   C.win = WIN and d.win != 0;
   C.rbb = d.rb + static_cast< size_t >( b ) * C.L * C.nct;
 
   // ================================================================ 0. stage the window
-  const uint32_t n_win = C.W * C.nct;
-  const uint32_t before_wrap = ( C.L - C.s0 ) * C.nct;
+  uint32_t w0 = C.s0 + C.wa;  // ring slot of the window's first slot
+  if ( w0 >= C.L )
+    w0 -= C.L;
+  const uint32_t n_win = C.wn * C.nct;
+  const uint32_t before_wrap = ( C.L - w0 ) * C.nct;
   if ( WIN and C.win )
   {
     for ( uint32_t i = t; i < n_win; i += BS )
-      S.win[ i ] = C.rbb[ i < before_wrap ? C.s0 * C.nct + i : i - before_wrap ];
+      S.win[ i ] = C.rbb[ i < before_wrap ? w0 * C.nct + i : i - before_wrap ];
   }
 
   // ================================================================ 1. delivery
@@ -482,6 +491,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         // takes an unordered place in its target's bucket
         uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ], ci[ RQ ];
         double val[ RQ ], cv[ RQ ];
+        bool lw[ RQ ];  // the event's cell is in the LDS window
         const uint32_t eb = e0 + t * RQ;
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
@@ -506,8 +516,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
             val[ q ] = v;
             key[ q ] = ( ( slot_rel * d.nch + ch ) << IDX_BITS ) | static_cast< uint32_t >( t * RQ + q );
             dst[ q ] = atomicAdd( &S.bcnt[ cur ][ local ], 1u );
-            if ( WIN and C.win )
-              ci[ q ] = ( slot_rel * d.nch + ch ) * tb + local;
+            lw[ q ] = C.in_window( slot_rel );
+            if ( lw[ q ] )
+              ci[ q ] = ( slot_rel - C.wa ) * C.nct + ch * tb + local;
             else
             {
               uint32_t slot = C.s0 + slot_rel;
@@ -521,13 +532,10 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         NG_PHASE( 2 );  // stage
         // every thread has finished the previous round (its cell stores included): the cells
         // of this round can be read now, while the buckets are formed
-        if ( not( WIN and C.win ) )
-        {
 #pragma unroll
-          for ( int q = 0; q < RQ; ++q )
-            if ( loc[ q ] != NONE )
-              cv[ q ] = C.rbb[ ci[ q ] ];
-        }
+        for ( int q = 0; q < RQ; ++q )
+          if ( loc[ q ] != NONE and not lw[ q ] )
+            cv[ q ] = C.rbb[ ci[ q ] ];
         // bucket starts: block scan of the per-target counts
         const uint32_t mycnt = t < tb ? S.bcnt[ cur ][ t ] : 0u;
         uint32_t mybase;
@@ -607,7 +615,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
             loc[ q ] = NONE;  // not the first of its run
             continue;
           }
-          double v = ( WIN and C.win ) ? S.win[ ci[ q ] ] : cv[ q ];
+          double v = lw[ q ] ? S.win[ ci[ q ] ] : cv[ q ];
           v = v + val[ q ];
           for ( uint32_t i = srt[ q ] + 1; i < hi; ++i )
           {
@@ -618,20 +626,15 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
           }
           cv[ q ] = v;
         }
-        if ( WIN and C.win )
-        {
 #pragma unroll
-          for ( int q = 0; q < RQ; ++q )
-            if ( loc[ q ] != NONE )
+        for ( int q = 0; q < RQ; ++q )
+          if ( loc[ q ] != NONE )
+          {
+            if ( lw[ q ] )
               S.win[ ci[ q ] ] = cv[ q ];
-        }
-        else
-        {
-#pragma unroll
-          for ( int q = 0; q < RQ; ++q )
-            if ( loc[ q ] != NONE )
+            else
               C.rbb[ ci[ q ] ] = cv[ q ];
-        }
+          }
         NG_PHASE( 5 );  // apply
       }
       }  // passes
@@ -814,7 +817,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   if ( WIN and C.win )
   {
     for ( uint32_t i = t; i < n_win; i += BS )
-      C.rbb[ i < before_wrap ? C.s0 * C.nct + i : i - before_wrap ] = S.win[ i ];
+      C.rbb[ i < before_wrap ? w0 * C.nct + i : i - before_wrap ] = S.win[ i ];
   }
   uint32_t* cnt = d.blk_cnt + static_cast< size_t >( par ) * P + b;
   if ( t == 0 )

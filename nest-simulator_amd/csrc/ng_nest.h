@@ -323,6 +323,7 @@ private:
   void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
   void build_spec( BuildSpec& b );
   void choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const;
+  static int window_slots( int max_delay, int nch, uint32_t tb );
   void collect_log();
 
   int device_;
