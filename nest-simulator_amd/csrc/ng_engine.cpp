@@ -84,6 +84,8 @@ Engine::build( const BuildSpec& b, bool keep_state )
     throw std::runtime_error( "window delivery: the window does not fit the LDS or the ring" );
   if ( b.syn.size() >= ( uint64_t( 1 ) << 32 ) )
     throw std::runtime_error( "NotImplemented: 2^32 or more synapses in one instance" );
+  if ( static_cast< uint64_t >( b.min_delay + b.max_delay ) * b.nch * b.tb * sizeof( double ) >= ( uint64_t( 1 ) << 31 ) )
+    throw std::runtime_error( "NotImplemented: ring buffer of one block at or above 2 GiB (delays too long)" );
   free_list( store_allocs_ );
   auto S = [ this ]( void* p )
   {

@@ -52,6 +52,61 @@ struct Tile
   static constexpr uint32_t PMAX = WIN ? PMAX_WIN : PMAX_GLOB;
 };
 constexpr uint32_t NONE = 0xffffffffu;
+
+// HBM views of the instance's arrays. The pointers in the instance record are generic, and
+// generic (FLAT) accesses are counted by the LDS wait counter too: every barrier and every
+// LDS read would then wait for all HBM loads in flight. Accessing HBM through global-space
+// pointers keeps those loads on the vector-memory counter, so they overlap the LDS phases.
+template < class T >
+using gptr = __attribute__( ( address_space( 1 ) ) ) T*;
+template < class T >
+using gref = __attribute__( ( address_space( 1 ) ) ) T&;
+template < class T >
+using lptr = __attribute__( ( address_space( 3 ) ) ) T*;
+template < class T >
+__device__ inline gptr< T >
+glob( T* p )
+{
+  return ( gptr< T > )p;
+}
+
+// device-scope relaxed atomics on HBM counters (what atomicAdd / atomicOr do, without FLAT)
+__device__ inline unsigned long long
+g_add( unsigned long long* p, unsigned long long v )
+{
+  return __hip_atomic_fetch_add( glob( p ), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+__device__ inline void
+g_or( unsigned long long* p, unsigned long long v )
+{
+  __hip_atomic_fetch_or( glob( p ), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+
+// The block's ring buffer as a buffer resource: an access at an offset past its end
+// (NO_CELL) moves no data (loads return 0, stores are dropped), so every lane can issue the
+// same loads and stores whether or not its event has an HBM cell.
+constexpr uint32_t NO_CELL = 0x80000000u;
+using v2u = __attribute__( ( ext_vector_type( 2 ) ) ) uint32_t;
+__device__ inline double
+cell_load( __amdgpu_buffer_rsrc_t r, uint32_t byte_off )
+{
+  return __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, byte_off, 0, 0 ) );
+}
+__device__ inline void
+cell_store( __amdgpu_buffer_rsrc_t r, uint32_t byte_off, double v )
+{
+  __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( v2u, v ), r, byte_off, 0, 0 );
+}
+
+__device__ inline SynRec
+load_rec( gptr< const SynRec > p )
+{
+  SynRec r;
+  r.w_lo = p->w_lo;
+  r.w_hi = p->w_hi;
+  r.word = p->word;
+  return r;
+}
 constexpr int LB = 8;  // lags of ring-buffer input loaded together in the update
 static_assert( Tile< false >::R <= ( 1 << IDX_BITS ) and Tile< true >::R <= ( 1 << IDX_BITS ),
   "round index must fit the bucket key" );
@@ -104,7 +159,7 @@ upper_bound_u32( const uint32_t* a, uint32_t n, uint32_t v )
 
 // number of records of a delay-ordered run [g0, g0 + len) whose delay offset is below t
 __device__ inline uint32_t
-run_split( const SynRec* __restrict__ syn, uint32_t g0, uint32_t len, int t )
+run_split( gptr< const SynRec > syn, uint32_t g0, uint32_t len, int t )
 {
   if ( t <= 0 )
     return 0;
@@ -150,8 +205,9 @@ block_scan( uint32_t v, uint32_t* excl, uint32_t* sh )
 }
 
 // Poisson count for seeded drive: splitmix64 of (seed, stream, step), inverse CDF
+template < class Table >
 __host__ __device__ inline uint32_t
-poisson_count( const PoissonTable& T, uint64_t stream, int64_t step )
+poisson_count( const Table& T, uint64_t stream, int64_t step )
 {
   uint64_t z = T.seed + stream * 0x9E3779B97F4A7C15ull + static_cast< uint64_t >( step ) * 0xD1B54A32D192ED03ull;
   z = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
@@ -170,8 +226,8 @@ poisson_count( const PoissonTable& T, uint64_t stream, int64_t step )
 template < bool WIN >
 struct Cells
 {
-  double* lds;
-  double* rbb;  // block's ring: [slot][channel][tb]
+  lptr< double > lds;
+  gptr< double > rbb;  // block's ring: [slot][channel][tb]
   uint32_t nct;  // doubles per slot (channels * tb)
   uint32_t tb;
   uint32_t s0;  // ring slot of the slice origin
@@ -184,15 +240,33 @@ struct Cells
   {
     return WIN and win and slot_rel - wa < wn;
   }
-  __device__ double*
-  at( uint32_t slot_rel, int ch, uint32_t local ) const
+  __device__ uint32_t
+  win_index( uint32_t slot_rel, uint32_t ch, uint32_t local ) const
   {
-    if ( in_window( slot_rel ) )
-      return lds + ( slot_rel - wa ) * nct + ch * tb + local;
+    return ( slot_rel - wa ) * nct + ch * tb + local;
+  }
+  __device__ uint32_t
+  ring_index( uint32_t slot_rel, uint32_t ch, uint32_t local ) const
+  {
     uint32_t slot = s0 + slot_rel;
     if ( slot >= L )
       slot -= L;
-    return rbb + static_cast< size_t >( slot ) * nct + ch * tb + local;
+    return slot * nct + ch * tb + local;
+  }
+  __device__ double
+  get( uint32_t slot_rel, uint32_t ch, uint32_t local ) const
+  {
+    if ( in_window( slot_rel ) )
+      return lds[ win_index( slot_rel, ch, local ) ];
+    return rbb[ ring_index( slot_rel, ch, local ) ];
+  }
+  __device__ void
+  put( uint32_t slot_rel, uint32_t ch, uint32_t local, double v ) const
+  {
+    if ( in_window( slot_rel ) )
+      lds[ win_index( slot_rel, ch, local ) ] = v;
+    else
+      rbb[ ring_index( slot_rel, ch, local ) ] = v;
   }
 };
 
@@ -200,17 +274,17 @@ struct Cells
 // Each lag adds into a different cell, so the cells of LB lags are read together.
 template < bool WIN >
 __device__ inline void
-drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
+drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
   int64_t origin, int from, int to )
 {
   if ( ps.drive_mode == 0 )
     return;
-  const uint32_t col = d.drive_idx[ dev ];
+  const uint32_t col = glob( d.drive_idx )[ dev ];
   if ( col == NONE )
     return;
   for ( int l0 = from; l0 < to; l0 += LB )
   {
-    double* cp[ LB ];
+    uint32_t at[ LB ], ch[ LB ];
     double add[ LB ];
     bool has[ LB ];
 #pragma unroll
@@ -223,33 +297,33 @@ drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t l
         continue;
       uint32_t c;
       if ( ps.drive_mode == 1 )
-        c = d.drive_counts[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
+        c = glob( d.drive_counts )[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
       else
-        c = poisson_count( d.ptables[ ps.drive_table ], col, e );
+        c = poisson_count( glob( d.ptables )[ ps.drive_table ], col, e );
       if ( c == 0 )
         continue;
       const double w = ps.drive_w;
       const double mult = static_cast< double >( c );
-      const uint32_t at = static_cast< uint32_t >( lag + ps.drive_d );
+      at[ q ] = static_cast< uint32_t >( lag + ps.drive_d );
       has[ q ] = true;
       if ( m == M_IAF_PSC_DELTA )
       {
-        cp[ q ] = C.at( at, 0, local );
+        ch[ q ] = 0;
         add[ q ] = w * mult;
       }
       else if ( m == M_IAF_PSC_EXP )
       {
         add[ q ] = w * mult;
-        cp[ q ] = C.at( at, add[ q ] > 0 ? 1 : 0, local );
+        ch[ q ] = add[ q ] > 0 ? 1 : 0;
       }
       else if ( w > 0.0 )
       {
-        cp[ q ] = C.at( at, 0, local );
+        ch[ q ] = 0;
         add[ q ] = w * mult;
       }
       else
       {
-        cp[ q ] = C.at( at, 1, local );
+        ch[ q ] = 1;
         add[ q ] = -w * mult;
       }
     }
@@ -257,11 +331,11 @@ drive_adds( const NetDev& d, const ParamSet& ps, int m, uint32_t dev, uint32_t l
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        old[ q ] = *cp[ q ];
+        old[ q ] = C.get( at[ q ], ch[ q ], local );
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        *cp[ q ] = old[ q ] + add[ q ];
+        C.put( at[ q ], ch[ q ], local, old[ q ] + add[ q ] );
   }
 }
 
@@ -287,8 +361,10 @@ __device__ unsigned long long g_phase_ticks[ 8 ];
   } while ( 0 )
 #endif
 
+// Two workgroups of 8 waves per CU need at most 128 VGPRs (4 waves per SIMD). The aeif
+// integrator needs more; its kernels run one workgroup per CU.
 template < int MASK, bool WIN >
-__global__ void __launch_bounds__( Tile< WIN >::BS )
+__global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   k_slice( const NetDev* __restrict__ nets, const uint32_t* __restrict__ blk_prefix, int n_nets, int s )
 {
 #ifdef NG_PROFILE
@@ -324,7 +400,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   const uint32_t t = threadIdx.x;
 
   Cells< WIN > C;
-  C.lds = S.win;
+  C.lds = ( lptr< double > )S.win;
   C.nct = static_cast< uint32_t >( d.nch ) * tb;
   C.tb = tb;
   C.L = static_cast< uint32_t >( d.L );
@@ -332,7 +408,10 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   C.wa = static_cast< uint32_t >( d.wa );
   C.wn = static_cast< uint32_t >( d.wn );
   C.win = WIN and d.win != 0;
-  C.rbb = d.rb + static_cast< size_t >( b ) * C.L * C.nct;
+  C.rbb = glob( d.rb ) + static_cast< size_t >( b ) * C.L * C.nct;
+  // ring bytes per block < 2^31 (checked on the host), so NO_CELL is always out of range
+  const __amdgpu_buffer_rsrc_t rb_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+    d.rb + static_cast< size_t >( b ) * C.L * C.nct, 0, static_cast< int >( C.L * C.nct * 8u ), 0x00020000 );
 
   // ================================================================ 0. stage the window
   uint32_t w0 = C.s0 + C.wa;  // ring slot of the window's first slot
@@ -352,14 +431,14 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
     S.bcnt[ 0 ][ t ] = 0;
     S.bcnt[ 1 ][ t ] = 0;
     int cur = 0;  // counter buffer of the current round
-    const bool flat = d.flat_meta[ 2 + ppar ] != 0;
+    const bool flat = glob( d.flat_meta )[ 2 + ppar ] != 0;
     uint32_t K;
     if ( flat )
-      K = d.flat_meta[ ppar ];
+      K = glob( d.flat_meta )[ ppar ];
     else
     {
       // prefix of the per-block spike counts of the delivered slice
-      const uint32_t* cnt = d.blk_cnt + static_cast< size_t >( ppar ) * P;
+      gptr< const uint32_t > cnt = glob( d.blk_cnt ) + static_cast< size_t >( ppar ) * P;
       uint32_t carry = 0;
       for ( uint32_t p0 = 0; p0 < P; p0 += BS )
       {
@@ -389,8 +468,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         for ( int q = 0; q < SPT; ++q )
           if ( kb + q < K )
           {
-            src[ q ] = d.flat_src[ kb + q ];
-            lag[ q ] = d.flat_lag[ kb + q ];
+            src[ q ] = glob( d.flat_src )[ kb + q ];
+            lag[ q ] = glob( d.flat_lag )[ kb + q ];
           }
       }
       else if ( kb < K )
@@ -405,7 +484,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
             while ( S.pref[ blk + 1 ] <= k )
               ++blk;
             const uint32_t e =
-              d.blk_spk[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - S.pref[ blk ] ) ];
+              glob( d.blk_spk )[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - S.pref[ blk ] ) ];
             src[ q ] = blk * tb + ( e >> 16 );
             lag[ q ] = e & 0xffu;
           }
@@ -418,7 +497,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         lenv[ q ] = 0;
         if ( kb + q < K )
         {
-          const uint32_t* bo = d.boff + static_cast< size_t >( src[ q ] ) * ( P + 1 ) + b;
+          gptr< const uint32_t > bo = glob( d.boff ) + static_cast< size_t >( src[ q ] ) * ( P + 1 ) + b;
           g0v[ q ] = bo[ 0 ];
           lenv[ q ] = bo[ 1 ] - g0v[ q ];
           S.c_lag[ t * SPT + q ] = static_cast< uint8_t >( lag[ q ] );
@@ -434,9 +513,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         s2v[ q ] = lenv[ q ];
         if ( d.np > 1 and lenv[ q ] > 0 )
         {
-          s1v[ q ] = run_split( d.syn, g0v[ q ], lenv[ q ], d.ws - static_cast< int >( lag[ q ] ) );
+          s1v[ q ] = run_split( glob( d.syn ), g0v[ q ], lenv[ q ], d.ws - static_cast< int >( lag[ q ] ) );
           if ( d.np > 2 )
-            s2v[ q ] = run_split( d.syn, g0v[ q ], lenv[ q ], 2 * d.ws - static_cast< int >( lag[ q ] ) );
+            s2v[ q ] = run_split( glob( d.syn ), g0v[ q ], lenv[ q ], 2 * d.ws - static_cast< int >( lag[ q ] ) );
         }
       }
       for ( int pass = 0; pass < d.np; ++pass )
@@ -470,26 +549,31 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
       // is applied
       SynRec rec[ RQ ];
       uint32_t rlag[ RQ ];
+      // Every thread issues RQ record loads (events past the end reload the last record):
+      // with a fixed number of loads in flight the waits count exactly, instead of draining
+      // all outstanding memory operations.
       auto load_round = [ & ]( uint32_t e0 )
       {
         const uint32_t eb = e0 + t * RQ;
-        uint32_t j = eb < E ? upper_bound_u32( S.c_off, CH + 1, eb ) - 1 : 0;
+        uint32_t j = upper_bound_u32( S.c_off, CH + 1, eb < E ? eb : E - 1 ) - 1;
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
-          if ( eb + q < E )
-          {
-            while ( S.c_off[ j + 1 ] <= eb + q )
-              ++j;
-            rec[ q ] = d.syn[ S.c_g0[ j ] + ( eb + q - S.c_off[ j ] ) ];
-            rlag[ q ] = S.c_lag[ j ];
-          }
+        {
+          const uint32_t e = eb + q < E ? eb + q : E - 1;
+          while ( S.c_off[ j + 1 ] <= e )
+            ++j;
+          rec[ q ] = load_rec( glob( d.syn ) + S.c_g0[ j ] + ( e - S.c_off[ j ] ) );
+          rlag[ q ] = S.c_lag[ j ];
+        }
       };
-      load_round( 0 );
+      if ( E )
+        load_round( 0 );
+      const int umodel = d.uniform_model;
       for ( uint32_t e0 = 0; e0 < E; e0 += R, cur ^= 1 )
       {
         // stage: thread t holds events e0 + t*RQ .. + RQ - 1 (key order = round index) and
         // takes an unordered place in its target's bucket
-        uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ], ci[ RQ ];
+        uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ], ci[ RQ ], goff[ RQ ];
         double val[ RQ ], cv[ RQ ];
         bool lw[ RQ ];  // the event's cell is in the LDS window
         const uint32_t eb = e0 + t * RQ;
@@ -497,12 +581,17 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         for ( int q = 0; q < RQ; ++q )
         {
           loc[ q ] = NONE;
+          goff[ q ] = NO_CELL;
           if ( eb + q < E )
           {
             const double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
             const uint32_t local = rec[ q ].word & 0xffffu;
             const uint32_t slot_rel = rlag[ q ] + ( rec[ q ].word >> 16 );
-            const int m = d.uniform_model >= 0 ? d.uniform_model : d.model[ b * tb + local ];
+            const int m = MASK == 1 ? M_IAF_PSC_DELTA
+              : MASK == 2           ? M_IAF_PSC_EXP
+              : MASK == 4           ? M_AEIF_COND_EXP
+              : umodel >= 0         ? umodel
+                                    : glob( d.model )[ b * tb + local ];
             uint32_t ch = 0;
             double v = w;  // static_synapse: weight * multiplicity 1
             if ( m == M_IAF_PSC_EXP )
@@ -525,6 +614,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
               if ( slot >= C.L )
                 slot -= C.L;
               ci[ q ] = slot * C.nct + ch * tb + local;
+              goff[ q ] = ci[ q ] * 8u;
             }
           }
         }
@@ -534,8 +624,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         // of this round can be read now, while the buckets are formed
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
-          if ( loc[ q ] != NONE and not lw[ q ] )
-            cv[ q ] = C.rbb[ ci[ q ] ];
+          cv[ q ] = cell_load( rb_rsrc, goff[ q ] );
         // bucket starts: block scan of the per-target counts
         const uint32_t mycnt = t < tb ? S.bcnt[ cur ][ t ] : 0u;
         uint32_t mybase;
@@ -599,8 +688,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
           }
         __syncthreads();
         NG_PHASE( 4 );  // place and rank
-        if ( e0 + R < E )
-          load_round( e0 + R );
+        load_round( e0 + R );  // past the last round: reloads record E - 1, unused
         // the first event of each cell's run adds the whole run in key order; runs of
         // different cells proceed in parallel
 #pragma unroll
@@ -613,6 +701,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
           if ( srt[ q ] > lo and ( S.bk[ S.perm[ srt[ q ] - 1 ] ] >> IDX_BITS ) == sc )
           {
             loc[ q ] = NONE;  // not the first of its run
+            goff[ q ] = NO_CELL;
             continue;
           }
           double v = lw[ q ] ? S.win[ ci[ q ] ] : cv[ q ];
@@ -628,19 +717,17 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         }
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
-          if ( loc[ q ] != NONE )
-          {
-            if ( lw[ q ] )
-              S.win[ ci[ q ] ] = cv[ q ];
-            else
-              C.rbb[ ci[ q ] ] = cv[ q ];
-          }
+        {
+          if ( loc[ q ] != NONE and lw[ q ] )
+            S.win[ ci[ q ] ] = cv[ q ];
+          cell_store( rb_rsrc, goff[ q ], cv[ q ] );  // NO_CELL unless a global run leader
+        }
         NG_PHASE( 5 );  // apply
       }
       }  // passes
     }
     if ( t == 0 and events )
-      atomicAdd( &d.counters[ 1 ], events );
+      g_add( &d.counters[ 1 ], events );
   }
   __syncthreads();  // window staged and delivered
 
@@ -650,7 +737,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
   if ( local < tb and dev < d.n )
   {
-    const ParamSet& ps = d.psets[ d.pidx[ dev ] ];
+    gref< const ParamSet > ps = glob( d.psets )[ glob( d.pidx )[ dev ] ];
     const int m = ( MASK == 1 ) ? M_IAF_PSC_DELTA
       : ( MASK == 2 )           ? M_IAF_PSC_EXP
       : ( MASK == 4 )           ? M_AEIF_COND_EXP
@@ -660,8 +747,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
       // models/iaf_psc_delta.cpp:272-331 (refractory_input = false)
       const double P30 = ps.p[ 0 ], P33 = ps.p[ 1 ], y0I = ps.p[ 2 ], V_th = ps.p[ 3 ], V_reset = ps.p[ 4 ],
                    V_min = ps.p[ 5 ];
-      double y3 = d.v[ 0 ][ dev ];
-      int32_t r = d.r[ dev ];
+      double y3 = glob( d.v[ 0 ] )[ dev ];
+      int32_t r = glob( d.r )[ dev ];
       // ring-buffer cells are read LB lags at a time (independent loads), then consumed in
       // lag order and cleared
       for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
@@ -670,13 +757,13 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
 #pragma unroll
         for ( int q = 0; q < LB; ++q )
           if ( l0 + q < ph.to )
-            win_[ q ] = *C.at( l0 + q, 0, local );
+            win_[ q ] = C.get( l0 + q, 0, local );
 #pragma unroll
         for ( int q = 0; q < LB; ++q )
           if ( l0 + q < ph.to )
           {
             const int lag = l0 + q;
-            *C.at( lag, 0, local ) = 0.0;
+            C.put( lag, 0, local, 0.0 );
             if ( r == 0 )
             {
               y3 = P30 * y0I + P33 * y3 + win_[ q ];
@@ -692,18 +779,18 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
             }
           }
       }
-      d.v[ 0 ][ dev ] = y3;
-      d.r[ dev ] = r;
+      glob( d.v[ 0 ] )[ dev ] = y3;
+      glob( d.r )[ dev ] = r;
     }
     if ( ( MASK & 2 ) and m == M_IAF_PSC_EXP )
     {
       // models/iaf_psc_exp.cpp:288-349 (delta = 0: deterministic threshold)
       const double P22 = ps.p[ 0 ], P21ex = ps.p[ 1 ], P21in = ps.p[ 2 ], P20 = ps.p[ 3 ], P11ex = ps.p[ 4 ],
                    P11in = ps.p[ 5 ], IeI0 = ps.p[ 6 ], Theta = ps.p[ 7 ], V_reset = ps.p[ 8 ], exI1 = ps.p[ 9 ];
-      double V = d.v[ 0 ][ dev ];
-      double iex = d.v[ 1 ][ dev ];
-      double iin = d.v[ 2 ][ dev ];
-      int32_t r = d.r[ dev ];
+      double V = glob( d.v[ 0 ] )[ dev ];
+      double iex = glob( d.v[ 1 ] )[ dev ];
+      double iin = glob( d.v[ 2 ] )[ dev ];
+      int32_t r = glob( d.r )[ dev ];
       for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
       {
         double w_in[ LB ], w_ex[ LB ];
@@ -711,8 +798,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         for ( int q = 0; q < LB; ++q )
           if ( l0 + q < ph.to )
           {
-            w_in[ q ] = *C.at( l0 + q, 0, local );
-            w_ex[ q ] = *C.at( l0 + q, 1, local );
+            w_in[ q ] = C.get( l0 + q, 0, local );
+            w_ex[ q ] = C.get( l0 + q, 1, local );
           }
 #pragma unroll
         for ( int q = 0; q < LB; ++q )
@@ -734,14 +821,14 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
               V = V_reset;
               bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
             }
-            *C.at( lag, 0, local ) = 0.0;
-            *C.at( lag, 1, local ) = 0.0;
+            C.put( lag, 0, local, 0.0 );
+            C.put( lag, 1, local, 0.0 );
           }
       }
-      d.v[ 0 ][ dev ] = V;
-      d.v[ 1 ][ dev ] = iex;
-      d.v[ 2 ][ dev ] = iin;
-      d.r[ dev ] = r;
+      glob( d.v[ 0 ] )[ dev ] = V;
+      glob( d.v[ 1 ] )[ dev ] = iex;
+      glob( d.v[ 2 ] )[ dev ] = iin;
+      glob( d.r )[ dev ] = r;
     }
     if ( ( MASK & 4 ) and m == M_AEIF_COND_EXP )
     {
@@ -765,9 +852,10 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
       A.tol = ps.p[ 15 ];
       const double V_peak_eff = ps.p[ 16 ];
       const double hstep = ps.p[ 17 ];
-      double y[ 4 ] = { d.v[ 0 ][ dev ], d.v[ 1 ][ dev ], d.v[ 2 ][ dev ], d.v[ 3 ][ dev ] };
-      double hint = d.v[ 4 ][ dev ];
-      int32_t r = d.r[ dev ];
+      double y[ 4 ] = { glob( d.v[ 0 ] )[ dev ], glob( d.v[ 1 ] )[ dev ], glob( d.v[ 2 ] )[ dev ],
+        glob( d.v[ 3 ] )[ dev ] };
+      double hint = glob( d.v[ 4 ] )[ dev ];
+      int32_t r = glob( d.r )[ dev ];
       for ( int lag = ph.from; lag < ph.to; ++lag )
       {
         double tt = 0.0;
@@ -776,7 +864,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
           aeif_evolve( A, r > 0, &tt, hstep, &hint, y );
           if ( y[ 0 ] < -1e3 or y[ 3 ] < -1e6 or y[ 3 ] > 1e6 )
           {
-            atomicOr( &d.counters[ 0 ], ERR_AEIF_INSTABILITY );
+            g_or( &d.counters[ 0 ], ERR_AEIF_INSTABILITY );
             tt = hstep;
             break;
           }
@@ -788,23 +876,21 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
             y[ 3 ] += A.b;
             r = ps.refractory_counts > 0 ? ps.refractory_counts + 1 : 0;
             if ( bits[ lag >> 6 ] & ( 1ull << ( lag & 63 ) ) )
-              atomicOr( &d.counters[ 0 ], ERR_AEIF_DOUBLE_SPIKE );
+              g_or( &d.counters[ 0 ], ERR_AEIF_DOUBLE_SPIKE );
             bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
           }
         }
         if ( r > 0 )
           --r;
-        double* cex = C.at( lag, 0, local );
-        double* cinh = C.at( lag, 1, local );
-        y[ 1 ] += *cex;
-        y[ 2 ] += *cinh;
-        *cex = 0.0;
-        *cinh = 0.0;
+        y[ 1 ] += C.get( lag, 0, local );
+        y[ 2 ] += C.get( lag, 1, local );
+        C.put( lag, 0, local, 0.0 );
+        C.put( lag, 1, local, 0.0 );
       }
       for ( int q = 0; q < 4; ++q )
-        d.v[ q ][ dev ] = y[ q ];
-      d.v[ 4 ][ dev ] = hint;
-      d.r[ dev ] = r;
+        glob( d.v[ q ] )[ dev ] = y[ q ];
+      glob( d.v[ 4 ] )[ dev ] = hint;
+      glob( d.r )[ dev ] = r;
     }
     drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, ph.to );
   }
@@ -819,22 +905,22 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
     for ( uint32_t i = t; i < n_win; i += BS )
       C.rbb[ i < before_wrap ? w0 * C.nct + i : i - before_wrap ] = S.win[ i ];
   }
-  uint32_t* cnt = d.blk_cnt + static_cast< size_t >( par ) * P + b;
+  gptr< uint32_t > cnt = glob( d.blk_cnt ) + static_cast< size_t >( par ) * P + b;
   if ( t == 0 )
   {
     const uint32_t base = ph.from == 0 ? 0u : *cnt;  // a new slice starts an empty list
     S.cnt_base = base;
     if ( base + tot > d.spk_cap )
-      atomicOr( &d.counters[ 0 ], ERR_SPIKE_LIST );
+      g_or( &d.counters[ 0 ], ERR_SPIKE_LIST );
     *cnt = base + tot < d.spk_cap ? base + tot : d.spk_cap;
     if ( ph.from == 0 and b == 0 )
-      d.flat_meta[ 2 + par ] = 0;
+      glob( d.flat_meta )[ 2 + par ] = 0;
     S.log_base = 0ull;
     if ( tot )
     {
-      atomicAdd( &d.counters[ 2 ], static_cast< unsigned long long >( tot ) );
+      g_add( &d.counters[ 2 ], static_cast< unsigned long long >( tot ) );
       if ( d.log_enabled )
-        S.log_base = atomicAdd( d.log_n, static_cast< unsigned long long >( tot ) );
+        S.log_base = g_add( d.log_n, static_cast< unsigned long long >( tot ) );
     }
   }
   __syncthreads();
@@ -843,7 +929,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
   {
     uint32_t o = S.cnt_base + ex;
     unsigned long long li = S.log_base + ex;
-    uint32_t* lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+    gptr< uint32_t > lst = glob( d.blk_spk ) + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
     for ( int w = 0; w < 4; ++w )
     {
       unsigned long long x = bits[ w ];
@@ -858,11 +944,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS )
         {
           if ( li < d.log_cap )
           {
-            d.log_src[ li ] = dev;
-            d.log_step[ li ] = ph.origin + lag + 1;
+            glob( d.log_src )[ li ] = dev;
+            glob( d.log_step )[ li ] = ph.origin + lag + 1;
           }
           else
-            atomicOr( &d.counters[ 0 ], ERR_LOG );
+            g_or( &d.counters[ 0 ], ERR_LOG );
           ++li;
         }
       }
