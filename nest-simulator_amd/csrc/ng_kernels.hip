@@ -175,6 +175,22 @@ run_split( gptr< const SynRec > syn, uint32_t g0, uint32_t len, int t )
   return lo;
 }
 
+// inclusive scan over the 64 lanes of a wave with DPP moves: Hillis-Steele inside each row
+// of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row read 0), then the
+// last lane of row 0 and 2 is added to row 1 and 3 (row_bcast:15), then the last lane of
+// row 1 to rows 2 and 3 (row_bcast:31)
+__device__ inline uint32_t
+wave_incl_scan( uint32_t x )
+{
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x111, 0xf, 0xf, false ) );
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x112, 0xf, 0xf, false ) );
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x114, 0xf, 0xf, false ) );
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x118, 0xf, 0xf, false ) );
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x142, 0xa, 0xf, false ) );
+  x += static_cast< uint32_t >( __builtin_amdgcn_update_dpp( 0, static_cast< int >( x ), 0x143, 0xc, 0xf, false ) );
+  return x;
+}
+
 // block-wide exclusive scan of one value per thread; returns the total
 template < int BS >
 __device__ inline uint32_t
@@ -182,13 +198,7 @@ block_scan( uint32_t v, uint32_t* excl, uint32_t* sh )
 {
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
-  uint32_t x = v;
-  for ( int off = 1; off < 64; off <<= 1 )
-  {
-    const uint32_t u = __shfl_up( x, off, 64 );
-    if ( lane >= off )
-      x += u;
-  }
+  const uint32_t x = wave_incl_scan( v );
   if ( lane == 63 )
     sh[ wid ] = x;
   __syncthreads();
@@ -560,7 +570,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         for ( int q = 0; q < RQ; ++q )
         {
           const uint32_t e = eb + q < E ? eb + q : E - 1;
-          while ( S.c_off[ j + 1 ] <= e )
+          while ( j + 1 < static_cast< uint32_t >( CH ) and S.c_off[ j + 1 ] <= e )
             ++j;
           rec[ q ] = load_rec( glob( d.syn ) + S.c_g0[ j ] + ( e - S.c_off[ j ] ) );
           rlag[ q ] = S.c_lag[ j ];
@@ -625,37 +635,31 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
           cv[ q ] = cell_load( rb_rsrc, goff[ q ] );
-        // bucket starts: block scan of the per-target counts
-        const uint32_t mycnt = t < tb ? S.bcnt[ cur ][ t ] : 0u;
-        uint32_t mybase;
+        // bucket starts: wave 0 scans the per-target counts, BS / 64 per lane; the others
+        // clear the counters of the next round
+        if ( t < 64 )
         {
-          const int lane = t & 63;
-          const int wid = t >> 6;
-          uint32_t x = mycnt;
-          for ( int off = 1; off < 64; off <<= 1 )
+          constexpr int CPL = BS / 64;
+          uint32_t c[ CPL ];
+          uint32_t sum = 0;
+#pragma unroll
+          for ( int j = 0; j < CPL; ++j )
           {
-            const uint32_t u = __shfl_up( x, off, 64 );
-            if ( lane >= off )
-              x += u;
+            c[ j ] = S.bcnt[ cur ][ CPL * t + j ];
+            sum += c[ j ];
           }
-          if ( lane == 63 )
-            S.sh[ wid ] = x;
-          S.bcnt[ cur ^ 1 ][ t ] = 0;  // counters of the next round
-          __syncthreads();
-          uint32_t pre = 0, tot = 0;
-          for ( int q = 0; q < BS / 64; ++q )
+          uint32_t run = wave_incl_scan( sum ) - sum;
+#pragma unroll
+          for ( int j = 0; j < CPL; ++j )
           {
-            if ( q < wid )
-              pre += S.sh[ q ];
-            tot += S.sh[ q ];
+            S.bst[ CPL * t + j ] = run;
+            run += c[ j ];
           }
-          mybase = pre + x - mycnt;
-          if ( t < tb )
-            S.bst[ t ] = mybase;
-          if ( t == 0 )
-            S.bst[ tb ] = tot;
-          __syncthreads();
+          if ( t == 63 )
+            S.bst[ BS ] = run;
         }
+        S.bcnt[ cur ^ 1 ][ t ] = 0;
+        __syncthreads();
         NG_PHASE( 3 );  // bucket starts
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
