@@ -38,16 +38,16 @@ iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
 void
 Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const
 {
-  // Block size: about two blocks per compute unit of an MI355X (256 CUs), multiple of 32,
-  // few enough blocks for the per-instance limit. Window delivery stages as many of the
-  // max_delay slots a slice can reach as fit the LDS window at that block size.
+  // Block size: at most two blocks per compute unit of an MI355X (256 CUs, two workgroups
+  // each), multiple of 8, at least 32, few enough blocks for the per-instance limit. The
+  // smallest such block leaves the most ring slots in the LDS window.
   const uint32_t pmax = delivery_mode_ == 1 ? PMAX_WIN : PMAX_GLOB;
   if ( targets_per_block_ > 0 )
     tb = static_cast< uint32_t >( targets_per_block_ );
   else
   {
     tb = ( n + 511 ) / 512;
-    tb = ( ( tb + 31 ) / 32 ) * 32;
+    tb = ( ( tb + 7 ) / 8 ) * 8;
     tb = std::max< uint32_t >( tb, 32 );
     tb = std::max< uint32_t >( tb, ( n + pmax - 1 ) / pmax );
     tb = std::min< uint32_t >( tb, MAX_TB );

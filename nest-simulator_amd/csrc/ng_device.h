@@ -67,9 +67,9 @@ struct SynRec
 
 // ring-buffer slots staged in LDS per workgroup and slice (window delivery), in doubles;
 // sized so that two window workgroups fit in a CU's LDS
-constexpr uint32_t WIN_CAP = 5888;
+constexpr uint32_t WIN_CAP = 6264;
 constexpr uint32_t MAX_TB = 512;      // targets per block = threads per workgroup
-constexpr uint32_t PMAX_WIN = 1024;   // blocks per instance, window delivery
+constexpr uint32_t PMAX_WIN = 512;    // blocks per instance, window delivery
 constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
 constexpr int MAX_PASSES = 3;         // slot-range passes of global delivery
 

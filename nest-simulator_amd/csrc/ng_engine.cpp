@@ -380,11 +380,8 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
   const uint32_t total = prefix[ n ];
   if ( total == 0 )
     return false;
-  // the window kernel keeps the block-count prefix of every instance in a smaller LDS table
-  for ( int i = 0; win and i < n; ++i )
-    if ( engines[ i ]->net().n_blocks > PMAX_WIN )
-      throw std::runtime_error( "NotImplemented: an instance with more than " + std::to_string( PMAX_WIN )
-        + " target blocks cannot run beside window-delivery instances; raise its targets_per_block" );
+  // the window kernel runs instances with a window (at most PMAX_WIN blocks, checked at
+  // build) and without one (their block prefix lives in the unused window)
   if ( prefix != prefix_ )
   {
     hip_check( hipMemcpy( d_prefix_, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice ), "prefix" );

@@ -110,6 +110,8 @@ load_rec( gptr< const SynRec > p )
 constexpr int LB = 8;  // lags of ring-buffer input loaded together in the update
 static_assert( Tile< false >::R <= ( 1 << IDX_BITS ) and Tile< true >::R <= ( 1 << IDX_BITS ),
   "round index must fit the bucket key" );
+static_assert( ( PMAX_GLOB + 1 ) * sizeof( uint32_t ) <= WIN_CAP * sizeof( double ),
+  "the window must hold the block prefix of an instance without a window" );
 
 template < bool WIN >
 struct SliceLds
@@ -438,6 +440,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   // ================================================================ 1. delivery
   if ( ph.deliver )
   {
+    // prefix of per-block spike counts: an instance without a window in the window kernel
+    // (up to PMAX_GLOB blocks) keeps it in the unused window
+    uint32_t* const pref = ( WIN and not C.win ) ? reinterpret_cast< uint32_t* >( S.win ) : S.pref;
     S.bcnt[ 0 ][ t ] = 0;
     S.bcnt[ 1 ][ t ] = 0;
     int cur = 0;  // counter buffer of the current round
@@ -456,11 +461,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         uint32_t ex;
         const uint32_t tot = block_scan< BS >( v, &ex, S.sh );
         if ( p0 + t < P )
-          S.pref[ p0 + t ] = carry + ex;
+          pref[ p0 + t ] = carry + ex;
         carry += tot;
       }
       if ( t == 0 )
-        S.pref[ P ] = carry;
+        pref[ P ] = carry;
       K = carry;
     }
     __syncthreads();
@@ -484,17 +489,17 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       }
       else if ( kb < K )
       {
-        uint32_t blk = upper_bound_u32( S.pref, P + 1, kb ) - 1;
+        uint32_t blk = upper_bound_u32( pref, P + 1, kb ) - 1;
 #pragma unroll
         for ( int q = 0; q < SPT; ++q )
         {
           const uint32_t k = kb + q;
           if ( k < K )
           {
-            while ( S.pref[ blk + 1 ] <= k )
+            while ( pref[ blk + 1 ] <= k )
               ++blk;
             const uint32_t e =
-              glob( d.blk_spk )[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - S.pref[ blk ] ) ];
+              glob( d.blk_spk )[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - pref[ blk ] ) ];
             src[ q ] = blk * tb + ( e >> 16 );
             lag[ q ] = e & 0xffu;
           }
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       NG_PHASE( 1 );
       events += E;
       // rounds of R events; the synapse records of round r + 1 are loaded while round r
-      // is applied
+      // is bucketed and applied
       SynRec rec[ RQ ];
       uint32_t rlag[ RQ ];
       // Every thread issues RQ record loads (events past the end reload the last record):
@@ -628,6 +633,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             }
           }
         }
+        // the records of this round are consumed: load the next round's now, so they have the
+        // whole round to arrive (past the last round: reloads record E - 1, unused)
+        load_round( e0 + R );
         __syncthreads();
         NG_PHASE( 2 );  // stage
         // every thread has finished the previous round (its cell stores included): the cells
@@ -692,7 +700,6 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           }
         __syncthreads();
         NG_PHASE( 4 );  // place and rank
-        load_round( e0 + R );  // past the last round: reloads record E - 1, unused
         // the first event of each cell's run adds the whole run in key order; runs of
         // different cells proceed in parallel
 #pragma unroll

@@ -264,10 +264,15 @@ class Kernel:
         self._chk(_lib.nestgpu_set_kernel_status(self._h, _to_json(params).encode()))
 
     def GetKernelStatus(self, key=None):
+        # a served kernel changes between calls: retry until the text fits
         n = ctypes.c_size_t(0)
         self._chk(_lib.nestgpu_get_kernel_status(self._h, None, 0, ctypes.byref(n)))
-        buf = ctypes.create_string_buffer(n.value + 1)
-        self._chk(_lib.nestgpu_get_kernel_status(self._h, buf, n.value + 1, ctypes.byref(n)))
+        while True:
+            cap = n.value + 64
+            buf = ctypes.create_string_buffer(cap)
+            self._chk(_lib.nestgpu_get_kernel_status(self._h, buf, cap, ctypes.byref(n)))
+            if n.value < cap:
+                break
         d = json.loads(buf.value.decode())
         return d if key is None else d[key]
 
@@ -402,12 +407,16 @@ class Server:
                 "events": ev.value}
 
     def stats(self):
+        # the server thread changes the figures between calls: retry until the text fits
         n = ctypes.c_size_t(0)
         _lib.nestgpu_server_stats(self._h, None, 0, ctypes.byref(n))
-        buf = ctypes.create_string_buffer(n.value + 1)
-        if _lib.nestgpu_server_stats(self._h, buf, n.value + 1, ctypes.byref(n)) != 0:
-            raise NESTError(_lib.nestgpu_last_error(None).decode())
-        return json.loads(buf.value.decode())
+        while True:
+            cap = n.value + 64
+            buf = ctypes.create_string_buffer(cap)
+            if _lib.nestgpu_server_stats(self._h, buf, cap, ctypes.byref(n)) != 0:
+                raise NESTError(_lib.nestgpu_last_error(None).decode())
+            if n.value < cap:
+                return json.loads(buf.value.decode())
 
 
 # ------------------------------------------------------------------ module-level kernel (PyNEST style)

@@ -89,3 +89,23 @@ def test_autotune_leaves_results_unchanged():
     for i in range(len(specs)):
         assert ka[i].GetKernelStatus("targets_per_block") == chosen[i]
         assert _same(_result(pa[i], ra[i]), _result(pp[i], rp[i])), i
+
+
+def test_batch_mixes_window_and_large_global_instances():
+    # a window-delivery instance and a global one with more blocks than a window instance may
+    # have (20,000 neurons in blocks of 32: 625 blocks) share the window kernel's launches
+    specs = _instances(2, seed=23, n_min=2000, n_max=3000)
+    specs.append(dict(specs[0], N=20000))
+    layouts = [None, None, {"delivery_mode": "global", "targets_per_block": 32}]
+    built = [_build([sp], lay) for sp, lay in zip(specs, layouts)]
+    plain = [_build([sp], lay) for sp, lay in zip(specs, layouts)]
+    kb = [b[0][0] for b in built]
+    nestgpu.batch_simulate(kb, 45.0)
+    for p in plain:
+        p[0][0].Simulate(45.0)
+    assert [k.GetKernelStatus("window_delivery") for k in kb] == [True, True, False]
+    assert kb[2].GetKernelStatus("targets_per_block") == 32
+    for b, p in zip(built, plain):
+        a, c = _result(b[1][0], b[2][0]), _result(p[1][0], p[2][0])
+        assert len(a[0]) > 0
+        assert _same(a, c)
