@@ -13,10 +13,11 @@
 //
 // Window delivery: the slots a slice can touch through recurrent synapses are the
 // max_delay slots starting at the slice origin (lag + delay - min_delay < max_delay), and
-// the update reads the first min_delay of them. In window mode the workgroup copies those
-// slots of its block into LDS, accumulates there and writes them back once, so the
-// read-modify-write of every synaptic event is an LDS operation. Global mode (short
-// slices with sparse events) updates the ring buffer in HBM directly.
+// the update reads the first min_delay of them. In window mode the workgroup copies the wn
+// consecutive slots [wa, wa + wn) of those that receive the most events into LDS,
+// accumulates there and writes them back once, so the read-modify-write of most synaptic
+// events is an LDS operation; cells outside the window are updated in HBM. Global mode
+// updates every cell in HBM.
 //
 // Determinism: NEST adds into one ring-buffer cell in the order (delivery batch, receive
 // order of the spike, position in the connector run). A workgroup takes events in that key
