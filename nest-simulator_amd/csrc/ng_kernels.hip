@@ -217,19 +217,34 @@ block_scan( uint32_t v, uint32_t* excl, uint32_t* sh )
   return tot;
 }
 
-// Poisson count for seeded drive: splitmix64 of (seed, stream, step), inverse CDF
-template < class Table >
-__host__ __device__ inline uint32_t
-poisson_count( const Table& T, uint64_t stream, int64_t step )
+// Poisson count for seeded drive: splitmix64 of (seed, stream, step) gives u in [0, 1), the
+// count is the first k with u < cdf[k] (at most kmax). The first KC entries of the table are
+// held in registers; the loop continues in HBM only past them.
+constexpr int KC = 8;
+struct PoissonCache
 {
-  uint64_t z = T.seed + stream * 0x9E3779B97F4A7C15ull + static_cast< uint64_t >( step ) * 0xD1B54A32D192ED03ull;
+  double c[ KC ];
+  uint64_t seed;
+  int32_t kmax;
+  gptr< const PoissonTable > T;
+};
+
+__device__ inline uint32_t
+poisson_count( const PoissonCache& P, uint64_t stream, int64_t step )
+{
+  uint64_t z = P.seed + stream * 0x9E3779B97F4A7C15ull + static_cast< uint64_t >( step ) * 0xD1B54A32D192ED03ull;
   z = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
   z = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
   z = z ^ ( z >> 31 );
   const double u = static_cast< double >( z >> 11 ) * ( 1.0 / 9007199254740992.0 );
   uint32_t k = 0;
-  while ( static_cast< int32_t >( k ) < T.kmax and u >= T.cdf[ k ] )
-    ++k;
+#pragma unroll
+  for ( int i = 0; i < KC; ++i )
+    if ( k == static_cast< uint32_t >( i ) and i < P.kmax and u >= P.c[ i ] )
+      k = i + 1;
+  if ( k == KC )
+    while ( static_cast< int32_t >( k ) < P.kmax and u >= P.T->cdf[ k ] )
+      ++k;
   return k;
 }
 
@@ -290,14 +305,51 @@ __device__ inline void
 drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
   int64_t origin, int from, int to )
 {
-  if ( ps.drive_mode == 0 )
+  const int32_t mode = ps.drive_mode;
+  if ( mode == 0 )
     return;
   const uint32_t col = glob( d.drive_idx )[ dev ];
   if ( col == NONE )
     return;
+  // the drive parameters in registers (the ring stores below could alias them in HBM)
+  const int64_t t0 = ps.drive_t0, n_steps = ps.drive_steps;
+  const double w = ps.drive_w;
+  const int32_t dd = ps.drive_d;
+  PoissonCache PC;
+  if ( mode == 2 )
+  {
+    PC.T = glob( d.ptables ) + ps.drive_table;
+    PC.seed = PC.T->seed;
+    PC.kmax = PC.T->kmax;
+#pragma unroll
+    for ( int i = 0; i < KC; ++i )
+      PC.c[ i ] = PC.T->cdf[ i ];
+  }
+  // the channel and multiplier of the weight: models/*::handle(SpikeEvent&)
+  uint32_t ch;
+  double sw;
+  if ( m == M_IAF_PSC_DELTA )
+  {
+    ch = 0;
+    sw = w;
+  }
+  else if ( m == M_IAF_PSC_EXP )
+  {
+    ch = w > 0.0 ? 1 : 0;
+    sw = w;
+  }
+  else if ( w > 0.0 )
+  {
+    ch = 0;
+    sw = w;
+  }
+  else
+  {
+    ch = 1;
+    sw = -w;
+  }
   for ( int l0 = from; l0 < to; l0 += LB )
   {
-    uint32_t at[ LB ], ch[ LB ];
     double add[ LB ];
     bool has[ LB ];
 #pragma unroll
@@ -306,49 +358,27 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
       has[ q ] = false;
       const int lag = l0 + q;
       const int64_t e = origin + lag;  // emission step; spike stamp e + 1
-      if ( lag >= to or e < ps.drive_t0 or e >= ps.drive_t0 + ps.drive_steps )
+      if ( lag >= to or e < t0 or e >= t0 + n_steps )
         continue;
       uint32_t c;
-      if ( ps.drive_mode == 1 )
-        c = glob( d.drive_counts )[ static_cast< uint64_t >( e - ps.drive_t0 ) * d.drive_cols + col ];
+      if ( mode == 1 )
+        c = glob( d.drive_counts )[ static_cast< uint64_t >( e - t0 ) * d.drive_cols + col ];
       else
-        c = poisson_count( glob( d.ptables )[ ps.drive_table ], col, e );
+        c = poisson_count( PC, col, e );
       if ( c == 0 )
         continue;
-      const double w = ps.drive_w;
-      const double mult = static_cast< double >( c );
-      at[ q ] = static_cast< uint32_t >( lag + ps.drive_d );
       has[ q ] = true;
-      if ( m == M_IAF_PSC_DELTA )
-      {
-        ch[ q ] = 0;
-        add[ q ] = w * mult;
-      }
-      else if ( m == M_IAF_PSC_EXP )
-      {
-        add[ q ] = w * mult;
-        ch[ q ] = add[ q ] > 0 ? 1 : 0;
-      }
-      else if ( w > 0.0 )
-      {
-        ch[ q ] = 0;
-        add[ q ] = w * mult;
-      }
-      else
-      {
-        ch[ q ] = 1;
-        add[ q ] = -w * mult;
-      }
+      add[ q ] = sw * static_cast< double >( c );
     }
     double old[ LB ];
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        old[ q ] = C.get( at[ q ], ch[ q ], local );
+        old[ q ] = C.get( static_cast< uint32_t >( l0 + q + dd ), ch, local );
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        C.put( at[ q ], ch[ q ], local, old[ q ] + add[ q ] );
+        C.put( static_cast< uint32_t >( l0 + q + dd ), ch, local, old[ q ] + add[ q ] );
   }
 }
 
