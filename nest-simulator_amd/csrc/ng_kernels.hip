@@ -303,7 +303,7 @@ struct Cells
 template < bool WIN >
 __device__ inline void
 drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uint32_t local, const Cells< WIN >& C,
-  int64_t origin, int from, int to )
+  int64_t origin, int from, int stride, int to )
 {
   const int32_t mode = ps.drive_mode;
   if ( mode == 0 )
@@ -348,7 +348,7 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
     ch = 1;
     sw = -w;
   }
-  for ( int l0 = from; l0 < to; l0 += LB )
+  for ( int l0 = from; l0 < to; l0 += LB * stride )
   {
     double add[ LB ];
     bool has[ LB ];
@@ -356,7 +356,7 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
     for ( int q = 0; q < LB; ++q )
     {
       has[ q ] = false;
-      const int lag = l0 + q;
+      const int lag = l0 + q * stride;
       const int64_t e = origin + lag;  // emission step; spike stamp e + 1
       if ( lag >= to or e < t0 or e >= t0 + n_steps )
         continue;
@@ -374,11 +374,11 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        old[ q ] = C.get( static_cast< uint32_t >( l0 + q + dd ), ch, local );
+        old[ q ] = C.get( static_cast< uint32_t >( l0 + q * stride + dd ), ch, local );
 #pragma unroll
     for ( int q = 0; q < LB; ++q )
       if ( has[ q ] )
-        C.put( static_cast< uint32_t >( l0 + q + dd ), ch, local, old[ q ] + add[ q ] );
+        C.put( static_cast< uint32_t >( l0 + q * stride + dd ), ch, local, old[ q ] + add[ q ] );
   }
 }
 
@@ -774,9 +774,27 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   __syncthreads();  // window staged and delivered
 
   // ================================================================ 2. update
+  // Threads [0, tb) update one neuron each. The spike_generator drive of the slice lands in
+  // cells at least min_delay ahead (every delay, the drive's included, is >= min_delay), which
+  // the update does not touch, and in distinct cells per lag: when tb <= BS / 2 the spare
+  // threads add it concurrently, G groups of tb threads splitting the lags.
   const uint32_t local = t;
   const uint32_t dev = b * tb + local;
+  const uint32_t G = ( static_cast< uint32_t >( BS ) - tb ) / tb;
   uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
+  if ( local >= tb and local < tb + G * tb )
+  {
+    const uint32_t dl = ( local - tb ) % tb, g = ( local - tb ) / tb, ddev = b * tb + dl;
+    if ( ddev < d.n )
+    {
+      gref< const ParamSet > dps = glob( d.psets )[ glob( d.pidx )[ ddev ] ];
+      const int dm = ( MASK == 1 ) ? M_IAF_PSC_DELTA
+        : ( MASK == 2 )            ? M_IAF_PSC_EXP
+        : ( MASK == 4 )            ? M_AEIF_COND_EXP
+                                   : dps.model;
+      drive_adds( d, dps, dm, ddev, dl, C, ph.origin, ph.from + static_cast< int >( g ), static_cast< int >( G ), ph.to );
+    }
+  }
   if ( local < tb and dev < d.n )
   {
     gref< const ParamSet > ps = glob( d.psets )[ glob( d.pidx )[ dev ] ];
@@ -934,7 +952,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       glob( d.v[ 4 ] )[ dev ] = hint;
       glob( d.r )[ dev ] = r;
     }
-    drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, ph.to );
+    if ( G == 0 )
+      drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, 1, ph.to );
   }
   NG_PHASE( 6 );
 
