@@ -17,6 +17,7 @@ on the same network for a short biological window on this host's cores.
 """
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -133,7 +134,8 @@ def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True):
 def run_batched(args, nestgpu):
     """Batched form: every instance advances in the same slice launches (one grid over all
     blocks of all instances, replayed from a hipGraph). value = aggregate recurrent synaptic
-    events/s of the batch; the same instances then continue one by one on the plain path."""
+    events/s of the batch; every 4th instance is then built again with the library's default
+    layout and run one by one on the plain path."""
     from concurrent.futures import ThreadPoolExecutor
 
     specs = c4_specs(args.instances, args.seed)
@@ -175,17 +177,28 @@ def run_batched(args, nestgpu):
     rtf = wall / (args.steps * args.t_step * 1e-3)
     print("batched: %d x %g ms, %.3f s wall, RTF %.3f, %.3g events/s" % (args.steps, args.t_step, wall, rtf, value),
           file=sys.stderr, flush=True)
-    # the same instances, continued one at a time through the plain path
+    # the plain path: every 4th instance, built again with the library's default layout (the
+    # layout a single run would get), warmed up, then run one at a time
+    kernels = None
+    gc.collect()
+    sub = specs[::4]
+    plain = [nestgpu.Kernel(0) for _ in sub]
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False), zip(plain, sub)))
+    for k in plain:
+        k.Prepare()
+        k.Run(args.plain_warm_ms)
+    ev0 = sum(k.GetKernelStatus("synaptic_events") for k in plain)
     plain_wall = 0.0
-    ev0 = events()
-    for k in kernels:
+    for k in plain:
         t0 = time.perf_counter()
         k.Run(args.plain_ms)
         plain_wall += time.perf_counter() - t0
-    ev_p = events() - ev0
+    ev_p = sum(k.GetKernelStatus("synaptic_events") for k in plain) - ev0
     plain_value = ev_p / plain_wall
-    print("plain one by one: %g ms each, %.3f s wall, %.3g events/s; batched / plain = %.2f" % (
-        args.plain_ms, plain_wall, plain_value, value / plain_value), file=sys.stderr, flush=True)
+    print("plain one by one (%d instances, default layout, after %g ms warm-up): %g ms each, %.3f s wall, "
+          "%.3g events/s; batched / plain = %.2f" % (len(plain), args.plain_warm_ms, args.plain_ms, plain_wall,
+                                                    plain_value, value / plain_value), file=sys.stderr, flush=True)
     models = {m: sum(1 for sp in specs if sp["model"] == m) for m in ("iaf_psc_exp", "aeif_cond_exp")}
     line = {
         "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
@@ -198,7 +211,8 @@ def run_batched(args, nestgpu):
                    "n_instances": len(specs), "n_neurons": n_neurons, "n_synapses": n_syn, "build_s": build_s,
                    "targets_per_block": sorted(set(chosen)) if chosen else "library default"},
         "batched_vs_plain": value / plain_value,
-        "plain_path": {"value": plain_value, "wall_s": plain_wall, "ms_per_instance": args.plain_ms},
+        "plain_path": {"value": plain_value, "wall_s": plain_wall, "ms_per_instance": args.plain_ms,
+                       "instances": len(plain), "layout": "library default", "warm_ms": args.plain_warm_ms},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -368,6 +382,8 @@ def main():
                          "8-population circuit with min_delay one step; c4: batched sweep of many instances")
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
     ap.add_argument("--plain-ms", type=float, default=100.0, help="c4: biological ms per instance on the plain path")
+    ap.add_argument("--plain-warm-ms", type=float, default=100.0,
+                    help="c4: untimed biological ms per instance before the plain-path timing")
     ap.add_argument("--autotune", action="store_true", help="c4: pick targets_per_block per size class first")
     ap.add_argument("--probe-ms", type=float, default=30.0, help="c4: autotuner probe length")
     ap.add_argument("--calibrate-copy", action="store_true",

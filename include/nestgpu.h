@@ -90,11 +90,13 @@ int nestgpu_recorder_events( nestgpu_kernel* k, int64_t recorder_id, int64_t* se
 int nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms );
 
 /* Layout autotuner of the batched form (no counterpart in the reference): groups the kernels
- * (none simulated yet) into size classes by neuron count (one per octave), times probe runs of
- * probe_ms on up to 4 instances of each class for every candidate targets_per_block, and sets
- * the fastest on every instance of the class (global delivery). Probe runs are discarded: every
- * kernel returns to its state before the first Prepare. chosen[i] (may be NULL) receives the
- * value picked for kernel i. */
+ * (none simulated yet) into classes by neuron models and neuron count (one per octave), times
+ * probe runs of probe_ms on up to 4 instances of each class for every candidate
+ * targets_per_block, and sets the cheapest on every instance of the class (each keeps its
+ * delivery_mode). A probe's cost is its block-time, wall time x min(blocks, 2 x compute units):
+ * its share of a batch that fills the device. Probe runs are discarded: every kernel returns
+ * to its state before the first Prepare. chosen[i] (may be NULL) receives the value picked for
+ * kernel i. */
 int nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand,
   int* chosen );
 

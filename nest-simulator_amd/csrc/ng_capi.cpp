@@ -284,8 +284,8 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
     if ( n_cand <= 0 or candidates == nullptr )
       throw std::runtime_error( "batch_autotune needs at least one candidate targets_per_block" );
     const int dev = ks[ 0 ]->device;
-    // size classes: one per octave of the neuron count
-    std::map< int, std::vector< int > > classes;
+    // classes: one per set of neuron models and octave of the neuron count
+    std::map< std::pair< int, int >, std::vector< int > > classes;
     for ( int i = 0; i < n; ++i )
     {
       if ( ks[ i ]->device != dev )
@@ -294,8 +294,16 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
       int c = 0;
       while ( ( uint64_t( 2 ) << c ) <= nn )
         ++c;
-      classes[ c ].push_back( i );
+      classes[ { ks[ i ]->k->model_mask(), c } ].push_back( i );
     }
+    // A probe of a few instances may not fill the device, so its wall time measures one
+    // block's latency rather than throughput. The score is the probe's block-time,
+    // ms * min(blocks, slots) with two workgroups per compute unit: what the class costs
+    // inside a batch that fills the device.
+    int n_cu = 0;
+    if ( hipDeviceGetAttribute( &n_cu, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess or n_cu <= 0 )
+      n_cu = 256;
+    const double slots = 2.0 * n_cu;
     nestgpu::Runner runner( dev, 64 );
     for ( auto& kv : classes )
     {
@@ -322,7 +330,7 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
           keep.push_back( ks[ i ]->k->keep_host_connections() );
           ks[ i ]->k->set_keep_host_connections( true );  // probes rebuild the device copy
           ks[ i ]->k->reset_device();
-          ks[ i ]->k->set_layout( tb, 2 );
+          ks[ i ]->k->set_layout( tb, ks[ i ]->k->delivery_mode() );
           ks[ i ]->k->prepare_for_batch();
           engines.push_back( &ks[ i ]->k->engine() );
           steps.push_back( ks[ i ]->k->steps_for( probe_ms ) );
@@ -330,7 +338,11 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
         runner.run( engines, steps );  // warm-up: graph capture, caches
         const auto t0 = std::chrono::steady_clock::now();
         runner.run( engines, steps );
-        const double ms = std::chrono::duration< double, std::milli >( std::chrono::steady_clock::now() - t0 ).count();
+        double blocks = 0;
+        for ( int i : sample )
+          blocks += static_cast< double >( ( ks[ i ]->k->num_neurons() + tb - 1 ) / tb );
+        const double ms = std::chrono::duration< double, std::milli >( std::chrono::steady_clock::now() - t0 ).count()
+          * std::min( blocks, slots );
         if ( best < 0 or ms < best_ms )
         {
           best = tb;
@@ -347,7 +359,7 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
       for ( int i : members )
       {
         ks[ i ]->k->reset_device();
-        ks[ i ]->k->set_layout( best, 2 );
+        ks[ i ]->k->set_layout( best, ks[ i ]->k->delivery_mode() );
         if ( chosen )
           chosen[ i ] = best;
       }

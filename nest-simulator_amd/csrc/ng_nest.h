@@ -299,6 +299,17 @@ public:
   {
     return params_.size();
   }
+  int delivery_mode() const  // 0 automatic, 1 window, 2 global
+  {
+    return delivery_mode_;
+  }
+  int model_mask() const  // bit m set when a neuron of model id m exists
+  {
+    int mask = 0;
+    for ( const auto& p : params_ )
+      mask |= 1 << p.model;
+    return mask;
+  }
   void set_runner( Runner* r )
   {
     runner_ = r;

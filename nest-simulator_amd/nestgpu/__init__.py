@@ -353,8 +353,9 @@ def batch_simulate(kernels, times):
 
 
 def batch_autotune(kernels, probe_ms=20.0, candidates=(64, 128, 256, 512)):
-    """Pick targets_per_block per size class of not yet simulated kernels by timing probe
-    runs (discarded). Returns the value chosen for each kernel."""
+    """Pick targets_per_block per class (neuron models, size octave) of not yet simulated
+    kernels from probe runs (discarded), scored by block-time. Returns the value chosen for
+    each kernel."""
     arr = (_vp * len(kernels))(*[k._h for k in kernels])
     cand = (ctypes.c_int * len(candidates))(*[int(c) for c in candidates])
     out = (ctypes.c_int * len(kernels))()
