@@ -916,36 +916,57 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         glob( d.v[ 3 ] )[ dev ] };
       double hint = glob( d.v[ 4 ] )[ dev ];
       int32_t r = glob( d.r )[ dev ];
-      for ( int lag = ph.from; lag < ph.to; ++lag )
+      // One loop over (lag, accepted step): each iteration makes one adaptive step of the
+      // lane's neuron, or, once the neuron has reached the end of its lag, closes the lag.
+      // Lanes advance through their own lags independently, so a wave costs its slowest
+      // neuron's total number of steps, not the sum over lags of the slowest per lag. Each
+      // neuron's sequence of operations is that of the lag loop with an inner step loop.
+      int lag = ph.from;
+      double tt = 0.0;
+      int guard = 0;
+      while ( lag < ph.to )
       {
-        double tt = 0.0;
-        for ( int guard = 0; tt < hstep and guard < 1000000; ++guard )
+        if ( tt < hstep and guard < 1000000 )
         {
+          ++guard;
           aeif_evolve( A, r > 0, &tt, hstep, &hint, y );
           if ( y[ 0 ] < -1e3 or y[ 3 ] < -1e6 or y[ 3 ] > 1e6 )
           {
             g_or( &d.counters[ 0 ], ERR_AEIF_INSTABILITY );
             tt = hstep;
-            break;
           }
-          if ( r > 0 )
+          else if ( r > 0 )
             y[ 0 ] = A.V_reset;
           else if ( y[ 0 ] >= V_peak_eff )
           {
             y[ 0 ] = A.V_reset;
             y[ 3 ] += A.b;
             r = ps.refractory_counts > 0 ? ps.refractory_counts + 1 : 0;
-            if ( bits[ lag >> 6 ] & ( 1ull << ( lag & 63 ) ) )
+            const uint64_t bit = 1ull << ( lag & 63 );
+            uint64_t had = 0;
+#pragma unroll
+            for ( int w = 0; w < 4; ++w )  // lag differs per lane: no dynamic register index
+              if ( ( lag >> 6 ) == w )
+              {
+                had = bits[ w ] & bit;
+                bits[ w ] |= bit;
+              }
+            if ( had )
               g_or( &d.counters[ 0 ], ERR_AEIF_DOUBLE_SPIKE );
-            bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
           }
         }
-        if ( r > 0 )
-          --r;
-        y[ 1 ] += C.get( lag, 0, local );
-        y[ 2 ] += C.get( lag, 1, local );
-        C.put( lag, 0, local, 0.0 );
-        C.put( lag, 1, local, 0.0 );
+        else
+        {
+          if ( r > 0 )
+            --r;
+          y[ 1 ] += C.get( lag, 0, local );
+          y[ 2 ] += C.get( lag, 1, local );
+          C.put( lag, 0, local, 0.0 );
+          C.put( lag, 1, local, 0.0 );
+          ++lag;
+          tt = 0.0;
+          guard = 0;
+        }
       }
       for ( int q = 0; q < 4; ++q )
         glob( d.v[ q ] )[ dev ] = y[ q ];
