@@ -1057,39 +1057,64 @@ nestgpu_profile_read( unsigned long long* out8, int reset )
 
 // A slice split across Run calls: NEST's per-thread registers hold spikes part by part,
 // so its receive order is (VP, part, node, lag). Rebuild that order once the slice is
-// complete (rare path, one thread).
-__global__ void
-k_flatten( NetDev* net, int par, int n_parts )
+// complete. One workgroup: for each (VP, part), each thread takes one block of the VP's
+// range, counts its matching entries, a block scan gives the output offsets, and each
+// thread copies its entries in list order.
+constexpr int FLATTEN_BS = 1024;
+
+__global__ void __launch_bounds__( FLATTEN_BS )
+  k_flatten( NetDev* net, int par, int n_parts )
 {
-  if ( blockIdx.x != 0 or threadIdx.x != 0 )
-    return;
+  __shared__ uint32_t sh[ FLATTEN_BS / 64 ];
   const NetDev& d = *net;
   const uint32_t P = d.n_blocks;
-  uint32_t o = 0;
+  const uint32_t t = threadIdx.x;
+  uint32_t o = 0;  // output entries so far (the same in every thread)
   for ( int vp = 0; vp < d.n_vp; ++vp )
   {
     const uint32_t lo = d.vp_begin[ vp ], hi = d.vp_begin[ vp + 1 ];
     if ( lo >= hi )
       continue;
+    const uint32_t b0 = lo / d.tb;
+    const uint32_t b1 = min( ( hi - 1 ) / d.tb + 1, P );
     for ( int p = 0; p < n_parts; ++p )
-      for ( uint32_t b = lo / d.tb; b <= ( hi - 1 ) / d.tb and b < P; ++b )
+      for ( uint32_t c0 = b0; c0 < b1; c0 += FLATTEN_BS )
       {
-        const uint32_t n = d.blk_cnt[ static_cast< size_t >( par ) * P + b ];
-        const uint32_t* lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
-        for ( uint32_t q = 0; q < n and q < d.spk_cap; ++q )
+        const uint32_t b = c0 + t;
+        uint32_t n = 0, cnt = 0;
+        const uint32_t* lst = nullptr;
+        if ( b < b1 )
+        {
+          n = min( d.blk_cnt[ static_cast< size_t >( par ) * P + b ], d.spk_cap );
+          lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+          for ( uint32_t q = 0; q < n; ++q )
+          {
+            const uint32_t e = lst[ q ];
+            const uint32_t dev = b * d.tb + ( e >> 16 );
+            cnt += static_cast< int >( ( e >> 8 ) & 0xffu ) == p and dev >= lo and dev < hi ? 1u : 0u;
+          }
+        }
+        uint32_t ex;
+        const uint32_t tot = block_scan< FLATTEN_BS >( cnt, &ex, sh );
+        uint32_t w = o + ex;
+        for ( uint32_t q = 0; q < n; ++q )
         {
           const uint32_t e = lst[ q ];
           const uint32_t dev = b * d.tb + ( e >> 16 );
           if ( static_cast< int >( ( e >> 8 ) & 0xffu ) != p or dev < lo or dev >= hi )
             continue;
-          d.flat_src[ o ] = dev;
-          d.flat_lag[ o ] = e & 0xffu;
-          ++o;
+          d.flat_src[ w ] = dev;
+          d.flat_lag[ w ] = e & 0xffu;
+          ++w;
         }
+        o += tot;
       }
   }
-  d.flat_meta[ par ] = o;
-  d.flat_meta[ 2 + par ] = 1;
+  if ( t == 0 )
+  {
+    d.flat_meta[ par ] = o;
+    d.flat_meta[ 2 + par ] = 1;
+  }
 }
 
 // ------------------------------------------------------------------ launch wrappers
@@ -1165,7 +1190,7 @@ launch_status( const NetDev* nets, int n_nets, unsigned long long* out, hipStrea
 hipError_t
 launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st )
 {
-  k_flatten<<< 1, 64, 0, st >>>( net, par, n_parts );
+  k_flatten<<< 1, FLATTEN_BS, 0, st >>>( net, par, n_parts );
   return hipGetLastError();
 }
 
