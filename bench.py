@@ -412,8 +412,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X and the step "
-              "does not shard across GPUs here (see DESIGN.md for how it would).", file=sys.stderr)
+        print("bench.py measures one GPU: the slice step runs on a single MI355X here and is not "
+              "sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
 
     import nestgpu
@@ -473,7 +473,7 @@ def main():
     steps_r = int(round(args.roofline_ms / 0.1))
     min_delay = st2["min_delay_steps"]
     slices_r = steps_r // min_delay
-    # algorithmic bytes of the fused slice kernel (per-unit figures in DESIGN.md):
+    # algorithmic bytes of the fused slice kernel, per unit:
     #   28 B per recurrent synaptic event (12 B synapse word + weight, 16 B ring-buffer RMW)
     #   32 B per neuron-step (two ring-buffer channels read and cleared)
     #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
