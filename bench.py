@@ -254,11 +254,12 @@ def run_served(args, nestgpu):
         r = reqs[i]
         k = nestgpu.Kernel(0)
         build_instance(k, nestgpu, r["spec"], keep_host_connections=False)
+        built = time.perf_counter() - t_origin
         rid = srv.submit(k, r["calls"])
         info = srv.wait(rid)
         done = time.perf_counter() - t_origin
         with lock:
-            results[i] = dict(arrival=r["arrival"], done=done, events=info["events"], server=info)
+            results[i] = dict(arrival=r["arrival"], built=built, done=done, events=info["events"], server=info)
         del k
 
     with ThreadPoolExecutor(args.builders) as ex:
@@ -279,6 +280,15 @@ def run_served(args, nestgpu):
     print("served: %d requests, makespan %.1f s, p50 %.2f s, p99 %.2f s, %.3g events/s, max concurrent %d" % (
         len(reqs), makespan, np.percentile(lat, 50), np.percentile(lat, 99), value, stats["max_concurrent"]),
         file=sys.stderr, flush=True)
+    # where a request's latency goes (medians): host construction including the wait for a
+    # builder, the wait for HBM budget, device preparation up to the next chunk boundary, run
+    stages = {
+        "host_build_s": float(np.median([x["built"] - x["arrival"] for x in results])),
+        "admission_wait_s": float(np.median([x["server"]["t_admit"] - x["server"]["t_submit"] for x in results])),
+        "prepare_s": float(np.median([x["server"]["t_start"] - x["server"]["t_admit"] for x in results])),
+        "run_s": float(np.median([x["server"]["t_done"] - x["server"]["t_start"] for x in results])),
+    }
+    print("median stages: " + ", ".join("%s %.2f" % kv for kv in stages.items()), file=sys.stderr, flush=True)
     # serial plain path: each request alone (host build, then its Simulate calls)
     service = []
     for r in reqs:
@@ -304,7 +314,8 @@ def run_served(args, nestgpu):
         "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": len(reqs),
         "warmup": 0, "ms_per_step": 1e3 * makespan / len(reqs), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "median_stages": stages},
         "config": {"workload": "served: %d requests at %g QPS (Poisson arrivals), batched-sweep-style networks "
                                "N log-uniform [2k, 100k], T uniform [100, 2000] ms in 1-4 Simulate calls, HBM "
                                "budget %g GiB, %d host builders" % (len(reqs), args.qps, args.hbm_budget_gib,

@@ -15,7 +15,10 @@ Server::Server( int device, uint64_t hbm_budget_bytes, int slices_per_chunk )
   , S_( std::max( 1, slices_per_chunk ) )
   , t0_( std::chrono::steady_clock::now() )
 {
-  admit_thread_ = std::thread( [ this ] { admit_loop(); } );
+  // admission stays FIFO against the budget; the admitted requests' device copies are built
+  // by several preparers at once, so one large network does not hold up the ones behind it
+  for ( int i = 0; i < PREPARERS; ++i )
+    admit_threads_.emplace_back( [ this ] { admit_loop(); } );
   run_thread_ = std::thread( [ this ] { run_loop(); } );
 }
 
@@ -26,7 +29,8 @@ Server::~Server()
     stop_ = true;
   }
   cv_.notify_all();
-  admit_thread_.join();
+  for ( auto& th : admit_threads_ )
+    th.join();
   run_thread_.join();
 }
 
@@ -155,6 +159,11 @@ void
 Server::run_loop()
 {
   Runner runner( device_, S_ );
+  // slices are launched directly, not replayed from a captured graph: the preparers allocate
+  // and upload on the device while this loop runs, and HIP fails a stream capture when other
+  // threads make such calls during it (the batch changes at most chunk boundaries, so a
+  // graph would be captured often anyway)
+  runner.use_graph = false;
   std::vector< Req* > active;
   for ( ;; )
   {

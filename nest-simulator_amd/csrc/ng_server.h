@@ -44,6 +44,8 @@ struct RequestInfo
 class Server
 {
 public:
+  static constexpr int PREPARERS = 4;  // admitted requests whose device copies build at once
+
   Server( int device, uint64_t hbm_budget_bytes, int slices_per_chunk );
   ~Server();
   Server( const Server& ) = delete;
@@ -86,7 +88,8 @@ private:
   uint64_t chunks_ = 0, events_done_ = 0;
   double busy_s_ = 0;
   size_t max_concurrent_ = 0;
-  std::thread admit_thread_, run_thread_;
+  std::vector< std::thread > admit_threads_;  // preparers: admitted requests build concurrently
+  std::thread run_thread_;
 };
 
 }  // namespace nestgpu
