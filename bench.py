@@ -406,6 +406,7 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=128.0, help="c5: device memory budget of the server")
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
+    ap.add_argument("--window-slots", type=int, default=0, help="cap on ring slots in LDS; 0: as many as fit")
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="0: as many as fit")
     args = ap.parse_args()
 
@@ -441,7 +442,7 @@ def main():
     t_build = time.perf_counter()
     k = nestgpu.Kernel(0)
     k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block,
-                       "workgroups_per_cu": args.workgroups_per_cu})
+                       "workgroups_per_cu": args.workgroups_per_cu, "window_slots": args.window_slots})
     if args.workload == "c2":
         pop, sg = build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
         n_neurons = args.n
@@ -516,7 +517,7 @@ def main():
                          "delays clipped at dt, Poisson background 8 Hz x K_ext, dt 0.1 ms, one step = Run(%g ms)"
                          % (n_syn, args.t_step))
     default_layout = (args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workgroups_per_cu == 0
-                      and args.workload == "c2")
+                      and args.window_slots == 0 and args.workload == "c2")
     line = {
         "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",

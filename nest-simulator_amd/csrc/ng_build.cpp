@@ -91,6 +91,8 @@ Kernel::build_spec( BuildSpec& b )
     // emission lag uniform over the slice, a synapse of delay d feeds slots
     // [d - min_delay, d - 1] equally.
     b.wn = window_slots( b.max_delay, b.nch, b.tb );
+    if ( window_slots_ > 0 )
+      b.wn = std::min( b.wn, window_slots_ );
     std::vector< double > per_delay( b.max_delay + 1, 0.0 ), density( b.max_delay, 0.0 );
     for ( const auto& C : conns_ )
       for ( uint16_t dd : C.d )

@@ -286,6 +286,7 @@ Kernel::reset()
   targets_per_block_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
+  window_slots_ = 0;
   delivery_passes_ = 0;
   keep_host_connections_ = true;
   host_connections_freed_ = false;
@@ -376,6 +377,16 @@ Kernel::set_kernel_status( const Json& d )
       delivery_passes_ = v;
       store_dirty_ = true;
     }
+    else if ( k == "window_slots" )
+    {
+      const int v = static_cast< int >( kv.second.number() );
+      if ( v < 0 )
+        throw std::runtime_error( "BadProperty: window_slots must be >= 0 (0 = as many as fit)" );
+      if ( simulated_ and v != window_slots_ )
+        throw std::runtime_error( "KernelException: window_slots cannot change after Simulate" );
+      window_slots_ = v;
+      store_dirty_ = true;
+    }
     else if ( k == "workgroups_per_cu" )
     {
       const int v = static_cast< int >( kv.second.number() );
@@ -428,6 +439,7 @@ Kernel::get_kernel_status_json() const
     << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
     << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )
+    << ", \"window_slots\": " << ( engine_ and engine_->built() ? engine_->net().wn : window_slots_ )
     << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
   static const char* kn[] = { "slice", "flatten" };
   for ( int q = 0; q < N_KINDS; ++q )

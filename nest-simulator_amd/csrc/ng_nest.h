@@ -373,6 +373,7 @@ private:
   double last_kernel_ms_ = 0, last_wall_ms_ = 0;
   int slices_per_chunk_ = 64;
   int wg_per_cu_ = 0;
+  int window_slots_ = 0;  // cap on the ring slots staged in LDS (0: as many as fit)
   int delivery_passes_ = 0;  // global delivery in this many slot-range passes (0, 1: one)
   bool time_kernels_ = false;
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
