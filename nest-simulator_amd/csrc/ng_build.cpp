@@ -90,9 +90,11 @@ Kernel::build_spec( BuildSpec& b )
     // The window covers the wn consecutive slots that expect the most events: with the
     // emission lag uniform over the slice, a synapse of delay d feeds slots
     // [d - min_delay, d - 1] equally.
+    // Staging costs the same for every slice while a slice's events grow with its length:
+    // by default the window holds at most 6 slots per step of min_delay (the measured best on
+    // a one-step min_delay circuit; ten-step slices keep every slot that fits).
     b.wn = window_slots( b.max_delay, b.nch, b.tb );
-    if ( window_slots_ > 0 )
-      b.wn = std::min( b.wn, window_slots_ );
+    b.wn = std::min( b.wn, window_slots_ > 0 ? window_slots_ : 6 * b.min_delay );
     std::vector< double > per_delay( b.max_delay + 1, 0.0 ), density( b.max_delay, 0.0 );
     for ( const auto& C : conns_ )
       for ( uint16_t dd : C.d )
