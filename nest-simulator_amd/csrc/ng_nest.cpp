@@ -435,7 +435,7 @@ Kernel::get_kernel_status_json() const
     << ", \"time_update\": " << time_kernel_ms_[ K_SLICE ] * 1e-3
     << ", \"time_gather_spike_data\": " << time_kernel_ms_[ K_FLATTEN ] * 1e-3
     << ", \"time_deliver_spike_data\": 0, \"fused_deliver_update\": true"
-    << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? engine_->net().tb : 0 )
+    << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().tb ) : targets_per_block_ )
     << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
     << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )

@@ -13,9 +13,10 @@ pytestmark = pytest.mark.gpu
 
 
 # device layouts: automatic choice, LDS window with many small blocks, global ring buffer
-# win512 stages only a few of the ring's slots in LDS (partial window), win48 all of them
+# win512 and win2 stage only a few of the ring's slots in LDS (partial window), win48 all of them
 LAYOUTS = {"auto": {}, "win48": {"delivery_mode": "window", "targets_per_block": 48},
            "win512": {"delivery_mode": "window", "targets_per_block": 512},
+           "win2": {"delivery_mode": "window", "window_slots": 2},
            "glob32": {"delivery_mode": "global", "targets_per_block": 32},
            "glob64p3": {"delivery_mode": "global", "targets_per_block": 64, "delivery_passes": 3}}
 layouts = pytest.mark.parametrize("layout", list(LAYOUTS))
@@ -93,7 +94,7 @@ def test_split_equals_single_run(layout):
     assert np.array_equal(pops1[0].get("V_m"), pops2[0].get("V_m"))
 
 
-@pytest.mark.parametrize("layout", ["auto", "glob32", "win48", "win512", "glob64p3"])
+@pytest.mark.parametrize("layout", ["auto", "glob32", "win48", "win512", "win2", "glob64p3"])
 def test_bernoulli_min_delay_one_step(layout):
     # C3-like: pairwise_bernoulli, normal delays clipped at dt -> min_delay = 1 step
     params = {"tau_m": 10.0, "tau_syn_ex": 0.5, "tau_syn_in": 0.5, "C_m": 250.0, "V_th": -55.0,
@@ -155,7 +156,7 @@ def test_aeif_network(layout):
     _compare_tol(nb.aeif_random(N=1000, T=100.0), fields=("V_m", "w", "g_ex", "g_in"), layout=layout)
 
 
-@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "glob32", "glob64p3"])
+@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "win2", "glob32", "glob64p3"])
 def test_mixed_models(layout):
     desc = nb.mixed_models(T=100.0)
     k, pops, sr = nb.build_gpu(desc, layout=LAYOUTS[layout])

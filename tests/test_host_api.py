@@ -167,3 +167,18 @@ def test_poisson_counts_statistics_and_determinism():
     exp = stats.poisson(lam).pmf(ks) * c1.size
     assert np.all(np.abs(obs - exp) < 6 * np.sqrt(exp) + 5)
     assert not np.array_equal(c1, nestgpu.poisson_counts(12, lam, 100, 500, 0, 400))
+
+
+def test_layout_keys_validated_and_reported():
+    # device-layout kernel-status keys (no counterpart in the reference): accepted values are
+    # echoed back before a build, out-of-range values raise BadProperty
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"targets_per_block": 64, "delivery_mode": "window", "window_slots": 3,
+                       "delivery_passes": 1, "workgroups_per_cu": 2})
+    st = k.GetKernelStatus()
+    assert st["targets_per_block"] == 64 and st["delivery_mode"] == "window"
+    assert st["window_slots"] == 3 and st["workgroups_per_cu"] == 2
+    for bad in ({"window_slots": -1}, {"targets_per_block": 4096}, {"delivery_mode": "ring"},
+                {"delivery_passes": 9}, {"workgroups_per_cu": 9}):
+        with pytest.raises(nestgpu.NESTError, match="BadProperty"):
+            k.SetKernelStatus(bad)
