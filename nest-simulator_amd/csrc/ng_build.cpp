@@ -258,6 +258,11 @@ Kernel::build_spec( BuildSpec& b )
       ps.p[ 3 ] = Pm.V_th;
       ps.p[ 4 ] = Pm.V_reset;
       ps.p[ 5 ] = Pm.V_min;
+      // refractory_input: input arriving while refractory decays by exp(-r h / tau_m) and is
+      // added when the neuron becomes active again (iaf_psc_delta.cpp:284-299)
+      ps.p[ 6 ] = Pm.refr_input ? 1.0 : 0.0;
+      ps.p[ 7 ] = h;
+      ps.p[ 8 ] = Pm.tau_m;
     }
     else if ( Pm.model == M_IAF_PSC_EXP )
     {

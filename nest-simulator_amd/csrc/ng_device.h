@@ -26,6 +26,7 @@ enum ModelId : int32_t
 // Per parameter-set constants, computed on the host in pre_run_hook order with glibc libm.
 // Field meaning per model:
 //   iaf_psc_delta: p[0]=P30 p[1]=P33 p[2]=y0+I_e (=0.0+I_e) p[3]=V_th p[4]=V_reset p[5]=V_min
+//                  p[6]=refractory_input (1 or 0) p[7]=h p[8]=tau_m
 //   iaf_psc_exp:   p[0]=P22 p[1]=P21ex p[2]=P21in p[3]=P20 p[4]=P11ex p[5]=P11in
 //                  p[6]=I_e+i_0 (=I_e+0.0) p[7]=Theta p[8]=V_reset p[9]=(1-P11ex)*i_1 (=+0.0)
 //   aeif_cond_exp: p[0]=V_peak p[1]=V_reset p[2]=g_L p[3]=C_m p[4]=E_ex p[5]=E_in p[6]=E_L
@@ -90,7 +91,8 @@ struct NetDev
   int32_t np;             // global delivery: passes over slot ranges per chunk (1..MAX_PASSES)
   int32_t ws;             // slots per pass (np * ws >= max_delay)
   // ---- state by device index
-  double* v[ 5 ];     // delta: y3 | exp: V, I_ex, I_in | aeif: V, g_ex, g_in, w, IntegrationStep
+  double* v[ 5 ];     // delta: y3, refr_spikes_buffer | exp: V, I_ex, I_in | aeif: V, g_ex, g_in, w,
+                      // IntegrationStep
   int32_t* r;
   const uint16_t* pidx;
   const uint8_t* model;  // per neuron model id (used when uniform_model < 0)

@@ -804,10 +804,13 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
                                 : ps.model;
     if ( ( MASK & 1 ) and m == M_IAF_PSC_DELTA )
     {
-      // models/iaf_psc_delta.cpp:272-331 (refractory_input = false)
+      // models/iaf_psc_delta.cpp:272-331
       const double P30 = ps.p[ 0 ], P33 = ps.p[ 1 ], y0I = ps.p[ 2 ], V_th = ps.p[ 3 ], V_reset = ps.p[ 4 ],
                    V_min = ps.p[ 5 ];
+      const bool refr_input = ps.p[ 6 ] != 0.0;
+      const double h = ps.p[ 7 ], tau_m = ps.p[ 8 ];
       double y3 = glob( d.v[ 0 ] )[ dev ];
+      double refr_buf = refr_input ? glob( d.v[ 1 ] )[ dev ] : 0.0;  // refr_spikes_buffer_
       int32_t r = glob( d.r )[ dev ];
       // ring-buffer cells are read LB lags at a time (independent loads), then consumed in
       // lag order and cleared
@@ -827,10 +830,19 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             if ( r == 0 )
             {
               y3 = P30 * y0I + P33 * y3 + win_[ q ];
+              if ( refr_input and refr_buf != 0.0 )
+              {
+                y3 += refr_buf;
+                refr_buf = 0.0;
+              }
               y3 = ( y3 < V_min ? V_min : y3 );
             }
             else
+            {
+              if ( refr_input )
+                refr_buf += win_[ q ] * ng_exp( -static_cast< double >( r ) * h / tau_m );
               --r;
+            }
             if ( y3 >= V_th )
             {
               r = ps.refractory_counts;
@@ -840,6 +852,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           }
       }
       glob( d.v[ 0 ] )[ dev ] = y3;
+      if ( refr_input )
+        glob( d.v[ 1 ] )[ dev ] = refr_buf;
       glob( d.r )[ dev ] = r;
     }
     if ( ( MASK & 2 ) and m == M_IAF_PSC_EXP )

@@ -633,11 +633,7 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
     if ( q.tau_m <= 0 or ( q.model == M_IAF_PSC_EXP and ( q.tau_ex <= 0 or q.tau_in <= 0 ) ) )
       throw std::runtime_error( "BadProperty: Membrane and synapse time constants must be strictly positive." );
     if ( q.model == M_IAF_PSC_DELTA and has( "refractory_input" ) )
-    {
       q.refr_input = d.at( "refractory_input" ).number() != 0;
-      if ( q.refr_input )
-        throw std::runtime_error( "NotImplemented: refractory_input is not supported on the device path." );
-    }
     if ( q.model == M_IAF_PSC_EXP )
     {
       if ( has( "rho" ) )

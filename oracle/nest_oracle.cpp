@@ -1125,7 +1125,7 @@ struct Kernel
           const double in = n.rb[ slot ];
           n.rb[ slot ] = 0.0;
           if ( n.refr_input )
-            n.refr_buf += in * std::exp( -n.r * h / n.tau_m );
+            n.refr_buf += in * mm.exp( -n.r * h / n.tau_m );  // std::exp unless portable math
           --n.r;
         }
         if ( n.y3 >= n.V_th )

@@ -94,6 +94,15 @@ def test_split_equals_single_run(layout):
     assert np.array_equal(pops1[0].get("V_m"), pops2[0].get("V_m"))
 
 
+@pytest.mark.parametrize("layout", ["auto", "glob32"])
+def test_brunel_delta_refractory_input(layout):
+    # refractory_input: input during the refractory period is decayed and added afterwards
+    # (the decay factor uses the restated exp on both sides: portable math in the oracle)
+    desc = nb.brunel_delta(N=500, K=50, T=200.0, seed=5)
+    desc["pops"] = [(m, n, dict(p, refractory_input=True)) for m, n, p in desc["pops"]]
+    _compare(desc, portable_math=True, layout=layout)
+
+
 @pytest.mark.parametrize("layout", ["auto", "glob32", "win48", "win512", "win2", "glob64p3"])
 def test_bernoulli_min_delay_one_step(layout):
     # C3-like: pairwise_bernoulli, normal delays clipped at dt -> min_delay = 1 step

@@ -96,6 +96,34 @@ def test_delta_refractory_and_reset():
     assert list(s) == [25, 56]
 
 
+def test_delta_refractory_input_closed_form():
+    # refractory_input (models/iaf_psc_delta.cpp:284-299): input arriving while refractory is
+    # scaled by exp(-r h / tau_m), r the refractory steps left at its arrival, and added at the
+    # first active step. The first input fires the neuron at step 24 (r = 20 from step 25 on);
+    # the second arrives at step 35 with r = 10 and joins V at step 45 without firing it.
+    o = Oracle()
+    n = o.create("iaf_psc_delta", 1, {"E_L": 0.0, "V_th": 40.0, "V_reset": 10.0, "t_ref": 2.0, "tau_m": 20.0,
+                                       "C_m": 1.0, "refractory_input": True})
+    o.set_values(n, "V_m", [20.0])
+    counts = np.zeros((200, 1), np.uint8)
+    counts[9, 0] = 1   # arrives at step 24: 20 e^(-0.005 * 24) + 25 > 40
+    counts[20, 0] = 1  # arrives at step 35, refractory
+    o.add_drive(n[0], counts, 25.0, 1.5)
+    o.simulate(4.6)    # through the update of step 45
+    g, s = o.spikes()
+    assert list(s) == [25]
+    expected = 10.0 * np.exp(-0.1 / 20.0) + 25.0 * np.exp(-10 * 0.1 / 20.0)
+    assert abs(o.get(n, "V_m")[0] - expected) < 1e-12
+    # without refractory_input the refractory input is lost
+    o2 = Oracle()
+    n2 = o2.create("iaf_psc_delta", 1, {"E_L": 0.0, "V_th": 40.0, "V_reset": 10.0, "t_ref": 2.0, "tau_m": 20.0,
+                                         "C_m": 1.0})
+    o2.set_values(n2, "V_m", [20.0])
+    o2.add_drive(n2[0], counts, 25.0, 1.5)
+    o2.simulate(4.6)
+    assert abs(o2.get(n2, "V_m")[0] - 10.0 * np.exp(-0.1 / 20.0)) < 1e-12
+
+
 # ------------------------------------------------------------------ iaf_psc_exp
 def test_iaf_psc_exp_dc_closed_form():
     # testsuite/pytests/neurons/test_iaf_psc_exp.py: DC response follows
