@@ -306,30 +306,46 @@ Runner::ensure_capacity( size_t n )
   prefix_.clear();
 }
 
-void
+bool
 Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
 {
   if ( exec_ and exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win
        and exec_wg_ == wg_per_cu )
-    return;
+    return true;
   if ( exec_ )
   {
     hipGraphExecDestroy( exec_ );
     exec_ = nullptr;
   }
-  hipGraph_t g;
-  hip_check( hipStreamBeginCapture( stream_, hipStreamCaptureModeThreadLocal ), "hipStreamBeginCapture" );
+  // A capture fails when another host thread makes a call HIP does not allow during it (an
+  // allocation or a synchronous copy, e.g. while other networks are prepared): the capture is
+  // then dropped, and the caller launches the slices directly from then on.
+  hipGraph_t g = nullptr;
+  if ( hipStreamBeginCapture( stream_, hipStreamCaptureModeThreadLocal ) != hipSuccess )
+  {
+    (void)hipGetLastError();
+    return false;
+  }
   for ( int s = 0; s < S_; ++s )
     launch_slice( d_nets_, d_prefix_, n_nets, total_blocks, mask, win, s, slice_lds_for_occupancy( wg_per_cu, win ),
       stream_ );
-  hip_check( hipStreamEndCapture( stream_, &g ), "hipStreamEndCapture" );
-  hip_check( hipGraphInstantiate( &exec_, g, nullptr, nullptr, 0 ), "hipGraphInstantiate" );
+  const hipError_t ended = hipStreamEndCapture( stream_, &g );
+  if ( ended != hipSuccess or g == nullptr or hipGraphInstantiate( &exec_, g, nullptr, nullptr, 0 ) != hipSuccess )
+  {
+    (void)hipGetLastError();
+    if ( g )
+      hipGraphDestroy( g );
+    exec_ = nullptr;
+    exec_n_ = -1;
+    return false;
+  }
   hip_check( hipGraphDestroy( g ), "hipGraphDestroy" );
   exec_n_ = n_nets;
   exec_total_ = total_blocks;
   exec_mask_ = mask;
   exec_win_ = win;
   exec_wg_ = wg_per_cu;
+  return true;
 }
 
 void
@@ -407,8 +423,8 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
   }
   if ( not any )
     return false;
-  if ( use_graph and not time_kernels and not split )
-    capture( n, total, mask, win );
+  if ( use_graph and not time_kernels and not split and not capture( n, total, mask, win ) )
+    use_graph = false;
   hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
   hip_check( hipEventRecord( ev_chunk_[ 0 ], stream_ ), "hipEventRecord" );
   if ( split or time_kernels or not use_graph )

@@ -185,7 +185,7 @@ public:
 
 private:
   void ensure_capacity( size_t n );
-  void capture( int n_nets, uint32_t total_blocks, int mask, bool win );
+  bool capture( int n_nets, uint32_t total_blocks, int mask, bool win );  // false: launch directly
   void launch_one( int n, uint32_t total, int mask, bool win, int s, hipEvent_t* ev );
   int device_;
   int S_;
