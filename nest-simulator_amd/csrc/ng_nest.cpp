@@ -158,7 +158,7 @@ Param::value( VpRng& rng, int vp )
   case NORMAL:
     if ( b_ == 0 )
       return a_;
-    return normals_[ vp ]( rng.engine );
+    return normals_[ vp ].dist( rng.engine );
   case SUM:
   {
     const double x = l_->value( rng, vp );
@@ -1141,6 +1141,17 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       VpRng& rng = rngs_[ vp ];
       const long K = static_cast< long >( std::round( indegree ) );
       std::vector< unsigned long > chosen;
+      size_t local = 0;
+      for ( const int64_t t : post )
+        local += vp_of( t ) == vp;
+      if ( rule == "one_to_one" )
+        conns_[ vp ].reserve_more( local );
+      else if ( rule == "all_to_all" )
+        conns_[ vp ].reserve_more( local * pre.size() );
+      else if ( rule == "fixed_indegree" )
+        conns_[ vp ].reserve_more( local * static_cast< size_t >( K ) );
+      else
+        conns_[ vp ].reserve_more( static_cast< size_t >( p * static_cast< double >( local * pre.size() ) * 1.02 ) + 1024 );
       for ( size_t ti = 0; ti < post.size(); ++ti )
       {
         const int64_t t = post[ ti ];

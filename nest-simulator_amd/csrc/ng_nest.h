@@ -8,6 +8,7 @@
 // (simulation_manager.cpp:507-671). The simulation itself runs on the device engine.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -88,7 +89,9 @@ struct SeedSeqFe128
 };
 
 // RandomGenerator< std::mt19937_64 > (random_generators.h:141-297)
-struct VpRng
+// Per-VP objects are written on every draw or connection by their own thread: each sits on
+// cache lines of its own so that the VP threads of Connect do not share lines.
+struct alignas( 64 ) VpRng
 {
   std::mt19937_64 engine;
   std::uniform_int_distribution< unsigned long > ulong_dist;
@@ -140,7 +143,15 @@ private:
   Op op_ = CONST;
   double a_ = 0, b_ = 0;
   std::unique_ptr< Param > l_, r_;
-  std::vector< std::normal_distribution<> > normals_;  // one per VP (parameter.cpp:95-117)
+  struct alignas( 64 ) VpNormal
+  {
+    std::normal_distribution<> dist;
+    VpNormal( double mean, double sd )
+      : dist( mean, sd )
+    {
+    }
+  };
+  std::vector< VpNormal > normals_;  // one per VP (parameter.cpp:95-117)
 };
 
 struct NeuronParams
@@ -179,7 +190,7 @@ struct DenseBlock
 };
 
 // connection in creation order of its VP (SoA storage per VP)
-struct ConnList
+struct alignas( 64 ) ConnList
 {
   std::vector< uint32_t > src, tgt;
   std::vector< double > w;
@@ -188,6 +199,19 @@ struct ConnList
   size() const
   {
     return src.size();
+  }
+  // room for `extra` more connections: exact for one large addition, geometric for many small
+  void
+  reserve_more( size_t extra )
+  {
+    const size_t need = src.size() + extra;
+    if ( need <= src.capacity() )
+      return;
+    const size_t cap = std::max( need, src.capacity() + src.capacity() / 2 );
+    src.reserve( cap );
+    tgt.reserve( cap );
+    w.reserve( cap );
+    d.reserve( cap );
   }
 };
 
