@@ -354,16 +354,14 @@ Kernel::build_spec( BuildSpec& b )
     throw std::runtime_error( "NotImplemented: delay range above 65535 steps" );
   if ( E >= ( int64_t( 1 ) << 32 ) )
     throw std::runtime_error( "NotImplemented: 2^32 or more synapses in one instance" );
-  std::vector< uint64_t > row_ptr( N + 1, 0 );
-  b.boff.assign( static_cast< size_t >( N ) * ( P + 1 ), 0 );
-  b.syn.resize( E );
-  // Worker w owns the sources [N*w/T, N*(w+1)/T): it scans every connection in creation
-  // order (VP order, then within a VP) and counts, then fills, only its own rows, so each
-  // (source, block) run keeps creation order with no synchronisation between workers.
-  std::vector< int64_t > vp_first( conns_.size() + 1, 0 );
-  for ( size_t v = 0; v < conns_.size(); ++v )
-    vp_first[ v + 1 ] = vp_first[ v ] + static_cast< int64_t >( conns_[ v ].size() );
-  std::vector< uint32_t > src_dev( E ), tgt_dev( E );
+  // Three parallel passes over the connections, each worker on a contiguous range of
+  // creation order (VP order, then within a VP):
+  //   1. device source of every connection, counts per (worker, source);
+  //   2. the connections placed source-major into syn, a worker's connections of one source
+  //      after those of the workers before it, i.e. every row in creation order;
+  //   3. per row: grouped by target block (counting sort, stable), each block run ordered
+  //      by delay (stable, so a cell (target, slot) receives a spike's weights in creation
+  //      order), and the final record words written.
   const int n_workers = static_cast< int >(
     std::max< unsigned >( 1u, std::min< unsigned >( 16u, std::thread::hardware_concurrency() ) ) );
   auto parallel = [ n_workers ]( auto&& fn )
@@ -375,83 +373,133 @@ Kernel::build_spec( BuildSpec& b )
     for ( auto& x : th )
       x.join();
   };
+  std::vector< int64_t > vp_first( conns_.size() + 1, 0 );
+  for ( size_t v = 0; v < conns_.size(); ++v )
+    vp_first[ v + 1 ] = vp_first[ v ] + static_cast< int64_t >( conns_[ v ].size() );
+  // the VP whose list holds connection e (the first with vp_first[v + 1] > e)
+  auto vp_of_edge = [ & ]( int64_t e )
+  {
+    return static_cast< size_t >(
+      std::upper_bound( vp_first.begin() + 1, vp_first.end(), e ) - ( vp_first.begin() + 1 ) );
+  };
+  HostArray< uint32_t > src_dev( E );
+  HostArray< uint16_t > delay( E );
+  // offsets fit 32 bits: fewer than 2^32 connections
+  std::vector< uint32_t > start( static_cast< size_t >( n_workers ) * N, 0 );
   parallel(
     [ & ]( int w )
     {
       const int64_t lo = E * w / n_workers, hi = E * ( w + 1 ) / n_workers;
-      size_t v = 0;
+      uint32_t* cnt = &start[ static_cast< size_t >( w ) * N ];
+      size_t v = lo < hi ? vp_of_edge( lo ) : 0;
       for ( int64_t e = lo; e < hi; ++e )
       {
         while ( e >= vp_first[ v + 1 ] )
           ++v;
-        const size_t i = static_cast< size_t >( e - vp_first[ v ] );
-        src_dev[ e ] = static_cast< uint32_t >( gid_to_dev_[ conns_[ v ].src[ i ] ] );
-        tgt_dev[ e ] = static_cast< uint32_t >( gid_to_dev_[ conns_[ v ].tgt[ i ] ] );
+        const uint32_t s = static_cast< uint32_t >( gid_to_dev_[ conns_[ v ].src[ e - vp_first[ v ] ] ] );
+        src_dev[ e ] = s;
+        ++cnt[ s ];
       }
     } );
-  const uint32_t tbs = b.tb;
-  // counts per (source, block), then the row-relative prefix
-  parallel(
-    [ & ]( int w )
-    {
-      const uint32_t s_lo = static_cast< uint32_t >( static_cast< uint64_t >( N ) * w / n_workers );
-      const uint32_t s_hi = static_cast< uint32_t >( static_cast< uint64_t >( N ) * ( w + 1 ) / n_workers );
-      for ( int64_t e = 0; e < E; ++e )
-      {
-        const uint32_t s = src_dev[ e ];
-        if ( s >= s_lo and s < s_hi )
-          b.boff[ static_cast< size_t >( s ) * ( P + 1 ) + tgt_dev[ e ] / tbs + 1 ]++;
-      }
-      for ( uint32_t s = s_lo; s < s_hi; ++s )
-      {
-        uint32_t* row = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
-        for ( uint32_t q = 0; q < P; ++q )
-          row[ q + 1 ] += row[ q ];
-      }
-    } );
+  std::vector< uint64_t > row_ptr( N + 1, 0 );
   for ( uint32_t s = 0; s < N; ++s )
-    row_ptr[ s + 1 ] = row_ptr[ s ] + b.boff[ static_cast< size_t >( s ) * ( P + 1 ) + P ];
+  {
+    uint64_t at = row_ptr[ s ];
+    for ( int w = 0; w < n_workers; ++w )
+    {
+      uint32_t& c = start[ static_cast< size_t >( w ) * N + s ];
+      const uint32_t n_ws = c;
+      c = static_cast< uint32_t >( at );
+      at += n_ws;
+    }
+    row_ptr[ s + 1 ] = at;
+  }
+  b.syn.resize( E );
   parallel(
     [ & ]( int w )
     {
-      const uint32_t s_lo = static_cast< uint32_t >( static_cast< uint64_t >( N ) * w / n_workers );
-      const uint32_t s_hi = static_cast< uint32_t >( static_cast< uint64_t >( N ) * ( w + 1 ) / n_workers );
-      // row-relative offsets become absolute run starts in syn
-      for ( uint32_t s = s_lo; s < s_hi; ++s )
-      {
-        uint32_t* row = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
-        for ( uint32_t q = 0; q <= P; ++q )
-          row[ q ] += static_cast< uint32_t >( row_ptr[ s ] );
-      }
-      std::vector< uint32_t > cur( b.boff.begin() + static_cast< size_t >( s_lo ) * ( P + 1 ),
-        b.boff.begin() + static_cast< size_t >( s_hi ) * ( P + 1 ) );
-      size_t v = 0;
-      for ( int64_t e = 0; e < E; ++e )
+      const int64_t lo = E * w / n_workers, hi = E * ( w + 1 ) / n_workers;
+      uint32_t* cur = &start[ static_cast< size_t >( w ) * N ];
+      size_t v = lo < hi ? vp_of_edge( lo ) : 0;
+      for ( int64_t e = lo; e < hi; ++e )
       {
         while ( e >= vp_first[ v + 1 ] )
           ++v;
-        const uint32_t s = src_dev[ e ];
-        if ( s < s_lo or s >= s_hi )
-          continue;
-        const uint32_t t = tgt_dev[ e ];
-        const uint32_t blk = t / tbs;
-        const uint32_t pos = cur[ static_cast< size_t >( s - s_lo ) * ( P + 1 ) + blk ]++;
+        const ConnList& C = conns_[ v ];
         const size_t i = static_cast< size_t >( e - vp_first[ v ] );
+        const uint32_t pos = cur[ src_dev[ e ] ]++;
         uint64_t wbits;
-        std::memcpy( &wbits, &conns_[ v ].w[ i ], sizeof wbits );
+        std::memcpy( &wbits, &C.w[ i ], sizeof wbits );
         SynRec& rec = b.syn[ pos ];
         rec.w_lo = static_cast< uint32_t >( wbits );
         rec.w_hi = static_cast< uint32_t >( wbits >> 32 );
-        rec.word = ( t - blk * tbs ) | ( static_cast< uint32_t >( conns_[ v ].d[ i ] - min_delay_ ) << 16 );
+        rec.word = static_cast< uint32_t >( gid_to_dev_[ C.tgt[ i ] ] );  // device target, for now
+        delay[ pos ] = static_cast< uint16_t >( C.d[ i ] - min_delay_ );
       }
-      // each (source, block) run ordered by delay, creation order among equal delays: a cell
-      // (target, slot) still receives a spike's weights in creation order
+    } );
+  start = std::vector< uint32_t >();
+  src_dev = HostArray< uint32_t >();
+  const uint32_t tbs = b.tb;
+  b.boff.resize( static_cast< size_t >( N ) * ( P + 1 ) );
+  parallel(
+    [ & ]( int w )
+    {
+      // rows split by connection count: worker w takes the rows that start in its share
+      const uint64_t e_lo = static_cast< uint64_t >( E ) * w / n_workers;
+      const uint64_t e_hi = static_cast< uint64_t >( E ) * ( w + 1 ) / n_workers;
+      const uint32_t s_lo =
+        w == 0 ? 0 : static_cast< uint32_t >( std::lower_bound( row_ptr.begin(), row_ptr.end() - 1, e_lo ) - row_ptr.begin() );
+      const uint32_t s_hi = w == n_workers - 1
+        ? N
+        : static_cast< uint32_t >( std::lower_bound( row_ptr.begin(), row_ptr.end() - 1, e_hi ) - row_ptr.begin() );
+      std::vector< SynRec > row_syn;
+      std::vector< uint16_t > row_delay;
+      std::vector< uint32_t > cur( P );
       for ( uint32_t s = s_lo; s < s_hi; ++s )
       {
-        const uint32_t* row = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
+        const uint64_t r0 = row_ptr[ s ], r1 = row_ptr[ s + 1 ];
+        uint32_t* bo = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
+        std::fill( bo, bo + P + 1, 0u );
+        for ( uint64_t j = r0; j < r1; ++j )
+          ++bo[ b.syn[ j ].word / tbs + 1 ];
+        bo[ 0 ] = static_cast< uint32_t >( r0 );
         for ( uint32_t q = 0; q < P; ++q )
-          std::stable_sort( b.syn.begin() + row[ q ], b.syn.begin() + row[ q + 1 ],
-            []( const SynRec& x, const SynRec& y ) { return ( x.word >> 16 ) < ( y.word >> 16 ); } );
+        {
+          bo[ q + 1 ] += bo[ q ];
+          cur[ q ] = bo[ q ];
+        }
+        row_syn.assign( b.syn.begin() + r0, b.syn.begin() + r1 );
+        row_delay.assign( delay.begin() + r0, delay.begin() + r1 );
+        for ( size_t j = 0; j < row_syn.size(); ++j )
+        {
+          const uint32_t t = row_syn[ j ].word;
+          const uint32_t blk = t / tbs;
+          SynRec& rec = b.syn[ cur[ blk ]++ ];
+          rec.w_lo = row_syn[ j ].w_lo;
+          rec.w_hi = row_syn[ j ].w_hi;
+          rec.word = ( t - blk * tbs ) | ( static_cast< uint32_t >( row_delay[ j ] ) << 16 );
+        }
+        for ( uint32_t q = 0; q < P; ++q )
+        {
+          SynRec* run = &b.syn[ bo[ q ] ];
+          const uint32_t len = bo[ q + 1 ] - bo[ q ];
+          if ( len <= 1 )
+            continue;
+          auto by_delay = []( const SynRec& x, const SynRec& y ) { return ( x.word >> 16 ) < ( y.word >> 16 ); };
+          if ( len > 32 )
+          {
+            std::stable_sort( run, run + len, by_delay );
+            continue;
+          }
+          for ( uint32_t j = 1; j < len; ++j )  // insertion sort: stable
+          {
+            const SynRec x = run[ j ];
+            uint32_t k = j;
+            for ( ; k > 0 and by_delay( x, run[ k - 1 ] ); --k )
+              run[ k ] = run[ k - 1 ];
+            run[ k ] = x;
+          }
+        }
       }
     } );
   // spikes are logged only while a spike_recorder observes some neuron

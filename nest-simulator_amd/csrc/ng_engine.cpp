@@ -52,9 +52,9 @@ Engine::alloc( size_t bytes )
   return p;
 }
 
-template < typename T >
+template < typename T, typename A >
 T*
-Engine::upload( const std::vector< T >& v )
+Engine::upload( const std::vector< T, A >& v )
 {
   T* p = static_cast< T* >( alloc( v.size() * sizeof( T ) ) );
   if ( not v.empty() )

@@ -6,9 +6,45 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ng_device.h"
+
+namespace nestgpu
+{
+// Allocator whose value-initialisation leaves elements default-initialised (uninitialised for
+// trivial types): the store builder writes every element of its large tables itself, from
+// several threads, so the pages are first touched there instead of by one zeroing pass.
+template < typename T >
+struct NoInitAlloc : std::allocator< T >
+{
+  template < typename U >
+  struct rebind
+  {
+    using other = NoInitAlloc< U >;
+  };
+  NoInitAlloc() = default;
+  template < typename U >
+  NoInitAlloc( const NoInitAlloc< U >& ) noexcept
+  {
+  }
+  template < typename U >
+  void
+  construct( U* p )
+  {
+    ::new ( static_cast< void* >( p ) ) U;
+  }
+  template < typename U, typename... Args >
+  void
+  construct( U* p, Args&&... args )
+  {
+    ::new ( static_cast< void* >( p ) ) U( std::forward< Args >( args )... );
+  }
+};
+template < typename T >
+using HostArray = std::vector< T, NoInitAlloc< T > >;
+}  // namespace nestgpu
 
 namespace nestgpu
 {
@@ -54,8 +90,8 @@ struct BuildSpec
   std::vector< int32_t > r;
   // out-CSR by source device index, each row grouped by target block (creation order
   // within a block); boff[s * (n_blocks + 1) + b] = index in syn of block b's run of row s
-  std::vector< uint32_t > boff;
-  std::vector< SynRec > syn;
+  HostArray< uint32_t > boff;
+  HostArray< SynRec > syn;
   // drive
   std::vector< uint8_t > drive_counts;
   int64_t drive_cols = 0;
@@ -137,8 +173,8 @@ public:
 
 private:
   void* alloc( size_t bytes );
-  template < typename T >
-  T* upload( const std::vector< T >& v );
+  template < typename T, typename A >
+  T* upload( const std::vector< T, A >& v );
   void free_list( std::vector< void* >& l );
 
   int device_;
