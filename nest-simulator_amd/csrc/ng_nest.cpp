@@ -158,7 +158,7 @@ Param::value( VpRng& rng, int vp )
   case NORMAL:
     if ( b_ == 0 )
       return a_;
-    return normals_[ vp ].dist( rng.engine );
+    return normals_[ vp ]( rng );
   case SUM:
   {
     const double x = l_->value( rng, vp );

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -95,12 +96,22 @@ struct alignas( 64 ) VpRng
 {
   std::mt19937_64 engine;
   std::uniform_int_distribution< unsigned long > ulong_dist;
-  std::uniform_real_distribution<> unit_dist { 0.0, 1.0 };
   explicit VpRng( const std::vector< uint32_t >& seed );
+  // std::generate_canonical<double, 53>(engine) in closed form: for a 64-bit engine it takes
+  // one output and divides it by 2^64, clamping a result that rounds to 1 to the double below
+  // (libstdc++ bits/random.tcc). The library's generic form evaluates two long-double
+  // logarithms per draw unless the compiler folds them, which hipcc does not.
+  double
+  canonical()
+  {
+    const double u = static_cast< double >( engine() ) * 0x1p-64;
+    return u < 1.0 ? u : std::nextafter( 1.0, 0.0 );
+  }
+  // std::uniform_real_distribution<>(0, 1): canonical * (1 - 0) + 0, i.e. canonical itself
   double
   drand()
   {
-    return unit_dist( engine );
+    return canonical();
   }
   unsigned long
   ulrand( unsigned long n )
@@ -143,12 +154,42 @@ private:
   Op op_ = CONST;
   double a_ = 0, b_ = 0;
   std::unique_ptr< Param > l_, r_;
+  // std::normal_distribution<double> (libstdc++ bits/random.tcc): Marsaglia's polar method
+  // over pairs of canonical draws, the second value of a pair kept for the next call
   struct alignas( 64 ) VpNormal
   {
-    std::normal_distribution<> dist;
-    VpNormal( double mean, double sd )
-      : dist( mean, sd )
+    double mean, sd;
+    double saved = 0;
+    bool saved_available = false;
+    VpNormal( double mean_, double sd_ )
+      : mean( mean_ )
+      , sd( sd_ )
     {
+    }
+    double
+    operator()( VpRng& rng )
+    {
+      double ret;
+      if ( saved_available )
+      {
+        saved_available = false;
+        ret = saved;
+      }
+      else
+      {
+        double x, y, r2;
+        do
+        {
+          x = 2.0 * rng.canonical() - 1.0;
+          y = 2.0 * rng.canonical() - 1.0;
+          r2 = x * x + y * y;
+        } while ( r2 > 1.0 or r2 == 0.0 );
+        const double mult = std::sqrt( -2.0 * std::log( r2 ) / r2 );
+        saved = x * mult;
+        saved_available = true;
+        ret = y * mult;
+      }
+      return ret * sd + mean;
     }
   };
   std::vector< VpNormal > normals_;  // one per VP (parameter.cpp:95-117)
