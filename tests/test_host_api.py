@@ -46,6 +46,8 @@ CASES = [
     (3, 8, 12, 17, "all_to_all", 0, ("normal", -2.0, 0.5), 2.0, True, True, False),
     (2, 9, 15, 15, "all_to_all", 0, 0.5, 1.0, False, True, True),
     (6, 123, 40, 30, "fixed_indegree", 25, ("min", ("normal", 0.0, 2.0), 1.0), ("uniform", 0.1, 3.0), True, False, False),
+    # many normal and uniform draws per VP stream: the canonical and polar-method draws
+    (2, 31, 400, 300, "fixed_indegree", 50, ("normal", 0.0, 1.0), ("uniform", 0.1, 5.0), True, True, False),
 ]
 
 
