@@ -58,7 +58,9 @@ int nestgpu_get_values( nestgpu_kernel* k, const int64_t* ids, int64_t n, const 
  * {"min": [a, v]}, {"redraw": [a, min, max]}). Random draws follow NEST's per-VP streams. */
 int nestgpu_connect( nestgpu_kernel* k, const int64_t* pre, int64_t n_pre, const int64_t* post, int64_t n_post,
   const char* conn_spec_json, const char* syn_spec_json );
-/* nest::connect_arrays (nestkernel/nest.cpp:454); weights/delays may be NULL (defaults). */
+/* nest::connect_arrays (nestkernel/nest.cpp:454); weights/delays may be NULL (defaults).
+ * Every element is checked first: a call that fails (IllegalConnection, BadDelay) adds no
+ * connection. */
 int nestgpu_connect_arrays( nestgpu_kernel* k, const int64_t* sources, const int64_t* targets, const double* weights,
   const double* delays_ms, int64_t n );
 /* spike_generator input as dense multiplicities: counts[s * n + i] spikes of generator

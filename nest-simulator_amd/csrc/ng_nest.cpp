@@ -1248,29 +1248,58 @@ Kernel::connect_arrays( const int64_t* src, const int64_t* tgt, const double* w,
   if ( host_connections_freed_ )
     throw std::runtime_error( "KernelException: the host connection table was released; Connect is not possible" );
   // ConnectionManager::connect_arrays: each VP creates the connections of its targets in
-  // array order (connection_manager.cpp:624-760)
+  // array order (connection_manager.cpp:624-760). Every element is checked before any
+  // connection is added, so a call that fails adds none.
   for ( int64_t i = 0; i < n; ++i )
   {
     if ( node( src[ i ] ).kind != 0 or node( tgt[ i ] ).kind != 0 )
       throw std::runtime_error( "IllegalConnection: connect_arrays connects neurons only" );
   }
-  for ( int vp = 0; vp < n_vp_; ++vp )
+  std::vector< uint16_t > steps( static_cast< size_t >( n ) );
+  std::vector< size_t > per_vp( n_vp_, 0 );
+  long dmin = -1, dmax = 1;
+  for ( int64_t i = 0; i < n; ++i )
+  {
+    const long ds = tb_.delay_ms_to_steps( d_ms ? d_ms[ i ] : 1.0 );
+    check_delay( ds );
+    if ( ds > 65535 )
+      throw std::runtime_error( "BadDelay: delays above 65535 steps are not supported" );
+    steps[ i ] = static_cast< uint16_t >( ds );
+    if ( dmin < 0 or ds < dmin )
+      dmin = ds;
+    dmax = std::max( dmax, ds );
+    ++per_vp[ vp_of( tgt[ i ] ) ];
+  }
+  auto fill = [ & ]( int vp )
   {
     ConnList& C = conns_[ vp ];
+    C.reserve_more( per_vp[ vp ] );
     for ( int64_t i = 0; i < n; ++i )
     {
       if ( vp_of( tgt[ i ] ) != vp )
         continue;
-      const long ds = tb_.delay_ms_to_steps( d_ms ? d_ms[ i ] : 1.0 );
-      check_delay( ds );
-      if ( d_min_seen_ < 0 or ds < d_min_seen_ )
-        d_min_seen_ = ds;
-      d_max_seen_ = std::max( d_max_seen_, ds );
       C.src.push_back( static_cast< uint32_t >( src[ i ] ) );
       C.tgt.push_back( static_cast< uint32_t >( tgt[ i ] ) );
       C.w.push_back( w ? w[ i ] : 1.0 );
-      C.d.push_back( static_cast< uint16_t >( ds ) );
+      C.d.push_back( steps[ i ] );
     }
+  };
+  if ( n_vp_ == 1 or n < 65536 )
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      fill( vp );
+  else
+  {
+    std::vector< std::thread > th;
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      th.emplace_back( fill, vp );
+    for ( auto& x : th )
+      x.join();
+  }
+  if ( n > 0 )
+  {
+    if ( d_min_seen_ < 0 or dmin < d_min_seen_ )
+      d_min_seen_ = dmin;
+    d_max_seen_ = std::max( d_max_seen_, dmax );
   }
   store_dirty_ = true;
 }

@@ -88,6 +88,42 @@ def test_connectivity_independent_of_call_granularity_per_vp():
     assert np.array_equal(s1[keep], s2) and np.array_equal(w1[keep], w2) and np.array_equal(d1[keep], d2)
 
 
+@pytest.mark.parametrize("n_vp, n", [(1, 20000), (3, 20000), (8, 70000)])
+def test_connect_arrays_connectivity_identical_to_oracle(n_vp, n):
+    # connect_arrays: every VP creates its targets' connections in array order
+    # (connection_manager.cpp:624-760); a bad delay anywhere adds nothing. Arrays of 65,536
+    # or more elements are filled by one thread per VP.
+    N = 300
+    rng = np.random.default_rng(n_vp)
+    src = rng.integers(1, N + 1, size=n)
+    tgt = rng.integers(1, N + 1, size=n)
+    w = rng.normal(10.0, 5.0, size=n)
+    d = np.round(rng.uniform(0.1, 4.0, size=n), 1)
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"total_num_virtual_procs": n_vp})
+    k.Create("iaf_psc_exp", N)
+    bad = d.copy()
+    bad[n // 2] = 0.01
+    with pytest.raises(nestgpu.NESTError, match="BadDelay"):
+        k.ConnectArrays(src, tgt, w, bad)
+    assert len(k.GetConnections()["source"]) == 0
+    k.ConnectArrays(src, tgt, w, d)
+    k.ConnectArrays(src[:50], tgt[:50])  # default weight 1 and delay 1 ms
+    o = Oracle(n_vp=n_vp)
+    o.create("iaf_psc_exp", N, {})
+    steps = np.floor(d * 10 + 0.5).astype(np.int64)
+    for vp in range(n_vp):
+        sel = (tgt % n_vp) == vp
+        o.add_connections(src[sel], tgt[sel], w[sel], steps[sel])
+    for vp in range(n_vp):
+        sel = (tgt[:50] % n_vp) == vp
+        o.add_connections(src[:50][sel], tgt[:50][sel], np.ones(sel.sum()), np.full(sel.sum(), 10))
+    s1, t1, w1, d1 = _conns_gpu(k)
+    s2, t2, w2, d2 = o.connections()
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+    assert np.array_equal(w1, w2) and np.array_equal(d1, d2)
+
+
 def test_get_set_status_roundtrip_and_relative_voltages():
     k = nestgpu.Kernel(0)
     n = k.Create("iaf_psc_delta", 3, {"E_L": -65.0, "V_th": -50.0})
