@@ -1000,6 +1000,11 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
 {
   if ( host_connections_freed_ )
     throw std::runtime_error( "KernelException: the host connection table was released; Connect is not possible" );
+  // ConnectionManager::connect (connection_manager.cpp:440-447)
+  if ( pre.empty() )
+    throw std::runtime_error( "IllegalConnection: Presynaptic nodes cannot be an empty NodeCollection" );
+  if ( post.empty() )
+    throw std::runtime_error( "IllegalConnection: Postsynaptic nodes cannot be an empty NodeCollection" );
   if ( rng_dirty_ )
     init_rngs();
   std::string rule = "all_to_all";

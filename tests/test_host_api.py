@@ -160,6 +160,48 @@ def test_error_paths(bad, msg):
         bad(k)
 
 
+@pytest.mark.parametrize("case, msg", [
+    # connect/test_connect_node_collection.py:55-69
+    ("empty_pre", "IllegalConnection"),
+    ("empty_post", "IllegalConnection"),
+    ("empty_both", "IllegalConnection"),
+    # connect/test_connect_fixed_indegree.py:52-62 (indegree above the population, no multapses)
+    ("indegree_too_large", "BadProperty"),
+    ("indegree_negative", "BadProperty"),
+    ("p_above_one", "BadProperty"),
+    ("one_to_one_sizes", "DimensionMismatch"),
+    ("unknown_rule", "BadProperty"),
+    ("unknown_conn_key", "UnaccessedDictionaryEntry"),
+    ("unknown_synapse", "UnknownSynapseType"),
+    ("generator_target", "IllegalConnection"),
+    ("recorder_source", "IllegalConnection"),
+])
+def test_connect_error_paths(case, msg):
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 5)
+    sg = k.Create("spike_generator", 5)
+    sr = k.Create("spike_recorder")
+    calls = {
+        "empty_pre": lambda: k.Connect(a[0:0], a),
+        "empty_post": lambda: k.Connect(a, a[0:0]),
+        "empty_both": lambda: k.Connect(a[0:0], a[0:0]),
+        "indegree_too_large": lambda: k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 6,
+                                                       "allow_autapses": True, "allow_multapses": False}),
+        "indegree_negative": lambda: k.Connect(a, a, {"rule": "fixed_indegree", "indegree": -1}),
+        "p_above_one": lambda: k.Connect(a, a, {"rule": "pairwise_bernoulli", "p": 1.5}),
+        "one_to_one_sizes": lambda: k.Connect(a, a[:3], "one_to_one"),
+        "unknown_rule": lambda: k.Connect(a, a, {"rule": "no_such_rule"}),
+        "unknown_conn_key": lambda: k.Connect(a, a, {"rule": "all_to_all", "no_such_key": 1}),
+        "unknown_synapse": lambda: k.Connect(a, a, "all_to_all", {"synapse_model": "no_such_synapse"}),
+        "generator_target": lambda: k.Connect(a, sg, "one_to_one"),
+        "recorder_source": lambda: k.Connect(sr, a),
+    }
+    with pytest.raises(nestgpu.NESTError, match=msg):
+        calls[case]()
+    # a failed Connect adds nothing
+    assert len(k.GetConnections()["source"]) == 0
+
+
 def test_bad_delay_rejected():
     k = nestgpu.Kernel(0)
     a = k.Create("iaf_psc_delta", 2)
