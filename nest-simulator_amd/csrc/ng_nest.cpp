@@ -421,11 +421,15 @@ Kernel::get_kernel_status_json() const
   o.precision( 17 );
   const int64_t clk = engine_ ? engine_->clock : 0;
   const int32_t fr = engine_ ? engine_->from_step : 0;
+  // until the first simulation the delay extrema are those of the connections made so far,
+  // the resolution when there are none (update_delay_extrema_, connection_manager.cpp:485-527)
+  const long dmin = simulated_ ? min_delay_ : ( d_min_seen_ < 0 ? 1 : d_min_seen_ );
+  const long dmax = simulated_ ? max_delay_ : std::max( d_max_seen_, 1L );
   o << "{\"resolution\": " << tb_.resolution_ms() << ", \"tics_per_step\": " << tb_.tics_per_step
     << ", \"rng_seed\": " << seed_ << ", \"total_num_virtual_procs\": " << n_vp_
-    << ", \"local_num_threads\": " << n_vp_ << ", \"min_delay\": " << min_delay_ * tb_.ms_per_step
-    << ", \"max_delay\": " << max_delay_ * tb_.ms_per_step << ", \"min_delay_steps\": " << min_delay_
-    << ", \"max_delay_steps\": " << max_delay_ << ", \"num_connections\": " << num_connections()
+    << ", \"local_num_threads\": " << n_vp_ << ", \"min_delay\": " << dmin * tb_.ms_per_step
+    << ", \"max_delay\": " << dmax * tb_.ms_per_step << ", \"min_delay_steps\": " << dmin
+    << ", \"max_delay_steps\": " << dmax << ", \"num_connections\": " << num_connections()
     << ", \"network_size\": " << nodes_.size() << ", \"biological_time\": " << ( clk + fr ) * tb_.ms_per_step
     << ", \"clock_steps\": " << clk << ", \"from_step\": " << fr << ", \"local_spike_counter\": " << local_spike_counter()
     << ", \"time_simulate_kernel_ms\": " << last_kernel_ms_ << ", \"time_last_run_ms\": " << last_wall_ms_

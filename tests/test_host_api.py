@@ -209,6 +209,19 @@ def test_bad_delay_rejected():
         k.Connect(a, a, "all_to_all", {"delay": 0.04})  # rounds to 0 steps at 0.1 ms
 
 
+def test_delay_extrema_reported_from_connections():
+    # connect/test_delay_check.py: min_delay/max_delay follow the delays connected so far
+    k = nestgpu.Kernel(0)
+    assert k.GetKernelStatus("min_delay") == 0.1 and k.GetKernelStatus("max_delay") == 0.1
+    n1, n2 = k.Create("iaf_psc_delta", 2)
+    k.Connect(n1, n2, syn_spec={"weight": 1.0, "delay": 2.0})
+    assert k.GetKernelStatus("min_delay") == 2.0 and k.GetKernelStatus("max_delay") == 2.0
+    k.Connect(n2, n1, syn_spec={"weight": 1.0, "delay": 6.0})
+    assert k.GetKernelStatus("min_delay") == 2.0 and k.GetKernelStatus("max_delay") == 6.0
+    k.Connect(n2, n1, syn_spec={"weight": 1.0, "delay": 1.0})
+    assert k.GetKernelStatus("min_delay_steps") == 10 and k.GetKernelStatus("max_delay_steps") == 60
+
+
 def test_resolution_fixed_once_nodes_exist():
     k = nestgpu.Kernel(0)
     k.Create("iaf_psc_delta", 1)
