@@ -74,13 +74,26 @@ VpRng::VpRng( const std::vector< uint32_t >& seed )
 
 // ------------------------------------------------------------------ parameters
 std::unique_ptr< Param >
-Param::from_json( const Json& j, int n_vp )
+Param::from_json( const Json& j, int n_vp, bool top_level )
 {
   auto p = std::make_unique< Param >();
   if ( j.kind == Json::NUM or j.kind == Json::BOOL )
   {
     p->op_ = CONST;
     p->a_ = j.number();
+    return p;
+  }
+  if ( j.kind == Json::ARR and top_level )
+  {
+    p->op_ = ARRAY;
+    p->values_.reserve( j.arr.size() );
+    for ( const Json& x : j.arr )
+    {
+      if ( x.kind != Json::NUM )
+        throw std::runtime_error( "BadProperty: an array parameter holds numbers only" );
+      p->values_.push_back( x.num );
+    }
+    p->cursor_.resize( n_vp );
     return p;
   }
   if ( j.kind != Json::OBJ or j.obj.size() != 1 )
@@ -173,6 +186,13 @@ Param::value( VpRng& rng, int vp )
     return std::max( l_->value( rng, vp ), a_ );
   case MIN:
     return std::min( l_->value( rng, vp ), a_ );
+  case ARRAY:
+  {
+    size_t& i = cursor_[ vp ].next;
+    if ( i >= values_.size() )
+      throw std::runtime_error( "KernelException: Parameter values exhausted." );
+    return values_[ i++ ];
+  }
   case REDRAW:
     for ( int i = 0; i < 1000; ++i )
     {
@@ -183,6 +203,17 @@ Param::value( VpRng& rng, int vp )
     throw std::runtime_error( "KernelException: Number of redraws exceeded limit." );
   }
   return 0;
+}
+
+void
+Param::skip( int vp, size_t n )
+{
+  if ( op_ != ARRAY )
+    return;
+  size_t& i = cursor_[ vp ].next;
+  if ( i >= values_.size() )
+    throw std::runtime_error( "KernelException: Parameter values exhausted." );
+  i += n;
 }
 
 // ------------------------------------------------------------------ Poisson drive
@@ -1044,9 +1075,9 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
         throw std::runtime_error( "UnknownSynapseType: '" + sm + "' is not available on the device path." );
     }
     else if ( kv.first == "weight" )
-      wp = Param::from_json( kv.second, n_vp_ );
+      wp = Param::from_json( kv.second, n_vp_, true );
     else if ( kv.first == "delay" )
-      dp = Param::from_json( kv.second, n_vp_ );
+      dp = Param::from_json( kv.second, n_vp_, true );
     else
       throw std::runtime_error( "UnaccessedDictionaryEntry: syn_spec key '" + kv.first + "'" );
   }
@@ -1093,6 +1124,36 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     throw std::runtime_error( "BadProperty: Connection probability 0 <= p <= 1 required." );
   if ( rule != "one_to_one" and rule != "all_to_all" and rule != "fixed_indegree" and rule != "pairwise_bernoulli" )
     throw std::runtime_error( "BadProperty: unknown or unsupported connection rule '" + rule + "'" );
+  // array parameters hold one value per connection in PyNEST's layout: one_to_one [pair],
+  // all_to_all [target][source], fixed_indegree [target][draw]
+  // (pynest/nest/lib/hl_api_connection_helpers.py:48-130)
+  const bool arrays = ( wp and wp->is_array() ) or ( dp and dp->is_array() );
+  if ( arrays )
+  {
+    size_t expect = 0;
+    if ( rule == "one_to_one" )
+      expect = pre.size();
+    else if ( rule == "all_to_all" )
+      expect = pre.size() * post.size();
+    else if ( rule == "fixed_indegree" and pre_kind == 0 )
+      expect = post.size() * static_cast< size_t >( std::max( 0L, std::lround( indegree ) ) );
+    else
+      throw std::runtime_error( "BadProperty: array parameters need the rule one_to_one, all_to_all or fixed_indegree" );
+    for ( const Param* q : { wp.get(), dp.get() } )
+      if ( q and q->is_array() and q->array_size() != expect )
+        throw std::runtime_error(
+          "BadProperty: an array parameter of this connection needs " + std::to_string( expect ) + " values" );
+  }
+  // an array's values of connections that another VP creates: each VP's cursor passes over them
+  auto skip = [ &wp, &dp ]( int vp, size_t n )
+  {
+    if ( n == 0 )
+      return;
+    if ( wp )
+      wp->skip( vp, n );
+    if ( dp )
+      dp->skip( vp, n );
+  };
 
   if ( pre_kind == 1 )
   {
@@ -1105,7 +1166,11 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       {
         const int64_t t = post[ ti ];
         if ( vp_of( t ) != vp )
+        {
+          if ( arrays )
+            skip( vp, rule == "one_to_one" ? 1 : pre.size() );
           continue;
+        }
         auto one = [ & ]( int64_t s )
         {
           double delay = 1.0, weight = 1.0;
@@ -1165,7 +1230,23 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       {
         const int64_t t = post[ ti ];
         if ( vp_of( t ) != vp )
+        {
+          // conn_builder.cpp:985-996 (one_to_one checks autapses first and skips no value for
+          // one), 1186-1192 (all_to_all), 1466-1474 (fixed_indegree)
+          if ( arrays )
+          {
+            if ( rule == "one_to_one" )
+            {
+              if ( not( pre[ ti ] == t and not autapses ) )
+                skip( vp, 1 );
+            }
+            else if ( rule == "all_to_all" )
+              skip( vp, pre.size() );
+            else if ( rule == "fixed_indegree" )
+              skip( vp, static_cast< size_t >( K ) );
+          }
           continue;
+        }
         if ( rule == "one_to_one" )
         {
           const int64_t s = pre[ ti ];
@@ -1178,7 +1259,11 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
           for ( int64_t s : pre )
           {
             if ( s == t and not autapses )
+            {
+              if ( arrays )
+                skip( vp, 1 );  // conn_builder.cpp:1244-1251
               continue;
+            }
             single_connect( s, t, vp, rng, wp.get(), dp.get() );
           }
         }

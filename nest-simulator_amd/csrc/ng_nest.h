@@ -135,15 +135,31 @@ public:
     PRODUCT,
     MAX,
     MIN,
-    REDRAW
+    REDRAW,
+    ARRAY
   };
-  static std::unique_ptr< Param > from_json( const Json& j, int n_vp );
+  // top_level: a JSON array is an array parameter (one value per connection, a syn_spec
+  // value of its own, not an operand)
+  static std::unique_ptr< Param > from_json( const Json& j, int n_vp, bool top_level = false );
   double value( VpRng& rng, int vp );
   bool
   is_const() const
   {
     return op_ == CONST;
   }
+  bool
+  is_array() const
+  {
+    return op_ == ARRAY;
+  }
+  size_t
+  array_size() const
+  {
+    return values_.size();
+  }
+  // passes over n values of VP vp's cursor: the connections another VP creates
+  // (ArrayDoubleParameter::skip, conn_parameter.h:242-252)
+  void skip( int vp, size_t n );
   double
   const_value() const
   {
@@ -193,6 +209,14 @@ private:
     }
   };
   std::vector< VpNormal > normals_;  // one per VP (parameter.cpp:95-117)
+  // array parameter: the values and each VP's next index (ArrayDoubleParameter,
+  // conn_parameter.h:231-300)
+  struct alignas( 64 ) VpCursor
+  {
+    size_t next = 0;
+  };
+  std::vector< double > values_;
+  std::vector< VpCursor > cursor_;
 };
 
 struct NeuronParams

@@ -173,6 +173,46 @@ def _to_json(d):
     return json.dumps(conv(d if d is not None else {}))
 
 
+def _process_syn_spec(syn_spec, conn_spec, prelength, postlength):
+    """Array-valued syn_spec entries checked and laid out as PyNEST does
+    (pynest/nest/lib/hl_api_connection_helpers.py:48-130): 1-D with one value per pair for
+    one_to_one; 2-D [n_target, n_source] for all_to_all and [n_target, indegree] for
+    fixed_indegree, flattened target-major."""
+    if not syn_spec:
+        return syn_spec
+    rule = conn_spec if isinstance(conn_spec, str) else conn_spec.get("rule", "all_to_all")
+    out = dict(syn_spec)
+    for key, value in syn_spec.items():
+        if isinstance(value, (list, tuple)):
+            value = np.asarray(value)
+        if not isinstance(value, np.ndarray) or value.ndim == 0:
+            continue
+        if value.ndim == 1:
+            if rule != "one_to_one":
+                raise TypeError(f"'{key}' has the wrong type. One-dimensional parameter arrays can only be used in "
+                                "conjunction with rule 'one_to_one'.")
+            if value.shape[0] != prelength:
+                raise ValueError(f"'{key}' has to be an array of dimension {prelength}, a scalar or a dictionary.")
+            out[key] = value
+        elif value.ndim == 2:
+            if rule == "all_to_all":
+                if value.shape != (postlength, prelength):
+                    raise ValueError(f"'{key}' has to be an array of dimension {postlength}x{prelength} "
+                                     "(n_target x n_sources), a scalar or a dictionary.")
+            elif rule == "fixed_indegree":
+                indegree = conn_spec["indegree"]
+                if value.shape != (postlength, indegree):
+                    raise ValueError(f"'{key}' has to be an array of dimension {postlength}x{indegree} "
+                                     "(n_target x indegree), a scalar or a dictionary.")
+            else:
+                raise TypeError(f"'{key}' has the wrong type. Two-dimensional parameter arrays can only be used in "
+                                "conjunction with rules 'all_to_all' or 'fixed_indegree'.")
+            out[key] = value.flatten()
+        else:
+            raise TypeError(f"'{key}' has the wrong type: parameter arrays have one or two dimensions.")
+    return out
+
+
 # ------------------------------------------------------------------ nodes
 class NodeCollection:
     """Ids of nodes of one kernel (pynest NodeCollection subset)."""
@@ -289,6 +329,7 @@ class Kernel:
             conn_spec = "all_to_all"
         if isinstance(syn_spec, str):
             syn_spec = {"synapse_model": syn_spec}
+        syn_spec = _process_syn_spec(syn_spec, conn_spec, len(pre_ids), len(post_ids))
         self._chk(_lib.nestgpu_connect(self._h, _ptr(pre_ids, ctypes.c_int64), len(pre_ids),
                                        _ptr(post_ids, ctypes.c_int64), len(post_ids),
                                        _to_json(conn_spec).encode(), _to_json(syn_spec or {}).encode()))

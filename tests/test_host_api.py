@@ -124,6 +124,117 @@ def test_connect_arrays_connectivity_identical_to_oracle(n_vp, n):
     assert np.array_equal(w1, w2) and np.array_equal(d1, d2)
 
 
+def _weighted_matrix(k, pre, post, field):
+    c = k.GetConnections()
+    vals = c["weight"] if field == "weight" else c["delay_steps"] * 0.1
+    M = np.zeros((len(post), len(pre)))
+    first_pre, first_post = pre.tolist()[0], post.tolist()[0]
+    for s_, t_, v in zip(c["source"], c["target"], vals):
+        if first_pre <= s_ < first_pre + len(pre) and first_post <= t_ < first_post + len(post):
+            M[t_ - first_post, s_ - first_pre] += v
+    return M
+
+
+@pytest.mark.parametrize("n_vp", [1, 3])
+def test_one_to_one_parameter_arrays(n_vp):
+    # connect/test_connect_one_to_one.py:63-74: one value per pair
+    N = 50
+    for label, arr in [("weight", np.arange(N, dtype=float)), ("delay", np.arange(1, N + 1) * 0.1)]:
+        k = nestgpu.Kernel(0)
+        k.SetKernelStatus({"total_num_virtual_procs": n_vp})
+        a = k.Create("iaf_psc_exp", N)
+        b = k.Create("iaf_psc_exp", N)
+        k.Connect(a, b, "one_to_one", {label: arr})
+        np.testing.assert_allclose(_weighted_matrix(k, a, b, label), np.diag(arr), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n_vp", [1, 4])
+def test_all_to_all_parameter_arrays(n_vp):
+    # connect/test_connect_all_to_all.py:56-71: [n_target, n_source] arrays
+    N1, N2 = 500, 10
+    for label, arr in [("weight", np.arange(N1 * N2, dtype=float).reshape(N2, N1)),
+                       ("delay", np.arange(1, N1 * N2 + 1).reshape(N2, N1) * 0.1)]:
+        k = nestgpu.Kernel(0)
+        k.SetKernelStatus({"total_num_virtual_procs": n_vp})
+        a = k.Create("iaf_psc_exp", N1)
+        b = k.Create("iaf_psc_exp", N2)
+        k.Connect(a, b, "all_to_all", {label: arr})
+        np.testing.assert_allclose(_weighted_matrix(k, a, b, label), arr, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("n_vp", [1, 3])
+def test_all_to_all_parameter_arrays_without_autapses(n_vp):
+    # connect/test_connect_all_to_all.py:73-85: an autapse's value is passed over
+    N = 20
+    for label, arr in [("weight", np.arange(N * N, dtype=float).reshape(N, N)),
+                       ("delay", np.arange(1, N * N + 1).reshape(N, N) * 0.1)]:
+        k = nestgpu.Kernel(0)
+        k.SetKernelStatus({"total_num_virtual_procs": n_vp})
+        a = k.Create("iaf_psc_exp", N)
+        k.Connect(a, a, {"rule": "all_to_all", "allow_autapses": False}, {label: arr})
+        expected = arr.copy()
+        np.fill_diagonal(expected, 0)
+        np.testing.assert_allclose(_weighted_matrix(k, a, a, label), expected, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("n_vp", [1, 3])
+def test_fixed_indegree_parameter_arrays(n_vp):
+    # connect/test_connect_array_fixed_indegree.py: [n_target, indegree] arrays, each target
+    # receives its row; delay = weight + 1 pairs the two arrays
+    N, K = 20, 5
+    Warr = [[y * K + x for x in range(K)] for y in range(N)]
+    Darr = [[y * K + x + 1 for x in range(K)] for y in range(N)]
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"total_num_virtual_procs": n_vp})
+    net1 = k.Create("iaf_psc_exp", N)
+    net2 = k.Create("iaf_psc_exp", N)
+    k.Connect(net1, net2, {"rule": "fixed_indegree", "indegree": K},
+              {"synapse_model": "static_synapse", "weight": Warr, "delay": Darr})
+    c = k.GetConnections()
+    first = net2.tolist()[0]
+    for i in range(N):
+        sel = c["target"] == first + i
+        w, d = c["weight"][sel], c["delay_steps"][sel] * 0.1
+        np.testing.assert_allclose(d - w, 1.0, rtol=0, atol=1e-9)
+        assert len(w) == K and sorted(w) == sorted(Warr[i])
+
+
+def test_parameter_arrays_draw_nothing_from_the_streams():
+    # an array value takes no random draw: sources and normal weights are the oracle's with a
+    # constant delay, and every target's delays are its row of the array in draw order
+    n_vp, N, K = 3, 60, 7
+    darr = np.round(np.random.default_rng(4).uniform(0.1, 3.0, size=(N, K)), 1)
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": 9})
+    a = k.Create("iaf_psc_delta", N)
+    k.Connect(a, a, {"rule": "fixed_indegree", "indegree": K},
+              {"weight": nestgpu.random.normal(1.0, 0.5), "delay": darr})
+    o = Oracle(n_vp=n_vp, seed=9)
+    g = o.create("iaf_psc_delta", N, {})
+    o.connect(g, g, "fixed_indegree", K, weight=("normal", 1.0, 0.5), delay=1.0)
+    s1, t1, w1, d1 = _conns_gpu(k)
+    s2, t2, w2, d2 = o.connections()
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2) and np.array_equal(w1, w2)
+    first = a.tolist()[0]
+    for i in range(N):
+        np.testing.assert_array_equal(d1[t1 == first + i], np.floor(darr[i] * 10 + 0.5).astype(np.int64))
+
+
+@pytest.mark.parametrize("bad, exc", [
+    (lambda k, a: k.Connect(a, a, "one_to_one", {"weight": np.ones(4)}), ValueError),
+    (lambda k, a: k.Connect(a, a, "all_to_all", {"weight": np.ones(5)}), TypeError),
+    (lambda k, a: k.Connect(a, a, "all_to_all", {"weight": np.ones((5, 4))}), ValueError),
+    (lambda k, a: k.Connect(a, a, {"rule": "pairwise_bernoulli", "p": 0.5}, {"weight": np.ones((5, 5))}), TypeError),
+])
+def test_parameter_array_shapes_checked(bad, exc):
+    # hl_api_connection_helpers.py:48-130
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 5)
+    with pytest.raises(exc):
+        bad(k, a)
+    assert len(k.GetConnections()["source"]) == 0
+
+
 def test_get_set_status_roundtrip_and_relative_voltages():
     k = nestgpu.Kernel(0)
     n = k.Create("iaf_psc_delta", 3, {"E_L": -65.0, "V_th": -50.0})
