@@ -350,7 +350,23 @@ class Kernel:
         self._chk(_lib.nestgpu_set_spike_counts(self._h, int(ids[0]), len(ids), int(first_step), c.shape[0],
                                                 _ptr(c, ctypes.c_uint8)))
 
-    def GetConnections(self):
+    def GetConnections(self, source=None, target=None, synapse_model=None):
+        """Connections in creation order (VP, then creation within the VP), as arrays. source,
+        target: node collections (or ids) that the connections' ends must belong to;
+        synapse_model: only static_synapse exists here (hl_api_connections.py GetConnections)."""
+        c = self._all_connections()
+        keep = np.ones(len(c["source"]), bool)
+        if source is not None:
+            keep &= np.isin(c["source"], source._ids if isinstance(source, NodeCollection) else _i64(source))
+        if target is not None:
+            keep &= np.isin(c["target"], target._ids if isinstance(target, NodeCollection) else _i64(target))
+        if synapse_model is not None and synapse_model not in ("static_synapse", "static_synapse_hpc"):
+            keep[:] = False
+        if keep.all():
+            return c
+        return {key: v[keep] for key, v in c.items()}
+
+    def _all_connections(self):
         n = _lib.nestgpu_num_connections(self._h)
         s = np.zeros(n, np.int64)
         t = np.zeros(n, np.int64)
@@ -492,8 +508,8 @@ def Connect(pre, post, conn_spec=None, syn_spec=None):
     _k().Connect(pre, post, conn_spec, syn_spec)
 
 
-def GetConnections():
-    return _k().GetConnections()
+def GetConnections(source=None, target=None, synapse_model=None):
+    return _k().GetConnections(source, target, synapse_model)
 
 
 def Simulate(t):

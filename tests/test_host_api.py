@@ -235,6 +235,24 @@ def test_parameter_array_shapes_checked(bad, exc):
     assert len(k.GetConnections()["source"]) == 0
 
 
+def test_get_connections_filters():
+    # hl_api_connections.py GetConnections(source, target, synapse_model): filters keep order
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"total_num_virtual_procs": 3})
+    a = k.Create("iaf_psc_exp", 30)
+    k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 4}, {"weight": nestgpu.random.normal(1.0, 0.2)})
+    c = k.GetConnections()
+    src, tgt = a[:10], a[5:20]
+    f = k.GetConnections(source=src, target=tgt)
+    keep = np.isin(c["source"], src.tolist()) & np.isin(c["target"], tgt.tolist())
+    assert keep.any() and not keep.all()
+    for key in c:
+        np.testing.assert_array_equal(f[key], c[key][keep])
+    assert np.array_equal(k.GetConnections(target=a[3])["target"], c["target"][c["target"] == a.tolist()[3]])
+    assert len(k.GetConnections(synapse_model="static_synapse")["source"]) == len(c["source"])
+    assert len(k.GetConnections(synapse_model="stdp_synapse")["source"]) == 0
+
+
 def test_get_set_status_roundtrip_and_relative_voltages():
     k = nestgpu.Kernel(0)
     n = k.Create("iaf_psc_delta", 3, {"E_L": -65.0, "V_th": -50.0})
