@@ -344,11 +344,21 @@ Kernel::set_kernel_status( const Json& d )
     const std::string& k = kv.first;
     if ( k == "resolution" )
     {
+      // SimulationManager::set_status (simulation_manager.cpp:183-266)
       if ( not nodes_.empty() )
-        throw std::runtime_error( "KernelException: Cannot change resolution after nodes have been created." );
+        throw std::runtime_error(
+          "KernelException: Nodes have already been created: time representation cannot be changed." );
       const double r = kv.second.number();
-      if ( r <= 0 )
-        throw std::runtime_error( "BadProperty: resolution must be greater than zero." );
+      if ( not( r >= tb_.ms_per_tic ) )
+        throw std::runtime_error( "KernelException: Resolution must be greater than or equal to one tic. Value unchanged." );
+      // is_integer (libnestutil/numerics.cpp:62-76): within two bits of rounding of an integer
+      const double n_tics = r / tb_.ms_per_tic;
+      double int_part;
+      double frac_part = std::modf( n_tics, &int_part );
+      if ( frac_part > 0.5 )
+        frac_part = 1 - frac_part;
+      if ( not( frac_part < 4 * n_tics * std::numeric_limits< double >::epsilon() ) )
+        throw std::runtime_error( "KernelException: Resolution must be a multiple of the tic length. Value unchanged." );
       tb_.set_resolution( r );
     }
     else if ( k == "rng_seed" )

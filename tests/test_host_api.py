@@ -354,8 +354,21 @@ def test_delay_extrema_reported_from_connections():
 def test_resolution_fixed_once_nodes_exist():
     k = nestgpu.Kernel(0)
     k.Create("iaf_psc_delta", 1)
-    with pytest.raises(nestgpu.NESTError, match="resolution"):
+    with pytest.raises(nestgpu.NESTError, match="time representation cannot be changed"):
         k.SetKernelStatus({"resolution": 0.01})
+
+
+@pytest.mark.parametrize("res, msg", [(0.0005, "at least one tic|greater than or equal to one tic"),
+                                      (0.1005, "multiple of the tic length"), (-0.1, "one tic")])
+def test_resolution_checked_against_the_tic(res, msg):
+    # simulation_manager.cpp:257-266: at least one tic (1 us) and a whole number of tics
+    k = nestgpu.Kernel(0)
+    with pytest.raises(nestgpu.NESTError, match=msg):
+        k.SetKernelStatus({"resolution": res})
+    assert k.GetKernelStatus("resolution") == 0.1
+    for ok in (0.01, 0.25, 1.0):
+        k.SetKernelStatus({"resolution": ok})
+        assert k.GetKernelStatus("resolution") == ok
 
 
 def test_reset_kernel_clears_network():
