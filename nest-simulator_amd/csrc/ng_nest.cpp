@@ -608,12 +608,23 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
       if ( has( keys[ i ] ) )
         *qdst[ i ] = get( keys[ i ] );
     (void) dst;
+    // Parameters_::set and State_::set (aeif_cond_exp.cpp:233-304), checked in that order;
+    // nothing is changed unless every check passes
     if ( q.V_peak < q.V_th )
       throw std::runtime_error( "BadProperty: V_peak >= V_th required." );
-    if ( q.V_reset >= q.V_peak )
-      throw std::runtime_error( "BadProperty: Ensure that: V_reset < V_peak ." );
     if ( q.Delta_T < 0. )
       throw std::runtime_error( "BadProperty: Delta_T must be positive." );
+    if ( q.Delta_T > 0. )
+    {
+      // the exponential term must not overflow at spike time, with a margin of 1e20
+      const double max_exp_arg = std::log( std::numeric_limits< double >::max() / 1e20 );
+      if ( ( q.V_peak - q.V_th ) / q.Delta_T >= max_exp_arg )
+        throw std::runtime_error( "BadProperty: The current combination of V_peak, V_th and Delta_T will lead to "
+                                  "numerical overflow at spike time; try for instance to increase Delta_T or to "
+                                  "reduce V_peak to avoid this problem." );
+    }
+    if ( q.V_reset >= q.V_peak )
+      throw std::runtime_error( "BadProperty: Ensure that: V_reset < V_peak ." );
     if ( q.c_m <= 0 )
       throw std::runtime_error( "BadProperty: Ensure that C_m > 0" );
     if ( q.t_ref < 0 )
@@ -622,13 +633,15 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
       throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
     if ( q.tol <= 0. )
       throw std::runtime_error( "BadProperty: The gsl_error_tol must be strictly positive." );
+    const double g_ex = has( "g_ex" ) ? get( "g_ex" ) : s.v1;
+    const double g_in = has( "g_in" ) ? get( "g_in" ) : s.v2;
+    if ( g_ex < 0 or g_in < 0 )
+      throw std::runtime_error( "BadProperty: Conductances must not be negative." );
     p = q;
     if ( has( "V_m" ) )
       s.v0 = get( "V_m" );
-    if ( has( "g_ex" ) )
-      s.v1 = get( "g_ex" );
-    if ( has( "g_in" ) )
-      s.v2 = get( "g_in" );
+    s.v1 = g_ex;
+    s.v2 = g_in;
     if ( has( "w" ) )
       s.v3 = get( "w" );
   }

@@ -280,6 +280,9 @@ def test_get_set_status_roundtrip_and_relative_voltages():
     (lambda k: k.Create("iaf_psc_delta", 1, {"C_m": -1.0}), "Capacitance"),
     (lambda k: k.Create("iaf_psc_exp", 1, {"V_reset": -40.0, "V_th": -50.0}), "Reset potential"),
     (lambda k: k.Create("aeif_cond_exp", 1, {"V_peak": -60.0}), "V_peak"),
+    # aeif_cond_exp.cpp:244-256 (exponential overflow guard) and :294-304 (state)
+    (lambda k: k.Create("aeif_cond_exp", 1, {"V_peak": 1000.0, "Delta_T": 0.01}), "numerical overflow"),
+    (lambda k: k.Create("aeif_cond_exp", 1, {"g_ex": -1.0}), "Conductances must not be negative"),
     (lambda k: k.SetKernelStatus({"rng_seed": 0}), "RNG seed"),
     (lambda k: k.SetKernelStatus({"no_such_key": 1}), "UnaccessedDictionaryEntry"),
 ])
@@ -329,6 +332,19 @@ def test_connect_error_paths(case, msg):
         calls[case]()
     # a failed Connect adds nothing
     assert len(k.GetConnections()["source"]) == 0
+
+
+def test_failed_set_changes_nothing():
+    # Node::set_status sets temporaries and commits them only when every check passes
+    k = nestgpu.Kernel(0)
+    e = k.Create("aeif_cond_exp", 2)
+    a0, g0 = e.get("a").copy(), e.get("g_ex").copy()
+    with pytest.raises(nestgpu.NESTError, match="Conductances"):
+        e.set({"a": 7.0, "g_ex": -1.0})
+    np.testing.assert_array_equal(e.get("a"), a0)
+    np.testing.assert_array_equal(e.get("g_ex"), g0)
+    e.set({"Delta_T": 0.0})  # Delta_T = 0 is allowed: the model without the exponential term
+    np.testing.assert_array_equal(e.get("Delta_T"), [0.0, 0.0])
 
 
 def test_bad_delay_rejected():
