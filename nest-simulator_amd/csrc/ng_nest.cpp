@@ -626,9 +626,9 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
     if ( q.V_reset >= q.V_peak )
       throw std::runtime_error( "BadProperty: Ensure that: V_reset < V_peak ." );
     if ( q.c_m <= 0 )
-      throw std::runtime_error( "BadProperty: Ensure that C_m > 0" );
+      throw std::runtime_error( "BadProperty: Ensure that C_m >0" );
     if ( q.t_ref < 0 )
-      throw std::runtime_error( "BadProperty: Ensure that t_ref >= 0" );
+      throw std::runtime_error( "BadProperty: Refractory time cannot be negative." );
     if ( q.tau_ex <= 0 or q.tau_in <= 0 or q.tau_w <= 0 )
       throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
     if ( q.tol <= 0. )
@@ -684,12 +684,26 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
       q.t_ref = get( "t_ref" );
     if ( q.V_reset >= q.V_th )
       throw std::runtime_error( "BadProperty: Reset potential must be smaller than threshold." );
-    if ( q.c_m <= 0 )
-      throw std::runtime_error( "BadProperty: Capacitance must be > 0." );
-    if ( q.t_ref < 0 )
-      throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
-    if ( q.tau_m <= 0 or ( q.model == M_IAF_PSC_EXP and ( q.tau_ex <= 0 or q.tau_in <= 0 ) ) )
-      throw std::runtime_error( "BadProperty: Membrane and synapse time constants must be strictly positive." );
+    if ( q.model == M_IAF_PSC_EXP )
+    {
+      // iaf_psc_exp::Parameters_::set (iaf_psc_exp.cpp:149-164)
+      if ( q.c_m <= 0 )
+        throw std::runtime_error( "BadProperty: Capacitance must be strictly positive." );
+      if ( q.tau_m <= 0 or q.tau_ex <= 0 or q.tau_in <= 0 )
+        throw std::runtime_error( "BadProperty: Membrane and synapse time constants must be strictly positive." );
+      if ( q.t_ref < 0 )
+        throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
+    }
+    else
+    {
+      // iaf_psc_delta::Parameters_::set (iaf_psc_delta.cpp:148-163)
+      if ( q.c_m <= 0 )
+        throw std::runtime_error( "BadProperty: Capacitance must be > 0." );
+      if ( q.t_ref < 0 )
+        throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
+      if ( q.tau_m <= 0 )
+        throw std::runtime_error( "BadProperty: Membrane time constant must be > 0." );
+    }
     if ( q.model == M_IAF_PSC_DELTA and has( "refractory_input" ) )
       q.refr_input = d.at( "refractory_input" ).number() != 0;
     if ( q.model == M_IAF_PSC_EXP )

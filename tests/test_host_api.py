@@ -277,7 +277,14 @@ def test_get_set_status_roundtrip_and_relative_voltages():
 @pytest.mark.parametrize("bad, msg", [
     (lambda k: k.Create("no_such_model", 1), "UnknownModelName"),
     (lambda k: k.Create("iaf_psc_delta", 1, {"no_such_param": 1.0}), "UnaccessedDictionaryEntry"),
-    (lambda k: k.Create("iaf_psc_delta", 1, {"C_m": -1.0}), "Capacitance"),
+    (lambda k: k.Create("iaf_psc_delta", 1, {"C_m": -1.0}), "Capacitance must be > 0"),
+    # messages and check order of each model's Parameters_::set
+    (lambda k: k.Create("iaf_psc_exp", 1, {"C_m": 0.0, "tau_m": 0.0}), "Capacitance must be strictly positive"),
+    (lambda k: k.Create("iaf_psc_exp", 1, {"tau_syn_in": 0.0, "t_ref": -1.0}), "time constants must be strictly"),
+    (lambda k: k.Create("iaf_psc_delta", 1, {"t_ref": -1.0, "tau_m": 0.0}), "Refractory time must not be negative"),
+    (lambda k: k.Create("iaf_psc_delta", 1, {"tau_m": 0.0}), "Membrane time constant must be > 0"),
+    (lambda k: k.Create("aeif_cond_exp", 1, {"t_ref": -1.0}), "Refractory time cannot be negative"),
+    (lambda k: k.Create("aeif_cond_exp", 1, {"C_m": 0.0}), "Ensure that C_m >0"),
     (lambda k: k.Create("iaf_psc_exp", 1, {"V_reset": -40.0, "V_th": -50.0}), "Reset potential"),
     (lambda k: k.Create("aeif_cond_exp", 1, {"V_peak": -60.0}), "V_peak"),
     # aeif_cond_exp.cpp:244-256 (exponential overflow guard) and :294-304 (state)
