@@ -299,6 +299,7 @@ Kernel::reset()
   gen_gid_.clear();
   dense_.clear();
   recorders_.clear();
+  rec_window_.clear();
   rec_gid_.clear();
   conns_.assign( 1, ConnList() );
   dev_conns_.clear();
@@ -782,7 +783,15 @@ Kernel::create( const std::string& model, int64_t n, const Json& params )
     {
       nodes_.push_back( NodeRec { 2, static_cast< uint32_t >( recorders_.size() ) } );
       recorders_.emplace_back();
+      rec_window_.emplace_back();
       rec_gid_.push_back( first + i );
+    }
+    if ( params.kind == Json::OBJ and not params.obj.empty() )
+    {
+      std::vector< int64_t > g( n );
+      for ( int64_t i = 0; i < n; ++i )
+        g[ i ] = first + i;
+      set_status( g, params );
     }
   }
   else
@@ -873,7 +882,49 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
       store_dirty_ = true;
     }
     else
-      throw std::runtime_error( "BadProperty: spike_recorder has no settable parameters on the device path" );
+    {
+      RecWindow w = rec_window_[ r.idx ];
+      for ( const auto& kv : d.obj )
+      {
+        const std::string& k = kv.first;
+        if ( k == "origin" or k == "start" or k == "stop" )
+        {
+          // Device::Parameters_::update_ (device.cpp:83-101): finite values on the time grid
+          const double ms = kv.second.number();
+          const bool infinite = not std::isfinite( ms ) or std::fabs( ms ) >= 1e15;
+          long steps = 0;
+          if ( not infinite )
+          {
+            const long tics = tb_.ms_to_tics( ms );
+            if ( tics % tb_.tics_per_step != 0 )
+              throw std::runtime_error( "BadProperty: " + k + " must be a multiple of the simulation resolution." );
+            steps = tics / tb_.tics_per_step;
+          }
+          else if ( k != "stop" or ms < 0 )
+            throw std::runtime_error( "BadProperty: " + k + " must be finite on the device path." );
+          if ( k == "origin" )
+            w.origin = steps;
+          else if ( k == "start" )
+            w.start = steps;
+          else
+          {
+            w.stop = steps;
+            w.stop_inf = infinite;
+          }
+          w.set = true;
+        }
+        else if ( k == "record_to" )
+        {
+          if ( kv.second.string() != "memory" )
+            throw std::runtime_error( "BadProperty: spike_recorder records to memory only on the device path." );
+        }
+        else
+          throw std::runtime_error( "UnaccessedDictionaryEntry: spike_recorder parameter '" + k + "'" );
+      }
+      if ( not w.stop_inf and w.stop < w.start )
+        throw std::runtime_error( "BadProperty: stop >= start required." );
+      rec_window_[ r.idx ] = w;
+    }
   }
 }
 
@@ -935,6 +986,22 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
   for ( size_t i = 0; i < gids.size(); ++i )
   {
     const NodeRec& r = node( gids[ i ] );
+    if ( r.kind == 2 and ( key == "origin" or key == "start" or key == "stop" or key == "n_events" ) )
+    {
+      const RecWindow& w = rec_window_[ r.idx ];
+      if ( key == "n_events" )
+      {
+        std::vector< int64_t > senders, steps;
+        recorder_events( gids[ i ], senders, steps );
+        out[ i ] = static_cast< double >( senders.size() );
+      }
+      else if ( key == "stop" and w.stop_inf )
+        out[ i ] = std::numeric_limits< double >::infinity();
+      else
+        out[ i ] = static_cast< double >( key == "origin" ? w.origin : key == "start" ? w.start : w.stop )
+          * tb_.ms_per_tic * static_cast< double >( tb_.tics_per_step );
+      continue;
+    }
     if ( r.kind != 0 )
       throw std::runtime_error( "BadProperty: '" + key + "' is not a property of " + model_name( gids[ i ] ) );
     const NeuronParams& p = params_[ r.idx ];
@@ -1730,10 +1797,16 @@ Kernel::recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::
     throw std::runtime_error( "BadProperty: node is not a spike_recorder" );
   collect_log();
   const auto& rec = recorders_[ r.idx ];
+  const RecWindow& w = rec_window_[ r.idx ];
+  const long t_min = w.origin + w.start, t_max = w.origin + w.stop;
   std::vector< std::pair< int64_t, int64_t > > ev;
   for ( size_t i = 0; i < spike_sender_.size(); ++i )
     if ( std::binary_search( rec.begin(), rec.end(), spike_sender_[ i ] ) )
+    {
+      if ( w.set and not( t_min < spike_step_[ i ] and ( w.stop_inf or spike_step_[ i ] <= t_max ) ) )
+        continue;
       ev.emplace_back( spike_step_[ i ], spike_sender_[ i ] );
+    }
   std::sort( ev.begin(), ev.end() );
   senders.resize( ev.size() );
   steps.resize( ev.size() );

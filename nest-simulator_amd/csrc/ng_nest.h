@@ -440,6 +440,15 @@ private:
   std::vector< int64_t > gen_gid_;
   std::vector< DenseBlock > dense_;
   std::vector< std::vector< int64_t > > recorders_;  // recorded senders (sorted) per recorder
+  // spike_recorder activity window (Device::Parameters_, device.cpp:83-118, recording_device.cpp:
+  // 206-211): a spike with stamp origin + start < stamp <= origin + stop is recorded; in steps
+  struct RecWindow
+  {
+    long origin = 0, start = 0, stop = 0;
+    bool stop_inf = true;
+    bool set = false;  // default window: every spike
+  };
+  std::vector< RecWindow > rec_window_;
   std::vector< int64_t > rec_gid_;
   std::vector< ConnList > conns_;  // per VP
   struct DevConn

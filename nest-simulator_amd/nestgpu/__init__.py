@@ -163,7 +163,9 @@ def _to_json(d):
         if isinstance(x, np.ndarray):
             return x.tolist()
         if isinstance(x, (np.floating, np.integer)):
-            return x.item()
+            x = x.item()
+        if isinstance(x, float) and np.isinf(x):
+            return 1e308 if x > 0 else -1e308  # JSON has no infinity; the library reads these as infinite
         if isinstance(x, dict):
             return {k: conv(v) for k, v in x.items()}
         if isinstance(x, (list, tuple)):

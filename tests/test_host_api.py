@@ -354,6 +354,27 @@ def test_failed_set_changes_nothing():
     np.testing.assert_array_equal(e.get("Delta_T"), [0.0, 0.0])
 
 
+def test_spike_recorder_window_parameters():
+    # Device::Parameters_ (device.cpp:83-118): origin, start, stop on the time grid, stop >= start
+    k = nestgpu.Kernel(0)
+    sr = k.Create("spike_recorder", 1, {"start": 10.0, "stop": 50.0, "origin": 5.0})
+    assert (sr.get("start"), sr.get("stop"), sr.get("origin")) == (10.0, 50.0, 5.0)
+    assert sr.get("n_events") == 0
+    sr.set({"stop": float("inf")})
+    assert np.isinf(sr.get("stop"))
+    d = k.Create("spike_recorder")
+    assert d.get("start") == 0.0 and np.isinf(d.get("stop")) and d.get("origin") == 0.0
+    for bad, msg in [({"start": 0.05}, "multiple of the simulation resolution"),
+                     ({"start": 20.0, "stop": 10.0}, "stop >= start"),
+                     ({"record_to": "ascii"}, "memory"),
+                     ({"no_such_key": 1.0}, "UnaccessedDictionaryEntry")]:
+        with pytest.raises(nestgpu.NESTError, match=msg):
+            k.Create("spike_recorder", 1, bad)
+    with pytest.raises(nestgpu.NESTError, match="stop >= start"):
+        sr.set({"start": 100.0, "stop": 60.0})
+    assert sr.get("start") == 10.0  # a failed set changes nothing
+
+
 def test_bad_delay_rejected():
     k = nestgpu.Kernel(0)
     a = k.Create("iaf_psc_delta", 2)
