@@ -170,7 +170,7 @@ Kernel::build_spec( BuildSpec& b )
       if ( G.dense_block < 0 and G.poisson_lambda == 0 and not G.stamps.empty() )
       {
         any = true;
-        tmax = std::max( tmax, G.stamps.back() );
+        tmax = std::max( tmax, G.window.origin + G.stamps.back() );
       }
     if ( any )
     {
@@ -192,7 +192,12 @@ Kernel::build_spec( BuildSpec& b )
           const int64_t m = G.mults.empty() ? 1 : G.mults[ k ];
           if ( m < 0 or m > 255 )
             throw std::runtime_error( "NotImplemented: spike multiplicities above 255" );
-          sb.counts[ static_cast< size_t >( ( G.stamps[ k ] - 1 ) * sb.cols + g ) ] = static_cast< uint8_t >( m );
+          // emitted at origin + stamp while the window is active; stamps before the first step
+          // are never sent (spike_generator.cpp:334-352)
+          const int64_t stamp = G.window.origin + G.stamps[ k ];
+          if ( stamp < 1 or ( G.window.set and not G.window.active( stamp ) ) )
+            continue;
+          sb.counts[ static_cast< size_t >( ( stamp - 1 ) * sb.cols + g ) ] = static_cast< uint8_t >( m );
         }
       }
       blocks.push_back( std::move( sb ) );

@@ -827,10 +827,13 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
     }
     else if ( r.kind == 1 )
     {
-      SpikeGen& G = gens_[ r.idx ];
+      // set on a copy, kept only when every check passes (Node::set_status)
+      SpikeGen G = gens_[ r.idx ];
       for ( const auto& kv : d.obj )
       {
         const std::string& k = kv.first;
+        if ( update_window( G.window, k, kv.second ) )
+          continue;
         if ( k == "spike_times" )
         {
           G.stamps.clear();
@@ -879,41 +882,20 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
       }
       if ( not G.mults.empty() and G.mults.size() != G.stamps.size() )
         throw std::runtime_error( "BadProperty: spike_multiplicities must have the same length as spike_times." );
+      if ( not G.window.stop_inf and G.window.stop < G.window.start )
+        throw std::runtime_error( "BadProperty: stop >= start required." );
+      gens_[ r.idx ] = std::move( G );
       store_dirty_ = true;
     }
     else
     {
-      RecWindow w = rec_window_[ r.idx ];
+      DeviceWindow w = rec_window_[ r.idx ];
       for ( const auto& kv : d.obj )
       {
         const std::string& k = kv.first;
-        if ( k == "origin" or k == "start" or k == "stop" )
-        {
-          // Device::Parameters_::update_ (device.cpp:83-101): finite values on the time grid
-          const double ms = kv.second.number();
-          const bool infinite = not std::isfinite( ms ) or std::fabs( ms ) >= 1e15;
-          long steps = 0;
-          if ( not infinite )
-          {
-            const long tics = tb_.ms_to_tics( ms );
-            if ( tics % tb_.tics_per_step != 0 )
-              throw std::runtime_error( "BadProperty: " + k + " must be a multiple of the simulation resolution." );
-            steps = tics / tb_.tics_per_step;
-          }
-          else if ( k != "stop" or ms < 0 )
-            throw std::runtime_error( "BadProperty: " + k + " must be finite on the device path." );
-          if ( k == "origin" )
-            w.origin = steps;
-          else if ( k == "start" )
-            w.start = steps;
-          else
-          {
-            w.stop = steps;
-            w.stop_inf = infinite;
-          }
-          w.set = true;
-        }
-        else if ( k == "record_to" )
+        if ( update_window( w, k, kv.second ) )
+          continue;
+        if ( k == "record_to" )
         {
           if ( kv.second.string() != "memory" )
             throw std::runtime_error( "BadProperty: spike_recorder records to memory only on the device path." );
@@ -926,6 +908,38 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
       rec_window_[ r.idx ] = w;
     }
   }
+}
+
+bool
+Kernel::update_window( DeviceWindow& w, const std::string& k, const Json& v ) const
+{
+  // Device::Parameters_::update_ (device.cpp:83-101): finite values on the time grid, or an
+  // infinite stop
+  if ( k != "origin" and k != "start" and k != "stop" )
+    return false;
+  const double ms = v.number();
+  const bool infinite = not std::isfinite( ms ) or std::fabs( ms ) >= 1e15;
+  long steps = 0;
+  if ( not infinite )
+  {
+    const long tics = tb_.ms_to_tics( ms );
+    if ( tics % tb_.tics_per_step != 0 )
+      throw std::runtime_error( "BadProperty: " + k + " must be a multiple of the simulation resolution." );
+    steps = tics / tb_.tics_per_step;
+  }
+  else if ( k != "stop" or ms < 0 )
+    throw std::runtime_error( "BadProperty: " + k + " must be finite on the device path." );
+  if ( k == "origin" )
+    w.origin = steps;
+  else if ( k == "start" )
+    w.start = steps;
+  else
+  {
+    w.stop = steps;
+    w.stop_inf = infinite;
+  }
+  w.set = true;
+  return true;
 }
 
 void
@@ -986,9 +1000,9 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
   for ( size_t i = 0; i < gids.size(); ++i )
   {
     const NodeRec& r = node( gids[ i ] );
-    if ( r.kind == 2 and ( key == "origin" or key == "start" or key == "stop" or key == "n_events" ) )
+    if ( ( r.kind == 2 and key == "n_events" ) or ( r.kind != 0 and ( key == "origin" or key == "start" or key == "stop" ) ) )
     {
-      const RecWindow& w = rec_window_[ r.idx ];
+      const DeviceWindow& w = r.kind == 2 ? rec_window_[ r.idx ] : gens_[ r.idx ].window;
       if ( key == "n_events" )
       {
         std::vector< int64_t > senders, steps;
@@ -1797,13 +1811,12 @@ Kernel::recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::
     throw std::runtime_error( "BadProperty: node is not a spike_recorder" );
   collect_log();
   const auto& rec = recorders_[ r.idx ];
-  const RecWindow& w = rec_window_[ r.idx ];
-  const long t_min = w.origin + w.start, t_max = w.origin + w.stop;
+  const DeviceWindow& w = rec_window_[ r.idx ];
   std::vector< std::pair< int64_t, int64_t > > ev;
   for ( size_t i = 0; i < spike_sender_.size(); ++i )
     if ( std::binary_search( rec.begin(), rec.end(), spike_sender_[ i ] ) )
     {
-      if ( w.set and not( t_min < spike_step_[ i ] and ( w.stop_inf or spike_step_[ i ] <= t_max ) ) )
+      if ( w.set and not w.active( spike_step_[ i ] ) )
         continue;
       ev.emplace_back( spike_step_[ i ], spike_sender_[ i ] );
     }

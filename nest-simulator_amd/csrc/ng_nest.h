@@ -235,8 +235,25 @@ struct NeuronState
   int32_t r;
 };
 
+// activity window of a device (Device::Parameters_, device.cpp:83-118) in steps: a spike with
+// stamp origin + start < stamp <= origin + stop is recorded or emitted
+// (RecordingDevice::is_active, recording_device.cpp:206-211; StimulationDevice::is_active,
+// stimulation_device.cpp:47-57)
+struct DeviceWindow
+{
+  long origin = 0, start = 0, stop = 0;
+  bool stop_inf = true;
+  bool set = false;  // the default window passes every spike
+  bool
+  active( long stamp ) const
+  {
+    return origin + start < stamp and ( stop_inf or stamp <= origin + stop );
+  }
+};
+
 struct SpikeGen
 {
+  DeviceWindow window;  // emitted stamps are window.origin + stamps[k]
   std::vector< int64_t > stamps;  // spike stamps in steps (emission step = stamp - 1)
   std::vector< int64_t > mults;
   // dense per-step counts: column in a kernel-wide block
@@ -440,15 +457,9 @@ private:
   std::vector< int64_t > gen_gid_;
   std::vector< DenseBlock > dense_;
   std::vector< std::vector< int64_t > > recorders_;  // recorded senders (sorted) per recorder
-  // spike_recorder activity window (Device::Parameters_, device.cpp:83-118, recording_device.cpp:
-  // 206-211): a spike with stamp origin + start < stamp <= origin + stop is recorded; in steps
-  struct RecWindow
-  {
-    long origin = 0, start = 0, stop = 0;
-    bool stop_inf = true;
-    bool set = false;  // default window: every spike
-  };
-  std::vector< RecWindow > rec_window_;
+  std::vector< DeviceWindow > rec_window_;  // per spike_recorder
+  // origin, start or stop into w; false for any other key
+  bool update_window( DeviceWindow& w, const std::string& k, const Json& v ) const;
   std::vector< int64_t > rec_gid_;
   std::vector< ConnList > conns_;  // per VP
   struct DevConn

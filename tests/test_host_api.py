@@ -375,6 +375,23 @@ def test_spike_recorder_window_parameters():
     assert sr.get("start") == 10.0  # a failed set changes nothing
 
 
+def test_spike_generator_window_parameters():
+    # StimulationDevice window (device.cpp:83-118): checked like the recorder's, and a failed
+    # set keeps the generator's spike times
+    k = nestgpu.Kernel(0)
+    g = k.Create("spike_generator", 1, {"spike_times": [1.0, 2.0], "origin": 1.0, "start": 0.5, "stop": 2.0})
+    assert (g.get("origin"), g.get("start"), g.get("stop")) == (1.0, 0.5, 2.0)
+    with pytest.raises(nestgpu.NESTError, match="stop >= start"):
+        g.set({"spike_times": [5.0], "start": 3.0})
+    with pytest.raises(nestgpu.NESTError, match="multiple of the simulation resolution"):
+        g.set({"origin": -2.0})  # Time(ms) truncates toward zero: off the grid, as in the reference
+    with pytest.raises(nestgpu.NESTError, match="sorted"):
+        g.set({"spike_times": [3.0, 1.0]})
+    assert g.get("start") == 0.5
+    n = k.Create("iaf_psc_exp", 1)
+    k.Connect(g, n, "one_to_one", {"weight": 1.0, "delay": 1.0})  # spike times kept: still valid
+
+
 def test_bad_delay_rejected():
     k = nestgpu.Kernel(0)
     a = k.Create("iaf_psc_delta", 2)
