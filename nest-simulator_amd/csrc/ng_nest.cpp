@@ -1482,18 +1482,26 @@ Kernel::connect_arrays( const int64_t* src, const int64_t* tgt, const double* w,
     dmax = std::max( dmax, ds );
     ++per_vp[ vp_of( tgt[ i ] ) ];
   }
+  std::vector< std::string > errors( n_vp_ );
   auto fill = [ & ]( int vp )
   {
-    ConnList& C = conns_[ vp ];
-    C.reserve_more( per_vp[ vp ] );
-    for ( int64_t i = 0; i < n; ++i )
+    try
     {
-      if ( vp_of( tgt[ i ] ) != vp )
-        continue;
-      C.src.push_back( static_cast< uint32_t >( src[ i ] ) );
-      C.tgt.push_back( static_cast< uint32_t >( tgt[ i ] ) );
-      C.w.push_back( w ? w[ i ] : 1.0 );
-      C.d.push_back( steps[ i ] );
+      ConnList& C = conns_[ vp ];
+      C.reserve_more( per_vp[ vp ] );
+      for ( int64_t i = 0; i < n; ++i )
+      {
+        if ( vp_of( tgt[ i ] ) != vp )
+          continue;
+        C.src.push_back( static_cast< uint32_t >( src[ i ] ) );
+        C.tgt.push_back( static_cast< uint32_t >( tgt[ i ] ) );
+        C.w.push_back( w ? w[ i ] : 1.0 );
+        C.d.push_back( steps[ i ] );
+      }
+    }
+    catch ( const std::exception& e )
+    {
+      errors[ vp ] = e.what();
     }
   };
   if ( n_vp_ == 1 or n < 65536 )
@@ -1507,6 +1515,9 @@ Kernel::connect_arrays( const int64_t* src, const int64_t* tgt, const double* w,
     for ( auto& x : th )
       x.join();
   }
+  for ( const auto& e : errors )
+    if ( not e.empty() )
+      throw std::runtime_error( e );
   if ( n > 0 )
   {
     if ( d_min_seen_ < 0 or dmin < d_min_seen_ )
