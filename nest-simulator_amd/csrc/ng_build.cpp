@@ -369,14 +369,30 @@ Kernel::build_spec( BuildSpec& b )
   //      order), and the final record words written.
   const int n_workers = static_cast< int >(
     std::max< unsigned >( 1u, std::min< unsigned >( 16u, std::thread::hardware_concurrency() ) ) );
+  // runs fn(w) for every worker; an error in any worker is raised after all have joined
   auto parallel = [ n_workers ]( auto&& fn )
   {
+    std::vector< std::string > errors( n_workers );
+    auto guarded = [ & ]( int w )
+    {
+      try
+      {
+        fn( w );
+      }
+      catch ( const std::exception& e )
+      {
+        errors[ w ] = e.what();
+      }
+    };
     std::vector< std::thread > th;
     for ( int w = 1; w < n_workers; ++w )
-      th.emplace_back( fn, w );
-    fn( 0 );
+      th.emplace_back( guarded, w );
+    guarded( 0 );
     for ( auto& x : th )
       x.join();
+    for ( const auto& e : errors )
+      if ( not e.empty() )
+        throw std::runtime_error( e );
   };
   std::vector< int64_t > vp_first( conns_.size() + 1, 0 );
   for ( size_t v = 0; v < conns_.size(); ++v )
