@@ -734,6 +734,8 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
 int64_t
 Kernel::create( const std::string& model, int64_t n, const Json& params )
 {
+  if ( params.kind != Json::OBJ and params.kind != Json::NUL )
+    throw std::runtime_error( "BadProperty: node parameters must be a dictionary" );
   if ( n < 1 )
     throw std::runtime_error( "BadProperty: Create requires n >= 1." );
   if ( simulated_ )
@@ -803,6 +805,8 @@ Kernel::create( const std::string& model, int64_t n, const Json& params )
 void
 Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
 {
+  if ( d.kind != Json::OBJ )
+    throw std::runtime_error( "BadProperty: node parameters must be a dictionary" );
   for ( int64_t g : gids )
   {
     NodeRec& r = node( g );

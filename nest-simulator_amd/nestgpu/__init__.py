@@ -175,6 +175,10 @@ def _to_json(d):
     return json.dumps(conv(d if d is not None else {}))
 
 
+# node parameters whose value is itself a list (not one value per node)
+_LIST_PARAMS = ("spike_times", "spike_multiplicities")
+
+
 def _process_syn_spec(syn_spec, conn_spec, prelength, postlength):
     """Array-valued syn_spec entries checked and laid out as PyNEST does
     (pynest/nest/lib/hl_api_connection_helpers.py:48-130): 1-D with one value per pair for
@@ -256,10 +260,17 @@ class NodeCollection:
         return out[0] if len(out) == 1 else out
 
     def set(self, params=None, **kwargs):
+        if isinstance(params, (list, tuple)):
+            # one dictionary per node (hl_api_types.py NodeCollection.set)
+            if len(params) != len(self._ids):
+                raise TypeError("list of params must have one dictionary per node")
+            for i, d in enumerate(params):
+                self[i].set(d)
+            return
         params = dict(params or {}, **kwargs)
         scalar = {}
         for k, v in params.items():
-            if np.ndim(v) > 0 and k not in ("spike_times", "spike_multiplicities"):
+            if np.ndim(v) > 0 and k not in _LIST_PARAMS:
                 vals = np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
                 if len(vals) != len(self._ids):
                     raise NESTError("value array length does not match the NodeCollection")
@@ -319,10 +330,30 @@ class Kernel:
         return d if key is None else d[key]
 
     def Create(self, model, n=1, params=None):
+        """hl_api_nodes.py Create: params is a dict (scalars, or arrays with one value per
+        node) or a list of dicts, one per node."""
+        if isinstance(n, dict):
+            if params is not None:
+                raise ValueError("A parameter dictionary can be passed as second argument only of Create() is "
+                                 "called with two arguments.")
+            params, n = n, 1
+        if int(n) != n:
+            raise TypeError("n must have an integer value")
+        n = int(n)
+        if isinstance(params, (list, tuple)) and len(params) != n:
+            raise TypeError("list of params must have one dictionary per node")
+        if params is not None and not (isinstance(params, dict) or (isinstance(params, (list, tuple))
+                                                                    and all(isinstance(e, dict) for e in params))):
+            raise TypeError("params must be either a dict of parameters or a list or tuple of dicts")
+        per_node = isinstance(params, (list, tuple)) or (isinstance(params, dict) and any(
+            np.ndim(v) > 0 and k not in _LIST_PARAMS for k, v in params.items()))
         first = ctypes.c_int64(0)
-        self._chk(_lib.nestgpu_create(self._h, model.encode(), int(n), _to_json(params or {}).encode(),
-                                      ctypes.byref(first)))
-        return NodeCollection(self, np.arange(first.value, first.value + n))
+        self._chk(_lib.nestgpu_create(self._h, model.encode(), n,
+                                      _to_json({} if per_node else (params or {})).encode(), ctypes.byref(first)))
+        nc = NodeCollection(self, np.arange(first.value, first.value + n))
+        if per_node:
+            nc.set(params)
+        return nc
 
     def Connect(self, pre, post, conn_spec=None, syn_spec=None):
         pre_ids = pre._ids if isinstance(pre, NodeCollection) else _i64(pre)

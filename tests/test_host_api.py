@@ -392,6 +392,25 @@ def test_spike_generator_window_parameters():
     k.Connect(g, n, "one_to_one", {"weight": 1.0, "delay": 1.0})  # spike times kept: still valid
 
 
+def test_create_with_per_node_parameters():
+    # hl_api_nodes.py Create: a list of dicts or per-node arrays, applied after creation
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 3, [{"I_e": 1.0}, {"I_e": 2.0}, {"I_e": 3.0, "tau_m": 5.0}])
+    np.testing.assert_array_equal(a.get("I_e"), [1.0, 2.0, 3.0])
+    np.testing.assert_array_equal(a.get("tau_m"), [10.0, 10.0, 5.0])
+    b = k.Create("iaf_psc_exp", 3, {"I_e": np.array([4.0, 5.0, 6.0]), "C_m": 200.0})
+    np.testing.assert_array_equal(b.get("I_e"), [4.0, 5.0, 6.0])
+    np.testing.assert_array_equal(b.get("C_m"), [200.0, 200.0, 200.0])
+    c = k.Create("iaf_psc_delta", {"V_th": -50.0})  # a dict as second argument: one node
+    assert len(c) == 1 and c.get("V_th") == -50.0
+    with pytest.raises(TypeError, match="one dictionary per node"):
+        k.Create("iaf_psc_exp", 2, [{"I_e": 1.0}])
+    with pytest.raises(TypeError, match="dict of parameters"):
+        k.Create("iaf_psc_exp", 2, 5.0)
+    with pytest.raises(nestgpu.NESTError, match="must be a dictionary"):
+        k._chk(nestgpu._lib.nestgpu_create(k._h, b"iaf_psc_exp", 1, b"[1, 2]", ctypes.byref(ctypes.c_int64(0))))
+
+
 def test_bad_delay_rejected():
     k = nestgpu.Kernel(0)
     a = k.Create("iaf_psc_delta", 2)
