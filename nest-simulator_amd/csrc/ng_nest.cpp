@@ -1090,6 +1090,10 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       out[ i ] = p.tau_in;
     else if ( key == "global_id" )
       out[ i ] = static_cast< double >( gids[ i ] );
+    else if ( key == "vp" or key == "thread" )
+      out[ i ] = vp_of( gids[ i ] );  // one process: the node's thread is its VP (vp_manager_impl.h:43-52)
+    else if ( key == "refractory_input" and p.model == M_IAF_PSC_DELTA )
+      out[ i ] = p.refr_input;
     else if ( key == "V_min" and p.model == M_IAF_PSC_DELTA )
       out[ i ] = p.V_min + p.E_L;
     else if ( aeif and key == "V_peak" )

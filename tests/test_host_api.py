@@ -411,6 +411,18 @@ def test_create_with_per_node_parameters():
         k._chk(nestgpu._lib.nestgpu_create(k._h, b"iaf_psc_exp", 1, b"[1, 2]", ctypes.byref(ctypes.c_int64(0))))
 
 
+def test_node_placement_and_flags_readable():
+    # nodes are placed on VP gid % n_vp (vp_manager_impl.h:43-52); refractory_input reads back
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"total_num_virtual_procs": 3})
+    a = k.Create("iaf_psc_delta", 5, {"refractory_input": True})
+    np.testing.assert_array_equal(a.get("vp"), np.array(a.tolist()) % 3)
+    np.testing.assert_array_equal(a.get("thread"), a.get("vp"))
+    np.testing.assert_array_equal(a.get("refractory_input"), 1.0)
+    a.set({"refractory_input": False})
+    np.testing.assert_array_equal(a.get("refractory_input"), 0.0)
+
+
 def test_bad_delay_rejected():
     k = nestgpu.Kernel(0)
     a = k.Create("iaf_psc_delta", 2)
