@@ -451,6 +451,14 @@ Kernel::set_kernel_status( const Json& d )
     }
     else if ( k == "print_time" or k == "overwrite_files" or k == "data_path" )
       continue;
+    else if ( k == "biological_time" )
+    {
+      // SimulationManager::set_status (simulation_manager.cpp:151-158)
+      if ( kv.second.number() != 0.0 )
+        throw std::runtime_error( "BadProperty: The simulation time can only be set to 0.0." );
+      if ( simulated_ and engine_ and engine_->clock + engine_->from_step > 0 )
+        throw std::runtime_error( "NotImplemented: resetting the simulation time after Simulate on the device path." );
+    }
     else
       throw std::runtime_error( "UnaccessedDictionaryEntry: kernel status key '" + k + "'" );
   }

@@ -443,6 +443,15 @@ def test_delay_extrema_reported_from_connections():
     assert k.GetKernelStatus("min_delay_steps") == 10 and k.GetKernelStatus("max_delay_steps") == 60
 
 
+def test_biological_time_only_settable_to_zero():
+    # simulation_manager.cpp:151-158
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"biological_time": 0.0})
+    with pytest.raises(nestgpu.NESTError, match="can only be set to 0.0"):
+        k.SetKernelStatus({"biological_time": 5.0})
+    assert k.GetKernelStatus("biological_time") == 0.0
+
+
 def test_resolution_fixed_once_nodes_exist():
     k = nestgpu.Kernel(0)
     k.Create("iaf_psc_delta", 1)
