@@ -1212,6 +1212,22 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       wp = Param::from_json( kv.second, n_vp_, true );
     else if ( kv.first == "delay" )
       dp = Param::from_json( kv.second, n_vp_, true );
+    else if ( kv.first == "receptor_type" )
+    {
+      // the models here have the one receptor 0 (handles_test_event, e.g. iaf_psc_exp.h:466-474)
+      const auto check = [ & ]( const Json& v )
+      {
+        const long rt = static_cast< long >( v.number() );
+        if ( rt != 0 )
+          throw std::runtime_error( "UnknownReceptorType: Receptor type " + std::to_string( rt )
+            + " is not available in " + model_name( post[ 0 ] ) + "." );
+      };
+      if ( kv.second.kind == Json::ARR )
+        for ( const Json& v : kv.second.arr )
+          check( v );
+      else
+        check( kv.second );
+    }
     else
       throw std::runtime_error( "UnaccessedDictionaryEntry: syn_spec key '" + kv.first + "'" );
   }

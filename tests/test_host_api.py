@@ -314,6 +314,8 @@ def test_error_paths(bad, msg):
     ("unknown_synapse", "UnknownSynapseType"),
     ("generator_target", "IllegalConnection"),
     ("recorder_source", "IllegalConnection"),
+    # iaf_psc_exp.h:466-474: receptor 0 only
+    ("receptor_type", "UnknownReceptorType: Receptor type 1 is not available in iaf_psc_exp"),
 ])
 def test_connect_error_paths(case, msg):
     k = nestgpu.Kernel(0)
@@ -334,6 +336,7 @@ def test_connect_error_paths(case, msg):
         "unknown_synapse": lambda: k.Connect(a, a, "all_to_all", {"synapse_model": "no_such_synapse"}),
         "generator_target": lambda: k.Connect(a, sg, "one_to_one"),
         "recorder_source": lambda: k.Connect(sr, a),
+        "receptor_type": lambda: k.Connect(a, a, "all_to_all", {"receptor_type": 1}),
     }
     with pytest.raises(nestgpu.NESTError, match=msg):
         calls[case]()
@@ -421,6 +424,14 @@ def test_node_placement_and_flags_readable():
     np.testing.assert_array_equal(a.get("refractory_input"), 1.0)
     a.set({"refractory_input": False})
     np.testing.assert_array_equal(a.get("refractory_input"), 0.0)
+
+
+def test_receptor_type_zero_accepted():
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 3)
+    k.Connect(a, a, "one_to_one", {"receptor_type": 0, "weight": 2.0})
+    k.Connect(a, a, "one_to_one", {"receptor_type": [0, 0, 0]})
+    assert len(k.GetConnections()["source"]) == 6
 
 
 def test_bad_delay_rejected():
