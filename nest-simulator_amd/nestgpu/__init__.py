@@ -327,7 +327,14 @@ class Kernel:
             if n.value < cap:
                 break
         d = json.loads(buf.value.decode())
-        return d if key is None else d[key]
+        # hl_api_simulation.py GetKernelStatus: one key, or a list of keys giving a tuple
+        if key is None:
+            return d
+        if isinstance(key, str):
+            return d[key]
+        if isinstance(key, (list, tuple)):
+            return tuple(d[k] for k in key)
+        raise TypeError("keys should be either a string or an iterable")
 
     def Create(self, model, n=1, params=None):
         """hl_api_nodes.py Create: params is a dict (scalars, or arrays with one value per

@@ -463,6 +463,15 @@ def test_biological_time_only_settable_to_zero():
     assert k.GetKernelStatus("biological_time") == 0.0
 
 
+def test_get_kernel_status_keys():
+    k = nestgpu.Kernel(0)
+    assert k.GetKernelStatus(["resolution", "total_num_virtual_procs"]) == (0.1, 1)
+    with pytest.raises(KeyError):
+        k.GetKernelStatus("no_such_key")
+    with pytest.raises(TypeError):
+        k.GetKernelStatus(5)
+
+
 def test_resolution_fixed_once_nodes_exist():
     k = nestgpu.Kernel(0)
     k.Create("iaf_psc_delta", 1)
