@@ -1179,6 +1179,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
   std::string rule = "all_to_all";
   bool autapses = true, multapses = true;
   double indegree = 0, p = 0;
+  bool has_indegree = false, has_p = false;
   if ( conn_spec.kind == Json::STR )
     rule = conn_spec.str;
   else if ( conn_spec.kind == Json::OBJ )
@@ -1192,9 +1193,15 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       else if ( kv.first == "allow_multapses" )
         multapses = kv.second.number() != 0;
       else if ( kv.first == "indegree" )
+      {
         indegree = kv.second.number();
+        has_indegree = true;
+      }
       else if ( kv.first == "p" )
+      {
         p = kv.second.number();
+        has_p = true;
+      }
       else
         throw std::runtime_error( "UnaccessedDictionaryEntry: conn_spec key '" + kv.first + "'" );
     }
@@ -1274,6 +1281,16 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     throw std::runtime_error( "BadProperty: Connection probability 0 <= p <= 1 required." );
   if ( rule != "one_to_one" and rule != "all_to_all" and rule != "fixed_indegree" and rule != "pairwise_bernoulli" )
     throw std::runtime_error( "BadProperty: unknown or unsupported connection rule '" + rule + "'" );
+  // a key the rule does not read is an error (conn_spec.all_entries_accessed,
+  // connection_manager.cpp:470); the rule's own key must be there (conn_builder.cpp:1403,1861)
+  if ( has_indegree and rule != "fixed_indegree" )
+    throw std::runtime_error( "UnaccessedDictionaryEntry: conn_spec key 'indegree'" );
+  if ( has_p and rule != "pairwise_bernoulli" )
+    throw std::runtime_error( "UnaccessedDictionaryEntry: conn_spec key 'p'" );
+  if ( rule == "fixed_indegree" and not has_indegree )
+    throw std::runtime_error( "BadProperty: rule fixed_indegree needs 'indegree'" );
+  if ( rule == "pairwise_bernoulli" and not has_p )
+    throw std::runtime_error( "BadProperty: rule pairwise_bernoulli needs 'p'" );
   // array parameters hold one value per connection in PyNEST's layout: one_to_one [pair],
   // all_to_all [target][source], fixed_indegree [target][draw]
   // (pynest/nest/lib/hl_api_connection_helpers.py:48-130)

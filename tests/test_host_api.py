@@ -316,6 +316,11 @@ def test_error_paths(bad, msg):
     ("recorder_source", "IllegalConnection"),
     # iaf_psc_exp.h:466-474: receptor 0 only
     ("receptor_type", "UnknownReceptorType: Receptor type 1 is not available in iaf_psc_exp"),
+    # connection_manager.cpp:470 (unread keys), conn_builder.cpp:1403,1861 (required keys)
+    ("indegree_unread", "UnaccessedDictionaryEntry"),
+    ("p_unread", "UnaccessedDictionaryEntry"),
+    ("indegree_missing", "needs 'indegree'"),
+    ("p_missing", "needs 'p'"),
 ])
 def test_connect_error_paths(case, msg):
     k = nestgpu.Kernel(0)
@@ -337,6 +342,10 @@ def test_connect_error_paths(case, msg):
         "generator_target": lambda: k.Connect(a, sg, "one_to_one"),
         "recorder_source": lambda: k.Connect(sr, a),
         "receptor_type": lambda: k.Connect(a, a, "all_to_all", {"receptor_type": 1}),
+        "indegree_unread": lambda: k.Connect(a, a, {"rule": "all_to_all", "indegree": 2}),
+        "p_unread": lambda: k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 2, "p": 0.5}),
+        "indegree_missing": lambda: k.Connect(a, a, {"rule": "fixed_indegree"}),
+        "p_missing": lambda: k.Connect(a, a, {"rule": "pairwise_bernoulli"}),
     }
     with pytest.raises(nestgpu.NESTError, match=msg):
         calls[case]()
