@@ -127,6 +127,26 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     for ( int i = 0; i < n_vp; ++i )
       p->normals_.emplace_back( p->a_, p->b_ );
   }
+  else if ( k == "lognormal" )
+  {
+    // LognormalParameter (parameter.cpp:120-157): std::lognormal_distribution, i.e.
+    // exp(std * z + mean) with z from its own standard normal draw
+    p->op_ = LOGNORMAL;
+    p->a_ = v.has( "mean" ) ? v.at( "mean" ).number() : 0.0;
+    p->b_ = v.has( "std" ) ? v.at( "std" ).number() : 1.0;
+    if ( p->b_ < 0 )
+      throw std::runtime_error( "BadProperty: nest::LognormalParameter: std ≥ 0 required." );
+    for ( int i = 0; i < n_vp; ++i )
+      p->normals_.emplace_back( 0.0, 1.0 );
+  }
+  else if ( k == "exponential" )
+  {
+    // ExponentialParameter (parameter.h:313-346)
+    p->op_ = EXPONENTIAL;
+    p->a_ = v.has( "beta" ) ? v.at( "beta" ).number() : 1.0;
+    if ( p->a_ < 0 )
+      throw std::runtime_error( "BadProperty: nest::ExponentialParameter: beta ≥ 0 required." );
+  }
   else if ( k == "sum" or k == "product" )
   {
     if ( v.kind != Json::ARR or v.arr.size() != 2 )
@@ -186,6 +206,12 @@ Param::value( VpRng& rng, int vp )
     return std::max( l_->value( rng, vp ), a_ );
   case MIN:
     return std::min( l_->value( rng, vp ), a_ );
+  case LOGNORMAL:
+    if ( b_ == 0 )
+      return std::exp( a_ );
+    return std::exp( b_ * normals_[ vp ]( rng ) + a_ );
+  case EXPONENTIAL:
+    return a_ * ( -std::log( 1 - rng.drand() ) );
   case ARRAY:
   {
     size_t& i = cursor_[ vp ].next;

@@ -136,7 +136,9 @@ public:
     MAX,
     MIN,
     REDRAW,
-    ARRAY
+    ARRAY,
+    LOGNORMAL,
+    EXPONENTIAL
   };
   // top_level: a JSON array is an array parameter (one value per connection, a syn_spec
   // value of its own, not an operand)

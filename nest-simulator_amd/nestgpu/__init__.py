@@ -137,6 +137,14 @@ class _Random:
     def normal(mean=0.0, std=1.0):
         return Parameter({"normal": {"mean": float(mean), "std": float(std)}})
 
+    @staticmethod
+    def lognormal(mean=0.0, std=1.0):
+        return Parameter({"lognormal": {"mean": float(mean), "std": float(std)}})
+
+    @staticmethod
+    def exponential(beta=1.0):
+        return Parameter({"exponential": {"beta": float(beta)}})
+
 
 class _Math:
     @staticmethod

@@ -198,6 +198,7 @@ struct Rng
 // Prefix-encoded expression tree over doubles; op codes:
 //   0 const v | 1 uniform min max | 2 uniform_int max | 3 normal mean std
 //   4 sum A B | 5 product A B     | 6 max A v          | 7 min A v | 8 redraw A min max
+//   9 lognormal mean std | 10 exponential beta
 // nestkernel/parameter.h:131-286 (leaves), 468-600 (product/sum), 878-990 (max/redraw)
 struct Param
 {
@@ -205,6 +206,7 @@ struct Param
   double a = 0, b = 0;
   std::unique_ptr< Param > l, r;
   std::vector< std::normal_distribution<> > normals;  // one per VP (parameter.cpp:95-117)
+  std::vector< std::lognormal_distribution<> > lognormals;  // one per VP (parameter.cpp:120-157)
 
   static std::unique_ptr< Param >
   parse( const double*& p, const double* end, int n_vp )
@@ -253,6 +255,20 @@ struct Param
       x->a = *p++;
       x->b = *p++;
       break;
+    case 9:
+      x->a = *p++;
+      x->b = *p++;
+      if ( x->b < 0 )
+        throw std::runtime_error( "nest::LognormalParameter: std >= 0 required." );
+      if ( x->b > 0 )
+        for ( int i = 0; i < n_vp; ++i )
+          x->lognormals.emplace_back( x->a, x->b );
+      break;
+    case 10:
+      x->a = *p++;
+      if ( x->a < 0 )
+        throw std::runtime_error( "nest::ExponentialParameter: beta >= 0 required." );
+      break;
     default:
       throw std::runtime_error( "unknown parameter op" );
     }
@@ -288,6 +304,14 @@ struct Param
       return std::max( l->value( rng, vp ), a );
     case 7:
       return std::min( l->value( rng, vp ), a );
+    case 9:
+      // LognormalParameter::value (parameter.cpp:147-157)
+      if ( b == 0 )
+        return std::exp( a );
+      return lognormals[ vp ]( rng.eng );
+    case 10:
+      // ExponentialParameter::value (parameter.h:339-342)
+      return a * ( -std::log( 1 - rng.drand() ) );
     case 8:
     {
       // nestkernel/parameter.cpp RedrawParameter::value: redraw until inside [min, max]

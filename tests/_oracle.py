@@ -75,6 +75,10 @@ def spec(x):
             out.extend([2, v[1]])
         elif op == "normal":
             out.extend([3, v[1], v[2]])
+        elif op == "lognormal":
+            out.extend([9, v[1], v[2]])
+        elif op == "exponential":
+            out.extend([10, v[1]])
         elif op in ("sum", "product"):
             out.append(4 if op == "sum" else 5)
             enc(v[1])

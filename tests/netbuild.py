@@ -26,6 +26,10 @@ def to_param(x):
         return nestgpu.Parameter({"uniform_int": {"max": x[1]}})
     if op == "normal":
         return nestgpu.Parameter({"normal": {"mean": x[1], "std": x[2]}})
+    if op == "lognormal":
+        return nestgpu.Parameter({"lognormal": {"mean": x[1], "std": x[2]}})
+    if op == "exponential":
+        return nestgpu.Parameter({"exponential": {"beta": x[1]}})
     if op in ("sum", "product"):
         return nestgpu.Parameter({op: [nestgpu.Parameter._wrap(to_param(x[1])), nestgpu.Parameter._wrap(to_param(x[2]))]})
     if op in ("max", "min"):

@@ -48,6 +48,9 @@ CASES = [
     (6, 123, 40, 30, "fixed_indegree", 25, ("min", ("normal", 0.0, 2.0), 1.0), ("uniform", 0.1, 3.0), True, False, False),
     # many normal and uniform draws per VP stream: the canonical and polar-method draws
     (2, 31, 400, 300, "fixed_indegree", 50, ("normal", 0.0, 1.0), ("uniform", 0.1, 5.0), True, True, False),
+    # lognormal weights, exponential delays
+    (3, 41, 120, 90, "fixed_indegree", 20, ("lognormal", 0.5, 0.3), ("sum", 0.5, ("exponential", 1.5)), True, True,
+     False),
 ]
 
 

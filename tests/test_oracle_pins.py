@@ -290,3 +290,39 @@ def test_delay_rounding_and_min_delay():
     o.simulate(1.0)
     st = o.status()
     assert st["min_delay"] == 1 and st["max_delay"] == 3
+
+
+def _weights(weight, n_vp=2, seed=17, N=200, K=50):
+    o = Oracle(n_vp=n_vp, seed=seed)
+    g = o.create("iaf_psc_delta", N)
+    o.connect(g, g, "fixed_indegree", K, weight=weight, delay=1.0)
+    s, t, w, d = o.connections()
+    return s, t, w
+
+
+def test_exponential_parameter_maps_one_uniform_draw():
+    # ExponentialParameter::value (parameter.h:339-342): beta * -log(1 - u), one uniform draw
+    # u per value: the same stream drawn as uniform(0, 1) gives the same sources and u
+    import math
+    beta = 2.5
+    s1, t1, u = _weights(("uniform", 0.0, 1.0))
+    s2, t2, w = _weights(("exponential", beta))
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+    expected = np.array([beta * -math.log(1 - x) for x in u])
+    np.testing.assert_allclose(w, expected, rtol=1e-15, atol=0)
+    assert abs(w.mean() / beta - 1) < 0.02  # 20,000 draws: mean beta within 4 sigma
+
+
+def test_lognormal_parameter_is_exp_of_a_normal_draw():
+    # std::lognormal_distribution: exp(s * z + m), z from its own N(0, 1) polar-method draws,
+    # the same consumption as a normal(0, 1) parameter on the same stream
+    import math
+    m, sd = 0.3, 0.4
+    s1, t1, z = _weights(("normal", 0.0, 1.0))
+    s2, t2, w = _weights(("lognormal", m, sd))
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+    np.testing.assert_allclose(w, [math.exp(sd * x + m) for x in z], rtol=1e-15, atol=0)
+    assert abs(w.mean() / math.exp(m + sd * sd / 2) - 1) < 0.01
+    # std = 0 draws nothing and gives exp(mean) (parameter.cpp:149-152)
+    s3, t3, w3 = _weights(("lognormal", m, 0.0))
+    assert np.all(w3 == math.exp(m))
