@@ -147,11 +147,27 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( p->a_ < 0 )
       throw std::runtime_error( "BadProperty: nest::ExponentialParameter: beta ≥ 0 required." );
   }
-  else if ( k == "sum" or k == "product" )
+  else if ( k == "exp" or k == "sin" or k == "cos" )
   {
+    // ExpParameter, SinParameter, CosParameter (parameter.h:991,1037,1082)
+    p->op_ = k == "exp" ? EXP : k == "sin" ? SIN : COS;
+    p->l_ = from_json( v, n_vp );
+  }
+  else if ( k == "pow" )
+  {
+    // PowParameter (parameter.h:1130): a parameter to a fixed exponent
     if ( v.kind != Json::ARR or v.arr.size() != 2 )
-      throw std::runtime_error( "BadProperty: sum/product take two operands" );
-    p->op_ = k == "sum" ? SUM : PRODUCT;
+      throw std::runtime_error( "BadProperty: pow takes a parameter and an exponent" );
+    p->op_ = POW;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->a_ = v.arr[ 1 ].number();
+  }
+  else if ( k == "sum" or k == "product" or k == "quotient" )
+  {
+    // SumParameter, ProductParameter, QuotientParameter (parameter.h:496,546 and the like)
+    if ( v.kind != Json::ARR or v.arr.size() != 2 )
+      throw std::runtime_error( "BadProperty: sum/product/quotient take two operands" );
+    p->op_ = k == "sum" ? SUM : k == "product" ? PRODUCT : QUOTIENT;
     p->l_ = from_json( v.arr[ 0 ], n_vp );
     p->r_ = from_json( v.arr[ 1 ], n_vp );
   }
@@ -206,6 +222,19 @@ Param::value( VpRng& rng, int vp )
     return std::max( l_->value( rng, vp ), a_ );
   case MIN:
     return std::min( l_->value( rng, vp ), a_ );
+  case EXP:
+    return std::exp( l_->value( rng, vp ) );
+  case SIN:
+    return std::sin( l_->value( rng, vp ) );
+  case COS:
+    return std::cos( l_->value( rng, vp ) );
+  case QUOTIENT:
+  {
+    const double x = l_->value( rng, vp );
+    return x / r_->value( rng, vp );
+  }
+  case POW:
+    return std::pow( l_->value( rng, vp ), a_ );
   case LOGNORMAL:
     if ( b_ == 0 )
       return std::exp( a_ );

@@ -138,7 +138,12 @@ public:
     REDRAW,
     ARRAY,
     LOGNORMAL,
-    EXPONENTIAL
+    EXPONENTIAL,
+    EXP,
+    SIN,
+    COS,
+    QUOTIENT,
+    POW
   };
   // top_level: a JSON array is an array parameter (one value per connection, a syn_spec
   // value of its own, not an operand)

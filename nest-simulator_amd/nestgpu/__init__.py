@@ -123,6 +123,15 @@ class Parameter:
     def __neg__(self):
         return self * -1.0
 
+    def __truediv__(self, other):
+        return Parameter({"quotient": [self.spec, Parameter._wrap(other)]})
+
+    def __rtruediv__(self, lhs):
+        return self**-1 * lhs  # as hl_api_types.py Parameter.__rtruediv__
+
+    def __pow__(self, exponent):
+        return Parameter({"pow": [self.spec, float(exponent)]})
+
 
 class _Random:
     @staticmethod
@@ -147,6 +156,18 @@ class _Random:
 
 
 class _Math:
+    @staticmethod
+    def exp(p):
+        return Parameter({"exp": Parameter._wrap(p)})
+
+    @staticmethod
+    def sin(p):
+        return Parameter({"sin": Parameter._wrap(p)})
+
+    @staticmethod
+    def cos(p):
+        return Parameter({"cos": Parameter._wrap(p)})
+
     @staticmethod
     def max(p, value):
         return Parameter({"max": [Parameter._wrap(p), float(value)]})

@@ -199,6 +199,7 @@ struct Rng
 //   0 const v | 1 uniform min max | 2 uniform_int max | 3 normal mean std
 //   4 sum A B | 5 product A B     | 6 max A v          | 7 min A v | 8 redraw A min max
 //   9 lognormal mean std | 10 exponential beta
+//   11 exp A | 12 sin A | 13 cos A | 14 quotient A B | 15 pow A v
 // nestkernel/parameter.h:131-286 (leaves), 468-600 (product/sum), 878-990 (max/redraw)
 struct Param
 {
@@ -269,6 +270,19 @@ struct Param
       if ( x->a < 0 )
         throw std::runtime_error( "nest::ExponentialParameter: beta >= 0 required." );
       break;
+    case 11:
+    case 12:
+    case 13:
+      x->l = parse( p, end, n_vp );
+      break;
+    case 14:
+      x->l = parse( p, end, n_vp );
+      x->r = parse( p, end, n_vp );
+      break;
+    case 15:
+      x->l = parse( p, end, n_vp );
+      x->a = *p++;
+      break;
     default:
       throw std::runtime_error( "unknown parameter op" );
     }
@@ -312,6 +326,20 @@ struct Param
     case 10:
       // ExponentialParameter::value (parameter.h:339-342)
       return a * ( -std::log( 1 - rng.drand() ) );
+    case 11:
+      return std::exp( l->value( rng, vp ) );  // ExpParameter (parameter.h:991)
+    case 12:
+      return std::sin( l->value( rng, vp ) );  // SinParameter (parameter.h:1037)
+    case 13:
+      return std::cos( l->value( rng, vp ) );  // CosParameter (parameter.h:1082)
+    case 14:
+    {
+      // QuotientParameter (parameter.h:546)
+      const double x = l->value( rng, vp );
+      return x / r->value( rng, vp );
+    }
+    case 15:
+      return std::pow( l->value( rng, vp ), a );  // PowParameter (parameter.h:1130)
     case 8:
     {
       // nestkernel/parameter.cpp RedrawParameter::value: redraw until inside [min, max]

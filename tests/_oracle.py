@@ -79,6 +79,17 @@ def spec(x):
             out.extend([9, v[1], v[2]])
         elif op == "exponential":
             out.extend([10, v[1]])
+        elif op in ("exp", "sin", "cos"):
+            out.append({"exp": 11, "sin": 12, "cos": 13}[op])
+            enc(v[1])
+        elif op == "quotient":
+            out.append(14)
+            enc(v[1])
+            enc(v[2])
+        elif op == "pow":
+            out.append(15)
+            enc(v[1])
+            out.append(v[2])
         elif op in ("sum", "product"):
             out.append(4 if op == "sum" else 5)
             enc(v[1])

@@ -30,6 +30,13 @@ def to_param(x):
         return nestgpu.Parameter({"lognormal": {"mean": x[1], "std": x[2]}})
     if op == "exponential":
         return nestgpu.Parameter({"exponential": {"beta": x[1]}})
+    if op in ("exp", "sin", "cos"):
+        return nestgpu.Parameter({op: nestgpu.Parameter._wrap(to_param(x[1]))})
+    if op == "quotient":
+        return nestgpu.Parameter({"quotient": [nestgpu.Parameter._wrap(to_param(x[1])),
+                                               nestgpu.Parameter._wrap(to_param(x[2]))]})
+    if op == "pow":
+        return nestgpu.Parameter({"pow": [nestgpu.Parameter._wrap(to_param(x[1])), float(x[2])]})
     if op in ("sum", "product"):
         return nestgpu.Parameter({op: [nestgpu.Parameter._wrap(to_param(x[1])), nestgpu.Parameter._wrap(to_param(x[2]))]})
     if op in ("max", "min"):

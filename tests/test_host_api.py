@@ -51,6 +51,9 @@ CASES = [
     # lognormal weights, exponential delays
     (3, 41, 120, 90, "fixed_indegree", 20, ("lognormal", 0.5, 0.3), ("sum", 0.5, ("exponential", 1.5)), True, True,
      False),
+    # nest.math and parameter arithmetic: exp, sin, cos, quotient, pow
+    (2, 43, 80, 70, "fixed_indegree", 15, ("quotient", ("exp", ("normal", 0.0, 0.5)), ("sum", 2.0, ("sin", ("uniform", 0.0, 3.0)))),
+     ("sum", 0.5, ("pow", ("cos", ("uniform", 0.0, 1.0)), 2.0)), True, True, False),
 ]
 
 
@@ -254,6 +257,20 @@ def test_get_connections_filters():
     assert np.array_equal(k.GetConnections(target=a[3])["target"], c["target"][c["target"] == a.tolist()[3]])
     assert len(k.GetConnections(synapse_model="static_synapse")["source"]) == len(c["source"])
     assert len(k.GetConnections(synapse_model="stdp_synapse")["source"]) == 0
+
+
+def test_parameter_arithmetic_expressions():
+    # hl_api_types.py Parameter: / and ** build quotient and pow; lhs / p is p**-1 * lhs
+    p = nestgpu.random.uniform(1.0, 2.0)
+    assert (p / 2.0).spec == {"quotient": [p.spec, 2.0]}
+    assert (p ** 3).spec == {"pow": [p.spec, 3.0]}
+    assert (2.0 / p).spec == {"product": [{"pow": [p.spec, -1.0]}, 2.0]}
+    assert nestgpu.math.exp(p).spec == {"exp": p.spec}
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 4)
+    k.Connect(a, a, "all_to_all", {"weight": nestgpu.math.cos(p) / 2.0, "delay": 1.0 + nestgpu.math.sin(p) ** 2})
+    c = k.GetConnections()
+    assert np.all(np.abs(c["weight"]) <= 0.5) and np.all(c["delay_steps"] >= 10)
 
 
 def test_get_set_status_roundtrip_and_relative_voltages():

@@ -326,3 +326,18 @@ def test_lognormal_parameter_is_exp_of_a_normal_draw():
     # std = 0 draws nothing and gives exp(mean) (parameter.cpp:149-152)
     s3, t3, w3 = _weights(("lognormal", m, 0.0))
     assert np.all(w3 == math.exp(m))
+
+
+@pytest.mark.parametrize("op, f", [
+    ("exp", lambda z: __import__("math").exp(z)), ("sin", lambda z: __import__("math").sin(z)),
+    ("cos", lambda z: __import__("math").cos(z)), ("pow", lambda z: z ** 3.0), ("quotient", lambda z: z / 4.0),
+])
+def test_composite_parameters_apply_to_the_operand_draws(op, f):
+    # ExpParameter, SinParameter, CosParameter, PowParameter, QuotientParameter
+    # (parameter.h:546,991,1037,1082,1130): the function of the operand's value, no extra draw
+    s1, t1, z = _weights(("normal", 0.0, 1.0))
+    spec = {"pow": ("pow", ("normal", 0.0, 1.0), 3.0), "quotient": ("quotient", ("normal", 0.0, 1.0), 4.0)}.get(
+        op, (op, ("normal", 0.0, 1.0)))
+    s2, t2, w = _weights(spec)
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+    np.testing.assert_allclose(w, [f(x) for x in z], rtol=1e-15, atol=0)
