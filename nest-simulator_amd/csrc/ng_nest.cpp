@@ -153,6 +153,28 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     p->op_ = k == "exp" ? EXP : k == "sin" ? SIN : COS;
     p->l_ = from_json( v, n_vp );
   }
+  else if ( k == "compare" )
+  {
+    // ComparingParameter (parameter.h:680-752): 1 or 0 for a < b, <=, ==, !=, >=, > (0..5)
+    if ( v.kind != Json::ARR or v.arr.size() != 3 )
+      throw std::runtime_error( "BadProperty: compare takes two parameters and a comparator" );
+    p->op_ = COMPARE;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->r_ = from_json( v.arr[ 1 ], n_vp );
+    p->a_ = v.arr[ 2 ].number();
+    if ( p->a_ < 0 or p->a_ > 5 or p->a_ != std::floor( p->a_ ) )
+      throw std::runtime_error( "BadProperty: Comparator specification has to be in the range 0-5." );
+  }
+  else if ( k == "conditional" )
+  {
+    // ConditionalParameter (parameter.h:760-820): only the chosen branch is evaluated
+    if ( v.kind != Json::ARR or v.arr.size() != 3 )
+      throw std::runtime_error( "BadProperty: conditional takes a condition and two parameters" );
+    p->op_ = CONDITIONAL;
+    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->r_ = from_json( v.arr[ 1 ], n_vp );
+    p->f_ = from_json( v.arr[ 2 ], n_vp );
+  }
   else if ( k == "pow" )
   {
     // PowParameter (parameter.h:1130): a parameter to a fixed exponent
@@ -235,6 +257,28 @@ Param::value( VpRng& rng, int vp )
   }
   case POW:
     return std::pow( l_->value( rng, vp ), a_ );
+  case COMPARE:
+  {
+    const double x = l_->value( rng, vp );
+    const double y = r_->value( rng, vp );
+    switch ( static_cast< int >( a_ ) )
+    {
+    case 0:
+      return x < y;
+    case 1:
+      return x <= y;
+    case 2:
+      return x == y;
+    case 3:
+      return x != y;
+    case 4:
+      return x >= y;
+    default:
+      return x > y;
+    }
+  }
+  case CONDITIONAL:
+    return l_->value( rng, vp ) != 0 ? r_->value( rng, vp ) : f_->value( rng, vp );
   case LOGNORMAL:
     if ( b_ == 0 )
       return std::exp( a_ );

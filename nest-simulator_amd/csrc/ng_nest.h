@@ -143,7 +143,9 @@ public:
     SIN,
     COS,
     QUOTIENT,
-    POW
+    POW,
+    COMPARE,
+    CONDITIONAL
   };
   // top_level: a JSON array is an array parameter (one value per connection, a syn_spec
   // value of its own, not an operand)
@@ -176,7 +178,7 @@ public:
 private:
   Op op_ = CONST;
   double a_ = 0, b_ = 0;
-  std::unique_ptr< Param > l_, r_;
+  std::unique_ptr< Param > l_, r_, f_;  // f_: a conditional's value when false
   // std::normal_distribution<double> (libstdc++ bits/random.tcc): Marsaglia's polar method
   // over pairs of canonical draws, the second value of a pair kept for the next call
   struct alignas( 64 ) VpNormal

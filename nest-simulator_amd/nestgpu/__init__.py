@@ -132,6 +132,30 @@ class Parameter:
     def __pow__(self, exponent):
         return Parameter({"pow": [self.spec, float(exponent)]})
 
+    # comparisons give 1 or 0 (hl_api_types.py Parameter.__lt__ ..., comparator 0..5)
+    def _compare(self, other, comparator):
+        return Parameter({"compare": [self.spec, Parameter._wrap(other), comparator]})
+
+    def __lt__(self, other):
+        return self._compare(other, 0)
+
+    def __le__(self, other):
+        return self._compare(other, 1)
+
+    def __eq__(self, other):
+        return self._compare(other, 2)
+
+    def __ne__(self, other):
+        return self._compare(other, 3)
+
+    def __ge__(self, other):
+        return self._compare(other, 4)
+
+    def __gt__(self, other):
+        return self._compare(other, 5)
+
+    __hash__ = object.__hash__
+
 
 class _Random:
     @staticmethod
@@ -181,8 +205,17 @@ class _Math:
         return Parameter({"redraw": [Parameter._wrap(p), float(min), float(max)]})
 
 
+class _Logic:
+    @staticmethod
+    def conditional(condition, param_if_true, param_if_false):
+        """hl_api_logic.py conditional: the first parameter where the condition is nonzero."""
+        return Parameter({"conditional": [Parameter._wrap(condition), Parameter._wrap(param_if_true),
+                                          Parameter._wrap(param_if_false)]})
+
+
 random = _Random()
 math = _Math()
+logic = _Logic()
 
 
 def _to_json(d):

@@ -200,12 +200,13 @@ struct Rng
 //   4 sum A B | 5 product A B     | 6 max A v          | 7 min A v | 8 redraw A min max
 //   9 lognormal mean std | 10 exponential beta
 //   11 exp A | 12 sin A | 13 cos A | 14 quotient A B | 15 pow A v
+//   16 compare A B comparator | 17 conditional C T F
 // nestkernel/parameter.h:131-286 (leaves), 468-600 (product/sum), 878-990 (max/redraw)
 struct Param
 {
   int op = 0;
   double a = 0, b = 0;
-  std::unique_ptr< Param > l, r;
+  std::unique_ptr< Param > l, r, f;
   std::vector< std::normal_distribution<> > normals;  // one per VP (parameter.cpp:95-117)
   std::vector< std::lognormal_distribution<> > lognormals;  // one per VP (parameter.cpp:120-157)
 
@@ -283,6 +284,16 @@ struct Param
       x->l = parse( p, end, n_vp );
       x->a = *p++;
       break;
+    case 16:
+      x->l = parse( p, end, n_vp );
+      x->r = parse( p, end, n_vp );
+      x->a = *p++;
+      break;
+    case 17:
+      x->l = parse( p, end, n_vp );
+      x->r = parse( p, end, n_vp );
+      x->f = parse( p, end, n_vp );
+      break;
     default:
       throw std::runtime_error( "unknown parameter op" );
     }
@@ -340,6 +351,19 @@ struct Param
     }
     case 15:
       return std::pow( l->value( rng, vp ), a );  // PowParameter (parameter.h:1130)
+    case 16:
+    {
+      // ComparingParameter::value (parameter.h:714, 734-752)
+      const double x = l->value( rng, vp );
+      const double y = r->value( rng, vp );
+      const int c = static_cast< int >( a );
+      return c == 0 ? x < y : c == 1 ? x <= y : c == 2 ? x == y : c == 3 ? x != y : c == 4 ? x >= y : x > y;
+    }
+    case 17:
+      // ConditionalParameter::value (parameter.h:791-797): the condition, then one branch
+      if ( l->value( rng, vp ) )
+        return r->value( rng, vp );
+      return f->value( rng, vp );
     case 8:
     {
       // nestkernel/parameter.cpp RedrawParameter::value: redraw until inside [min, max]

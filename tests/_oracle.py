@@ -90,6 +90,16 @@ def spec(x):
             out.append(15)
             enc(v[1])
             out.append(v[2])
+        elif op == "compare":
+            out.append(16)
+            enc(v[1])
+            enc(v[2])
+            out.append(v[3])
+        elif op == "conditional":
+            out.append(17)
+            enc(v[1])
+            enc(v[2])
+            enc(v[3])
         elif op in ("sum", "product"):
             out.append(4 if op == "sum" else 5)
             enc(v[1])

@@ -37,6 +37,11 @@ def to_param(x):
                                                nestgpu.Parameter._wrap(to_param(x[2]))]})
     if op == "pow":
         return nestgpu.Parameter({"pow": [nestgpu.Parameter._wrap(to_param(x[1])), float(x[2])]})
+    if op == "compare":
+        return nestgpu.Parameter({"compare": [nestgpu.Parameter._wrap(to_param(x[1])),
+                                              nestgpu.Parameter._wrap(to_param(x[2])), int(x[3])]})
+    if op == "conditional":
+        return nestgpu.logic.conditional(to_param(x[1]), to_param(x[2]), to_param(x[3]))
     if op in ("sum", "product"):
         return nestgpu.Parameter({op: [nestgpu.Parameter._wrap(to_param(x[1])), nestgpu.Parameter._wrap(to_param(x[2]))]})
     if op in ("max", "min"):

@@ -54,6 +54,10 @@ CASES = [
     # nest.math and parameter arithmetic: exp, sin, cos, quotient, pow
     (2, 43, 80, 70, "fixed_indegree", 15, ("quotient", ("exp", ("normal", 0.0, 0.5)), ("sum", 2.0, ("sin", ("uniform", 0.0, 3.0)))),
      ("sum", 0.5, ("pow", ("cos", ("uniform", 0.0, 1.0)), 2.0)), True, True, False),
+    # nest.logic.conditional on a comparison: only the chosen branch draws
+    (3, 47, 60, 50, "fixed_indegree", 12,
+     ("conditional", ("compare", ("uniform", 0.0, 1.0), 0.3, 0), ("normal", 2.0, 0.5), ("exponential", 0.5)),
+     ("sum", 1.0, ("compare", ("uniform", 0.0, 1.0), 0.5, 5)), True, True, False),
 ]
 
 
@@ -271,6 +275,21 @@ def test_parameter_arithmetic_expressions():
     k.Connect(a, a, "all_to_all", {"weight": nestgpu.math.cos(p) / 2.0, "delay": 1.0 + nestgpu.math.sin(p) ** 2})
     c = k.GetConnections()
     assert np.all(np.abs(c["weight"]) <= 0.5) and np.all(c["delay_steps"] >= 10)
+
+
+def test_comparison_and_conditional_expressions():
+    # hl_api_types.py comparisons (comparator 0..5) and hl_api_logic.py conditional
+    p = nestgpu.random.uniform(0.0, 1.0)
+    assert [(p < 0.5).spec["compare"][2], (p <= 0.5).spec["compare"][2], (p == 0.5).spec["compare"][2],
+            (p != 0.5).spec["compare"][2], (p >= 0.5).spec["compare"][2], (p > 0.5).spec["compare"][2]] == [0, 1, 2, 3, 4, 5]
+    c = nestgpu.logic.conditional(p < 0.5, 1.0, -1.0)
+    assert c.spec == {"conditional": [(p < 0.5).spec, 1.0, -1.0]}
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 5)
+    k.Connect(a, a, "all_to_all", {"weight": c})
+    assert set(np.unique(k.GetConnections()["weight"])) <= {1.0, -1.0}
+    with pytest.raises(nestgpu.NESTError, match="range 0-5"):
+        k.Connect(a, a, "all_to_all", {"weight": nestgpu.Parameter({"compare": [p.spec, 0.5, 7]})})
 
 
 def test_get_set_status_roundtrip_and_relative_voltages():

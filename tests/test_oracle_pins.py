@@ -341,3 +341,22 @@ def test_composite_parameters_apply_to_the_operand_draws(op, f):
     s2, t2, w = _weights(spec)
     assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
     np.testing.assert_allclose(w, [f(x) for x in z], rtol=1e-15, atol=0)
+
+
+def test_comparison_gives_one_or_zero_of_the_operand_draws():
+    # ComparingParameter (parameter.h:714, 734-752): 1.0 where u < 0.4 for the same stream's u
+    s1, t1, u = _weights(("uniform", 0.0, 1.0))
+    for c, f in enumerate([np.less, np.less_equal, np.equal, np.not_equal, np.greater_equal, np.greater]):
+        s2, t2, w = _weights(("compare", ("uniform", 0.0, 1.0), 0.4, c))
+        assert np.array_equal(s1, s2) and np.array_equal(w, f(u, 0.4).astype(float))
+
+
+def test_conditional_draws_only_the_chosen_branch():
+    # ConditionalParameter::value (parameter.h:791-797): a branch not taken draws nothing, so a
+    # constant condition gives exactly the chosen branch's stream
+    s_n, t_n, w_n = _weights(("normal", 1.0, 0.5))
+    s_e, t_e, w_e = _weights(("exponential", 2.0))
+    s_t, t_t, w_t = _weights(("conditional", ("compare", 1.0, 0.0, 5), ("normal", 1.0, 0.5), ("exponential", 2.0)))
+    s_f, t_f, w_f = _weights(("conditional", ("compare", 1.0, 0.0, 0), ("normal", 1.0, 0.5), ("exponential", 2.0)))
+    assert np.array_equal(s_t, s_n) and np.array_equal(w_t, w_n)
+    assert np.array_equal(s_f, s_e) and np.array_equal(w_f, w_e)
