@@ -134,6 +134,9 @@ aeif_evolve( const AeifPar& P, bool refr, double* t, double t1, double* h, doubl
   for ( int i = 0; i < 4; ++i )
     y0[ i ] = y[ i ];
   aeif_dynamics( P, refr, y, dydt_in );
+  // A rejected attempt shrinks h0 by a factor of at most 0.9 * 1.1^(-1/5) < 0.89 and stops
+  // once t + h0 rounds to t, so fewer than 400 attempts can occur; the bound only makes the
+  // loop's exit evident.
   for ( int attempt = 0; attempt < 10000; ++attempt )
   {
     bool final_step = false;

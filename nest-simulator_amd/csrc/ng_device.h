@@ -162,7 +162,13 @@ enum ErrorFlag : unsigned long long
   ERR_AEIF_INSTABILITY = 1,
   ERR_AEIF_DOUBLE_SPIKE = 2,
   ERR_SPIKE_LIST = 4,
-  ERR_LOG = 8
+  ERR_LOG = 8,
+  ERR_AEIF_STEPS = 16
 };
+
+// accepted adaptive steps of one aeif_cond_exp neuron within one time step before the run
+// fails (the GSL evolve loop of the reference has no such limit; a neuron that needs this
+// many steps has a step size below 1e-7 of the resolution)
+constexpr int AEIF_MAX_STEPS = 1000000;
 
 }  // namespace nestgpu

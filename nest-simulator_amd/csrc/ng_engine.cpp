@@ -284,8 +284,9 @@ Runner::ensure_capacity( size_t n )
   {
     hipGraphExecDestroy( exec_ );
     exec_ = nullptr;
-    exec_n_ = -1;
   }
+  exec_n_ = -1;
+  exec_failed_ = false;
   if ( d_nets_ )
     hipFree( d_nets_ );
   if ( d_prefix_ )
@@ -309,9 +310,10 @@ Runner::ensure_capacity( size_t n )
 bool
 Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
 {
-  if ( exec_ and exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win
-       and exec_wg_ == wg_per_cu )
-    return true;
+  const bool same_key = exec_n_ == n_nets and exec_total_ == total_blocks and exec_mask_ == mask and exec_win_ == win
+    and exec_wg_ == wg_per_cu;
+  if ( same_key and ( exec_ or exec_failed_ ) )
+    return exec_ != nullptr;
   if ( exec_ )
   {
     hipGraphExecDestroy( exec_ );
@@ -319,11 +321,19 @@ Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
   }
   // A capture fails when another host thread makes a call HIP does not allow during it (an
   // allocation or a synchronous copy, e.g. while other networks are prepared): the capture is
-  // then dropped, and the caller launches the slices directly from then on.
+  // then dropped and the caller launches this launch shape's slices directly; a new launch
+  // shape is captured again. Failures are counted (graph_capture_failures in kernel status).
+  exec_n_ = n_nets;
+  exec_total_ = total_blocks;
+  exec_mask_ = mask;
+  exec_win_ = win;
+  exec_wg_ = wg_per_cu;
+  exec_failed_ = true;
   hipGraph_t g = nullptr;
   if ( hipStreamBeginCapture( stream_, hipStreamCaptureModeThreadLocal ) != hipSuccess )
   {
     (void)hipGetLastError();
+    ++capture_failures;
     return false;
   }
   for ( int s = 0; s < S_; ++s )
@@ -336,15 +346,11 @@ Runner::capture( int n_nets, uint32_t total_blocks, int mask, bool win )
     if ( g )
       hipGraphDestroy( g );
     exec_ = nullptr;
-    exec_n_ = -1;
+    ++capture_failures;
     return false;
   }
   hip_check( hipGraphDestroy( g ), "hipGraphDestroy" );
-  exec_n_ = n_nets;
-  exec_total_ = total_blocks;
-  exec_mask_ = mask;
-  exec_win_ = win;
-  exec_wg_ = wg_per_cu;
+  exec_failed_ = false;
   return true;
 }
 
@@ -423,11 +429,11 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
   }
   if ( not any )
     return false;
-  if ( use_graph and not time_kernels and not split and not capture( n, total, mask, win ) )
-    use_graph = false;
+  const bool graph = use_graph and not time_kernels and not split and capture( n, total, mask, win );
+  ++( graph ? graph_chunks : direct_chunks );
   hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
   hip_check( hipEventRecord( ev_chunk_[ 0 ], stream_ ), "hipEventRecord" );
-  if ( split or time_kernels or not use_graph )
+  if ( not graph )
   {
     for ( int s = 0; s < S_; ++s )
     {
@@ -473,6 +479,9 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
       std::string why;
       if ( flags & ERR_AEIF_INSTABILITY )
         why += " NumericalInstability in aeif_cond_exp;";
+      if ( flags & ERR_AEIF_STEPS )
+        why += " aeif_cond_exp needed more than " + std::to_string( AEIF_MAX_STEPS )
+          + " integration steps within one time step;";
       if ( flags & ERR_AEIF_DOUBLE_SPIKE )
         why += " aeif_cond_exp emitted two spikes within one step;";
       if ( flags & ERR_SPIKE_LIST )

@@ -212,7 +212,9 @@ public:
   {
     return stream_;
   }
-  bool use_graph = true;
+  bool use_graph = true;  // replay captured graphs where possible (false: direct launches)
+  // chunks run from a captured graph and by direct launches, and failed captures
+  int64_t graph_chunks = 0, direct_chunks = 0, capture_failures = 0;
   int wg_per_cu = 0;  // resident slice workgroups per CU (0: as many as fit)
   // per-kernel timing: one slice at a time with HIP events around every launch
   bool time_kernels = false;
@@ -235,6 +237,7 @@ private:
   hipEvent_t ev_kernel_[ 2 ] = { nullptr, nullptr };
   size_t cap_ = 0;
   hipGraphExec_t exec_ = nullptr;
+  bool exec_failed_ = false;  // the capture of the current launch shape failed
   int exec_n_ = -1;
   uint32_t exec_total_ = 0;
   int exec_mask_ = 0;

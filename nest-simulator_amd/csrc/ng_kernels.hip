@@ -935,12 +935,19 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       // Lanes advance through their own lags independently, so a wave costs its slowest
       // neuron's total number of steps, not the sum over lags of the slowest per lag. Each
       // neuron's sequence of operations is that of the lag loop with an inner step loop.
+      // More than AEIF_MAX_STEPS accepted steps within one time step stop the lane's lag and
+      // raise ERR_AEIF_STEPS, which fails the run (the oracle stops with the same limit).
       int lag = ph.from;
       double tt = 0.0;
       int guard = 0;
       while ( lag < ph.to )
       {
-        if ( tt < hstep and guard < 1000000 )
+        if ( tt < hstep and guard >= AEIF_MAX_STEPS )
+        {
+          g_or( &d.counters[ 0 ], ERR_AEIF_STEPS );
+          tt = hstep;
+        }
+        if ( tt < hstep )
         {
           ++guard;
           aeif_evolve( A, r > 0, &tt, hstep, &hint, y );

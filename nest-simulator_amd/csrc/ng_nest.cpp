@@ -259,8 +259,11 @@ Param::value( VpRng& rng, int vp )
     return std::pow( l_->value( rng, vp ), a_ );
   case COMPARE:
   {
-    const double x = l_->value( rng, vp );
+    // ComparingParameter::value (parameter.h:712-715) passes both operand values as the
+    // arguments of one call, compare_( p1->value(), p2->value() ). g++ on x86-64 evaluates
+    // call arguments right to left, so the reference draws the right operand first.
     const double y = r_->value( rng, vp );
+    const double x = l_->value( rng, vp );
     switch ( static_cast< int >( a_ ) )
     {
     case 0:
@@ -417,6 +420,8 @@ Kernel::reset()
   targets_per_block_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
+  graph_launch_ = true;
+  graph_chunks_ = direct_chunks_ = capture_failures_ = 0;
   window_slots_ = 0;
   delivery_passes_ = 0;
   keep_host_connections_ = true;
@@ -543,6 +548,8 @@ Kernel::set_kernel_status( const Json& d )
       if ( own_runner_ )
         own_runner_->time_kernels = time_kernels_;
     }
+    else if ( k == "graph_launch" )
+      graph_launch_ = kv.second.number() != 0;
     else if ( k == "rng_type" )
     {
       if ( kv.second.string() != "mt19937_64" )
@@ -593,7 +600,10 @@ Kernel::get_kernel_status_json() const
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
     << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )
     << ", \"window_slots\": " << ( engine_ and engine_->built() ? engine_->net().wn : window_slots_ )
-    << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" ) << ", \"kernel_ms\": {";
+    << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" )
+    << ", \"graph_launch\": " << ( graph_launch_ ? "true" : "false" ) << ", \"graph_chunks\": " << graph_chunks_
+    << ", \"direct_chunks\": " << direct_chunks_ << ", \"graph_capture_failures\": " << capture_failures_
+    << ", \"kernel_ms\": {";
   static const char* kn[] = { "slice", "flatten" };
   for ( int q = 0; q < N_KINDS; ++q )
     o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << time_kernel_ms_[ q ];
@@ -1917,6 +1927,8 @@ Kernel::run( double t_ms )
   }
   runner_->time_kernels = time_kernels_;
   runner_->wg_per_cu = wg_per_cu_;
+  if ( runner_ == own_runner_.get() )
+    runner_->use_graph = graph_launch_;
   double km0[ N_KINDS ];
   int64_t kl0[ N_KINDS ];
   for ( int q = 0; q < N_KINDS; ++q )
@@ -1924,6 +1936,7 @@ Kernel::run( double t_ms )
     km0[ q ] = runner_->kernel_ms[ q ];
     kl0[ q ] = runner_->kernel_launches[ q ];
   }
+  const int64_t gc0 = runner_->graph_chunks, dc0 = runner_->direct_chunks, cf0 = runner_->capture_failures;
   const auto t0 = std::chrono::steady_clock::now();
   runner_->run( { engine_.get() }, { steps } );
   const auto t1 = std::chrono::steady_clock::now();
@@ -1935,6 +1948,9 @@ Kernel::run( double t_ms )
     time_kernel_ms_[ q ] += runner_->kernel_ms[ q ] - km0[ q ];
     kernel_launches_[ q ] += runner_->kernel_launches[ q ] - kl0[ q ];
   }
+  graph_chunks_ += runner_->graph_chunks - gc0;
+  direct_chunks_ += runner_->direct_chunks - dc0;
+  capture_failures_ += runner_->capture_failures - cf0;
   collect_log();
 }
 

@@ -494,6 +494,8 @@ private:
   int window_slots_ = 0;  // cap on the ring slots staged in LDS (0: as many as fit)
   int delivery_passes_ = 0;  // global delivery in this many slot-range passes (0, 1: one)
   bool time_kernels_ = false;
+  bool graph_launch_ = true;  // replay chunks from captured hipGraphs (false: direct launches)
+  int64_t graph_chunks_ = 0, direct_chunks_ = 0, capture_failures_ = 0;
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
   int targets_per_block_ = 0;  // 0: chosen from the network size

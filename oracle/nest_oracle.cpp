@@ -353,9 +353,11 @@ struct Param
       return std::pow( l->value( rng, vp ), a );  // PowParameter (parameter.h:1130)
     case 16:
     {
-      // ComparingParameter::value (parameter.h:714, 734-752)
-      const double x = l->value( rng, vp );
+      // ComparingParameter::value (parameter.h:712-715, 734-752): compare_( p1->value(),
+      // p2->value() ) is one call, whose arguments g++ on x86-64 evaluates right to left, so
+      // the right operand draws first
       const double y = r->value( rng, vp );
+      const double x = l->value( rng, vp );
       const int c = static_cast< int >( a );
       return c == 0 ? x < y : c == 1 ? x <= y : c == 2 ? x == y : c == 3 ? x != y : c == 4 ? x >= y : x > y;
     }
@@ -1238,9 +1240,14 @@ struct Kernel
       else
       {
         // models/aeif_cond_exp.cpp:443-523
+        // the device path stops a neuron after 1,000,000 accepted steps within one time
+        // step and fails the run; the oracle stops at the same count
         double t = 0.0;
+        long n_steps = 0;
         while ( t < n.hstep )
         {
+          if ( ++n_steps > 1000000 )
+            throw std::runtime_error( "aeif_cond_exp needed more than 1000000 integration steps within one time step" );
           rkf45_evolve( n, &t, n.hstep, &n.integ, n.ay, mm );
           if ( n.ay[ 0 ] < -1e3 or n.ay[ 3 ] < -1e6 or n.ay[ 3 ] > 1e6 )
             throw std::runtime_error( "NumericalInstability: aeif_cond_exp" );

@@ -360,3 +360,295 @@ def test_conditional_draws_only_the_chosen_branch():
     s_f, t_f, w_f = _weights(("conditional", ("compare", 1.0, 0.0, 0), ("normal", 1.0, 0.5), ("exponential", 2.0)))
     assert np.array_equal(s_t, s_n) and np.array_equal(w_t, w_n)
     assert np.array_equal(s_f, s_e) and np.array_equal(w_f, w_e)
+
+
+# ------------------------------------------------------------------ draw order (conn_builder.cpp)
+# NEST's per-VP engines are std::mt19937_64 seeded with seed_seq_fe128{rng_seed, 0xb84c9bae, vp}
+# (random_manager.cpp:63-100). mt19937_64_streams.json holds the first outputs of such engines,
+# written by the reference's own randutils.hpp (tests/golden/make_seedseq_vectors.py). The
+# distributions map engine outputs by libstdc++'s published algorithms (GCC 11):
+#   uniform_real / drand: generate_canonical<double, 53> with one 64-bit draw, double(x) / 2^64,
+#   values >= 1 replaced by the largest double below 1 (bits/random.tcc generate_canonical);
+#   uniform_int over [0, n): Lemire's nearly divisionless method with 128-bit products
+#   (bits/uniform_int_dist.h _S_nd), which rejects a draw whose low word is below 2^64 mod n.
+class _Stream:
+    def __init__(self, seed):
+        with open(os.path.join(GOLDEN, "mt19937_64_streams.json")) as f:
+            self.x = next(c["mt19937_64"] for c in json.load(f) if c["seed"] == seed)
+        self.i = 0
+
+    def raw(self):
+        v = self.x[self.i]
+        self.i += 1
+        return v
+
+    def canonical(self):
+        u = float(self.raw()) / 18446744073709551616.0
+        return u if u < 1.0 else np.nextafter(1.0, 0.0)
+
+    def ulrand(self, n):
+        threshold = (1 << 64) % n
+        while True:
+            prod = self.raw() * n
+            if (prod & ((1 << 64) - 1)) >= threshold:
+                return prod >> 64
+
+
+SEED_VP0 = [143202461, 0xb84c9bae, 0]
+SEED_VP1 = [143202461, 0xb84c9bae, 1]
+
+
+def _delay_steps(ms):
+    # Time::delay_ms_to_steps (nest_time.h:539-542): ld_round(ms * 10) = floor(x + 0.5)
+    return int(np.floor(ms * 10.0 + 0.5))
+
+
+def test_fixed_indegree_draw_order_from_reference_engine_outputs():
+    # FixedInDegreeBuilder::inner_connect_ (conn_builder.cpp:1526-1545): per target in order,
+    # per connection: the source ulrand(n_sources), then single_connect_ (:419-420): the delay,
+    # then the weight. Fails for: delay and weight drawn in the other order; all K sources drawn
+    # before their weights; a source mapping other than Lemire's; uniform(min, max) computed as
+    # max - u * range; targets of other VPs drawing from this VP's stream.
+    for n_vp in (1, 2):
+        o = Oracle(n_vp=n_vp, seed=143202461)
+        g = o.create("iaf_psc_delta", 12)
+        pre, post = g[:9], g[9:]
+        o.connect(pre, post, "fixed_indegree", 4, weight=("uniform", 0.5, 1.5), delay=("uniform", 1.0, 3.0))
+        s, t, w, d = o.connections()
+        streams = {0: _Stream(SEED_VP0), 1: _Stream(SEED_VP1)}
+        exp = {}
+        for tgt in post:
+            vp = int(tgt % n_vp)
+            st = streams[vp]
+            for _ in range(4):
+                src = pre[st.ulrand(len(pre))]
+                dms = 1.0 + st.canonical() * 2.0
+                wv = 0.5 + st.canonical() * 1.0
+                exp.setdefault(vp, []).append((src, tgt, wv, _delay_steps(dms)))
+        got = list(zip(s.tolist(), t.tolist(), w.tolist(), d.tolist()))
+        # GetConnections order: per VP in creation order
+        want = [c for vp in sorted(exp) for c in exp[vp]]
+        assert sorted(got) == sorted(want)
+        for vp in exp:  # creation order within each VP
+            mine = [c for c in got if c[1] % n_vp == vp]
+            assert mine == exp[vp]
+
+
+def test_pairwise_bernoulli_draw_order_from_reference_engine_outputs():
+    # BernoulliBuilder::inner_connect_ (conn_builder.cpp:1937-1965): per target, per source in
+    # order, one drand(); a connection when it is below p, whose delay then weight are drawn
+    # at once (single_connect_). Fails for: the weight drawn before the Bernoulli trial of the
+    # next source; sources looped outside targets; drand() <= p instead of < p (golden draws
+    # are never equal to p, so that mistake shows only as a changed comparison on equal
+    # values: the test checks the strict one on the draws it has).
+    o = Oracle(n_vp=1, seed=143202461)
+    g = o.create("iaf_psc_delta", 10)
+    pre, post = g[:6], g[6:]
+    o.connect(pre, post, "pairwise_bernoulli", 0.5, weight=("normal", 0.0, 1.0), delay=("uniform", 0.5, 2.5))
+    s, t, w, d = o.connections()
+    st = _Stream(SEED_VP0)
+    want = []
+    saved = None
+    for tgt in post:
+        for src in pre:
+            if st.canonical() >= 0.5:
+                continue
+            dms = 0.5 + st.canonical() * 2.0
+            # std::normal_distribution: Marsaglia's polar method, the second value cached
+            if saved is not None:
+                z, saved = saved, None
+            else:
+                while True:
+                    x = 2.0 * st.canonical() - 1.0
+                    y = 2.0 * st.canonical() - 1.0
+                    r2 = x * x + y * y
+                    if r2 <= 1.0 and r2 != 0.0:
+                        break
+                import math
+                mult = math.sqrt(-2.0 * math.log(r2) / r2)
+                saved = x * mult
+                z = y * mult
+            want.append((src, tgt, z * 1.0 + 0.0, _delay_steps(dms)))
+    assert list(zip(s.tolist(), t.tolist(), w.tolist(), d.tolist())) == want
+    assert len(want) > 5
+
+
+def test_compare_draws_right_operand_first():
+    # ComparingParameter::value (parameter.h:712-715) evaluates both operands as the arguments
+    # of one call; g++ on x86-64 evaluates them right to left, so compare(uniform, uniform, <)
+    # on consecutive draws u0, u1 of the stream gives u1 < u0. Fails for left-first evaluation.
+    o = Oracle(n_vp=1, seed=143202461)
+    g = o.create("iaf_psc_delta", 40)
+    o.connect(g[:20], g[20:], "one_to_one", weight=("compare", ("uniform", 0.0, 1.0), ("uniform", 0.0, 1.0), 0),
+              delay=1.0)
+    s, t, w, d = o.connections()
+    st = _Stream(SEED_VP0)
+    want = []
+    for _ in range(20):
+        right = st.canonical()
+        left = st.canonical()
+        want.append(1.0 if left < right else 0.0)
+    assert w.tolist() == want
+    assert 0 < sum(want) < 20
+
+
+# ------------------------------------------------------------------ spike arrival (a1, a7-a10)
+def _relay_net(n_vp, n_pre, n_post, rule, d_steps, min_delay, max_delay, w=1.0, e_drive=4):
+    """iaf_psc_delta relays restating other/test_spike_transmission.py: the pre neurons fire at
+    stamp e_drive + 1 + min_delay (a drive event emitted at step e_drive with delay min_delay),
+    their spikes reach the post neurons with delay d_steps. Two idle neurons connected with
+    delays min_delay and max_delay fix the delay extrema."""
+    o = Oracle(n_vp=n_vp)
+    relay = {"E_L": 0.0, "V_th": 20.0, "V_reset": 0.0, "tau_m": 1e6, "C_m": 1.0, "t_ref": 2.0, "V_m": 0.0}
+    pre = o.create("iaf_psc_delta", n_pre, relay)
+    post = o.create("iaf_psc_delta", n_post, dict(relay, V_th=1e9))
+    idle = o.create("iaf_psc_delta", 2, relay)
+    o.add_connections([idle[0]], [idle[1]], [0.0], [min_delay])
+    o.add_connections([idle[1]], [idle[0]], [0.0], [max_delay])
+    if rule == "one_to_one":
+        o.add_connections(pre, post, [w] * n_pre, [d_steps] * n_pre)
+    else:
+        src = np.repeat(pre, n_post)
+        tgt = np.tile(post, n_pre)
+        o.add_connections(src, tgt, [w] * len(src), [d_steps] * len(src))
+    counts = np.zeros((e_drive + 1, n_pre), np.uint8)
+    counts[e_drive, :] = 1
+    o.add_drive(pre[0], counts, 25.0, min_delay * 0.1)
+    return o, pre, post
+
+
+@pytest.mark.parametrize("n_vp", [1, 2, 3, 4])
+@pytest.mark.parametrize("rule", ["one_to_one", "all_to_all"])
+@pytest.mark.parametrize("min_delay, max_delay, d", [(1, 1, 1), (1, 12, 10), (3, 3, 3), (3, 9, 7), (5, 20, 12),
+                                                       (10, 10, 10)])
+def test_spike_arrival_time(n_vp, rule, min_delay, max_delay, d):
+    # other/test_spike_transmission.py: a spike of stamp S over a connection of delay d is
+    # read in the update of step S + d - 1 (slot lag + d - min_delay of the delivering slice,
+    # event.h:1006-1013), so V_m of the post neuron rises by the weight between times
+    # (S + d - 1) h and (S + d) h and by exactly n_pre weights for all_to_all. Fails for an
+    # off-by-one in the slot, a delivery in the wrong slice, or a lost or doubled event.
+    n = 4 if rule == "one_to_one" else 3
+    o, pre, post = _relay_net(n_vp, n, n, rule, d, min_delay, max_delay)
+    S = 4 + 1 + min_delay  # pre spike stamp
+    steps_before = S + d - 1
+    o.run_steps(steps_before)
+    g, s = o.spikes()
+    assert sorted(g.tolist()) == sorted(pre.tolist()) and np.all(s == S)
+    assert np.all(o.get(post, "V_m") == 0.0)
+    o.run_steps(1)
+    assert np.all(o.get(post, "V_m") == (1.0 if rule == "one_to_one" else float(n)))
+    st = o.status()
+    assert st["min_delay"] == min_delay and st["max_delay"] == max_delay
+
+
+# ------------------------------------------------------------------ split Simulate (a1)
+@pytest.mark.parametrize("pieces", [[3.0], [1.0, 1.0, 1.0], [0.3, 0.2, 1.7, 0.8], [0.1] * 30])
+def test_split_simulate_equals_single_run(pieces):
+    # other/test_stepped_simulation.py: Simulate split into several calls gives the same spikes
+    # as one call; here also the same V_m, on a recurrent network whose min_delay (1.5 ms) is
+    # cut by the splits. Fails for a split that drops or repeats a partial slice, or moves the
+    # delivery of a slice split across calls.
+    import netbuild as nb
+
+    desc = nb.brunel_delta(N=200, K=20, T=3.0, n_vp=2, seed=31)
+    ref, pr = nb.build_oracle(desc)
+    ref.simulate(sum(pieces))
+    o, po = nb.build_oracle(desc)
+    for p in pieces:
+        o.simulate(p)
+    g1, s1 = ref.spikes()
+    g2, s2 = o.spikes()
+    assert sorted(zip(s1.tolist(), g1.tolist())) == sorted(zip(s2.tolist(), g2.tolist()))
+    assert len(s1) > 0
+    assert np.array_equal(ref.get(pr[0], "V_m"), o.get(po[0], "V_m"))
+
+
+# ------------------------------------------------------------------ iaf_psc_exp synaptic current (a4)
+@pytest.mark.parametrize("w", [300.0, -300.0])
+def test_iaf_psc_exp_single_psc_closed_form(w):
+    # One input of weight w (SYN_EX for w > 0, SYN_IN for w < 0, iaf_psc_exp.h:372-380) read in
+    # the update of step k: the current is added after that step's V update
+    # (iaf_psc_exp.cpp:313-330), so n steps later I_syn = w exp(-n h / tau_s) and
+    # V_m - E_L = w / C_m * tau_s tau_m / (tau_m - tau_s) * (exp(-t/tau_m) - exp(-t/tau_s)),
+    # t = n h, the exact solution. Fails for the channels swapped, the input added before the
+    # decay, or the current reaching V_m in the step it arrives.
+    tau_m, tau_s, C, E_L, h = 10.0, 2.0, 250.0, -70.0, 0.1
+    o = Oracle()
+    n = o.create("iaf_psc_exp", 1, {"tau_m": tau_m, "tau_syn_ex": tau_s, "tau_syn_in": tau_s, "C_m": C,
+                                     "E_L": E_L, "V_m": E_L, "V_th": 1e9})
+    counts = np.zeros((10, 1), np.uint8)
+    counts[4, 0] = 1  # emission step 4, delay 1 step: read in the update of step 5
+    o.add_drive(n[0], counts, w, 0.1)
+    o.run_steps(5)
+    assert o.get(n, "V_m")[0] == E_L
+    assert o.get(n, "I_syn_ex")[0] == 0.0 and o.get(n, "I_syn_in")[0] == 0.0
+    o.run_steps(1)  # through step 5
+    ch, other = ("I_syn_ex", "I_syn_in") if w > 0 else ("I_syn_in", "I_syn_ex")
+    assert o.get(n, ch)[0] == w and o.get(n, other)[0] == 0.0
+    assert o.get(n, "V_m")[0] == E_L
+    for k in range(1, 120):
+        o.run_steps(1)
+        t = k * h
+        i_exp = w * np.exp(-t / tau_s)
+        v_exp = w / C * tau_s * tau_m / (tau_m - tau_s) * (np.exp(-t / tau_m) - np.exp(-t / tau_s))
+        assert abs(o.get(n, ch)[0] - i_exp) <= 1e-12 * abs(w)
+        assert abs(o.get(n, "V_m")[0] - E_L - v_exp) <= 1e-12 * (abs(v_exp) + 1e-3)
+        assert o.get(n, other)[0] == 0.0
+
+
+# ------------------------------------------------------------------ receive-order summation (a11)
+B = 2.0 ** 53
+
+
+def _sum_net(n_vp, pre_weights, creation=None):
+    """Sources (iaf_psc_delta, one spike each, all in the same step) into one target whose V_m
+    after the arrival step equals the sum of the cell: tau_m huge, V_th huge, E_L = 0."""
+    o = Oracle(n_vp=n_vp)
+    relay = {"E_L": 0.0, "V_th": 20.0, "V_reset": 0.0, "tau_m": 1e6, "C_m": 1.0, "t_ref": 2.0, "V_m": 0.0}
+    pre = o.create("iaf_psc_delta", len(pre_weights), relay)
+    post = o.create("iaf_psc_delta", 1, dict(relay, V_th=1e300, V_min=-1e300))
+    for i, ws in (creation or enumerate(pre_weights)):
+        for wv in (ws if isinstance(ws, (list, tuple)) else [ws]):
+            o.add_connections([pre[i]], [post[0]], [wv], [2])
+    counts = np.zeros((3, len(pre_weights)), np.uint8)
+    counts[2, :] = 1
+    o.add_drive(pre[0], counts, 25.0, 0.1)
+    o.run_steps(3 + 1 + 2)  # pre spikes at stamp 4, read in the update of step 5
+    return o.get(post, "V_m")[0]
+
+
+def test_receive_order_summation_one_vp():
+    # One VP: spikes of one slice are added in ascending source gid (event_delivery_manager.cpp:
+    # 470-497, 599-776); 0 + a + b + c rounds differently depending on which weight comes
+    # last: with (-2^53, 0.75, 2^53) in gid order the cell is 1.0; the reversed order gives 0.0,
+    # 2^53 + 0.75 - 2^53 gives 0.0 and (2^53 - 2^53) + 0.75 gives 0.75.
+    assert _sum_net(1, [-B, 0.75, B]) == 1.0
+    assert _sum_net(1, [B, 0.75, -B]) == 0.0
+
+
+def _fold(ws):
+    acc = 0.0
+    for x in ws:
+        acc = acc + x
+    return acc
+
+
+def test_receive_order_summation_across_vps():
+    # Two VPs: the receive buffer holds VP 0's spikes, then VP 1's, each in ascending gid
+    # (collocate_spike_data_buffers_, event_delivery_manager.cpp:470-497; vp = gid % 2). Source
+    # gids 1..4 sit on VPs 1, 0, 1, 0, so the cell adds gid 2, 4, 1, 3. The weights are chosen
+    # so that gid order (1, 2, 3, 4) and VP-1-first order (1, 3, 2, 4) give other sums.
+    for w, expected in (([0.0, -B, B, 0.75], 1.0), ([B, 0.0, 0.75, -B], 0.75)):
+        receive = [w[1], w[3], w[0], w[2]]
+        assert _fold(receive) == expected
+        assert _fold(w) != expected and _fold([w[0], w[2], w[1], w[3]]) != expected
+        assert _sum_net(2, w) == expected
+
+
+def test_connector_run_order_within_a_source():
+    # Connections of one source to one target are added in connector run order, which is
+    # creation order here (connector_base.h:410-427): source gid 1 = -2^53, then gid 2 with two
+    # connections created 0.75 first and 2^53 second gives (-2^53 + 0.75) + 2^53 = 1.0;
+    # the reversed run gives (-2^53 + 2^53) + 0.75 = 0.75.
+    assert _sum_net(1, [-B, [0.75, B]]) == 1.0
+    assert _sum_net(1, [-B, [B, 0.75]]) == 0.75
