@@ -73,8 +73,30 @@ VpRng::VpRng( const std::vector< uint32_t >& seed )
 }
 
 // ------------------------------------------------------------------ parameters
+// The per-VP distributions of a normal or lognormal leaf: those of the parameter object its
+// "id" names when the kernel has seen it before (same VP count), else new ones.
+static std::shared_ptr< std::vector< VpNormal > >
+shared_normals( const Json& v, int n_vp, double mean, double sd, ParamStates* states )
+{
+  std::shared_ptr< std::vector< VpNormal > >* slot = nullptr;
+  if ( states and v.has( "id" ) )
+  {
+    slot = &states->normals[ static_cast< int64_t >( v.at( "id" ).number() ) ];
+    if ( *slot and static_cast< int >( ( *slot )->size() ) == n_vp and ( **slot )[ 0 ].mean == mean
+         and ( **slot )[ 0 ].sd == sd )
+      return *slot;
+  }
+  auto d = std::make_shared< std::vector< VpNormal > >();
+  d->reserve( n_vp );
+  for ( int i = 0; i < n_vp; ++i )
+    d->emplace_back( mean, sd );
+  if ( slot )
+    *slot = d;
+  return d;
+}
+
 std::unique_ptr< Param >
-Param::from_json( const Json& j, int n_vp, bool top_level )
+Param::from_json( const Json& j, int n_vp, bool top_level, ParamStates* states )
 {
   auto p = std::make_unique< Param >();
   if ( j.kind == Json::NUM or j.kind == Json::BOOL )
@@ -124,8 +146,7 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     p->b_ = v.has( "std" ) ? v.at( "std" ).number() : 1.0;
     if ( p->b_ < 0 )
       throw std::runtime_error( "BadProperty: nest::NormalParameter: std >= 0 required." );
-    for ( int i = 0; i < n_vp; ++i )
-      p->normals_.emplace_back( p->a_, p->b_ );
+    p->normals_ = shared_normals( v, n_vp, p->a_, p->b_, states );
   }
   else if ( k == "lognormal" )
   {
@@ -136,8 +157,7 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     p->b_ = v.has( "std" ) ? v.at( "std" ).number() : 1.0;
     if ( p->b_ < 0 )
       throw std::runtime_error( "BadProperty: nest::LognormalParameter: std ≥ 0 required." );
-    for ( int i = 0; i < n_vp; ++i )
-      p->normals_.emplace_back( 0.0, 1.0 );
+    p->normals_ = shared_normals( v, n_vp, 0.0, 1.0, states );
   }
   else if ( k == "exponential" )
   {
@@ -151,7 +171,7 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
   {
     // ExpParameter, SinParameter, CosParameter (parameter.h:991,1037,1082)
     p->op_ = k == "exp" ? EXP : k == "sin" ? SIN : COS;
-    p->l_ = from_json( v, n_vp );
+    p->l_ = from_json( v, n_vp, false, states );
   }
   else if ( k == "compare" )
   {
@@ -159,8 +179,8 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( v.kind != Json::ARR or v.arr.size() != 3 )
       throw std::runtime_error( "BadProperty: compare takes two parameters and a comparator" );
     p->op_ = COMPARE;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
-    p->r_ = from_json( v.arr[ 1 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
+    p->r_ = from_json( v.arr[ 1 ], n_vp, false, states );
     p->a_ = v.arr[ 2 ].number();
     if ( p->a_ < 0 or p->a_ > 5 or p->a_ != std::floor( p->a_ ) )
       throw std::runtime_error( "BadProperty: Comparator specification has to be in the range 0-5." );
@@ -171,9 +191,9 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( v.kind != Json::ARR or v.arr.size() != 3 )
       throw std::runtime_error( "BadProperty: conditional takes a condition and two parameters" );
     p->op_ = CONDITIONAL;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
-    p->r_ = from_json( v.arr[ 1 ], n_vp );
-    p->f_ = from_json( v.arr[ 2 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
+    p->r_ = from_json( v.arr[ 1 ], n_vp, false, states );
+    p->f_ = from_json( v.arr[ 2 ], n_vp, false, states );
   }
   else if ( k == "pow" )
   {
@@ -181,7 +201,7 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( v.kind != Json::ARR or v.arr.size() != 2 )
       throw std::runtime_error( "BadProperty: pow takes a parameter and an exponent" );
     p->op_ = POW;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
     p->a_ = v.arr[ 1 ].number();
   }
   else if ( k == "sum" or k == "product" or k == "quotient" )
@@ -190,15 +210,15 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( v.kind != Json::ARR or v.arr.size() != 2 )
       throw std::runtime_error( "BadProperty: sum/product/quotient take two operands" );
     p->op_ = k == "sum" ? SUM : k == "product" ? PRODUCT : QUOTIENT;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
-    p->r_ = from_json( v.arr[ 1 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
+    p->r_ = from_json( v.arr[ 1 ], n_vp, false, states );
   }
   else if ( k == "max" or k == "min" )
   {
     if ( v.kind != Json::ARR or v.arr.size() != 2 )
       throw std::runtime_error( "BadProperty: max/min take a parameter and a value" );
     p->op_ = k == "max" ? MAX : MIN;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
     p->a_ = v.arr[ 1 ].number();
   }
   else if ( k == "redraw" )
@@ -206,7 +226,7 @@ Param::from_json( const Json& j, int n_vp, bool top_level )
     if ( v.kind != Json::ARR or v.arr.size() != 3 )
       throw std::runtime_error( "BadProperty: redraw takes a parameter, min and max" );
     p->op_ = REDRAW;
-    p->l_ = from_json( v.arr[ 0 ], n_vp );
+    p->l_ = from_json( v.arr[ 0 ], n_vp, false, states );
     p->a_ = v.arr[ 1 ].number();
     p->b_ = v.arr[ 2 ].number();
   }
@@ -229,7 +249,7 @@ Param::value( VpRng& rng, int vp )
   case NORMAL:
     if ( b_ == 0 )
       return a_;
-    return normals_[ vp ]( rng );
+    return ( *normals_ )[ vp ]( rng );
   case SUM:
   {
     const double x = l_->value( rng, vp );
@@ -285,7 +305,7 @@ Param::value( VpRng& rng, int vp )
   case LOGNORMAL:
     if ( b_ == 0 )
       return std::exp( a_ );
-    return std::exp( b_ * normals_[ vp ]( rng ) + a_ );
+    return std::exp( b_ * ( *normals_ )[ vp ]( rng ) + a_ );
   case EXPONENTIAL:
     return a_ * ( -std::log( 1 - rng.drand() ) );
   case ARRAY:
@@ -420,6 +440,7 @@ Kernel::reset()
   targets_per_block_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
+  param_states_.normals.clear();
   graph_launch_ = true;
   graph_chunks_ = direct_chunks_ = capture_failures_ = 0;
   window_slots_ = 0;
@@ -1325,9 +1346,9 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
         throw std::runtime_error( "UnknownSynapseType: '" + sm + "' is not available on the device path." );
     }
     else if ( kv.first == "weight" )
-      wp = Param::from_json( kv.second, n_vp_, true );
+      wp = Param::from_json( kv.second, n_vp_, true, &param_states_ );
     else if ( kv.first == "delay" )
-      dp = Param::from_json( kv.second, n_vp_, true );
+      dp = Param::from_json( kv.second, n_vp_, true, &param_states_ );
     else if ( kv.first == "receptor_type" )
     {
       // the models here have the one receptor 0 (handles_test_event, e.g. iaf_psc_exp.h:466-474)

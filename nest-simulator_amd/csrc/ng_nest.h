@@ -121,6 +121,54 @@ struct alignas( 64 ) VpRng
   }
 };
 
+// std::normal_distribution<double> (libstdc++ bits/random.tcc): Marsaglia's polar method
+// over pairs of canonical draws, the second value of a pair kept for the next call
+struct alignas( 64 ) VpNormal
+{
+  double mean, sd;
+  double saved = 0;
+  bool saved_available = false;
+  VpNormal( double mean_, double sd_ )
+    : mean( mean_ )
+    , sd( sd_ )
+  {
+  }
+  double
+  operator()( VpRng& rng )
+  {
+    double ret;
+    if ( saved_available )
+    {
+      saved_available = false;
+      ret = saved;
+    }
+    else
+    {
+      double x, y, r2;
+      do
+      {
+        x = 2.0 * rng.canonical() - 1.0;
+        y = 2.0 * rng.canonical() - 1.0;
+        r2 = x * x + y * y;
+      } while ( r2 > 1.0 or r2 == 0.0 );
+      const double mult = std::sqrt( -2.0 * std::log( r2 ) / r2 );
+      saved = x * mult;
+      saved_available = true;
+      ret = y * mult;
+    }
+    return ret * sd + mean;
+  }
+};
+
+// A parameter object keeps its per-VP distributions across Connect calls: NormalParameter and
+// LognormalParameter hold one distribution per thread (parameter.cpp:95-157), whose cached
+// second polar value carries over to the next call that uses the same object. A normal or
+// lognormal leaf with an "id" names such an object; the kernel's registry holds its state.
+struct ParamStates
+{
+  std::map< int64_t, std::shared_ptr< std::vector< VpNormal > > > normals;
+};
+
 // nest::Parameter expression trees used by syn_spec weights/delays (parameter.h)
 class Param
 {
@@ -149,7 +197,8 @@ public:
   };
   // top_level: a JSON array is an array parameter (one value per connection, a syn_spec
   // value of its own, not an operand)
-  static std::unique_ptr< Param > from_json( const Json& j, int n_vp, bool top_level = false );
+  static std::unique_ptr< Param > from_json(
+    const Json& j, int n_vp, bool top_level = false, ParamStates* states = nullptr );
   double value( VpRng& rng, int vp );
   bool
   is_const() const
@@ -179,45 +228,8 @@ private:
   Op op_ = CONST;
   double a_ = 0, b_ = 0;
   std::unique_ptr< Param > l_, r_, f_;  // f_: a conditional's value when false
-  // std::normal_distribution<double> (libstdc++ bits/random.tcc): Marsaglia's polar method
-  // over pairs of canonical draws, the second value of a pair kept for the next call
-  struct alignas( 64 ) VpNormal
-  {
-    double mean, sd;
-    double saved = 0;
-    bool saved_available = false;
-    VpNormal( double mean_, double sd_ )
-      : mean( mean_ )
-      , sd( sd_ )
-    {
-    }
-    double
-    operator()( VpRng& rng )
-    {
-      double ret;
-      if ( saved_available )
-      {
-        saved_available = false;
-        ret = saved;
-      }
-      else
-      {
-        double x, y, r2;
-        do
-        {
-          x = 2.0 * rng.canonical() - 1.0;
-          y = 2.0 * rng.canonical() - 1.0;
-          r2 = x * x + y * y;
-        } while ( r2 > 1.0 or r2 == 0.0 );
-        const double mult = std::sqrt( -2.0 * std::log( r2 ) / r2 );
-        saved = x * mult;
-        saved_available = true;
-        ret = y * mult;
-      }
-      return ret * sd + mean;
-    }
-  };
-  std::vector< VpNormal > normals_;  // one per VP (parameter.cpp:95-117)
+  // one per VP (parameter.cpp:95-117), shared by every use of the same parameter object
+  std::shared_ptr< std::vector< VpNormal > > normals_;
   // array parameter: the values and each VP's next index (ArrayDoubleParameter,
   // conn_parameter.h:231-300)
   struct alignas( 64 ) VpCursor
@@ -471,6 +483,7 @@ private:
   bool update_window( DeviceWindow& w, const std::string& k, const Json& v ) const;
   std::vector< int64_t > rec_gid_;
   std::vector< ConnList > conns_;  // per VP
+  ParamStates param_states_;       // parameter objects used by more than one Connect call
   struct DevConn
   {
     int64_t gen, tgt;

@@ -157,7 +157,13 @@ class Parameter:
     __hash__ = object.__hash__
 
 
+_param_ids = iter(range(1, 1 << 62))
+
+
 class _Random:
+    # normal and lognormal parameters carry an id: the library keeps one set of per-VP
+    # distributions per parameter object, so an object used in several Connect calls continues
+    # its draws from call to call as NEST's NormalParameter does (parameter.cpp:95-157)
     @staticmethod
     def uniform(min=0.0, max=1.0):
         return Parameter({"uniform": {"min": float(min), "max": float(max)}})
@@ -168,11 +174,11 @@ class _Random:
 
     @staticmethod
     def normal(mean=0.0, std=1.0):
-        return Parameter({"normal": {"mean": float(mean), "std": float(std)}})
+        return Parameter({"normal": {"mean": float(mean), "std": float(std), "id": next(_param_ids)}})
 
     @staticmethod
     def lognormal(mean=0.0, std=1.0):
-        return Parameter({"lognormal": {"mean": float(mean), "std": float(std)}})
+        return Parameter({"lognormal": {"mean": float(mean), "std": float(std), "id": next(_param_ids)}})
 
     @staticmethod
     def exponential(beta=1.0):

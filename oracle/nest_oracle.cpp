@@ -198,20 +198,31 @@ struct Rng
 // Prefix-encoded expression tree over doubles; op codes:
 //   0 const v | 1 uniform min max | 2 uniform_int max | 3 normal mean std
 //   4 sum A B | 5 product A B     | 6 max A v          | 7 min A v | 8 redraw A min max
-//   9 lognormal mean std | 10 exponential beta
+//   9 lognormal mean std | 10 exponential beta | 18 normal id mean std | 19 lognormal id mean std
 //   11 exp A | 12 sin A | 13 cos A | 14 quotient A B | 15 pow A v
 //   16 compare A B comparator | 17 conditional C T F
 // nestkernel/parameter.h:131-286 (leaves), 468-600 (product/sum), 878-990 (max/redraw)
+// A parameter object keeps its per-VP distributions across Connect calls: NormalParameter and
+// LognormalParameter hold one std::normal_distribution / lognormal_distribution per thread
+// (parameter.cpp:95-157), whose cached second polar value carries over to the next call that
+// uses the same object. Ops 18 and 19 name such an object by id; the registry holds its state.
+struct ParamStates
+{
+  std::map< long, std::shared_ptr< std::vector< std::normal_distribution<> > > > normals;
+  std::map< long, std::shared_ptr< std::vector< std::lognormal_distribution<> > > > lognormals;
+};
+
 struct Param
 {
   int op = 0;
   double a = 0, b = 0;
   std::unique_ptr< Param > l, r, f;
-  std::vector< std::normal_distribution<> > normals;  // one per VP (parameter.cpp:95-117)
-  std::vector< std::lognormal_distribution<> > lognormals;  // one per VP (parameter.cpp:120-157)
+  // one per VP (parameter.cpp:95-117, 120-157), shared by every use of the same object
+  std::shared_ptr< std::vector< std::normal_distribution<> > > normals;
+  std::shared_ptr< std::vector< std::lognormal_distribution<> > > lognormals;
 
   static std::unique_ptr< Param >
-  parse( const double*& p, const double* end, int n_vp )
+  parse( const double*& p, const double* end, int n_vp, ParamStates* states = nullptr )
   {
     if ( p >= end )
       throw std::runtime_error( "truncated parameter spec" );
@@ -235,37 +246,68 @@ struct Param
         throw std::runtime_error( "nest::UniformIntParameter: max > 0 required." );
       break;
     case 3:
+    case 18:
+    {
+      const long id = x->op == 18 ? static_cast< long >( *p++ ) : -1;
+      x->op = 3;
       x->a = *p++;
       x->b = *p++;
       if ( x->b < 0 )
         throw std::runtime_error( "nest::NormalParameter: std >= 0 required." );
-      for ( int i = 0; i < n_vp; ++i )
-        x->normals.emplace_back( x->a, x->b );
+      std::shared_ptr< std::vector< std::normal_distribution<> > >* slot =
+        id >= 0 and states ? &states->normals[ id ] : nullptr;
+      if ( slot and *slot and static_cast< int >( ( *slot )->size() ) == n_vp )
+        x->normals = *slot;
+      else
+      {
+        x->normals = std::make_shared< std::vector< std::normal_distribution<> > >();
+        if ( x->b > 0 )
+          for ( int i = 0; i < n_vp; ++i )
+            x->normals->emplace_back( x->a, x->b );
+        if ( slot )
+          *slot = x->normals;
+      }
       break;
+    }
     case 4:
     case 5:
-      x->l = parse( p, end, n_vp );
-      x->r = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
+      x->r = parse( p, end, n_vp, states );
       break;
     case 6:
     case 7:
-      x->l = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
       x->a = *p++;
       break;
     case 8:
-      x->l = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
       x->a = *p++;
       x->b = *p++;
       break;
     case 9:
+    case 19:
+    {
+      const long id = x->op == 19 ? static_cast< long >( *p++ ) : -1;
+      x->op = 9;
       x->a = *p++;
       x->b = *p++;
       if ( x->b < 0 )
         throw std::runtime_error( "nest::LognormalParameter: std >= 0 required." );
-      if ( x->b > 0 )
-        for ( int i = 0; i < n_vp; ++i )
-          x->lognormals.emplace_back( x->a, x->b );
+      std::shared_ptr< std::vector< std::lognormal_distribution<> > >* slot =
+        id >= 0 and states ? &states->lognormals[ id ] : nullptr;
+      if ( slot and *slot and static_cast< int >( ( *slot )->size() ) == n_vp )
+        x->lognormals = *slot;
+      else
+      {
+        x->lognormals = std::make_shared< std::vector< std::lognormal_distribution<> > >();
+        if ( x->b > 0 )
+          for ( int i = 0; i < n_vp; ++i )
+            x->lognormals->emplace_back( x->a, x->b );
+        if ( slot )
+          *slot = x->lognormals;
+      }
       break;
+    }
     case 10:
       x->a = *p++;
       if ( x->a < 0 )
@@ -274,25 +316,25 @@ struct Param
     case 11:
     case 12:
     case 13:
-      x->l = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
       break;
     case 14:
-      x->l = parse( p, end, n_vp );
-      x->r = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
+      x->r = parse( p, end, n_vp, states );
       break;
     case 15:
-      x->l = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
       x->a = *p++;
       break;
     case 16:
-      x->l = parse( p, end, n_vp );
-      x->r = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
+      x->r = parse( p, end, n_vp, states );
       x->a = *p++;
       break;
     case 17:
-      x->l = parse( p, end, n_vp );
-      x->r = parse( p, end, n_vp );
-      x->f = parse( p, end, n_vp );
+      x->l = parse( p, end, n_vp, states );
+      x->r = parse( p, end, n_vp, states );
+      x->f = parse( p, end, n_vp, states );
       break;
     default:
       throw std::runtime_error( "unknown parameter op" );
@@ -314,7 +356,7 @@ struct Param
     case 3:
       if ( b == 0 )
         return a;
-      return normals[ vp ]( rng.eng );
+      return ( *normals )[ vp ]( rng.eng );
     case 4:
     {
       const double x = l->value( rng, vp );
@@ -333,7 +375,7 @@ struct Param
       // LognormalParameter::value (parameter.cpp:147-157)
       if ( b == 0 )
         return std::exp( a );
-      return lognormals[ vp ]( rng.eng );
+      return ( *lognormals )[ vp ]( rng.eng );
     case 10:
       // ExponentialParameter::value (parameter.h:339-342)
       return a * ( -std::log( 1 - rng.drand() ) );
@@ -678,6 +720,7 @@ struct Kernel
   std::vector< long > gid2idx;  // -1 for devices
   std::vector< std::vector< Conn > > conns;  // per VP, creation order
   std::vector< Drive > drives;
+  ParamStates param_states;  // parameter objects used by more than one Connect call
   long min_delay = 0, max_delay = 0;
   long d_min_seen = std::numeric_limits< long >::max(), d_max_seen = 1;
   bool simulated = false, prepared = false;
@@ -939,12 +982,12 @@ struct Kernel
     if ( wn > 0 )
     {
       const double* p = ws;
-      wp = Param::parse( p, ws + wn, n_vp );
+      wp = Param::parse( p, ws + wn, n_vp, &param_states );
     }
     if ( dn > 0 )
     {
       const double* p = ds;
-      dp = Param::parse( p, ds + dn, n_vp );
+      dp = Param::parse( p, ds + dn, n_vp, &param_states );
     }
     for ( long i = 0; i < n_post; ++i )
       nrn( post[ i ] );

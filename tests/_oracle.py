@@ -2,7 +2,8 @@
 
 Test infrastructure: the oracle is the checker the MI355X path is compared with.
 Parameter specs use the prefix encoding documented in oracle/nest_oracle.cpp
-(0 const, 1 uniform, 2 uniform_int, 3 normal, 4 sum, 5 product, 6 max, 7 min, 8 redraw).
+(0 const, 1 uniform, 2 uniform_int, 3 normal, 4 sum, 5 product, 6 max, 7 min, 8 redraw, ...; 18/19
+normal/lognormal parameter objects named by an id).
 """
 
 import ctypes
@@ -74,9 +75,11 @@ def spec(x):
         elif op == "uniform_int":
             out.extend([2, v[1]])
         elif op == "normal":
-            out.extend([3, v[1], v[2]])
+            # ("normal", mean, std[, id]): with an id, the same parameter object in every
+            # connect call that names it (its per-VP distribution state carries over)
+            out.extend([3, v[1], v[2]] if len(v) < 4 else [18, v[3], v[1], v[2]])
         elif op == "lognormal":
-            out.extend([9, v[1], v[2]])
+            out.extend([9, v[1], v[2]] if len(v) < 4 else [19, v[3], v[1], v[2]])
         elif op == "exponential":
             out.extend([10, v[1]])
         elif op in ("exp", "sin", "cos"):

@@ -16,6 +16,9 @@ import nestgpu
 from _oracle import Oracle
 
 
+PARAM_ID_BASE = 1 << 40  # JSON numbers are doubles: ids stay below 2^53
+
+
 def to_param(x):
     if isinstance(x, (int, float, np.floating, np.integer)):
         return float(x)
@@ -24,10 +27,18 @@ def to_param(x):
         return nestgpu.Parameter({"uniform": {"min": x[1], "max": x[2]}})
     if op == "uniform_int":
         return nestgpu.Parameter({"uniform_int": {"max": x[1]}})
+    # ("normal", mean, std, id): one parameter object shared by every connection spec naming
+    # the id (ids offset so that they never meet the ids nestgpu.random hands out)
     if op == "normal":
-        return nestgpu.Parameter({"normal": {"mean": x[1], "std": x[2]}})
+        d = {"mean": x[1], "std": x[2]}
+        if len(x) > 3:
+            d["id"] = PARAM_ID_BASE + int(x[3])
+        return nestgpu.Parameter({"normal": d})
     if op == "lognormal":
-        return nestgpu.Parameter({"lognormal": {"mean": x[1], "std": x[2]}})
+        d = {"mean": x[1], "std": x[2]}
+        if len(x) > 3:
+            d["id"] = PARAM_ID_BASE + int(x[3])
+        return nestgpu.Parameter({"lognormal": d})
     if op == "exponential":
         return nestgpu.Parameter({"exponential": {"beta": x[1]}})
     if op in ("exp", "sin", "cos"):

@@ -98,6 +98,39 @@ def test_connectivity_independent_of_call_granularity_per_vp():
     assert np.array_equal(s1[keep], s2) and np.array_equal(w1[keep], w2) and np.array_equal(d1[keep], d2)
 
 
+def test_parameter_object_continues_across_connect_calls():
+    # one normal parameter object in several Connect calls (as the 8-population circuit's
+    # builder uses its weight and delay objects): its per-VP distributions, with the cached
+    # second polar value, carry over from call to call (parameter.cpp:95-117); a second object
+    # with the same mean and std starts its own
+    w = ("normal", 1.0, 0.5, 7)
+    d = ("max", ("normal", 1.5, 0.75, 8), 0.1)
+    desc = dict(n_vp=3, seed=19, pops=[("iaf_psc_exp", 90, {})],
+                conns=[((0, 0, 30), (0, 0, 30), "pairwise_bernoulli", 0.1, w, d),
+                       ((0, 0, 30), (0, 30, 60), "pairwise_bernoulli", 0.13, w, d),
+                       ((0, 0, 30), (0, 60, 90), "fixed_indegree", 3, ("normal", 1.0, 0.5), d)])
+    k, kp, sr = nb.build_gpu(desc)
+    o, op = nb.build_oracle(desc)
+    s1, t1, w1, d1 = _conns_gpu(k)
+    keep = t1 != sr.global_id
+    s2, t2, w2, d2 = o.connections()
+    assert np.array_equal(s1[keep], s2) and np.array_equal(t1[keep], t2)
+    assert np.array_equal(w1[keep], w2) and np.array_equal(d1[keep], d2)
+    # the same build through nestgpu.random objects held by Python variables
+    import nestgpu
+    k2 = nestgpu.Kernel(0)
+    k2.SetKernelStatus({"total_num_virtual_procs": 3, "rng_seed": 19})
+    g = k2.Create("iaf_psc_exp", 90)
+    wo = nestgpu.random.normal(1.0, 0.5)
+    do = nestgpu.math.max(nestgpu.random.normal(1.5, 0.75), 0.1)
+    k2.Connect(g[0:30], g[0:30], {"rule": "pairwise_bernoulli", "p": 0.1}, {"weight": wo, "delay": do})
+    k2.Connect(g[0:30], g[30:60], {"rule": "pairwise_bernoulli", "p": 0.13}, {"weight": wo, "delay": do})
+    k2.Connect(g[0:30], g[60:90], {"rule": "fixed_indegree", "indegree": 3},
+               {"weight": nestgpu.random.normal(1.0, 0.5), "delay": do})
+    s3, t3, w3, d3 = _conns_gpu(k2)
+    assert np.array_equal(s3, s2) and np.array_equal(w3, w2) and np.array_equal(d3, d2)
+
+
 @pytest.mark.parametrize("n_vp, n", [(1, 20000), (3, 20000), (8, 70000)])
 def test_connect_arrays_connectivity_identical_to_oracle(n_vp, n):
     # connect_arrays: every VP creates its targets' connections in array order

@@ -652,3 +652,45 @@ def test_connector_run_order_within_a_source():
     # the reversed run gives (-2^53 + 2^53) + 0.75 = 0.75.
     assert _sum_net(1, [-B, [0.75, B]]) == 1.0
     assert _sum_net(1, [-B, [B, 0.75]]) == 0.75
+
+
+def _polar_normals(st, n, mean=0.0, sd=1.0):
+    """n values of one std::normal_distribution (libstdc++: Marsaglia's polar method, the
+    second value of each pair cached) over the stream st."""
+    import math
+    out = []
+    while len(out) < n:
+        while True:
+            x = 2.0 * st.canonical() - 1.0
+            y = 2.0 * st.canonical() - 1.0
+            r2 = x * x + y * y
+            if r2 <= 1.0 and r2 != 0.0:
+                break
+        mult = math.sqrt(-2.0 * math.log(r2) / r2)
+        out.extend([y * mult * sd + mean, x * mult * sd + mean])
+    return out[:n]
+
+
+def test_normal_parameter_object_carries_its_state_across_connect_calls():
+    # NormalParameter (parameter.cpp:95-117) owns one normal_distribution per thread: used in two
+    # Connect calls, the second call continues with the value cached by the first. Three
+    # one_to_one connections per call: the six weights are six consecutive values of one
+    # distribution on the VP stream. Fails for a parameter whose state restarts per call (the
+    # fourth weight would then come from a new pair).
+    o = Oracle(n_vp=1, seed=143202461)
+    g = o.create("iaf_psc_delta", 9)
+    w = ("normal", 2.0, 0.5, 1)
+    o.connect(g[:3], g[3:6], "one_to_one", weight=w, delay=1.0)
+    o.connect(g[:3], g[6:9], "one_to_one", weight=w, delay=1.0)
+    s, t, wv, d = o.connections()
+    assert wv.tolist() == _polar_normals(_Stream(SEED_VP0), 6, 2.0, 0.5)
+    # two objects: the second call starts a pair of its own and discards nothing on the stream
+    o2 = Oracle(n_vp=1, seed=143202461)
+    g = o2.create("iaf_psc_delta", 9)
+    o2.connect(g[:3], g[3:6], "one_to_one", weight=("normal", 2.0, 0.5, 1), delay=1.0)
+    o2.connect(g[:3], g[6:9], "one_to_one", weight=("normal", 2.0, 0.5, 2), delay=1.0)
+    s, t, wv2, d = o2.connections()
+    st = _Stream(SEED_VP0)
+    first = _polar_normals(st, 4, 2.0, 0.5)[:3]
+    assert wv2.tolist()[:3] == first and wv2.tolist()[3:] == _polar_normals(st, 3, 2.0, 0.5)
+    assert wv2.tolist() != wv.tolist()
