@@ -1,19 +1,25 @@
-"""Throughput of the MI355X simulation step on the exponential-synapse random network.
+"""Throughput of the MI355X simulation step (one GPU).
 
-Workload (one GPU): 100,000 iaf_psc_exp neurons (tau_m 10 ms, tau_syn 0.5 ms, C_m 250 pF,
-V_th -55 mV, V_reset -70 mV, t_ref 2 ms), 80 % excitatory, fixed_indegree 1000 E + 250 I
-(1.25e8 static synapses), weights normal(87.8, 8.8) pA and -4x for inhibitory sources,
-delays uniform over the integer steps 1.0..3.0 ms, seeded Poisson drive of 8 kHz per
-neuron through one spike_generator each (weight 87.8 pA, delay 1.0 ms), dt = 0.1 ms, FP64.
+Default workload: the 8-population layered circuit -- 77,169 iaf_psc_exp neurons (tau_m 10 ms,
+tau_syn 0.5 ms, C_m 250 pF, V_th -55 mV, V_reset -70 mV, t_ref 2 ms) in populations of 20683,
+5834, 21915, 5479, 4850, 1065, 14395 and 2948 (excitatory and inhibitory alternating),
+pairwise_bernoulli connections with an 8x8 probability matrix drawn from the seed (uniform in
+[0, 0.14], about 3.2e8 synapses), weights normal(87.8, 8.78) pA and -4x from inhibitory
+populations, delays normal(1.5, 0.75) ms (excitatory) and normal(0.75, 0.375) ms (inhibitory)
+clipped at 0.1 ms, so min_delay is one step; seeded Poisson background of 8 Hz x K_ext per
+neuron with K_ext drawn from [1500, 2500] per population; dt = 0.1 ms, FP64.
 
-A step is Run(1000 ms) -- one biological second, 1,000 min_delay slices -- continuing the
-same network. value = recurrent synaptic events delivered per wall-clock second over the
-timed steps; RTF (wall / biological time) is reported beside it.
+A step is Run(1000 ms) -- one biological second, 10,000 one-step slices -- continuing the same
+network. value = recurrent synaptic events delivered per wall-clock second over the timed
+steps; RTF (wall / biological time) is reported beside it. The exponential-synapse random
+network (100,000 neurons, fixed_indegree 1000 E + 250 I, delays 1-3 ms) is measured after it
+and reported under "c2". --workload c4 / c5 run the batched sweep and the served stream.
 
-The roofline pass re-runs part of a step with per-kernel HIP events on the simulation
-stream and divides each kernel's algorithmic bytes by its average launch duration.
-The CPU baseline times the oracle (C++/OpenMP restatement of the reference algorithm)
-on the same network for a short biological window on this host's cores.
+The roofline pass re-runs part of a step with HIP events around every slice launch on the
+simulation stream and divides the algorithmic bytes by the average launch duration; the
+workgroups' own clock splits the launch into delivery and the rest. The CPU baseline times
+the oracle (C++/OpenMP restatement of the reference algorithm) on the same network, after its
+start-up transient, on this host's cores.
 """
 
 import argparse
@@ -52,35 +58,75 @@ def build_c2(k, nestgpu, N, KE, KI, n_vp, seed, drive_seed, rate_hz=8000.0, t_st
 
 
 C3_POPS = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]  # E/I alternating
+C3_W, C3_W_SD = 87.8, 8.78  # pA
+C3_DELAY = ((1.5, 0.75), (0.75, 0.375))  # ms, (mean, sd) of excitatory and inhibitory sources
+C3_DRIVE_W, C3_DRIVE_D = 87.8, 0.1
 
 
-def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9):
-    """8-population layered circuit: 77,169 iaf_psc_exp, pairwise_bernoulli with an 8x8
+def c3_description(drive_seed, scale=1.0):
+    """Draws of the 8-population circuit, in the order the builders use them: initial V_m of
+    each population, the 8x8 connection probabilities (uniform in [0, 0.14]; prob[i, j] from
+    population j to i) and the background in-degrees K_ext (uniform integers in [1500, 2500]).
+    scale shrinks every population (at least one neuron) and keeps the probabilities."""
+    rng = np.random.default_rng(drive_seed)
+    sizes = [max(1, int(round(n * scale))) for n in C3_POPS]
+    v0 = [rng.uniform(-70.0, -55.0, n) for n in sizes]
+    prob = rng.uniform(0.0, 0.14, size=(8, 8))
+    k_ext = rng.integers(1500, 2501, size=8)
+    return dict(sizes=sizes, v0=v0, prob=prob, k_ext=k_ext, drive_seed=drive_seed)
+
+
+def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9, scale=1.0):
+    """8-population layered circuit: 77,169 iaf_psc_exp (scale 1), pairwise_bernoulli with an 8x8
     probability matrix drawn from the seed (uniform in [0, 0.14]), weights normal(87.8, 10 %)
     and x -4 from inhibitory populations, delays normal (E 1.5 ms, I 0.75 ms, sd 50 %)
     clipped at dt, background Poisson 8 Hz x K_ext with K_ext uniform in [1500, 2500] per
     population. Returns (populations, total neurons, expected drive events per step)."""
-    rng = np.random.default_rng(drive_seed)
+    c = c3_description(drive_seed, scale)
     k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": seed})
-    pops = [k.Create("iaf_psc_exp", n, NEURON) for n in C3_POPS]
-    for p in pops:
-        p.set(V_m=rng.uniform(-70.0, -55.0, len(p)))
-    prob = rng.uniform(0.0, 0.14, size=(8, 8))
+    pops = [k.Create("iaf_psc_exp", n, NEURON) for n in c["sizes"]]
+    for p, v in zip(pops, c["v0"]):
+        p.set(V_m=v)
     for j, pre in enumerate(pops):
         exc = j % 2 == 0
-        w = nestgpu.random.normal(87.8, 8.78) if exc else -4.0 * nestgpu.random.normal(87.8, 8.78)
-        d = nestgpu.math.max(nestgpu.random.normal(1.5, 0.75) if exc else nestgpu.random.normal(0.75, 0.375), 0.1)
+        w = nestgpu.random.normal(C3_W, C3_W_SD) if exc else -4.0 * nestgpu.random.normal(C3_W, C3_W_SD)
+        d = nestgpu.math.max(nestgpu.random.normal(*C3_DELAY[0 if exc else 1]), 0.1)
         for i, post in enumerate(pops):
-            k.Connect(pre, post, {"rule": "pairwise_bernoulli", "p": float(prob[i, j])}, {"weight": w, "delay": d})
-    k_ext = rng.integers(1500, 2501, size=8)
+            k.Connect(pre, post, {"rule": "pairwise_bernoulli", "p": float(c["prob"][i, j])}, {"weight": w, "delay": d})
     lam = 0.0
     for i, post in enumerate(pops):
-        rate = 8.0 * float(k_ext[i])
+        rate = 8.0 * float(c["k_ext"][i])
         sg = k.Create("spike_generator", len(post), {"poisson_rate": rate, "poisson_seed": drive_seed + i,
                                                      "poisson_stop": t_stop})
-        k.Connect(sg, post, "one_to_one", {"weight": 87.8, "delay": 0.1})
+        k.Connect(sg, post, "one_to_one", {"weight": C3_DRIVE_W, "delay": C3_DRIVE_D})
         lam += len(post) * (1.0 - np.exp(-rate * 0.1e-3))
-    return pops, sum(C3_POPS), lam
+    return pops, sum(c["sizes"]), lam
+
+
+def build_c3_oracle(o, nestgpu, drive_seed, n_steps, scale=1.0):
+    """The same circuit on the oracle (connections from the same per-VP streams; the
+    parameter objects that several Connect calls share carry their normal-draw state, as the
+    library's do), with the library's seeded Poisson counts of every generator as dense drive
+    for emission steps [0, n_steps). Returns the populations' node ids."""
+    c = c3_description(drive_seed, scale)
+    pops = [o.create("iaf_psc_exp", n, NEURON) for n in c["sizes"]]
+    for p, v in zip(pops, c["v0"]):
+        o.set_values(p, "V_m", v)
+    for j, pre in enumerate(pops):
+        exc = j % 2 == 0
+        wn = ("normal", C3_W, C3_W_SD, 2 * j)
+        w = wn if exc else ("product", wn, -4.0)
+        d = ("max", ("normal",) + C3_DELAY[0 if exc else 1] + (2 * j + 1,), 0.1)
+        for i, post in enumerate(pops):
+            o.connect(pre, post, "pairwise_bernoulli", float(c["prob"][i, j]), weight=w, delay=d)
+    n_total = sum(c["sizes"])
+    gen0 = n_total + 1  # generators are created after every population, population by population
+    for i, post in enumerate(pops):
+        lam = 8.0 * float(c["k_ext"][i]) * 0.1e-3
+        counts = nestgpu.poisson_counts(drive_seed + i, lam, gen0, len(post), 0, n_steps)
+        o.add_drive(post[0], counts, C3_DRIVE_W, C3_DRIVE_D)
+        gen0 += len(post)
+    return pops
 
 
 AEIF_C4 = {"a": 4.0, "b": 80.5, "tau_w": 144.0, "Delta_T": 2.0, "g_L": 30.0, "gsl_error_tol": 1e-3}
@@ -101,33 +147,39 @@ def c4_specs(n=256, seed=4242, n_min=1000, n_max=50000):
     return specs
 
 
-def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True):
-    """One batched-sweep instance. iaf_psc_exp: J = 87.8 pA x min(1, sqrt(250 / K)), drive at
+def instance_params(sp):
+    """Model parameters, initial V_m range, synaptic weight J, drive weight and drive rate (Hz)
+    of one batched-sweep instance. iaf_psc_exp: J = 87.8 pA x min(1, sqrt(250 / K)), drive at
     eta x nu_th x C_E (nu_th from the PSC amplitude); aeif_cond_exp: J = 6 nS x sqrt(0.8 / K),
-    drive eta x 10 kHz of 6 nS. Inhibitory weights -g J. Returns the population."""
+    drive eta x 10 kHz of 6 nS. Inhibitory weights are -g J."""
+    K = sp["K"]
+    if sp["model"] == "iaf_psc_exp":
+        J = 87.8 * min(1.0, np.sqrt(250.0 / K))
+        return dict(params=NEURON, v_range=(-70.0, -55.0), J=J, w_drive=J,
+                    rate_hz=sp["eta"] * 15.0 * 250.0 / (J * 0.5 * 10.0) * 1000.0)
+    return dict(params=AEIF_C4, v_range=(-70.6, -55.0), J=6.0 * np.sqrt(0.8 / K), w_drive=6.0,
+                rate_hz=sp["eta"] * 10000.0)
+
+
+def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True):
+    """One batched-sweep instance: N neurons (80 % excitatory), fixed_indegree 0.8 K from the
+    excitatory and 0.2 K from the inhibitory neurons, delay 1.5 ms, seeded Poisson drive through
+    one spike_generator per neuron (instance_params). Returns the population."""
     k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": sp["seed"],
                        "keep_host_connections": keep_host_connections})
     N, K = sp["N"], sp["K"]
     KE = int(0.8 * K)
     KI = K - KE
     NE = int(0.8 * N)
+    ip = instance_params(sp)
     rng = np.random.default_rng(sp["drive_seed"])
-    if sp["model"] == "iaf_psc_exp":
-        pop = k.Create("iaf_psc_exp", N, NEURON)
-        pop.set(V_m=rng.uniform(-70.0, -55.0, N))
-        J = 87.8 * min(1.0, np.sqrt(250.0 / K))
-        w_drive = J
-        rate_hz = sp["eta"] * 15.0 * 250.0 / (J * 0.5 * 10.0) * 1000.0
-    else:
-        pop = k.Create("aeif_cond_exp", N, AEIF_C4)
-        pop.set(V_m=rng.uniform(-70.6, -55.0, N))
-        J = 6.0 * np.sqrt(0.8 / K)
-        w_drive = 6.0
-        rate_hz = sp["eta"] * 10000.0
+    pop = k.Create(sp["model"], N, ip["params"])
+    pop.set(V_m=rng.uniform(*ip["v_range"], N))
+    J = ip["J"]
     k.Connect(pop[:NE], pop, {"rule": "fixed_indegree", "indegree": KE}, {"weight": J, "delay": 1.5})
     k.Connect(pop[NE:], pop, {"rule": "fixed_indegree", "indegree": KI}, {"weight": -sp["g"] * J, "delay": 1.5})
-    sg = k.Create("spike_generator", N, {"poisson_rate": rate_hz, "poisson_seed": sp["drive_seed"]})
-    k.Connect(sg, pop, "one_to_one", {"weight": w_drive, "delay": 1.5})
+    sg = k.Create("spike_generator", N, {"poisson_rate": ip["rate_hz"], "poisson_seed": sp["drive_seed"]})
+    k.Connect(sg, pop, "one_to_one", {"weight": ip["w_drive"], "delay": 1.5})
     return pop
 
 
@@ -330,67 +382,222 @@ def run_served(args, nestgpu):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(N, KE, KI, n_vp, seed, drive_seed, t_sample, threads):
+def _poisson_counts_c2(nestgpu, N, drive_seed, steps):
+    # the generators are created after the neurons: their node ids are N+1 .. 2N
+    return nestgpu.poisson_counts(drive_seed, 8000.0 * 0.1 * 1e-3, N + 1, N, 0, steps)
+
+
+def cpu_baseline(args, workload, threads):
+    """The oracle (C++/OpenMP restatement of the reference algorithm, test infrastructure) on
+    the same network with identical connections and drive, on this host's cores: the start-up
+    transient is simulated first, untimed, then a sample of biological time is timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import nestgpu
     from _oracle import Oracle
 
-    NE = int(0.8 * N)
-    o = Oracle(n_vp=n_vp, seed=seed, threads=threads)
-    g = o.create("iaf_psc_exp", N, NEURON)
-    rng = np.random.default_rng(drive_seed)
-    o.set_values(g, "V_m", rng.uniform(-70.0, -55.0, N))
-    delay = ("sum", ("product", ("uniform_int", 21), 0.1), 1.0)
-    o.connect(g[:NE], g, "fixed_indegree", KE, weight=("normal", 87.8, 8.8), delay=delay)
-    o.connect(g[NE:], g, "fixed_indegree", KI, weight=("product", ("normal", 87.8, 8.8), -4.0), delay=delay)
-    steps = int(round(t_sample / 0.1))
-    # the generators are created after the neurons: their node ids are N+1 .. 2N
-    counts = nestgpu.poisson_counts(drive_seed, 8000.0 * 0.1 * 1e-3, N + 1, N, 0, steps)
-    o.add_drive(g[0], counts, 87.8, 1.0)
+    t_warm, t_sample = args.cpu_transient_ms, args.cpu_sample_ms
+    steps = int(round((t_warm + t_sample) / 0.1))
+    o = Oracle(n_vp=args.n_vp, seed=args.seed, threads=threads)
+    t0 = time.perf_counter()
+    if workload == "c2":
+        N, KE, KI = args.n, args.ke, args.ki
+        NE = int(0.8 * N)
+        g = o.create("iaf_psc_exp", N, NEURON)
+        rng = np.random.default_rng(args.drive_seed)
+        o.set_values(g, "V_m", rng.uniform(-70.0, -55.0, N))
+        delay = ("sum", ("product", ("uniform_int", 21), 0.1), 1.0)
+        o.connect(g[:NE], g, "fixed_indegree", KE, weight=("normal", 87.8, 8.8), delay=delay)
+        o.connect(g[NE:], g, "fixed_indegree", KI, weight=("product", ("normal", 87.8, 8.8), -4.0), delay=delay)
+        o.add_drive(g[0], _poisson_counts_c2(nestgpu, N, args.drive_seed, steps), 87.8, 1.0)
+    else:
+        build_c3_oracle(o, nestgpu, args.drive_seed, steps)
     o.run_steps(0)  # prepare (store build) outside the timed window
+    setup_s = time.perf_counter() - t0
+    o.simulate(t_warm)
+    ev0 = o.synaptic_events()
     t0 = time.perf_counter()
     o.simulate(t_sample)
     wall = time.perf_counter() - t0
-    ev = o.synaptic_events()
+    ev = o.synaptic_events() - ev0
     return {"value": ev / wall, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-            "sample": "same network (oracle connectivity, identical draws), first %.0f ms of biological time; "
-                      "%d recurrent events in %.2f s wall, RTF %.1f" % (t_sample, ev, wall, wall / (t_sample * 1e-3))}
+            "sample": "same network (oracle connectivity from the same per-VP streams, identical drive), %d VPs; "
+                      "%.0f ms of biological time after an untimed %.0f ms start-up transient: %d recurrent events in "
+                      "%.2f s wall, RTF %.1f (oracle setup %.0f s, untimed)"
+                      % (args.n_vp, t_sample, t_warm, ev, wall, wall / (t_sample * 1e-3), setup_s)}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_slice_c2_hbm_traffic.txt")
-
-
-def measured_traffic():
-    """HBM bytes per slice launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+def measured_traffic(workload):
+    """HBM bytes per slice launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload, with the library source digest it was measured on; None when the built
+    library's sources differ (the figure would describe another build)."""
+    path = os.path.join(ROOT, "profiles", "k_slice_%s_hbm_traffic.txt" % workload)
+    vals = {}
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+        for line in open(path):
+            if ":" in line:
+                key, val = line.split(":", 1)
+                vals[key.strip()] = val.strip()
     except OSError:
-        pass
-    return None
+        return None, None
+    if "traffic_bytes_per_launch" not in vals:
+        return None, None
+    digest = vals.get("source_digest")
+    return (float(vals["traffic_bytes_per_launch"]) if digest == source_digest() else None), digest
+
+
+def source_digest():
+    """sha256 (16 hex digits) over the library's kernel and host sources, in name order."""
+    import hashlib
+
+    h = hashlib.sha256()
+    cdir = os.path.join(ROOT, "nest-simulator_amd", "csrc")
+    for name in sorted(os.listdir(cdir)):
+        with open(os.path.join(cdir, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
+    """One plain-path measurement of the exponential-synapse random network (c2) or the
+    8-population circuit (c3). Returns the bench line (without the outer keys)."""
+    t_build = time.perf_counter()
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block,
+                       "workgroups_per_cu": args.workgroups_per_cu, "window_slots": args.window_slots,
+                       "keep_host_connections": False, "graph_launch": bool(args.graph_launch)})
+    if workload == "c2":
+        build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
+        n_neurons = args.n
+        drive_nz_per_step = args.n * (1.0 - np.exp(-8000.0 * 0.1e-3))
+    else:
+        _, n_neurons, drive_nz_per_step = build_c3(k, nestgpu, args.n_vp, args.seed, args.drive_seed)
+    n_syn = k.GetKernelStatus("num_connections") - n_neurons
+    k.Prepare()
+    build_s = time.perf_counter() - t_build
+    print("%s built: %d neurons, %d synapses in %.1f s" % (workload, n_neurons, n_syn, build_s), file=sys.stderr,
+          flush=True)
+
+    for i in range(warmup):
+        k.Run(args.t_step)
+        print("%s warmup %d done" % (workload, i), file=sys.stderr, flush=True)
+    st0 = k.GetKernelStatus()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        k.Run(args.t_step)  # Run synchronizes the device before it returns
+    wall = time.perf_counter() - t0
+    st = k.GetKernelStatus()
+    events = st["synaptic_events"] - st0["synaptic_events"]
+    spikes = st["local_spike_counter"] - st0["local_spike_counter"]
+    graph_chunks = st["graph_chunks"] - st0["graph_chunks"]
+    direct_chunks = st["direct_chunks"] - st0["direct_chunks"]
+    bio_s = steps * args.t_step * 1e-3
+    value = events / wall
+    rtf = wall / bio_s
+    rate = spikes / n_neurons / bio_s
+    print("%s timed: %d steps, %.3f s wall, RTF %.4f, %.3g events/s, rate %.2f Hz" % (
+        workload, steps, wall, rtf, value, rate), file=sys.stderr, flush=True)
+
+    # ---- roofline pass: HIP events around every slice launch on the simulation stream, and
+    # the workgroups' own clock for the delivery phase
+    k.SetKernelStatus({"kernel_timing": True, "phase_timing": True})
+    evr0 = k.GetKernelStatus("synaptic_events")
+    k.Run(args.roofline_ms)
+    st2 = k.GetKernelStatus()
+    k.SetKernelStatus({"kernel_timing": False, "phase_timing": False})
+    events_r = st2["synaptic_events"] - evr0
+    kms = st2["kernel_ms"]
+    kl = st2["kernel_launches"]
+    steps_r = int(round(args.roofline_ms / 0.1))
+    min_delay = st2["min_delay_steps"]
+    slices_r = steps_r // min_delay
+    # algorithmic bytes of the fused slice kernel, per unit:
+    #   28 B per recurrent synaptic event (12 B synapse record, 16 B ring-buffer cell RMW)
+    #   32 B per neuron-step (two ring-buffer channels read and cleared)
+    #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
+    #   16 B per nonzero drive count (ring-buffer RMW; counts are generated in the kernel)
+    nz = drive_nz_per_step * steps_r
+    launches = kl["slice"]
+    deliv_bytes = 28.0 * events_r
+    alg_total = deliv_bytes + 32.0 * n_neurons * steps_r + 56.0 * n_neurons * slices_r + 16.0 * nz
+    alg_per_launch = alg_total / launches
+    avg_us = 1e3 * kms["slice"] / launches
+    achieved = alg_per_launch / (avg_us * 1e-6) / 1e9
+    ticks = st2["phase_ticks"]
+    deliv_share = ticks["deliver"] / ticks["total"] if ticks["total"] else 0.0
+    deliv_us = avg_us * deliv_share
+    deliv_achieved = (deliv_bytes / launches) / (deliv_us * 1e-6) / 1e9 if deliv_us > 0 else 0.0
+    per_kernel = {n: {"avg_us": 1e3 * ms / kl[n], "launches": kl[n], "total_ms": ms}
+                  for n, ms in kms.items() if kl.get(n)}
+    print("%s slice kernel: %.1f us avg over %d launches (delivery %.1f us, %.0f %%), %.3g events in roofline pass"
+          % (workload, avg_us, launches, deliv_us, 100 * deliv_share, events_r), file=sys.stderr, flush=True)
+    layout = {"delivery": "window" if st2["window_delivery"] else "global",
+              "targets_per_block": st2["targets_per_block"], "window_slots": st2["window_slots"]}
+    del k
+    gc.collect()
+
+    cpu = None
+    if with_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args, workload, threads)
+        print("%s cpu baseline: %.3g events/s on %d threads" % (workload, cpu["value"], threads), file=sys.stderr,
+              flush=True)
+
+    if workload == "c2":
+        workload_text = ("exponential-synapse random network: %d iaf_psc_exp, fixed_indegree %d E + %d I (%d synapses), "
+                         "delays 1.0-3.0 ms, seeded 8 kHz Poisson drive, dt 0.1 ms, one step = Run(%g ms)"
+                         % (args.n, args.ke, args.ki, n_syn, args.t_step))
+    else:
+        workload_text = ("8-population layered circuit: 77169 iaf_psc_exp, pairwise_bernoulli (%d synapses), normal "
+                         "delays clipped at dt (min_delay one step), Poisson background 8 Hz x K_ext, dt 0.1 ms, one "
+                         "step = Run(%g ms)" % (n_syn, args.t_step))
+    default_layout = (args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workgroups_per_cu == 0
+                      and args.window_slots == 0)
+    traffic, traffic_digest = measured_traffic(workload) if default_layout else (None, None)
+    return {
+        "value": value, "unit": "synaptic events/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps, "rtf": rtf,
+        "config": dict({"workload": workload_text, "n_neurons": n_neurons, "n_synapses": n_syn, "n_vp": args.n_vp,
+                        "t_step_ms": args.t_step, "min_delay_steps": min_delay, "mean_rate_hz": rate,
+                        "build_s": build_s, "graph_chunks": graph_chunks, "direct_chunks": direct_chunks}, **layout),
+        "roofline": {"bound": "hbm", "kernel": "k_slice (fused deliver + update + gather, iaf_psc_exp)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_source_digest": traffic_digest, "source_digest": source_digest(),
+                     "avg_launch_us": avg_us, "alg_bytes_per_launch": alg_per_launch,
+                     "events_per_launch": events_r / launches, "kernels": per_kernel,
+                     "delivery": {"achieved": deliv_achieved, "frac": deliv_achieved / HBM_PEAK_GBS,
+                                  "avg_us": deliv_us, "share_of_slice": deliv_share,
+                                  "alg_bytes_per_launch": deliv_bytes / launches,
+                                  "method": "28 B per recurrent event / (slice launch time x the workgroups' mean "
+                                            "delivery-phase share of their run time, 100 MHz clock)"}},
+        "cpu_baseline": cpu,
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--t-step", type=float, default=1000.0, help="biological ms per step")
-    ap.add_argument("--n", type=int, default=100000)
-    ap.add_argument("--ke", type=int, default=1000)
-    ap.add_argument("--ki", type=int, default=250)
+    ap.add_argument("--n", type=int, default=100000, help="c2: neurons")
+    ap.add_argument("--ke", type=int, default=1000, help="c2: excitatory in-degree")
+    ap.add_argument("--ki", type=int, default=250, help="c2: inhibitory in-degree")
     ap.add_argument("--n-vp", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20241)
     ap.add_argument("--drive-seed", type=int, default=777)
     ap.add_argument("--roofline-ms", type=float, default=128.0,
                     help="biological ms of the per-launch timed pass (a multiple of 64 slices avoids a short chunk)")
-    ap.add_argument("--cpu-sample-ms", type=float, default=30.0)
+    ap.add_argument("--cpu-transient-ms", type=float, default=100.0,
+                    help="untimed biological ms of the CPU baseline before its sample")
+    ap.add_argument("--cpu-sample-ms", type=float, default=100.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2: 100k-neuron exponential-synapse random network (default); c3: 77k-neuron "
-                         "8-population circuit with min_delay one step; c4: batched sweep of many instances")
+    ap.add_argument("--no-c2", action="store_true", help="c3: skip the extra c2 measurement")
+    ap.add_argument("--c2-steps", type=int, default=5, help="c3: timed steps of the extra c2 measurement")
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"],
+                    help="c3: 77k-neuron 8-population circuit with min_delay one step (default); c2: 100k-neuron "
+                         "exponential-synapse random network; c4: batched sweep of many instances; c5: served "
+                         "request stream")
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
     ap.add_argument("--plain-ms", type=float, default=100.0, help="c4: biological ms per instance on the plain path")
     ap.add_argument("--plain-warm-ms", type=float, default=100.0,
@@ -408,6 +615,9 @@ def main():
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     ap.add_argument("--window-slots", type=int, default=0, help="cap on ring slots in LDS; 0: as many as fit")
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="0: as many as fit")
+    ap.add_argument("--graph-launch", type=int, default=1, choices=[0, 1],
+                    help="1: replay each chunk of slice launches from a captured hipGraph (default); 0: launch "
+                         "every slice directly")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -439,100 +649,16 @@ def main():
         (run_batched if args.workload == "c4" else run_served)(args, nestgpu)
         return
 
-    t_build = time.perf_counter()
-    k = nestgpu.Kernel(0)
-    k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block,
-                       "workgroups_per_cu": args.workgroups_per_cu, "window_slots": args.window_slots})
-    if args.workload == "c2":
-        pop, sg = build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
-        n_neurons = args.n
-        drive_nz_per_step = args.n * (1.0 - np.exp(-8000.0 * 0.1e-3))
-    else:
-        pops, n_neurons, drive_nz_per_step = build_c3(k, nestgpu, args.n_vp, args.seed, args.drive_seed)
-    n_syn = k.GetKernelStatus("num_connections") - n_neurons
-    k.Prepare()
-    build_s = time.perf_counter() - t_build
-    print("built: %d neurons, %d synapses in %.1f s" % (n_neurons, n_syn, build_s), file=sys.stderr, flush=True)
-
-    for i in range(args.warmup):
-        k.Run(args.t_step)
-        print("warmup %d done" % i, file=sys.stderr, flush=True)
-    ev0 = k.GetKernelStatus("synaptic_events")
-    sp0 = k.GetKernelStatus("local_spike_counter")
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        k.Run(args.t_step)  # Run synchronizes the device before it returns
-    wall = time.perf_counter() - t0
-    st = k.GetKernelStatus()
-    events = st["synaptic_events"] - ev0
-    spikes = st["local_spike_counter"] - sp0
-    bio_s = args.steps * args.t_step * 1e-3
-    value = events / wall
-    rtf = wall / bio_s
-    rate = spikes / n_neurons / bio_s
-    print("timed: %d steps, %.3f s wall, RTF %.4f, %.3g events/s, rate %.2f Hz" % (args.steps, wall, rtf, value, rate),
-          file=sys.stderr, flush=True)
-
-    # ---- roofline pass: per-kernel HIP events on the simulation stream
-    k.SetKernelStatus({"kernel_timing": True})
-    evr0 = k.GetKernelStatus("synaptic_events")
-    k.Run(args.roofline_ms)
-    k.SetKernelStatus({"kernel_timing": False})
-    st2 = k.GetKernelStatus()
-    events_r = st2["synaptic_events"] - evr0
-    kms = st2["kernel_ms"]
-    kl = st2["kernel_launches"]
-    steps_r = int(round(args.roofline_ms / 0.1))
-    min_delay = st2["min_delay_steps"]
-    slices_r = steps_r // min_delay
-    # algorithmic bytes of the fused slice kernel, per unit:
-    #   28 B per recurrent synaptic event (12 B synapse word + weight, 16 B ring-buffer RMW)
-    #   32 B per neuron-step (two ring-buffer channels read and cleared)
-    #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
-    #   16 B per nonzero drive count (ring-buffer RMW; counts are generated in the kernel)
-    nz = drive_nz_per_step * steps_r
-    launches = kl["slice"]
-    alg_total = 28.0 * events_r + 32.0 * n_neurons * steps_r + 56.0 * n_neurons * slices_r + 16.0 * nz
-    alg_per_launch = alg_total / launches
-    avg_us = 1e3 * kms["slice"] / launches
-    achieved = alg_per_launch / (avg_us * 1e-6) / 1e9
-    dominant = "k_slice (fused deliver + update + gather, iaf_psc_exp)"
-    per_kernel = {n: {"avg_us": 1e3 * ms / kl[n], "launches": kl[n], "total_ms": ms}
-                  for n, ms in kms.items() if kl.get(n)}
-    print("slice kernel: %.1f us avg over %d launches, %.3g events in roofline pass" % (avg_us, launches, events_r),
-          file=sys.stderr, flush=True)
-
-    cpu = None
-    if not args.no_cpu_baseline and args.workload == "c2":
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed, args.cpu_sample_ms, threads)
-        print("cpu baseline: %.3g events/s on %d threads" % (cpu["value"], threads), file=sys.stderr, flush=True)
-
-    if args.workload == "c2":
-        workload_text = ("exponential-synapse random network: %d iaf_psc_exp, fixed_indegree %d E + %d I (%d synapses), "
-                         "delays 1.0-3.0 ms, seeded 8 kHz Poisson drive, dt 0.1 ms, one step = Run(%g ms)"
-                         % (args.n, args.ke, args.ki, n_syn, args.t_step))
-    else:
-        workload_text = ("8-population layered circuit: 77169 iaf_psc_exp, pairwise_bernoulli (%d synapses), normal "
-                         "delays clipped at dt, Poisson background 8 Hz x K_ext, dt 0.1 ms, one step = Run(%g ms)"
-                         % (n_syn, args.t_step))
-    default_layout = (args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workgroups_per_cu == 0
-                      and args.window_slots == 0 and args.workload == "c2")
-    line = {
-        "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "rtf": rtf,
-        "config": {"workload": workload_text, "n_neurons": n_neurons, "n_synapses": n_syn, "n_vp": args.n_vp,
-                   "t_step_ms": args.t_step, "min_delay_steps": min_delay, "mean_rate_hz": rate, "build_s": build_s,
-                   "delivery": "window" if st2["window_delivery"] else "global",
-                   "targets_per_block": st2["targets_per_block"]},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic() if default_layout else None,
-                     "avg_launch_us": avg_us, "alg_bytes_per_launch": alg_per_launch,
-                     "events_per_launch": events_r / launches, "kernels": per_kernel},
-        "cpu_baseline": cpu,
-    }
+    res = run_plain(args, nestgpu, args.workload, args.steps, args.warmup, not args.no_cpu_baseline)
+    line = {"metric": METRIC, "value": res["value"], "unit": res["unit"], "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic", "rtf": res["rtf"], "config": res["config"],
+            "roofline": res["roofline"], "cpu_baseline": res["cpu_baseline"]}
+    if args.workload == "c3" and not args.no_c2:
+        c2 = run_plain(args, nestgpu, "c2", args.c2_steps, 1, False)
+        line["c2"] = {k: c2[k] for k in ("value", "unit", "steps", "ms_per_step", "rtf", "config")}
+        line["c2"]["roofline"] = {k: c2["roofline"][k] for k in ("achieved", "frac", "avg_launch_us", "traffic",
+                                                                 "delivery")}
     print(json.dumps(line), flush=True)
 
 

@@ -41,7 +41,8 @@ int nestgpu_set_kernel_status( nestgpu_kernel* k, const char* json );
 int nestgpu_get_kernel_status( nestgpu_kernel* k, char* buf, size_t cap, size_t* len );
 
 /* nest::create (nestkernel/nest.cpp:362): models iaf_psc_delta, iaf_psc_exp, aeif_cond_exp,
- * spike_generator, spike_recorder. *first_id receives the id of the first new node. */
+ * spike_generator, spike_recorder, multimeter, voltmeter. *first_id receives the id of the first
+ * new node. */
 int nestgpu_create( nestgpu_kernel* k, const char* model, int64_t n, const char* params_json, int64_t* first_id );
 /* nest::set_nc_status (nestkernel/nest.cpp:210) with one dictionary for every node (NodeCollection.set(dict)). */
 int nestgpu_set_status( nestgpu_kernel* k, const int64_t* ids, int64_t n, const char* params_json );
@@ -86,6 +87,18 @@ int nestgpu_simulate( nestgpu_kernel* k, double t_ms );
 int nestgpu_recorder_events( nestgpu_kernel* k, int64_t recorder_id, int64_t* senders, double* times, int64_t cap,
   int64_t* n );
 
+/* multimeter / voltmeter events (models/multimeter.cpp:190-216 over the UniversalDataLogger
+ * samples, nestkernel/universal_data_logger_impl.h:343-375), sorted by (time, sender):
+ * values[i * n_fields + j] is record_from[j] of event i, with n_fields the length of the
+ * multimeter's record_from list (nestgpu_node_status gives it). Call with senders == NULL
+ * to get the count in *n. */
+int nestgpu_multimeter_events( nestgpu_kernel* k, int64_t multimeter_id, int64_t* senders, double* times,
+  double* values, int64_t cap, int64_t* n );
+/* Status dictionary of one node as JSON (nest::get_nc_status, nestkernel/nest.cpp:169):
+ * model and global_id, and a multimeter's interval, offset and record_from. Same buffer
+ * convention as nestgpu_get_kernel_status. */
+int nestgpu_node_status( nestgpu_kernel* k, int64_t id, char* buf, size_t cap, size_t* len );
+
 /* Batched form: advances several prepared kernels on one device together, one launch per
  * phase per min_delay slice for all of them (no counterpart in the reference, which runs
  * one kernel per process). t_ms[i] is the simulation time of kernel i. */
@@ -119,6 +132,17 @@ int nestgpu_server_submit( nestgpu_server* s, nestgpu_kernel* k, const double* t
  * since the server started. events: recurrent synaptic events of the request. */
 int nestgpu_server_wait( nestgpu_server* s, int64_t id, double timeout_s, int* state, double* times,
   uint64_t* events );
+/* Open request (a session, as nest-server clients use the kernel): like nestgpu_server_submit,
+ * but the network stays resident in HBM after its calls are done, until
+ * nestgpu_server_close. nestgpu_server_extend appends a Simulate call of t_ms, which joins the
+ * running batch at the next chunk boundary and continues from the resident state;
+ * nestgpu_server_wait_calls waits until n calls are done (the kernel's state and events are
+ * readable between calls). */
+int nestgpu_server_submit_open( nestgpu_server* s, nestgpu_kernel* k, const double* t_ms, int n_calls, int64_t* id );
+int nestgpu_server_extend( nestgpu_server* s, int64_t id, double t_ms );
+int nestgpu_server_close( nestgpu_server* s, int64_t id );
+int nestgpu_server_wait_calls( nestgpu_server* s, int64_t id, int64_t n, double timeout_s, int* state,
+  int64_t* calls_done );
 /* Server statistics as JSON (same buffer convention as nestgpu_get_kernel_status). */
 int nestgpu_server_stats( nestgpu_server* s, char* buf, size_t cap, size_t* len );
 

@@ -529,6 +529,33 @@ Kernel::build_spec( BuildSpec& b )
     recording = recording or not rec.empty();
   b.log_enabled = recording;
   b.log_cap = recording ? std::max< uint64_t >( 1u << 20, static_cast< uint64_t >( N ) * 16 ) : 1024;
+  // multimeters: the union of their targets is sampled on the device, one column per neuron
+  // in node-id order, at the interval and offset the multimeters share
+  collect_samples();  // samples of the previous device copy use its columns
+  std::vector< int64_t > sampled;
+  int64_t mm_int = 0, mm_off = 0;
+  for ( const Meter& m : meters_ )
+  {
+    if ( m.targets.empty() or m.record_from.empty() )
+      continue;
+    if ( mm_int == 0 )
+    {
+      mm_int = m.interval;
+      mm_off = m.offset;
+    }
+    else if ( m.interval != mm_int or m.offset != mm_off )
+      throw std::runtime_error( "NotImplemented: multimeters with different interval or offset on one network" );
+    sampled.insert( sampled.end(), m.targets.begin(), m.targets.end() );
+  }
+  std::sort( sampled.begin(), sampled.end() );
+  sampled.erase( std::unique( sampled.begin(), sampled.end() ), sampled.end() );
+  mm_col_gid_ = sampled;
+  b.mm_col.assign( sampled.empty() ? 0 : N, 0xffffffffu );
+  for ( size_t c = 0; c < sampled.size(); ++c )
+    b.mm_col[ gid_to_dev_[ sampled[ c ] ] ] = static_cast< uint32_t >( c );
+  b.mm_cols = static_cast< uint32_t >( sampled.size() );
+  b.mm_int = sampled.empty() ? 0 : static_cast< int32_t >( mm_int );
+  b.mm_off = mm_off;
 }
 
 }  // namespace nestgpu

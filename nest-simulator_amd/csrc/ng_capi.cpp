@@ -242,6 +242,54 @@ nestgpu_recorder_events( nestgpu_kernel* h, int64_t recorder_id, int64_t* sender
 }
 
 int
+nestgpu_multimeter_events( nestgpu_kernel* h, int64_t multimeter_id, int64_t* senders, double* times, double* values,
+  int64_t cap, int64_t* n )
+{
+  return guarded( h,
+    [ & ]
+    {
+      std::vector< int64_t > s, st;
+      std::vector< double > v;
+      h->k->multimeter_events( multimeter_id, s, st, v );
+      const size_t nf = h->k->multimeter_record_from( multimeter_id ).size();
+      if ( n )
+        *n = static_cast< int64_t >( s.size() );
+      if ( senders )
+      {
+        const double ms = h->k->timebase().ms_per_tic * h->k->timebase().tics_per_step;
+        const int64_t m = std::min< int64_t >( cap, static_cast< int64_t >( s.size() ) );
+        for ( int64_t i = 0; i < m; ++i )
+        {
+          senders[ i ] = s[ i ];
+          if ( times )
+            times[ i ] = ms * static_cast< double >( st[ i ] );
+          if ( values )
+            for ( size_t j = 0; j < nf; ++j )
+              values[ i * nf + j ] = v[ i * nf + j ];
+        }
+      }
+    } );
+}
+
+int
+nestgpu_node_status( nestgpu_kernel* h, int64_t id, char* buf, size_t cap, size_t* len )
+{
+  return guarded( h,
+    [ & ]
+    {
+      const std::string js = h->k->node_status_json( id );
+      if ( len )
+        *len = js.size();
+      if ( buf and cap > 0 )
+      {
+        const size_t m = std::min( cap - 1, js.size() );
+        std::memcpy( buf, js.data(), m );
+        buf[ m ] = 0;
+      }
+    } );
+}
+
+int
 nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms )
 {
   try
@@ -418,6 +466,89 @@ nestgpu_server_submit( nestgpu_server* s, nestgpu_kernel* k, const double* t_ms,
       steps.push_back( k->k->steps_for( t_ms[ i ] ) );
     }
     *id = s->s->submit( k->k.get(), steps );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_submit_open( nestgpu_server* s, nestgpu_kernel* k, const double* t_ms, int n_calls, int64_t* id )
+{
+  try
+  {
+    if ( s == nullptr or k == nullptr or id == nullptr or ( n_calls > 0 and t_ms == nullptr ) )
+      throw std::runtime_error( "null argument" );
+    std::vector< int64_t > steps;
+    for ( int i = 0; i < n_calls; ++i )
+      steps.push_back( k->k->checked_steps( t_ms[ i ] ) );
+    *id = s->s->submit_open( k->k.get(), steps );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_extend( nestgpu_server* s, int64_t id, double t_ms )
+{
+  try
+  {
+    if ( s == nullptr )
+      throw std::runtime_error( "null server" );
+    if ( t_ms < 0 )
+      throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
+    // the kernel's time base converts the duration; the request's kernel is not touched here
+    const nestgpu::TimeBase& tb = s->s->timebase_of( id );
+    if ( tb.ms_to_tics( t_ms ) % tb.tics_per_step != 0 )
+      throw std::runtime_error( "BadParameter: The simulation time must be a multiple of the simulation resolution." );
+    s->s->extend( id, tb.ms_to_steps( t_ms ) );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_close( nestgpu_server* s, int64_t id )
+{
+  try
+  {
+    if ( s == nullptr )
+      throw std::runtime_error( "null server" );
+    s->s->close( id );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
+nestgpu_server_wait_calls( nestgpu_server* s, int64_t id, int64_t n, double timeout_s, int* state, int64_t* calls_done )
+{
+  try
+  {
+    if ( s == nullptr )
+      throw std::runtime_error( "null server" );
+    const nestgpu::RequestInfo r = s->s->wait_calls( id, n, timeout_s );
+    if ( state )
+      *state = r.state;
+    if ( calls_done )
+      *calls_done = r.calls_done;
+    if ( r.state == nestgpu::REQ_FAILED )
+      throw std::runtime_error( r.error );
     return 0;
   }
   catch ( const std::exception& e )

@@ -73,6 +73,7 @@ constexpr uint32_t MAX_TB = 512;      // targets per block = threads per workgro
 constexpr uint32_t PMAX_WIN = 512;    // blocks per instance, window delivery
 constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
 constexpr int MAX_PASSES = 3;         // slot-range passes of global delivery
+constexpr uint32_t MM_ROWS = 256;     // multimeter sample rows per chunk
 
 struct NetDev
 {
@@ -119,13 +120,24 @@ struct NetDev
   int64_t* log_step;
   unsigned long long* log_n;
   uint64_t log_cap;
-  unsigned long long* counters;  // [0] error flags, [1] synaptic events delivered, [2] spikes
+  unsigned long long* counters;  // [0] error flags, [1] synaptic events delivered, [2] spikes,
+                                 // [4..6] phase timing: delivery ticks, total ticks, workgroups
+  // ---- multimeter sampling (UniversalDataLogger::record_data at the end of every step):
+  // a sampled neuron's state after step lag is stored when its stamp T = origin + lag + 1 is
+  // one of mm_off + k * mm_int (k >= 0, T >= 1), in row (T - mm_off) / mm_int - mm_row0
+  const uint32_t* mm_col;  // per neuron: sample column, ~0 not sampled
+  double* mm_buf;          // [row][column][4]: the model's state fields 0..3
+  int64_t mm_off;
+  int64_t mm_row0;         // host-written before each chunk
+  int32_t mm_int;          // sampling interval in steps; 0: no multimeter
+  uint32_t mm_cols, mm_rows;
   // ---- time (host-written before each chunk)
   int64_t clock;   // slice origin at chunk start
   int32_t from0;   // from_step at chunk start
   int32_t part0;   // index of the run part that starts at from0 within its slice
   int32_t active;
   int32_t log_enabled;  // a spike_recorder is connected: log every spike
+  int32_t phase_timing; // workgroups add their delivery and total durations to counters[4..6]
   int64_t todo;    // steps to simulate in this chunk
 };
 
@@ -163,8 +175,28 @@ enum ErrorFlag : unsigned long long
   ERR_AEIF_DOUBLE_SPIKE = 2,
   ERR_SPIKE_LIST = 4,
   ERR_LOG = 8,
-  ERR_AEIF_STEPS = 16
+  ERR_AEIF_STEPS = 16,
+  ERR_SAMPLES = 32
 };
+
+// First step of [from, to) whose stamp origin + lag + 1 is a sampling stamp of the instance's
+// multimeters, or `to` when none is.
+__host__ __device__ inline int
+first_sample_lag( const NetDev& d, int64_t origin, int from, int to )
+{
+  if ( d.mm_int <= 0 )
+    return to;
+  const int64_t x = origin + from + 1 - d.mm_off;  // stamp of step `from` relative to the offset
+  int64_t l;
+  if ( x <= 0 )
+    l = from - x;  // the stamp mm_off itself
+  else
+  {
+    const int64_t r = x % d.mm_int;
+    l = from + ( r == 0 ? 0 : d.mm_int - r );
+  }
+  return l < to ? static_cast< int >( l ) : to;
+}
 
 // accepted adaptive steps of one aeif_cond_exp neuron within one time step before the run
 // fails (the GSL evolve loop of the reference has no such limit; a neuron that needs this

@@ -28,6 +28,7 @@ Engine::~Engine()
     free_list( store_allocs_ );
     free_list( state_allocs_ );
     free_list( log_allocs_ );
+    free_list( mm_allocs_ );
   }
 }
 
@@ -145,7 +146,7 @@ Engine::build( const BuildSpec& b, bool keep_state )
     net_.flat_meta = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 4 ) ) );
     net_.log_cap = 0;
     net_.log_n = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) ) ) );
-    net_.counters = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) * 4 ) ) );
+    net_.counters = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) * N_COUNTERS ) ) );
     clock = 0;
     from_step = 0;
     part = 0;
@@ -164,6 +165,29 @@ Engine::build( const BuildSpec& b, bool keep_state )
     net_.log_step = static_cast< int64_t* >( Lg( alloc( sizeof( int64_t ) * b.log_cap ) ) );
     unsigned long long z = 0;
     hip_check( hipMemcpy( net_.log_n, &z, sizeof z, hipMemcpyHostToDevice ), "log reset" );
+  }
+  // multimeter sampling: columns and a sample buffer for the rows of one chunk (at most
+  // MM_ROWS rows; the runner shortens chunks that would sample more)
+  free_list( mm_allocs_ );
+  net_.mm_int = 0;
+  net_.mm_col = nullptr;
+  net_.mm_buf = nullptr;
+  net_.mm_cols = net_.mm_rows = 0;
+  net_.mm_off = b.mm_off;
+  if ( b.mm_int > 0 and b.mm_cols > 0 )
+  {
+    if ( b.mm_col.size() != b.n )
+      throw std::runtime_error( "multimeter: one sample column entry per neuron expected" );
+    auto M = [ this ]( void* p )
+    {
+      mm_allocs_.push_back( p );
+      return p;
+    };
+    net_.mm_int = b.mm_int;
+    net_.mm_cols = b.mm_cols;
+    net_.mm_rows = MM_ROWS;
+    net_.mm_col = static_cast< const uint32_t* >( M( upload( b.mm_col ) ) );
+    net_.mm_buf = static_cast< double* >( M( alloc( sizeof( double ) * 4 * MM_ROWS * static_cast< size_t >( b.mm_cols ) ) ) );
   }
   net_.clock = clock;
   net_.from0 = from_step;
@@ -207,6 +231,20 @@ Engine::read_counter( int i ) const
 }
 
 void
+Engine::read_counters( unsigned long long* out ) const
+{
+  hip_check( hipMemcpy( out, net_.counters, sizeof( unsigned long long ) * N_COUNTERS, hipMemcpyDeviceToHost ),
+    "read counters" );
+}
+
+void
+Engine::clear_phase_ticks()
+{
+  const unsigned long long z[ 3 ] = { 0, 0, 0 };
+  hip_check( hipMemcpy( net_.counters + 4, z, sizeof z, hipMemcpyHostToDevice ), "clear phase ticks" );
+}
+
+void
 Engine::clear_error_flags()
 {
   unsigned long long z = 0;
@@ -233,6 +271,23 @@ Engine::drain_log()
   }
   unsigned long long z = 0;
   hip_check( hipMemcpy( net_.log_n, &z, sizeof z, hipMemcpyHostToDevice ), "log reset" );
+}
+
+void
+Engine::drain_samples()
+{
+  if ( mm_chunk_rows <= 0 )
+    return;
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  const size_t per_row = 4 * static_cast< size_t >( net_.mm_cols );
+  const size_t old = mm_data.size();
+  mm_data.resize( old + per_row * mm_chunk_rows );
+  hip_check( hipMemcpy( mm_data.data() + old, net_.mm_buf, per_row * mm_chunk_rows * sizeof( double ),
+               hipMemcpyDeviceToHost ),
+    "multimeter samples" );
+  for ( int64_t r = 0; r < mm_chunk_rows; ++r )
+    mm_stamps.push_back( net_.mm_off + ( mm_chunk_row0 + r ) * net_.mm_int );
+  mm_chunk_rows = 0;
 }
 
 // ------------------------------------------------------------------ Runner
@@ -382,34 +437,30 @@ Runner::run( const std::vector< Engine* >& engines, const std::vector< int64_t >
   }
 }
 
-bool
-Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >& remaining )
+// ---- chunk pieces shared by Runner::run_chunk and run_noalloc
+
+namespace
 {
-  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+// Launch records of one chunk: every instance's clock, first step and steps to do. Returns
+// false when no instance has work; prefix, mask, win and split describe the launch.
+bool
+stage_chunk( const std::vector< Engine* >& engines, const std::vector< int64_t >& remaining, int S, NetDev* h_nets,
+  std::vector< uint32_t >& prefix, int& mask, bool& win, bool& split )
+{
   const int n = static_cast< int >( engines.size() );
-  if ( n == 0 )
-    return false;
-  ensure_capacity( n );
-  int mask = 0;
-  bool win = false;
-  std::vector< uint32_t > prefix( n + 1, 0 );
+  mask = 0;
+  win = false;
+  split = false;
+  prefix.assign( n + 1, 0 );
   for ( int i = 0; i < n; ++i )
   {
     mask |= engines[ i ]->model_mask();
     win = win or engines[ i ]->window();
     prefix[ i + 1 ] = prefix[ i ] + engines[ i ]->net().n_blocks;
   }
-  const uint32_t total = prefix[ n ];
-  if ( total == 0 )
+  if ( prefix[ n ] == 0 )
     return false;
-  // the window kernel runs instances with a window (at most PMAX_WIN blocks, checked at
-  // build) and without one (their block prefix lives in the unused window)
-  if ( prefix != prefix_ )
-  {
-    hip_check( hipMemcpy( d_prefix_, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice ), "prefix" );
-    prefix_ = prefix;
-  }
-  bool any = false, split = false;
+  bool any = false;
   for ( int i = 0; i < n; ++i )
   {
     Engine& E = *engines[ i ];
@@ -418,61 +469,74 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
     d.from0 = E.from_step;
     d.part0 = E.part;
     const int64_t span =
-      static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S_ - 1 ) * d.min_delay;
+      static_cast< int64_t >( d.min_delay - E.from_step ) + static_cast< int64_t >( S - 1 ) * d.min_delay;
     d.todo = std::min( remaining[ i ], span );
+    E.mm_chunk_rows = 0;
+    if ( d.mm_int > 0 and d.todo > 0 )
+    {
+      // sampling stamps off + k * int within the chunk's stamps [first, first + todo); a chunk
+      // holds at most mm_rows of them
+      const int64_t first = E.clock + E.from_step + 1;
+      int64_t s0 = first <= d.mm_off ? d.mm_off : first + ( d.mm_int - ( first - d.mm_off ) % d.mm_int ) % d.mm_int;
+      const int64_t last_allowed = s0 + static_cast< int64_t >( d.mm_rows - 1 ) * d.mm_int;
+      if ( first + d.todo - 1 > last_allowed )
+        d.todo = last_allowed - first + 1;
+      d.mm_row0 = ( s0 - d.mm_off ) / d.mm_int;
+      const int64_t last = first + d.todo - 1;
+      E.mm_chunk_row0 = d.mm_row0;
+      E.mm_chunk_rows = s0 > last ? 0 : ( last - s0 ) / d.mm_int + 1;
+    }
     d.active = d.todo > 0 ? 1 : 0;
     any = any or d.active;
     split = split or ( d.active and E.from_step > 0 );
     if ( d.part0 > 255 )
       throw std::runtime_error( "NotImplemented: a slice split into more than 256 Run parts" );
-    h_nets_[ i ] = d;
+    h_nets[ i ] = d;
   }
-  if ( not any )
-    return false;
-  const bool graph = use_graph and not time_kernels and not split and capture( n, total, mask, win );
-  ++( graph ? graph_chunks : direct_chunks );
-  hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
-  hip_check( hipEventRecord( ev_chunk_[ 0 ], stream_ ), "hipEventRecord" );
-  if ( not graph )
+  return any;
+}
+
+// Direct launches of a chunk's slices (no graph); launch(s) enqueues slice s. A slice that
+// completes a slice split across Run calls is followed by the receive-order rebuild.
+template < typename F >
+void
+launch_chunk_direct( const NetDev* h_nets, NetDev* d_nets, int n, int S, bool split, hipStream_t st, F&& launch )
+{
+  for ( int s = 0; s < S; ++s )
   {
-    for ( int s = 0; s < S_; ++s )
-    {
-      // launches where no instance has work are skipped (their workgroups would exit at once)
-      bool active = false;
-      for ( int i = 0; i < n and not active; ++i )
-        active = slice_phase( h_nets_[ i ], s ).active;
-      if ( not active )
-        break;
-      launch_one( n, total, mask, win, s, time_kernels ? ev_kernel_ : nullptr );
-      if ( s == 0 and split )
-        for ( int i = 0; i < n; ++i )
-        {
-          // the slice that continued a previous part completes here: restore NEST's order
-          const NetDev& d = h_nets_[ i ];
-          const SlicePhase ph = slice_phase( d, 0 );
-          if ( ph.active and d.from0 > 0 and ph.complete )
-            hip_check( launch_flatten( d_nets_ + i, ph.parity, d.part0 + 1, stream_ ), "launch_flatten" );
-        }
-    }
+    // launches where no instance has work are skipped (their workgroups would exit at once)
+    bool active = false;
+    for ( int i = 0; i < n and not active; ++i )
+      active = slice_phase( h_nets[ i ], s ).active;
+    if ( not active )
+      break;
+    launch( s );
+    if ( s == 0 and split )
+      for ( int i = 0; i < n; ++i )
+      {
+        // the slice that continued a previous part completes here: restore NEST's order
+        const NetDev& d = h_nets[ i ];
+        const SlicePhase ph = slice_phase( d, 0 );
+        if ( ph.active and d.from0 > 0 and ph.complete )
+          hip_check( launch_flatten( d_nets + i, ph.parity, d.part0 + 1, st ), "launch_flatten" );
+      }
   }
-  else
-    hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
-  hip_check( hipEventRecord( ev_chunk_[ 1 ], stream_ ), "hipEventRecord" );
-  // error flags and spike-log fill of every instance in one read-back
-  hip_check( launch_status( d_nets_, n, d_status_, stream_ ), "launch_status" );
-  hip_check( hipMemcpyAsync( h_status_, d_status_, 2 * n * sizeof( unsigned long long ), hipMemcpyDeviceToHost, stream_ ),
-    "status D2H" );
-  hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
-  float ms = 0;
-  hipEventElapsedTime( &ms, ev_chunk_[ 0 ], ev_chunk_[ 1 ] );
-  last_ms_ += ms;
+}
+
+// After the status read-back: raise device errors, advance every instance's clock and run
+// part, and drain spike logs that are half full.
+void
+finish_chunk( const std::vector< Engine* >& engines, const NetDev* h_nets, const unsigned long long* h_status,
+  std::vector< int64_t >& remaining )
+{
+  const int n = static_cast< int >( engines.size() );
   for ( int i = 0; i < n; ++i )
   {
     Engine& E = *engines[ i ];
-    const NetDev& d = h_nets_[ i ];
+    const NetDev& d = h_nets[ i ];
     if ( not d.active )
       continue;
-    const unsigned long long flags = h_status_[ 2 * i ];
+    const unsigned long long flags = h_status[ 2 * i ];
     if ( flags )
     {
       E.clear_error_flags();
@@ -488,6 +552,8 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
         why += " spike list capacity exceeded;";
       if ( flags & ERR_LOG )
         why += " spike log capacity exceeded;";
+      if ( flags & ERR_SAMPLES )
+        why += " multimeter sample buffer exceeded;";
       throw std::runtime_error( "device run failed:" + why );
     }
     const int64_t total_steps = E.from_step + d.todo;
@@ -502,9 +568,53 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
       E.part = 1;
     E.from_step = nf;
     remaining[ i ] -= d.todo;
-    if ( h_status_[ 2 * i + 1 ] > E.net().log_cap / 2 )
+    if ( h_status[ 2 * i + 1 ] > E.net().log_cap / 2 )
       E.drain_log();
+    E.drain_samples();
   }
+}
+}  // namespace
+
+bool
+Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >& remaining )
+{
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  const int n = static_cast< int >( engines.size() );
+  if ( n == 0 )
+    return false;
+  ensure_capacity( n );
+  int mask = 0;
+  bool win = false, split = false;
+  std::vector< uint32_t > prefix;
+  // the window kernel runs instances with a window (at most PMAX_WIN blocks, checked at
+  // build) and without one (their block prefix lives in the unused window)
+  if ( not stage_chunk( engines, remaining, S_, h_nets_, prefix, mask, win, split ) )
+    return false;
+  const uint32_t total = prefix[ n ];
+  if ( prefix != prefix_ )
+  {
+    hip_check( hipMemcpy( d_prefix_, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice ), "prefix" );
+    prefix_ = prefix;
+  }
+  const bool graph = use_graph and not time_kernels and not split and capture( n, total, mask, win );
+  ++( graph ? graph_chunks : direct_chunks );
+  hip_check( hipMemcpyAsync( d_nets_, h_nets_, n * sizeof( NetDev ), hipMemcpyHostToDevice, stream_ ), "nets H2D" );
+  hip_check( hipEventRecord( ev_chunk_[ 0 ], stream_ ), "hipEventRecord" );
+  if ( not graph )
+    launch_chunk_direct( h_nets_, d_nets_, n, S_, split, stream_,
+      [ & ]( int s ) { launch_one( n, total, mask, win, s, time_kernels ? ev_kernel_ : nullptr ); } );
+  else
+    hip_check( hipGraphLaunch( exec_, stream_ ), "hipGraphLaunch" );
+  hip_check( hipEventRecord( ev_chunk_[ 1 ], stream_ ), "hipEventRecord" );
+  // error flags and spike-log fill of every instance in one read-back
+  hip_check( launch_status( d_nets_, n, d_status_, stream_ ), "launch_status" );
+  hip_check( hipMemcpyAsync( h_status_, d_status_, 2 * n * sizeof( unsigned long long ), hipMemcpyDeviceToHost, stream_ ),
+    "status D2H" );
+  hip_check( hipStreamSynchronize( stream_ ), "hipStreamSynchronize" );
+  float ms = 0;
+  hipEventElapsedTime( &ms, ev_chunk_[ 0 ], ev_chunk_[ 1 ] );
+  last_ms_ += ms;
+  finish_chunk( engines, h_nets_, h_status_, remaining );
   return true;
 }
 

@@ -61,6 +61,8 @@ hipError_t launch_flatten( NetDev* net, int par, int n_parts, hipStream_t st );
 // out[2 i] = error flags, out[2 i + 1] = spike-log fill of instance i
 hipError_t launch_status( const NetDev* nets, int n_nets, unsigned long long* out, hipStream_t st );
 
+constexpr int N_COUNTERS = 8;  // words of NetDev::counters
+
 enum KernelKind
 {
   K_SLICE = 0,
@@ -99,6 +101,11 @@ struct BuildSpec
   std::vector< PoissonTable > ptables;
   uint64_t log_cap = 1 << 20;
   bool log_enabled = false;
+  // multimeter sampling: per neuron sample column (~0 none), interval and offset in steps
+  std::vector< uint32_t > mm_col;
+  uint32_t mm_cols = 0;
+  int32_t mm_int = 0;
+  int64_t mm_off = 0;
   uint32_t
   n_blocks() const
   {
@@ -126,6 +133,12 @@ public:
   void drain_log();
   std::vector< uint32_t > log_src;
   std::vector< int64_t > log_step;
+  // Multimeter samples of the last chunk: the rows its launches wrote are appended to
+  // mm_stamps (one stamp per row) and mm_data ([row][column][4]).
+  void drain_samples();
+  std::vector< int64_t > mm_stamps;
+  std::vector< double > mm_data;
+  int64_t mm_chunk_row0 = 0, mm_chunk_rows = 0;  // set when a chunk is staged
 
   NetDev&
   net()
@@ -163,7 +176,9 @@ public:
     return bytes_;
   }
   unsigned long long read_counter( int i ) const;
+  void read_counters( unsigned long long* out ) const;  // all N_COUNTERS words
   void clear_error_flags();
+  void clear_phase_ticks();
 
   // host bookkeeping of NEST's clock (SimulationManager::clock_, from_step_) and of the
   // run parts the current slice has been split into
@@ -181,7 +196,7 @@ private:
   bool built_ = false;
   int mask_ = 0;
   NetDev net_ {};
-  std::vector< void* > state_allocs_, store_allocs_, log_allocs_;
+  std::vector< void* > state_allocs_, store_allocs_, log_allocs_, mm_allocs_;
   uint64_t bytes_ = 0;
 };
 
@@ -248,5 +263,4 @@ private:
 };
 
 void hip_check( hipError_t e, const char* what );
-
 }  // namespace nestgpu

@@ -382,6 +382,24 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
   }
 }
 
+// Multimeter sample of one neuron after a step (UniversalDataLogger::record_data,
+// nestkernel/universal_data_logger_impl.h:343-375): the model's four state fields.
+__device__ inline void
+store_sample( const NetDev& d, uint32_t col, int64_t stamp, double f0, double f1, double f2, double f3 )
+{
+  const int64_t row = ( stamp - d.mm_off ) / d.mm_int - d.mm_row0;
+  if ( row < 0 or row >= static_cast< int64_t >( d.mm_rows ) )
+  {
+    g_or( &d.counters[ 0 ], ERR_SAMPLES );
+    return;
+  }
+  gptr< double > p = glob( d.mm_buf ) + ( static_cast< size_t >( row ) * d.mm_cols + col ) * 4;
+  p[ 0 ] = f0;
+  p[ 1 ] = f1;
+  p[ 2 ] = f2;
+  p[ 3 ] = f3;
+}
+
 // Phase timers of the slice kernel, compiled in only with -DNG_PROFILE: per phase, the sum
 // over workgroups of wall-clock ticks (read back with nestgpu_profile_read).
 #ifdef NG_PROFILE
@@ -436,6 +454,12 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   const SlicePhase ph = slice_phase( d, s );
   if ( not ph.active or b >= d.n_blocks )
     return;
+  // phase timing (counters[4..6]): thread 0 reads the 100 MHz wall clock at the start, after
+  // delivery and at the end of the workgroup's work
+  const bool ptime = d.phase_timing != 0;
+  unsigned long long tp0 = 0, tp1 = 0;
+  if ( ptime and threadIdx.x == 0 )
+    tp0 = wall_clock64();
   const uint32_t P = d.n_blocks;
   const uint32_t tb = d.tb;
   const int par = ph.parity;
@@ -772,6 +796,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       g_add( &d.counters[ 1 ], events );
   }
   __syncthreads();  // window staged and delivered
+  if ( ptime and t == 0 )
+    tp1 = wall_clock64();
 
   // ================================================================ 2. update
   // Threads [0, tb) update one neuron each. The spike_generator drive of the slice lands in
@@ -798,6 +824,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   if ( local < tb and dev < d.n )
   {
     gref< const ParamSet > ps = glob( d.psets )[ glob( d.pidx )[ dev ] ];
+    // multimeter: the next step of this part whose state is sampled (ph.to: none)
+    const uint32_t mcol = d.mm_int > 0 ? glob( d.mm_col )[ dev ] : NONE;
+    int next_sample = mcol != NONE ? first_sample_lag( d, ph.origin, ph.from, ph.to ) : ph.to;
     const int m = ( MASK == 1 ) ? M_IAF_PSC_DELTA
       : ( MASK == 2 )           ? M_IAF_PSC_EXP
       : ( MASK == 4 )           ? M_AEIF_COND_EXP
@@ -849,6 +878,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               y3 = V_reset;
               bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
             }
+            if ( lag == next_sample )
+            {
+              store_sample( d, mcol, ph.origin + lag + 1, y3, 0.0, 0.0, 0.0 );
+              next_sample += d.mm_int;
+            }
           }
       }
       glob( d.v[ 0 ] )[ dev ] = y3;
@@ -897,6 +931,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             }
             C.put( lag, 0, local, 0.0 );
             C.put( lag, 1, local, 0.0 );
+            if ( lag == next_sample )
+            {
+              store_sample( d, mcol, ph.origin + lag + 1, V, iex, iin, 0.0 );
+              next_sample += d.mm_int;
+            }
           }
       }
       glob( d.v[ 0 ] )[ dev ] = V;
@@ -984,6 +1023,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           y[ 2 ] += C.get( lag, 1, local );
           C.put( lag, 0, local, 0.0 );
           C.put( lag, 1, local, 0.0 );
+          if ( lag == next_sample )
+          {
+            store_sample( d, mcol, ph.origin + lag + 1, y[ 0 ], y[ 1 ], y[ 2 ], y[ 3 ] );
+            next_sample += d.mm_int;
+          }
           ++lag;
           tt = 0.0;
           guard = 0;
@@ -1058,6 +1102,17 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     }
   }
   NG_PHASE( 7 );
+  if ( ptime )
+  {
+    __syncthreads();
+    if ( t == 0 )
+    {
+      const unsigned long long tp2 = wall_clock64();
+      g_add( &d.counters[ 4 ], tp1 - tp0 );
+      g_add( &d.counters[ 5 ], tp2 - tp0 );
+      g_add( &d.counters[ 6 ], 1ull );
+    }
+  }
 }
 
 #ifdef NG_PROFILE

@@ -423,6 +423,10 @@ Kernel::reset()
   recorders_.clear();
   rec_window_.clear();
   rec_gid_.clear();
+  meters_.clear();
+  meter_gid_.clear();
+  mm_col_gid_.clear();
+  rec_events_.clear();
   conns_.assign( 1, ConnList() );
   dev_conns_.clear();
   d_min_seen_ = -1;
@@ -434,14 +438,14 @@ Kernel::reset()
   engine_.reset();
   gid_to_dev_.clear();
   dev_to_gid_.clear();
-  spike_sender_.clear();
-  spike_step_.clear();
+  rec_events_.clear();
   spike_count_ = 0;
   targets_per_block_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
   param_states_.normals.clear();
   graph_launch_ = true;
+  phase_timing_ = false;
   graph_chunks_ = direct_chunks_ = capture_failures_ = 0;
   window_slots_ = 0;
   delivery_passes_ = 0;
@@ -571,6 +575,18 @@ Kernel::set_kernel_status( const Json& d )
     }
     else if ( k == "graph_launch" )
       graph_launch_ = kv.second.number() != 0;
+    else if ( k == "phase_timing" )
+    {
+      // slice workgroups time their delivery phase and their whole work (100 MHz clock);
+      // switching it on clears the sums
+      phase_timing_ = kv.second.number() != 0;
+      if ( engine_ and engine_->built() )
+      {
+        engine_->net().phase_timing = phase_timing_ ? 1 : 0;
+        if ( phase_timing_ )
+          engine_->clear_phase_ticks();
+      }
+    }
     else if ( k == "rng_type" )
     {
       if ( kv.second.string() != "mt19937_64" )
@@ -631,7 +647,12 @@ Kernel::get_kernel_status_json() const
   o << "}, \"kernel_launches\": {";
   for ( int q = 0; q < N_KINDS; ++q )
     o << ( q ? ", " : "" ) << "\"" << kn[ q ] << "\": " << kernel_launches_[ q ];
-  o << "}, \"synaptic_events\": " << synaptic_events()
+  unsigned long long cn[ N_COUNTERS ] = {};
+  if ( engine_ and engine_->built() and not released_ )
+    engine_->read_counters( cn );
+  o << "}, \"phase_timing\": " << ( phase_timing_ ? "true" : "false" ) << ", \"phase_ticks\": {\"deliver\": " << cn[ 4 ]
+    << ", \"total\": " << cn[ 5 ] << ", \"workgroups\": " << cn[ 6 ] << ", \"clock_hz\": 100000000}";
+  o << ", \"synaptic_events\": " << synaptic_events()
     << ", \"device_bytes\": " << ( engine_ ? engine_->device_bytes() : 0 )
     << ", \"device_bytes_estimate\": " << estimate_device_bytes() << "}";
   return o.str();
@@ -661,6 +682,8 @@ Kernel::model_name( int64_t gid ) const
     return "spike_generator";
   if ( r.kind == 2 )
     return "spike_recorder";
+  if ( r.kind == 3 )
+    return meters_[ r.idx ].voltmeter ? "voltmeter" : "multimeter";
   static const char* names[] = { "iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp" };
   return names[ params_[ r.idx ].model ];
 }
@@ -924,7 +947,31 @@ Kernel::create( const std::string& model, int64_t n, const Json& params )
       nodes_.push_back( NodeRec { 2, static_cast< uint32_t >( recorders_.size() ) } );
       recorders_.emplace_back();
       rec_window_.emplace_back();
+      rec_events_.emplace_back();
       rec_gid_.push_back( first + i );
+    }
+    if ( params.kind == Json::OBJ and not params.obj.empty() )
+    {
+      std::vector< int64_t > g( n );
+      for ( int64_t i = 0; i < n; ++i )
+        g[ i ] = first + i;
+      set_status( g, params );
+    }
+  }
+  else if ( model == "multimeter" or model == "voltmeter" )
+  {
+    // multimeter::Parameters_ (models/multimeter.cpp:37-60): interval 1 ms, offset 0; a
+    // voltmeter records V_m (:227-233)
+    for ( int64_t i = 0; i < n; ++i )
+    {
+      nodes_.push_back( NodeRec { 3, static_cast< uint32_t >( meters_.size() ) } );
+      Meter m;
+      m.interval = tb_.ms_to_steps( 1.0 );
+      m.voltmeter = model == "voltmeter";
+      if ( m.voltmeter )
+        m.record_from = { "V_m" };
+      meters_.push_back( m );
+      meter_gid_.push_back( first + i );
     }
     if ( params.kind == Json::OBJ and not params.obj.empty() )
     {
@@ -1029,6 +1076,12 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
       gens_[ r.idx ] = std::move( G );
       store_dirty_ = true;
     }
+    else if ( r.kind == 3 )
+    {
+      Meter m = meters_[ r.idx ];
+      set_meter_status( m, d );
+      meters_[ r.idx ] = m;
+    }
     else
     {
       DeviceWindow w = rec_window_[ r.idx ];
@@ -1050,6 +1103,62 @@ Kernel::set_status( const std::vector< int64_t >& gids, const Json& d )
       rec_window_[ r.idx ] = w;
     }
   }
+}
+
+void
+Kernel::set_meter_status( Meter& m, const Json& d )
+{
+  // multimeter::Parameters_::set (models/multimeter.cpp:115-162), then the device window
+  // (nestkernel/device.cpp:83-118); a failed set changes nothing (the caller keeps a copy)
+  const bool fixed = d.has( "interval" ) or d.has( "offset" ) or d.has( "record_from" );
+  if ( not m.targets.empty() and fixed )
+    throw std::runtime_error( "BadProperty: The recording interval, the interval offset and the list of properties "
+                              "to record cannot be changed after the multimeter has been connected to nodes." );
+  for ( const auto& kv : d.obj )
+  {
+    const std::string& k = kv.first;
+    if ( k == "interval" )
+    {
+      const int64_t tics = tb_.ms_to_tics( kv.second.number() );
+      if ( tics < tb_.tics_per_step )
+        throw std::runtime_error( "BadProperty: The sampling interval must be at least as long as the simulation "
+                                  "resolution." );
+      if ( tics % tb_.tics_per_step != 0 )
+        throw std::runtime_error( "BadProperty: The sampling interval must be a multiple of the simulation "
+                                  "resolution." );
+      m.interval = tics / tb_.tics_per_step;
+    }
+    else if ( k == "offset" )
+    {
+      const double ms = kv.second.number();
+      if ( ms != 0 and tb_.ms_to_tics( ms ) < tb_.tics_per_step )
+        throw std::runtime_error( "BadProperty: The offset for the sampling interval must be at least as long as the "
+                                  "simulation resolution." );
+      m.offset = tb_.ms_to_steps( ms );  // Time::step( Time( Time::ms( offset ) ).get_steps() )
+    }
+    else if ( k == "record_from" )
+    {
+      if ( m.voltmeter )
+        throw std::runtime_error( "BadProperty: a voltmeter records V_m only" );
+      std::vector< std::string > rf;
+      if ( kv.second.kind != Json::ARR )
+        throw std::runtime_error( "BadProperty: record_from must be a list of names" );
+      for ( const Json& v : kv.second.arr )
+        rf.push_back( v.string() );
+      m.record_from = rf;
+    }
+    else if ( update_window( m.window, k, kv.second ) )
+      continue;
+    else if ( k == "record_to" )
+    {
+      if ( kv.second.string() != "memory" )
+        throw std::runtime_error( "BadProperty: multimeter records to memory only on the device path." );
+    }
+    else
+      throw std::runtime_error( "UnaccessedDictionaryEntry: multimeter parameter '" + k + "'" );
+  }
+  if ( not m.window.stop_inf and m.window.stop < m.window.start )
+    throw std::runtime_error( "BadProperty: stop >= start required." );
 }
 
 bool
@@ -1092,14 +1201,138 @@ Kernel::collect_log()
   engine_->drain_log();
   auto& src = engine_->log_src;
   auto& st = engine_->log_step;
-  for ( size_t i = 0; i < src.size(); ++i )
+  // spike_recorder::handle (models/spike_recorder.cpp:100-112): a recorder keeps the spikes of
+  // the senders it observes whose stamps lie in its window at this time
+  for ( size_t r = 0; r < recorders_.size(); ++r )
   {
-    spike_sender_.push_back( dev_to_gid_[ src[ i ] ] );
-    spike_step_.push_back( st[ i ] );
+    const auto& rec = recorders_[ r ];
+    if ( rec.empty() )
+      continue;
+    const DeviceWindow& w = rec_window_[ r ];
+    for ( size_t i = 0; i < src.size(); ++i )
+    {
+      const int64_t g = dev_to_gid_[ src[ i ] ];
+      if ( std::binary_search( rec.begin(), rec.end(), g ) and ( not w.set or w.active( st[ i ] ) ) )
+        rec_events_[ r ].emplace_back( st[ i ], g );
+    }
   }
   spike_count_ += src.size();
   src.clear();
   st.clear();
+  collect_samples();
+}
+
+void
+Kernel::collect_samples()
+{
+  if ( not engine_ or not engine_->built() )
+    return;
+  // multimeter::handle (models/multimeter.cpp:190-216): each multimeter keeps the samples of
+  // its targets whose stamps lie in its window at this time, the values in record_from order
+  auto& stamps = engine_->mm_stamps;
+  auto& data = engine_->mm_data;
+  const size_t cols = mm_col_gid_.size();
+  if ( stamps.empty() or cols == 0 )
+  {
+    stamps.clear();
+    data.clear();
+    return;
+  }
+  for ( Meter& m : meters_ )
+  {
+    if ( m.targets.empty() or m.record_from.empty() )
+      continue;
+    std::vector< int > fidx;
+    for ( const std::string& v : m.record_from )
+      fidx.push_back( v == "V_m" ? 0 : ( v == "I_syn_ex" or v == "g_ex" ) ? 1 : ( v == "I_syn_in" or v == "g_in" ) ? 2 : 3 );
+    std::vector< size_t > tcol;  // columns of the targets
+    for ( int64_t g : m.targets )
+    {
+      const auto it = std::lower_bound( mm_col_gid_.begin(), mm_col_gid_.end(), g );
+      if ( it != mm_col_gid_.end() and *it == g )
+        tcol.push_back( static_cast< size_t >( it - mm_col_gid_.begin() ) );
+    }
+    for ( size_t row = 0; row < stamps.size(); ++row )
+    {
+      const int64_t stamp = stamps[ row ];
+      if ( m.window.set and not m.window.active( stamp ) )
+        continue;
+      for ( size_t c : tcol )
+      {
+        const int64_t g = mm_col_gid_[ c ];
+        const NeuronParams& p = params_[ node( g ).idx ];
+        m.ev_gid.push_back( g );
+        m.ev_stamp.push_back( stamp );
+        for ( int f : fidx )
+        {
+          double v = data[ ( row * cols + c ) * 4 + f ];
+          if ( f == 0 and p.model != M_AEIF_COND_EXP )
+            v += p.E_L;  // the iaf models keep V_m relative to E_L
+          m.ev_vals.push_back( v );
+        }
+      }
+    }
+  }
+  stamps.clear();
+  data.clear();
+}
+
+const std::vector< std::string >&
+Kernel::multimeter_record_from( int64_t mm_gid ) const
+{
+  const NodeRec& r = node( mm_gid );
+  if ( r.kind != 3 )
+    throw std::runtime_error( "BadProperty: node is not a multimeter" );
+  return meters_[ r.idx ].record_from;
+}
+
+void
+Kernel::multimeter_events( int64_t mm_gid, std::vector< int64_t >& senders, std::vector< int64_t >& steps,
+  std::vector< double >& values )
+{
+  const NodeRec& r = node( mm_gid );
+  if ( r.kind != 3 )
+    throw std::runtime_error( "BadProperty: node is not a multimeter" );
+  collect_log();
+  const Meter& m = meters_[ r.idx ];
+  const size_t nf = m.record_from.size();
+  std::vector< size_t > order( m.ev_gid.size() );
+  for ( size_t i = 0; i < order.size(); ++i )
+    order[ i ] = i;
+  std::stable_sort( order.begin(), order.end(), [ &m ]( size_t a, size_t b ) {
+    return m.ev_stamp[ a ] != m.ev_stamp[ b ] ? m.ev_stamp[ a ] < m.ev_stamp[ b ] : m.ev_gid[ a ] < m.ev_gid[ b ];
+  } );
+  senders.resize( order.size() );
+  steps.resize( order.size() );
+  values.resize( order.size() * nf );
+  for ( size_t e = 0; e < order.size(); ++e )
+  {
+    const size_t i = order[ e ];
+    senders[ e ] = m.ev_gid[ i ];
+    steps[ e ] = m.ev_stamp[ i ];
+    for ( size_t j = 0; j < nf; ++j )
+      values[ e * nf + j ] = m.ev_vals[ i * nf + j ];
+  }
+}
+
+std::string
+Kernel::node_status_json( int64_t gid ) const
+{
+  const NodeRec& r = node( gid );
+  std::ostringstream o;
+  o.precision( 17 );
+  o << "{\"model\": \"" << model_name( gid ) << "\", \"global_id\": " << gid;
+  if ( r.kind == 3 )
+  {
+    const Meter& m = meters_[ r.idx ];
+    o << ", \"interval\": " << static_cast< double >( m.interval ) * tb_.ms_per_step
+      << ", \"offset\": " << static_cast< double >( m.offset ) * tb_.ms_per_step << ", \"record_from\": [";
+    for ( size_t j = 0; j < m.record_from.size(); ++j )
+      o << ( j ? ", " : "" ) << "\"" << m.record_from[ j ] << "\"";
+    o << "]";
+  }
+  o << "}";
+  return o.str();
 }
 
 void
@@ -1142,13 +1375,28 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
   for ( size_t i = 0; i < gids.size(); ++i )
   {
     const NodeRec& r = node( gids[ i ] );
-    if ( ( r.kind == 2 and key == "n_events" ) or ( r.kind != 0 and ( key == "origin" or key == "start" or key == "stop" ) ) )
+    if ( r.kind == 3 and ( key == "interval" or key == "offset" ) )
     {
-      const DeviceWindow& w = r.kind == 2 ? rec_window_[ r.idx ] : gens_[ r.idx ].window;
+      const Meter& m = meters_[ r.idx ];
+      out[ i ] = static_cast< double >( key == "interval" ? m.interval : m.offset ) * tb_.ms_per_step;
+      continue;
+    }
+    if ( ( ( r.kind == 2 or r.kind == 3 ) and key == "n_events" )
+         or ( r.kind != 0 and ( key == "origin" or key == "start" or key == "stop" ) ) )
+    {
+      const DeviceWindow& w = r.kind == 2 ? rec_window_[ r.idx ]
+        : r.kind == 3                     ? meters_[ r.idx ].window
+                                          : gens_[ r.idx ].window;
       if ( key == "n_events" )
       {
         std::vector< int64_t > senders, steps;
-        recorder_events( gids[ i ], senders, steps );
+        if ( r.kind == 2 )
+          recorder_events( gids[ i ], senders, steps );
+        else
+        {
+          std::vector< double > vals;
+          multimeter_events( gids[ i ], senders, steps, vals );
+        }
         out[ i ] = static_cast< double >( senders.size() );
       }
       else if ( key == "stop" and w.stop_inf )
@@ -1393,6 +1641,44 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       std::sort( v.begin(), v.end() );
       v.erase( std::unique( v.begin(), v.end() ), v.end() );
     }
+    return;
+  }
+  if ( pre_kind == 3 or post_kind == 3 )
+  {
+    // a multimeter samples the neurons it is connected to (multimeter -> neuron); each
+    // recordable must exist in the target model (UniversalDataLogger::connect_logging_device,
+    // nestkernel/universal_data_logger.h:280-295)
+    if ( post_kind == 3 or pre_kind != 3 or post_kind != 0 )
+      throw std::runtime_error( "IllegalConnection: a multimeter connects to neurons as their source" );
+    if ( rule != "all_to_all" and rule != "one_to_one" )
+      throw std::runtime_error( "IllegalConnection: multimeter connections use one_to_one or all_to_all" );
+    for ( size_t ti = 0; ti < post.size(); ++ti )
+    {
+      const int model = params_[ node( post[ ti ] ).idx ].model;
+      const std::vector< int64_t > meters = rule == "one_to_one" ? std::vector< int64_t > { pre[ ti ] } : pre;
+      for ( int64_t mg : meters )
+        for ( const std::string& v : meters_[ node( mg ).idx ].record_from )
+        {
+          const bool ok = v == "V_m"
+            or ( model == M_IAF_PSC_EXP and ( v == "I_syn_ex" or v == "I_syn_in" ) )
+            or ( model == M_AEIF_COND_EXP and ( v == "g_ex" or v == "g_in" or v == "w" ) );
+          if ( not ok )
+            throw std::runtime_error( "IllegalConnection: Cannot connect with unknown recordable " + v );
+        }
+    }
+    for ( size_t ti = 0; ti < post.size(); ++ti )
+    {
+      const std::vector< int64_t > meters = rule == "one_to_one" ? std::vector< int64_t > { pre[ ti ] } : pre;
+      for ( int64_t mg : meters )
+        meters_[ node( mg ).idx ].targets.push_back( post[ ti ] );
+    }
+    for ( int64_t mg : pre )
+    {
+      auto& v = meters_[ node( mg ).idx ].targets;
+      std::sort( v.begin(), v.end() );
+      v.erase( std::unique( v.begin(), v.end() ), v.end() );
+    }
+    store_dirty_ = true;
     return;
   }
   if ( pre_kind == 2 )
@@ -1796,6 +2082,7 @@ Kernel::prepare()
     BuildSpec b;
     build_spec( b );
     engine_->build( b, simulated_ );
+    engine_->net().phase_timing = phase_timing_ ? 1 : 0;
     store_dirty_ = false;
     state_dirty_ = false;
     if ( not keep_host_connections_ )
@@ -1821,8 +2108,14 @@ Kernel::reset_device()
   prepared_ = false;
   simulated_ = false;
   store_dirty_ = true;
-  spike_sender_.clear();
-  spike_step_.clear();
+  for ( auto& e : rec_events_ )
+    e.clear();
+  for ( Meter& m : meters_ )
+  {
+    m.ev_gid.clear();
+    m.ev_stamp.clear();
+    m.ev_vals.clear();
+  }
   spike_count_ = 0;
   last_kernel_ms_ = last_wall_ms_ = time_simulate_ms_ = 0;
   for ( int q = 0; q < N_KINDS; ++q )
@@ -1927,16 +2220,23 @@ Kernel::after_batch_run( int64_t )
   collect_log();
 }
 
+int64_t
+Kernel::checked_steps( double t_ms ) const
+{
+  // nest::run (nestkernel/nest.cpp:506-525)
+  if ( t_ms < 0 )
+    throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
+  if ( tb_.ms_to_tics( t_ms ) % tb_.tics_per_step != 0 )
+    throw std::runtime_error( "BadParameter: The simulation time must be a multiple of the simulation resolution." );
+  return tb_.ms_to_steps( t_ms );
+}
+
 void
 Kernel::run( double t_ms )
 {
   if ( not prepared_ )
     throw std::runtime_error( "KernelException: Run called without calling Prepare." );
-  if ( t_ms < 0 )
-    throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
-  const long steps = tb_.ms_to_steps( t_ms );
-  if ( tb_.ms_to_tics( t_ms ) % tb_.tics_per_step != 0 )
-    throw std::runtime_error( "BadParameter: The simulation time must be a multiple of the simulation resolution." );
+  const long steps = checked_steps( t_ms );
   simulated_ = true;
   if ( steps == 0 or params_.empty() )
     return;
@@ -2006,16 +2306,7 @@ Kernel::recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::
   if ( r.kind != 2 )
     throw std::runtime_error( "BadProperty: node is not a spike_recorder" );
   collect_log();
-  const auto& rec = recorders_[ r.idx ];
-  const DeviceWindow& w = rec_window_[ r.idx ];
-  std::vector< std::pair< int64_t, int64_t > > ev;
-  for ( size_t i = 0; i < spike_sender_.size(); ++i )
-    if ( std::binary_search( rec.begin(), rec.end(), spike_sender_[ i ] ) )
-    {
-      if ( w.set and not w.active( spike_step_[ i ] ) )
-        continue;
-      ev.emplace_back( spike_step_[ i ], spike_sender_[ i ] );
-    }
+  std::vector< std::pair< int64_t, int64_t > > ev = rec_events_[ r.idx ];
   std::sort( ev.begin(), ev.end() );
   senders.resize( ev.size() );
   steps.resize( ev.size() );

@@ -350,6 +350,13 @@ public:
 
   // spike_recorder events (senders, stamp steps), sorted by (time, sender)
   void recorder_events( int64_t rec_gid, std::vector< int64_t >& senders, std::vector< int64_t >& steps );
+  // multimeter events sorted by (time, sender): values[i * record_from.size() + j] is the
+  // value of record_from[j] of event i
+  void multimeter_events( int64_t mm_gid, std::vector< int64_t >& senders, std::vector< int64_t >& steps,
+    std::vector< double >& values );
+  const std::vector< std::string >& multimeter_record_from( int64_t mm_gid ) const;
+  // status dictionary of one node as JSON (model, global_id, and a device's settings)
+  std::string node_status_json( int64_t gid ) const;
 
   // batched / served use
   Engine& engine()
@@ -369,6 +376,8 @@ public:
     return device_;
   }
   void prepare_for_batch();  // prepare without running (idempotent)
+  // Steps of a Run of t_ms, with Run's checks (non-negative, a multiple of the resolution).
+  int64_t checked_steps( double t_ms ) const;
   // Discard the device copy and every simulation result, back to the state before the
   // first Prepare (used by the layout autotuner's probe runs).
   void reset_device();
@@ -445,7 +454,7 @@ public:
 private:
   struct NodeRec
   {
-    uint8_t kind;  // 0 neuron, 1 spike_generator, 2 spike_recorder
+    uint8_t kind;  // 0 neuron, 1 spike_generator, 2 spike_recorder, 3 multimeter
     uint32_t idx;
   };
   NodeRec& node( int64_t gid );
@@ -482,6 +491,24 @@ private:
   // origin, start or stop into w; false for any other key
   bool update_window( DeviceWindow& w, const std::string& k, const Json& v ) const;
   std::vector< int64_t > rec_gid_;
+  // multimeter (models/multimeter.cpp): the sampled neurons, the recordables in order, the
+  // sampling interval and offset in steps and the recording window
+  struct Meter
+  {
+    std::vector< int64_t > targets;  // sorted node ids
+    std::vector< std::string > record_from;
+    int64_t interval = 10, offset = 0;
+    DeviceWindow window;
+    bool voltmeter = false;
+    // recorded events: sender, stamp and the record_from values (row-major)
+    std::vector< int64_t > ev_gid, ev_stamp;
+    std::vector< double > ev_vals;
+  };
+  std::vector< Meter > meters_;
+  std::vector< int64_t > meter_gid_;
+  std::vector< int64_t > mm_col_gid_;  // node id of each sample column of the current device copy
+  void set_meter_status( Meter& m, const Json& d );
+  void collect_samples();
   std::vector< ConnList > conns_;  // per VP
   ParamStates param_states_;       // parameter objects used by more than one Connect call
   struct DevConn
@@ -499,7 +526,9 @@ private:
   Runner* runner_ = nullptr;
   std::vector< int32_t > gid_to_dev_;  // neuron gid -> device index (valid after prepare)
   std::vector< int64_t > dev_to_gid_;
-  std::vector< int64_t > spike_sender_, spike_step_;  // all spikes so far
+  // events of each spike_recorder (stamp, sender), recorded with the senders it observed and
+  // the window it had when the spikes were read back
+  std::vector< std::vector< std::pair< int64_t, int64_t > > > rec_events_;
   uint64_t spike_count_ = 0;
   double last_kernel_ms_ = 0, last_wall_ms_ = 0;
   int slices_per_chunk_ = 64;
@@ -507,6 +536,7 @@ private:
   int window_slots_ = 0;  // cap on the ring slots staged in LDS (0: as many as fit)
   int delivery_passes_ = 0;  // global delivery in this many slot-range passes (0, 1: one)
   bool time_kernels_ = false;
+  bool phase_timing_ = false;
   bool graph_launch_ = true;  // replay chunks from captured hipGraphs (false: direct launches)
   int64_t graph_chunks_ = 0, direct_chunks_ = 0, capture_failures_ = 0;
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };

@@ -43,6 +43,92 @@ Server::now() const
 int64_t
 Server::submit( Kernel* k, const std::vector< int64_t >& call_steps )
 {
+  return submit_req( k, call_steps, false );
+}
+
+int64_t
+Server::submit_open( Kernel* k, const std::vector< int64_t >& call_steps )
+{
+  return submit_req( k, call_steps, true );
+}
+
+void
+Server::extend( int64_t id, int64_t steps )
+{
+  if ( steps < 0 )
+    throw std::runtime_error( "BadParameter: The simulation time cannot be negative." );
+  std::lock_guard< std::mutex > g( mu_ );
+  auto it = reqs_.find( id );
+  if ( it == reqs_.end() )
+    throw std::runtime_error( "unknown request id" );
+  Req* r = it->second.get();
+  if ( not r->open or r->closing or r->info.state == REQ_DONE or r->info.state == REQ_FAILED )
+    throw std::runtime_error( "the request is not open" );
+  r->calls.push_back( steps );
+  if ( r->parked )
+  {
+    // rejoin the batch at the next chunk boundary with the new call (the first call of a
+    // request that was prepared without one, or the one after its last done call)
+    r->parked = false;
+    r->call = r->calls.size() == 1 ? 0 : r->call + 1;
+    r->remaining = r->calls[ r->call ];
+    ready_.push_back( r );
+  }
+  cv_.notify_all();
+}
+
+void
+Server::close( int64_t id )
+{
+  Req* r = nullptr;
+  {
+    std::lock_guard< std::mutex > g( mu_ );
+    auto it = reqs_.find( id );
+    if ( it == reqs_.end() )
+      throw std::runtime_error( "unknown request id" );
+    r = it->second.get();
+    if ( not r->open or r->closing )
+      return;
+    r->closing = true;
+    if ( not r->parked )
+    {
+      cv_.notify_all();
+      return;  // the run loop finishes it after its remaining calls
+    }
+    r->parked = false;
+  }
+  finish( r, "" );
+}
+
+const TimeBase&
+Server::timebase_of( int64_t id )
+{
+  std::lock_guard< std::mutex > g( mu_ );
+  auto it = reqs_.find( id );
+  if ( it == reqs_.end() )
+    throw std::runtime_error( "unknown request id" );
+  return it->second->k->timebase();
+}
+
+RequestInfo
+Server::wait_calls( int64_t id, int64_t n, double timeout_s )
+{
+  std::unique_lock< std::mutex > g( mu_ );
+  auto it = reqs_.find( id );
+  if ( it == reqs_.end() )
+    throw std::runtime_error( "unknown request id" );
+  Req* r = it->second.get();
+  auto done = [ r, n ] { return r->info.calls_done >= n or r->info.state == REQ_DONE or r->info.state == REQ_FAILED; };
+  if ( timeout_s < 0 )
+    cv_.wait( g, done );
+  else
+    cv_.wait_for( g, std::chrono::duration< double >( timeout_s ), done );
+  return r->info;
+}
+
+int64_t
+Server::submit_req( Kernel* k, const std::vector< int64_t >& call_steps, bool open )
+{
   if ( k == nullptr )
     throw std::runtime_error( "null kernel" );
   if ( k->prepared() or k->simulated() )
@@ -52,6 +138,7 @@ Server::submit( Kernel* k, const std::vector< int64_t >& call_steps )
   auto r = std::make_unique< Req >();
   r->k = k;
   r->calls = call_steps;
+  r->open = open;
   r->info.t_submit = now();
   r->info.bytes = k->estimate_device_bytes();
   if ( r->info.bytes > budget_ )
@@ -121,8 +208,17 @@ Server::admit_loop()
     else
     {
       r->call = 0;
-      r->remaining = r->calls.empty() ? 0 : r->calls[ 0 ];
-      ready_.push_back( r );
+      if ( r->calls.empty() and r->open )
+      {
+        r->info.state = REQ_RUNNING;
+        r->info.t_start = now();
+        r->parked = true;  // prepared, waiting for its first call
+      }
+      else
+      {
+        r->remaining = r->calls.empty() ? 0 : r->calls[ 0 ];
+        ready_.push_back( r );
+      }
     }
     cv_.notify_all();
   }
@@ -216,14 +312,36 @@ Server::run_loop()
       }
       r->remaining = remaining[ i ];
       // a Simulate call ends: the next one of the same request continues from its state
-      while ( r->remaining == 0 and r->call + 1 < r->calls.size() )
+      bool park = false;
       {
-        r->k->after_batch_run( 0 );
-        ++r->call;
-        r->remaining = r->calls[ r->call ];
+        std::lock_guard< std::mutex > g( mu_ );
+        while ( r->remaining == 0 and r->call + 1 < r->calls.size() )
+        {
+          r->k->after_batch_run( 0 );
+          ++r->info.calls_done;
+          ++r->call;
+          r->remaining = r->calls[ r->call ];
+        }
+        if ( r->remaining == 0 and r->open and not r->closing )
+        {
+          // an open request keeps its network resident and waits for its next call
+          r->k->after_batch_run( 0 );
+          ++r->info.calls_done;
+          r->parked = true;
+          park = true;
+          cv_.notify_all();
+        }
       }
+      if ( park )
+        continue;
       if ( r->remaining == 0 )
+      {
+        {
+          std::lock_guard< std::mutex > g( mu_ );
+          ++r->info.calls_done;
+        }
         finish( r, "" );
+      }
       else
         still.push_back( r );
     }

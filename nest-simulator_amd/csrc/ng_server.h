@@ -35,6 +35,7 @@ enum RequestState
 struct RequestInfo
 {
   int state = REQ_QUEUED;
+  int64_t calls_done = 0;  // Simulate calls completed
   double t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;  // seconds since the server started
   uint64_t bytes = 0;                                         // device bytes reserved
   uint64_t events = 0;                                        // recurrent synaptic events delivered
@@ -54,6 +55,18 @@ public:
   // k: a network built on the host (Create/Connect), not yet simulated; the caller must not
   // touch it until the request is done. call_steps: the Simulate calls, in steps.
   int64_t submit( Kernel* k, const std::vector< int64_t >& call_steps );
+  // An open request (a session): its network stays resident after its calls are done, until
+  // close(); extend() appends a Simulate call, which joins the batch at the next chunk
+  // boundary. Between calls the caller may read the kernel's state (no device work runs on
+  // it then).
+  int64_t submit_open( Kernel* k, const std::vector< int64_t >& call_steps );
+  void extend( int64_t id, int64_t steps );
+  void close( int64_t id );
+  // Waits up to timeout_s (negative: forever) until the request has completed n calls, or
+  // is done or failed; returns its info.
+  RequestInfo wait_calls( int64_t id, int64_t n, double timeout_s );
+  // time base of a request's kernel (fixed once the kernel is submitted)
+  const TimeBase& timebase_of( int64_t id );
   // Waits up to timeout_s (negative: forever) for the request to finish; returns its info.
   RequestInfo wait( int64_t id, double timeout_s );
   std::string stats_json();
@@ -67,6 +80,9 @@ private:
     size_t call = 0;
     int64_t remaining = 0;
     RequestInfo info;
+    bool open = false;    // stays resident between calls until closed
+    bool parked = false;  // open, on the device, all its calls done
+    bool closing = false;
   };
   double now() const;
   void admit_loop();
@@ -84,6 +100,7 @@ private:
   std::map< int64_t, std::unique_ptr< Req > > reqs_;
   std::deque< Req* > queue_;  // waiting for admission, FIFO
   std::vector< Req* > ready_;  // on the device, joining at the next chunk boundary
+  int64_t submit_req( Kernel* k, const std::vector< int64_t >& call_steps, bool open );
   // statistics
   uint64_t chunks_ = 0, events_done_ = 0;
   double busy_s_ = 0;

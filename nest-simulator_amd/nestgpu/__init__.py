@@ -50,6 +50,9 @@ _SIGS = {
     "nestgpu_cleanup": (ctypes.c_int, [_vp]),
     "nestgpu_simulate": (ctypes.c_int, [_vp, ctypes.c_double]),
     "nestgpu_recorder_events": (ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _f64p, ctypes.c_int64, _i64p]),
+    "nestgpu_multimeter_events": (ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _f64p, _f64p, ctypes.c_int64, _i64p]),
+    "nestgpu_node_status": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
     "nestgpu_batch_simulate": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
     "nestgpu_batch_autotune": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double,
                                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -58,6 +61,11 @@ _SIGS = {
     "nestgpu_server_submit": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, _i64p]),
     "nestgpu_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _f64p,
                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "nestgpu_server_submit_open": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, _i64p]),
+    "nestgpu_server_extend": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double]),
+    "nestgpu_server_close": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "nestgpu_server_wait_calls": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                                 ctypes.POINTER(ctypes.c_int), _i64p]),
     "nestgpu_server_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "nestgpu_poisson_counts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]),
@@ -244,7 +252,7 @@ def _to_json(d):
 
 
 # node parameters whose value is itself a list (not one value per node)
-_LIST_PARAMS = ("spike_times", "spike_multiplicities")
+_LIST_PARAMS = ("spike_times", "spike_multiplicities", "record_from")
 
 
 def _process_syn_spec(syn_spec, conn_spec, prelength, postlength):
@@ -320,6 +328,9 @@ class NodeCollection:
     def get(self, key, output=None):
         if key == "events":
             return self._k._events(self)
+        if key in ("record_from", "model"):
+            vals = [self._k._node_status(i)[key] for i in self._ids]
+            return vals[0] if len(vals) == 1 else vals
         if isinstance(key, (list, tuple)):
             return {k: self.get(k) for k in key}
         out = np.zeros(len(self._ids))
@@ -498,8 +509,29 @@ class Kernel:
     def Simulate(self, t):
         self._chk(_lib.nestgpu_simulate(self._h, float(t)))
 
+    def _node_status(self, node_id):
+        n = ctypes.c_size_t(0)
+        self._chk(_lib.nestgpu_node_status(self._h, int(node_id), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        self._chk(_lib.nestgpu_node_status(self._h, int(node_id), buf, n.value + 1, ctypes.byref(n)))
+        return json.loads(buf.value.decode())
+
     def _events(self, rec):
         rid = int(rec._ids[0])
+        st = self._node_status(rid)
+        if st["model"] in ("multimeter", "voltmeter"):
+            names = st["record_from"]
+            n = ctypes.c_int64(0)
+            self._chk(_lib.nestgpu_multimeter_events(self._h, rid, None, None, None, 0, ctypes.byref(n)))
+            s = np.zeros(n.value, np.int64)
+            tm = np.zeros(n.value)
+            vals = np.zeros((n.value, max(1, len(names))))
+            self._chk(_lib.nestgpu_multimeter_events(self._h, rid, _ptr(s, ctypes.c_int64), _ptr(tm, ctypes.c_double),
+                                                     _ptr(vals, ctypes.c_double), n.value, ctypes.byref(n)))
+            ev = {"senders": s, "times": tm}
+            for j, name in enumerate(names):
+                ev[name] = vals[:, j].copy()
+            return ev
         n = ctypes.c_int64(0)
         self._chk(_lib.nestgpu_recorder_events(self._h, rid, None, None, 0, ctypes.byref(n)))
         s = np.zeros(n.value, np.int64)
@@ -558,6 +590,34 @@ class Server:
             raise NESTError(_lib.nestgpu_last_error(None).decode())
         self._kernels[rid.value] = kernel
         return rid.value
+
+    def submit_open(self, kernel, times=()):
+        """Open request (a session): the network stays resident after its calls until
+        close_request(); extend() adds Simulate calls."""
+        t = np.ascontiguousarray(np.atleast_1d(np.asarray(times, dtype=np.float64)))
+        rid = ctypes.c_int64(0)
+        if _lib.nestgpu_server_submit_open(self._h, kernel._h, _ptr(t, ctypes.c_double), len(t),
+                                           ctypes.byref(rid)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        self._kernels[rid.value] = kernel
+        return rid.value
+
+    def extend(self, rid, t):
+        if _lib.nestgpu_server_extend(self._h, int(rid), float(t)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+
+    def close_request(self, rid):
+        if _lib.nestgpu_server_close(self._h, int(rid)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+
+    def wait_calls(self, rid, n, timeout=-1.0):
+        """Waits until the request has completed n Simulate calls; returns (state, calls done)."""
+        st = ctypes.c_int(0)
+        nd = ctypes.c_int64(0)
+        if _lib.nestgpu_server_wait_calls(self._h, int(rid), int(n), float(timeout), ctypes.byref(st),
+                                          ctypes.byref(nd)) != 0:
+            raise NESTError(_lib.nestgpu_last_error(None).decode())
+        return st.value, nd.value
 
     def wait(self, rid, timeout=-1.0):
         """Returns dict(state, t_submit, t_admit, t_start, t_done, events); raises if failed."""

@@ -48,6 +48,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -448,6 +449,10 @@ struct Neuron
   double tau_ex, tau_in;
   // aeif_cond_exp (absolute values)
   double V_peak, g_L, E_ex, E_in, Delta_T, tau_w, a, b, tol;
+  // multimeter sampling: samples (stamp, V_m and the three other state fields as the user
+  // reads them) taken after each sampled step (UniversalDataLogger::record_data)
+  bool sampled = false;
+  std::vector< std::array< double, 5 > > samples;
   // state
   double y0 = 0, y3 = 0, refr_buf = 0;          // delta
   double V = 0, i_ex = 0, i_in = 0, i0 = 0, i1 = 0;  // exp
@@ -955,9 +960,10 @@ struct Kernel
     d_max_seen = std::max( d_max_seen, d_steps );
   }
 
-  // BipartiteConnBuilder::single_connect_ (conn_builder.cpp:379-437): delay before weight
+  // BipartiteConnBuilder::single_connect_ (conn_builder.cpp:379-437): delay before weight.
+  // The delay range of the VP's new connections goes to lo/hi (merged after the VPs join).
   void
-  single_connect( long s, long t, int vp, Rng& rng, Param* wp, Param* dp )
+  single_connect( long s, long t, int vp, Rng& rng, Param* wp, Param* dp, long& lo, long& hi )
   {
     double delay = 1.0, weight = 1.0;  // static_synapse defaults
     if ( wp and dp )
@@ -970,7 +976,10 @@ struct Kernel
     else if ( wp )
       weight = wp->value( rng, vp );
     const long ds = tb.delay_ms_to_steps( delay );
-    check_delay( ds );
+    if ( ds < 1 )
+      throw std::runtime_error( "BadDelay: Delay must be greater than or equal to resolution" );
+    lo = std::min( lo, ds );
+    hi = std::max( hi, ds );
     conns[ vp ].push_back( Conn { s, t, weight, ds } );
   }
 
@@ -1003,10 +1012,18 @@ struct Kernel
       throw std::runtime_error( "Indegree cannot be larger than population size." );
     if ( rule == 3 and ( rule_param < 0 or rule_param > 1 ) )
       throw std::runtime_error( "Connection probability 0 <= p <= 1 required." );
-    // threads run the VPs; the per-VP work is independent, so a sequential sweep is equivalent
+    // Every VP draws from its own stream, appends to its own connection list and keeps its
+    // own normal-draw state in each parameter object, so the VPs run in parallel; the delay
+    // extrema and the first error (in VP order) are merged after the join.
+    std::vector< long > vlo( n_vp, std::numeric_limits< long >::max() ), vhi( n_vp, 0 );
+    std::vector< std::string > verr( n_vp );
+#pragma omp parallel for num_threads( std::max( 1, std::min( omp_get_num_procs(), n_vp ) ) ) schedule( dynamic, 1 )
     for ( int vp = 0; vp < n_vp; ++vp )
+    try
     {
       Rng& rng = vp_rng[ vp ];
+      long& lo = vlo[ vp ];
+      long& hi = vhi[ vp ];
       for ( long ti = 0; ti < n_post; ++ti )
       {
         const long t = post[ ti ];
@@ -1017,7 +1034,7 @@ struct Kernel
           const long s = pre[ ti ];
           if ( s == t and not autapses )
             continue;
-          single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          single_connect( s, t, vp, rng, wp.get(), dp.get(), lo, hi );
         }
         else if ( rule == 1 )
         {
@@ -1025,7 +1042,7 @@ struct Kernel
           {
             if ( pre[ si ] == t and not autapses )
               continue;
-            single_connect( pre[ si ], t, vp, rng, wp.get(), dp.get() );
+            single_connect( pre[ si ], t, vp, rng, wp.get(), dp.get(), lo, hi );
           }
         }
         else if ( rule == 2 )
@@ -1046,7 +1063,7 @@ struct Kernel
             } while ( skip_aut or skip_mult );
             if ( not multapses )
               chosen.push_back( static_cast< long >( s_id ) );
-            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+            single_connect( s, t, vp, rng, wp.get(), dp.get(), lo, hi );
           }
         }
         else if ( rule == 3 )
@@ -1058,13 +1075,26 @@ struct Kernel
               continue;
             if ( rng.drand() >= rule_param )
               continue;
-            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+            single_connect( s, t, vp, rng, wp.get(), dp.get(), lo, hi );
           }
         }
         else
           throw std::runtime_error( "unknown rule" );
       }
     }
+    catch ( const std::exception& e )
+    {
+      verr[ vp ] = e.what();
+    }
+    for ( int vp = 0; vp < n_vp; ++vp )
+      if ( not verr[ vp ].empty() )
+        throw std::runtime_error( verr[ vp ] );
+    for ( int vp = 0; vp < n_vp; ++vp )
+      if ( vhi[ vp ] > 0 )
+      {
+        check_delay( vlo[ vp ] );
+        check_delay( vhi[ vp ] );
+      }
     store_valid = false;
   }
 
@@ -1318,8 +1348,22 @@ struct Kernel
         mask[ lag >> 6 ] |= uint64_t( 1 ) << ( lag & 63 );
         ++spikes_local;
       }
+      // iaf_psc_exp.cpp:346, iaf_psc_delta.cpp:327, aeif_cond_exp.cpp:521: the logger samples
+      // the state after the step at stamps offset + k * interval
+      const long T = origin + lag + 1;
+      if ( n.sampled and T >= sample_off and ( T - sample_off ) % sample_int == 0 )
+      {
+        if ( n.model == IAF_PSC_DELTA )
+          n.samples.push_back( { static_cast< double >( T ), n.y3 + n.E_L, 0.0, 0.0, 0.0 } );
+        else if ( n.model == IAF_PSC_EXP )
+          n.samples.push_back( { static_cast< double >( T ), n.V + n.E_L, n.i_ex, n.i_in, 0.0 } );
+        else
+          n.samples.push_back( { static_cast< double >( T ), n.ay[ 0 ], n.ay[ 1 ], n.ay[ 2 ], n.ay[ 3 ] } );
+      }
     }
   }
+
+  long sample_int = 1, sample_off = 0;
 
   // --------------------------------------------------------------- simulate
   void
@@ -1694,6 +1738,42 @@ or_kernel_status( void* k, long* out5 )
 }
 
 // RNG probes for pinning the seeding against the reference's own randutils header
+// multimeter sampling of the given neurons every interval steps at stamps offset + k * interval
+int
+or_set_sampling( void* k, const long* gids, long n, long interval, long offset )
+{
+  OR_TRY( if ( interval < 1 ) throw std::runtime_error( "interval < 1" ); K->sample_int = interval;
+          K->sample_off = offset; for ( long i = 0; i < n; ++i ) K->nrn( gids[ i ] ).sampled = true; )
+}
+
+long
+or_num_samples( void* k )
+{
+  Kernel* K = static_cast< Kernel* >( k );
+  long c = 0;
+  for ( const auto& n : K->neurons )
+    c += static_cast< long >( n.samples.size() );
+  return c;
+}
+
+// samples sorted by (stamp, node id): gid, stamp and the four state fields per sample
+int
+or_get_samples( void* k, long* gid, long* stamp, double* f4 )
+{
+  OR_TRY( std::vector< std::tuple< long, long, const double* > > all;
+              for ( const auto& n : K->neurons ) for ( const auto& x : n.samples )
+                all.emplace_back( static_cast< long >( x[ 0 ] ), n.gid, &x[ 1 ] );
+              std::sort( all.begin(), all.end(), []( const auto& a, const auto& b ) {
+                return std::get< 0 >( a ) != std::get< 0 >( b ) ? std::get< 0 >( a ) < std::get< 0 >( b )
+                                                                : std::get< 1 >( a ) < std::get< 1 >( b );
+              } );
+              for ( size_t i = 0; i < all.size(); ++i ) {
+                gid[ i ] = std::get< 1 >( all[ i ] );
+                stamp[ i ] = std::get< 0 >( all[ i ] );
+                for ( int f = 0; f < 4; ++f ) f4[ 4 * i + f ] = std::get< 2 >( all[ i ] )[ f ];
+              } )
+}
+
 int
 or_seed_seq_generate( const uint32_t* seed, int n_seed, uint32_t* out, int n_out )
 {

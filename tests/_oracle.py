@@ -47,7 +47,8 @@ def lib():
         L.or_fd_log.argtypes = [ctypes.c_double]
         L.or_iaf_propagator.restype = ctypes.c_double
         L.or_iaf_propagator.argtypes = [ctypes.c_double] * 4
-        for name in ("or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
+        L.or_num_samples.restype = ctypes.c_long
+        for name in ("or_set_sampling", "or_get_samples", "or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
                      "or_add_connections", "or_get_connections", "or_add_drive", "or_simulate", "or_run_steps",
                      "or_get_spikes", "or_clear_spikes", "or_kernel_status"):
             getattr(L, name).restype = ctypes.c_int
@@ -220,6 +221,25 @@ class Oracle:
         s = np.zeros(n, np.int64)
         self._chk(self.L.or_get_spikes(ctypes.c_void_p(self.k), _p(g, ctypes.c_long), _p(s, ctypes.c_long)))
         return g, s
+
+    def set_sampling(self, gids, interval_steps, offset_steps=0):
+        """Sample the state of these neurons after every step whose stamp is
+        offset + k * interval (the multimeter's UniversalDataLogger)."""
+        gids = np.ascontiguousarray(np.atleast_1d(gids), dtype=np.int64)
+        self._chk(self.L.or_set_sampling(ctypes.c_void_p(self.k), _p(gids, ctypes.c_long), ctypes.c_long(len(gids)),
+                                         ctypes.c_long(int(interval_steps)), ctypes.c_long(int(offset_steps))))
+
+    def samples(self):
+        """(gid, stamp, fields[n, 4]) sorted by (stamp, gid); fields are V_m and the model's
+        other state variables as a user reads them (iaf_psc_exp: I_syn_ex, I_syn_in;
+        aeif_cond_exp: g_ex, g_in, w)."""
+        n = self.L.or_num_samples(ctypes.c_void_p(self.k))
+        g = np.zeros(n, np.int64)
+        st = np.zeros(n, np.int64)
+        f = np.zeros((n, 4))
+        self._chk(self.L.or_get_samples(ctypes.c_void_p(self.k), _p(g, ctypes.c_long), _p(st, ctypes.c_long),
+                                        _p(f, ctypes.c_double)))
+        return g, st, f
 
     def synaptic_events(self):
         return int(self.L.or_synaptic_events(ctypes.c_void_p(self.k)))

@@ -208,3 +208,26 @@ def mixed_models(T=100.0, seed=12, n_vp=3, drive_seed=5):
                (1, rng.poisson(0.8, size=(steps, ne)).astype(np.uint8), 87.8, 1.0),
                (2, rng.poisson(2.0, size=(steps, na)).astype(np.uint8), 6.0, 1.0)],
         T=T)
+
+
+def build_instance_oracle(sp, n_vp, n_steps, portable_math=True, threads=4):
+    """A batched-sweep instance (bench.build_instance) on the oracle: the same connections
+    from the same per-VP streams and the library's seeded Poisson counts as dense drive for
+    emission steps [0, n_steps) (the generators are created after the N neurons: node ids
+    N+1 .. 2N)."""
+    import bench
+
+    ip = bench.instance_params(sp)
+    N, K = sp["N"], sp["K"]
+    KE = int(0.8 * K)
+    KI = K - KE
+    NE = int(0.8 * N)
+    o = Oracle(n_vp=n_vp, seed=sp["seed"], threads=threads, portable_math=portable_math)
+    rng = np.random.default_rng(sp["drive_seed"])
+    g = o.create(sp["model"], N, ip["params"])
+    o.set_values(g, "V_m", rng.uniform(*ip["v_range"], N))
+    o.connect(g[:NE], g, "fixed_indegree", KE, weight=ip["J"], delay=1.5)
+    o.connect(g[NE:], g, "fixed_indegree", KI, weight=-sp["g"] * ip["J"], delay=1.5)
+    lam = ip["rate_hz"] * 0.1 * 1e-3
+    o.add_drive(g[0], nestgpu.poisson_counts(sp["drive_seed"], lam, N + 1, N, 0, n_steps), ip["w_drive"], 1.5)
+    return o, g

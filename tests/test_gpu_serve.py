@@ -81,3 +81,68 @@ def test_oversized_request_rejected():
     with pytest.raises(nestgpu.NESTError, match="budget"):
         srv.submit(k, [10.0])
     srv.close()
+
+
+def test_http_sessions_through_served_form_equal_plain():
+    """Two nest-server clients (NEST-Session a and b) build networks over HTTP and call
+    Simulate twice each from two threads; the served form batches their calls. Each session's
+    spikes and V_m equal the same network run alone on the plain path."""
+    import json
+    import threading
+    import urllib.request
+
+    from nestgpu import http_api
+
+    srv = http_api.NESTServer(port=0, slices_per_chunk=8).start()
+
+    def call(route, payload, session):
+        req = urllib.request.Request("http://127.0.0.1:%d/api/%s" % (srv.port, route), json.dumps(payload).encode())
+        req.add_header("Content-Type", "application/json")
+        req.add_header("NEST-Session", session)
+        with urllib.request.urlopen(req, timeout=120) as r:
+            return json.loads(r.read().decode())
+
+    nets = {"a": (700, 5), "b": (1300, 9)}
+    results = {}
+
+    def client(name):
+        n, seed = nets[name]
+        rng = np.random.default_rng(seed)
+        v0 = rng.uniform(-70.0, -55.0, n).tolist()
+        call("SetKernelStatus", {"params": {"rng_seed": seed, "total_num_virtual_procs": 2}}, name)
+        pop = call("Create", {"model": "iaf_psc_exp", "n": n, "params": {"I_e": 370.0}}, name)
+        call("SetStatus", {"nodes": pop, "params": [{"V_m": v} for v in v0]}, name)
+        call("Connect", {"pre": pop, "post": pop, "conn_spec": {"rule": "fixed_indegree", "indegree": 50},
+                         "syn_spec": {"weight": {"normal": {"mean": 40.0, "std": 5.0}}, "delay": 1.5}}, name)
+        sr = call("Create", {"model": "spike_recorder"}, name)
+        call("Connect", {"pre": pop, "post": sr}, name)
+        call("Simulate", {"t": 30.0}, name)
+        call("Simulate", {"t": 25.5}, name)
+        ev = call("GetStatus", {"nodes": sr, "keys": "events"}, name)[0]
+        vm = call("GetStatus", {"nodes": pop, "keys": "V_m"}, name)
+        results[name] = (v0, ev, vm)
+
+    threads = [threading.Thread(target=client, args=(nm,)) for nm in nets]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    srv.stop()
+    for name, (n, seed) in nets.items():
+        v0, ev, vm = results[name]
+        k = nestgpu.Kernel(0)
+        k.SetKernelStatus({"rng_seed": seed, "total_num_virtual_procs": 2})
+        pop = k.Create("iaf_psc_exp", n, {"I_e": 370.0})
+        pop.set(V_m=np.array(v0))
+        k.Connect(pop, pop, {"rule": "fixed_indegree", "indegree": 50},
+                  {"weight": nestgpu.Parameter({"normal": {"mean": 40.0, "std": 5.0}}), "delay": 1.5})
+        sr = k.Create("spike_recorder")
+        k.Connect(pop, sr)
+        k.Simulate(30.0)
+        k.Simulate(25.5)
+        e2 = sr.get("events")
+        print(name, "spikes", len(ev["senders"]), flush=True)
+        assert len(ev["senders"]) > 0
+        assert np.array_equal(np.asarray(ev["senders"]), e2["senders"])
+        assert np.array_equal(np.asarray(ev["times"]), e2["times"])
+        assert np.array_equal(np.asarray(vm), pop.get("V_m"))

@@ -619,3 +619,32 @@ def test_layout_keys_validated_and_reported():
                 {"delivery_passes": 9}, {"workgroups_per_cu": 9}):
         with pytest.raises(nestgpu.NESTError, match="BadProperty"):
             k.SetKernelStatus(bad)
+
+
+# ---------------------------------------------------------------- multimeter (host checks)
+def test_multimeter_parameters_and_checks():
+    # models/multimeter.cpp:115-162 and universal_data_logger.h:280-295
+    k = nestgpu.Kernel(0)
+    mm = k.Create("multimeter", params={"interval": 0.5, "offset": 0.3, "record_from": ["V_m", "I_syn_ex"]})
+    assert mm.get("record_from") == ["V_m", "I_syn_ex"]
+    assert mm.get("interval") == pytest.approx(0.5) and mm.get("offset") == pytest.approx(0.3)
+    assert mm.get("model") == "multimeter"
+    with pytest.raises(nestgpu.NESTError, match="at least as long as the simulation resolution"):
+        k.Create("multimeter", params={"interval": 0.05})
+    with pytest.raises(nestgpu.NESTError, match="multiple of the simulation resolution"):
+        k.Create("multimeter", params={"interval": 0.25})
+    with pytest.raises(nestgpu.NESTError, match="offset for the sampling interval must be at least"):
+        k.Create("multimeter", params={"offset": 0.01})
+    exp = k.Create("iaf_psc_exp", 2)
+    delta = k.Create("iaf_psc_delta", 1)
+    with pytest.raises(nestgpu.NESTError, match="unknown recordable I_syn_ex"):
+        k.Connect(mm, delta)
+    k.Connect(mm, exp)
+    with pytest.raises(nestgpu.NESTError, match="cannot be changed after the multimeter has been connected"):
+        mm.set(interval=1.0)
+    mm.set(start=2.0, stop=5.0)  # the window may change
+    vm = k.Create("voltmeter")
+    assert vm.get("record_from") == ["V_m"]
+    k.Connect(vm, delta)
+    with pytest.raises(nestgpu.NESTError, match="IllegalConnection"):
+        k.Connect(exp, mm)

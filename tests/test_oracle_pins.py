@@ -694,3 +694,39 @@ def test_normal_parameter_object_carries_its_state_across_connect_calls():
     first = _polar_normals(st, 4, 2.0, 0.5)[:3]
     assert wv2.tolist()[:3] == first and wv2.tolist()[3:] == _polar_normals(st, 3, 2.0, 0.5)
     assert wv2.tolist() != wv.tolist()
+
+
+# ---------------------------------------------------------------- multimeter sampling
+def test_sampling_stamps_follow_interval_and_offset():
+    # testsuite/pytests/recording/test_multimeter_offset.py::test_recorded_times_relative_to_offset:
+    # resolution 2^-3 ms, interval 3 ms, offset 5 ms, Simulate(15) -> times 5, 8, 11, 14
+    o = Oracle(resolution=0.125)
+    n = o.create("iaf_psc_exp", 1)
+    o.set_sampling(n, interval_steps=24, offset_steps=40)
+    o.simulate(15.0)
+    g, st, f = o.samples()
+    assert list(st * 0.125) == [5.0, 8.0, 11.0, 14.0]
+    # the default interval of 1 ms and offset 0 sample at every full millisecond after the start
+    o = Oracle()
+    n = o.create("iaf_psc_delta", 2)
+    o.set_sampling(n, interval_steps=10)
+    o.simulate(3.5)
+    g, st, f = o.samples()
+    assert list(st) == [10, 10, 20, 20, 30, 30] and list(g) == [1, 2] * 3
+
+
+def test_sampled_state_is_after_the_step():
+    # the logger records after the step's update (iaf_psc_exp.cpp:346): a DC-driven neuron's
+    # samples follow V(t) = E_L + (1 - exp(-t / tau_m)) R I at the sample stamps t; a sample taken
+    # before the update would lag by one step and miss this by about 0.4 mV
+    dt, I = 0.1, 200.0
+    o = Oracle(resolution=dt)
+    n = o.create("iaf_psc_exp", 1, {"I_e": I})
+    o.set_sampling(n, interval_steps=3, offset_steps=2)
+    o.simulate(6.0)
+    g, st, f = o.samples()
+    t = st * dt
+    assert st[0] == 2 and np.all(np.diff(st) == 3)
+    np.testing.assert_allclose(f[:, 0], -70.0 + (1.0 - np.exp(-t / 10.0)) * (10.0 / 250.0) * I, rtol=0, atol=1e-9)
+    # the synaptic currents of an unstimulated neuron stay at zero
+    assert np.all(f[:, 1:3] == 0.0)
