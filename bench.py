@@ -530,7 +530,7 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
                   for n, ms in kms.items() if kl.get(n)}
     print("%s slice kernel: %.1f us avg over %d launches (delivery %.1f us, %.0f %%), %.3g events in roofline pass"
           % (workload, avg_us, launches, deliv_us, 100 * deliv_share, events_r), file=sys.stderr, flush=True)
-    layout = {"delivery": "window" if st2["window_delivery"] else "global",
+    layout = {"delivery": "window" if st2["window_delivery"] else "cache" if st2["cache_delivery"] else "global",
               "targets_per_block": st2["targets_per_block"], "window_slots": st2["window_slots"]}
     del k
     gc.collect()
@@ -611,7 +611,7 @@ def main():
     ap.add_argument("--qps", type=float, default=1.0, help="c5: request arrival rate")
     ap.add_argument("--builders", type=int, default=6, help="c5: host threads building request networks")
     ap.add_argument("--hbm-budget-gib", type=float, default=128.0, help="c5: device memory budget of the server")
-    ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global"])
+    ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global", "cache"])
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     ap.add_argument("--window-slots", type=int, default=0, help="cap on ring slots in LDS; 0: as many as fit")
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="0: as many as fit")

@@ -41,7 +41,7 @@ Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& w
   // Block size: at most two blocks per compute unit of an MI355X (256 CUs, two workgroups
   // each), multiple of 8, at least 32, few enough blocks for the per-instance limit. The
   // smallest such block leaves the most ring slots in the LDS window.
-  const uint32_t pmax = delivery_mode_ == 1 ? PMAX_WIN : PMAX_GLOB;
+  const uint32_t pmax = delivery_mode_ == 1 or delivery_mode_ == 3 ? PMAX_WIN : PMAX_GLOB;
   if ( targets_per_block_ > 0 )
     tb = static_cast< uint32_t >( targets_per_block_ );
   else
@@ -56,8 +56,9 @@ Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& w
   // auto: window delivery wherever it fits (it keeps the densest slots' read-modify-writes in
   // LDS), global delivery for slot-range passes and for instances beyond the window's limits
   const bool fits = window_slots( max_delay, nch, tb ) >= 1 and P <= PMAX_WIN;
-  win = fits and ( delivery_mode_ == 1 or ( delivery_mode_ == 0 and delivery_passes_ <= 1 ) );
-  if ( delivery_mode_ == 1 and not win )
+  // (cache delivery starts from the window layout; build_spec turns it into the cell cache)
+  win = fits and ( delivery_mode_ == 1 or delivery_mode_ == 3 or ( delivery_mode_ == 0 and delivery_passes_ <= 1 ) );
+  if ( ( delivery_mode_ == 1 or delivery_mode_ == 3 ) and not win )
     throw std::runtime_error( "BadProperty: delivery_mode 'window' does not fit: channels * targets_per_block must "
                               "be <= "
       + std::to_string( WIN_CAP ) + " and the instance at most " + std::to_string( PMAX_WIN ) + " blocks" );
@@ -102,10 +103,11 @@ Kernel::build_spec( BuildSpec& b )
     for ( int dd = b.min_delay; dd <= b.max_delay; ++dd )
       for ( int l = 0; l < b.min_delay; ++l )
         density[ l + dd - b.min_delay ] += per_delay[ dd ];
-    double run = 0, best = -1;
+    double run = 0, best = -1, total = 0;
     b.wa = 0;
     for ( int s = 0; s < b.max_delay; ++s )
     {
+      total += density[ s ];
       run += density[ s ];
       if ( s >= b.wn )
         run -= density[ s - b.wn ];
@@ -115,7 +117,19 @@ Kernel::build_spec( BuildSpec& b )
         b.wa = s - b.wn + 1;
       }
     }
+    // cache delivery: the cells each slice touches are cached in LDS instead of a window,
+    // loaded once and written back once per slice
+    (void)total;
+    if ( delivery_mode_ == 3 )
+    {
+      b.win = false;
+      b.cache = true;
+      b.wa = b.wn = 0;
+    }
   }
+  else if ( delivery_mode_ == 3 )
+    throw std::runtime_error( "BadProperty: delivery_mode 'cache' needs at most " + std::to_string( PMAX_WIN )
+      + " target blocks" );
   // Global delivery can run in slot-range passes (delivery_passes), each touching a smaller
   // part of the ring. Finding each spike's split points costs dependent loads that measured
   // more than the smaller working set saves on the MI355X, so one pass is the default.

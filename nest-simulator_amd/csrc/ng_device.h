@@ -74,6 +74,11 @@ constexpr uint32_t PMAX_WIN = 512;    // blocks per instance, window delivery
 constexpr uint32_t PMAX_GLOB = 2048;  // blocks per instance, global delivery
 constexpr int MAX_PASSES = 3;         // slot-range passes of global delivery
 constexpr uint32_t MM_ROWS = 256;     // multimeter sample rows per chunk
+// LDS cell cache of cache delivery (in the window's LDS area): open addressing over the ring
+// cell index, CACHE_SLOTS keys then CACHE_SLOTS FP64 values, at most CACHE_PROBES probes
+constexpr uint32_t CACHE_SLOTS = 4096;
+constexpr uint32_t CACHE_PROBES = 32;
+static_assert( CACHE_SLOTS * ( 4 + 8 ) <= WIN_CAP * 8, "the cell cache must fit the window's LDS area" );
 
 struct NetDev
 {
@@ -88,6 +93,8 @@ struct NetDev
   uint32_t spk_cap;       // per-block spike list capacity (tb * min_delay)
   int32_t nch;            // ring-buffer channels per slot: 1 (iaf_psc_delta only) or 2
   int32_t win;            // 1: ring slots [wa, wa + wn) of the slice are staged in LDS
+  int32_t cache;          // 1: the ring cells a slice's delivery touches are cached in LDS
+                          //    (loaded before the delivery rounds, written back before the update)
   int32_t wa, wn;         // window: first slot relative to the slice origin, slots
   int32_t np;             // global delivery: passes over slot ranges per chunk (1..MAX_PASSES)
   int32_t ws;             // slots per pass (np * ws >= max_delay)

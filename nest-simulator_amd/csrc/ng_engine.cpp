@@ -79,6 +79,9 @@ Engine::build( const BuildSpec& b, bool keep_state )
   const uint32_t P = b.n_blocks();
   if ( P > ( b.win ? PMAX_WIN : PMAX_GLOB ) )
     throw std::runtime_error( "NotImplemented: too many target blocks in one instance; raise targets_per_block" );
+  if ( b.cache and ( b.win or P > PMAX_WIN or b.np != 1 ) )
+    throw std::runtime_error( "cache delivery: at most " + std::to_string( PMAX_WIN )
+      + " blocks, one pass and no window" );
   if ( b.win
        and ( b.wn < 1 or b.wa < 0 or b.wa + b.wn > b.max_delay
              or static_cast< uint64_t >( b.wn ) * b.nch * b.tb > WIN_CAP ) )
@@ -104,6 +107,7 @@ Engine::build( const BuildSpec& b, bool keep_state )
   net_.spk_cap = b.tb * static_cast< uint32_t >( b.min_delay );
   net_.nch = b.nch;
   net_.win = b.win ? 1 : 0;
+  net_.cache = b.cache ? 1 : 0;
   net_.wa = b.win ? b.wa : 0;
   net_.wn = b.win ? b.wn : 0;
   if ( b.np < 1 or b.np > MAX_PASSES or static_cast< int64_t >( b.np ) * b.ws < b.max_delay or ( b.win and b.np != 1 ) )
@@ -455,7 +459,8 @@ stage_chunk( const std::vector< Engine* >& engines, const std::vector< int64_t >
   for ( int i = 0; i < n; ++i )
   {
     mask |= engines[ i ]->model_mask();
-    win = win or engines[ i ]->window();
+    // the window kernel also carries the cell cache of cache delivery
+    win = win or engines[ i ]->window() or engines[ i ]->cache();
     prefix[ i + 1 ] = prefix[ i ] + engines[ i ]->net().n_blocks;
   }
   if ( prefix[ n ] == 0 )

@@ -81,6 +81,7 @@ struct BuildSpec
   uint32_t tb = 256;
   int nch = 2;       // ring-buffer channels per slot
   bool win = false;  // window delivery (LDS-staged ring-buffer slots)
+  bool cache = false;  // cache delivery (LDS cell cache per slice)
   int wa = 0, wn = 0;  // window: first slot relative to the slice origin, slots
   int np = 1, ws = 1;  // global delivery: slot-range passes and slots per pass
   int uniform_model = -1;
@@ -159,6 +160,11 @@ public:
   window() const
   {
     return net_.win != 0;
+  }
+  bool
+  cache() const
+  {
+    return net_.cache != 0;
   }
   int
   device() const

@@ -298,6 +298,48 @@ struct Cells
   }
 };
 
+// Cell cache of cache delivery: keys (ring cell index, ~0 empty) and values share the
+// window's LDS area.
+constexpr uint32_t EMPTY_KEY = 0xffffffffu;
+__device__ inline uint32_t
+cache_hash( uint32_t cell )
+{
+  return ( cell * 2654435761u ) >> 20;  // 12 bits: CACHE_SLOTS = 4096
+}
+// slot of cell in the cache, or NONE when the cell is not cached
+__device__ inline uint32_t
+cache_find( lptr< const uint32_t > keys, uint32_t cell )
+{
+  uint32_t h = cache_hash( cell );
+  for ( uint32_t p = 0; p < CACHE_PROBES; ++p )
+  {
+    const uint32_t k = keys[ h ];
+    if ( k == cell )
+      return h;
+    if ( k == EMPTY_KEY )
+      return NONE;
+    h = ( h + 1 ) & ( CACHE_SLOTS - 1 );
+  }
+  return NONE;
+}
+// inserts cell; returns its slot if this call inserted it (the caller loads its value), else NONE
+__device__ inline uint32_t
+cache_insert( lptr< uint32_t > keys, uint32_t cell )
+{
+  uint32_t h = cache_hash( cell );
+  for ( uint32_t p = 0; p < CACHE_PROBES; ++p )
+  {
+    uint32_t k = EMPTY_KEY;
+    if ( __hip_atomic_compare_exchange_strong( keys + h, &k, cell, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+           __HIP_MEMORY_SCOPE_WORKGROUP ) )
+      return h;
+    if ( k == cell )
+      return NONE;
+    h = ( h + 1 ) & ( CACHE_SLOTS - 1 );
+  }
+  return NONE;  // no free slot within the probe limit: the cell stays in the ring
+}
+
 // spike_generator drive of this part: handle() of the target with weight * multiplicity.
 // Each lag adds into a different cell, so the cells of LB lags are read together.
 template < bool WIN >
@@ -491,13 +533,20 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     for ( uint32_t i = t; i < n_win; i += BS )
       S.win[ i ] = C.rbb[ i < before_wrap ? w0 * C.nct + i : i - before_wrap ];
   }
+  // cache delivery: an empty cell cache for this slice (keys, then values, in the window area)
+  const bool cache = WIN and d.cache != 0 and ph.deliver;
+  const lptr< uint32_t > ckey = ( lptr< uint32_t > )reinterpret_cast< uint32_t* >( S.win );
+  const lptr< double > cval = ( lptr< double > )( S.win + CACHE_SLOTS / 2 );
+  if ( cache )
+    for ( uint32_t i = t; i < CACHE_SLOTS; i += BS )
+      ckey[ i ] = EMPTY_KEY;
 
   // ================================================================ 1. delivery
   if ( ph.deliver )
   {
     // prefix of per-block spike counts: an instance without a window in the window kernel
     // (up to PMAX_GLOB blocks) keeps it in the unused window
-    uint32_t* const pref = ( WIN and not C.win ) ? reinterpret_cast< uint32_t* >( S.win ) : S.pref;
+    uint32_t* const pref = ( WIN and not C.win and not cache ) ? reinterpret_cast< uint32_t* >( S.win ) : S.pref;
     S.bcnt[ 0 ][ t ] = 0;
     S.bcnt[ 1 ][ t ] = 0;
     int cur = 0;  // counter buffer of the current round
@@ -615,6 +664,67 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       __syncthreads();
       NG_PHASE( 1 );
       events += E;
+      if ( cache and E )
+      {
+        // The cells of this chunk's events join the slice's cell cache. Each thread takes a
+        // contiguous span of the events, reads their records (the rounds read them again, from
+        // L2) and loads the value of every cell it inserts, all loads in flight together: the
+        // rounds then add into LDS only. A cell stays cached for the rest of the slice, so all
+        // its later additions go to the cache, after the ones it already holds.
+        constexpr int HQ = 8;
+        const int umodel = d.uniform_model;
+        const uint32_t per = ( E + BS - 1 ) / BS;
+        const uint32_t e_beg = min( E, t * per ), e_end = min( E, e_beg + per );
+        uint32_t j = e_beg < e_end ? upper_bound_u32( S.c_off, CH + 1, e_beg ) - 1 : 0;
+        for ( uint32_t e0 = e_beg; e0 < e_end; e0 += HQ )
+        {
+          SynRec hr[ HQ ];
+          uint32_t hl[ HQ ];
+#pragma unroll
+          for ( int q = 0; q < HQ; ++q )
+          {
+            const uint32_t e = e0 + q < e_end ? e0 + q : e_end - 1;
+            while ( j + 1 < static_cast< uint32_t >( CH ) and S.c_off[ j + 1 ] <= e )
+              ++j;
+            hr[ q ] = load_rec( glob( d.syn ) + S.c_g0[ j ] + ( e - S.c_off[ j ] ) );
+            hl[ q ] = S.c_lag[ j ];
+          }
+          uint32_t hcell[ HQ ], hslot[ HQ ];
+#pragma unroll
+          for ( int q = 0; q < HQ; ++q )
+          {
+            hslot[ q ] = NONE;
+            hcell[ q ] = 0;
+            if ( e0 + q < e_end )
+            {
+              const double w = __hiloint2double( static_cast< int >( hr[ q ].w_hi ), static_cast< int >( hr[ q ].w_lo ) );
+              const uint32_t local = hr[ q ].word & 0xffffu;
+              const uint32_t slot_rel = hl[ q ] + ( hr[ q ].word >> 16 );
+              const int m = MASK == 1 ? M_IAF_PSC_DELTA
+                : MASK == 2           ? M_IAF_PSC_EXP
+                : MASK == 4           ? M_AEIF_COND_EXP
+                : umodel >= 0         ? umodel
+                                      : glob( d.model )[ b * tb + local ];
+              uint32_t ch = 0;
+              if ( m == M_IAF_PSC_EXP )
+                ch = w > 0 ? 1u : 0u;
+              else if ( m == M_AEIF_COND_EXP and not( w > 0.0 ) )
+                ch = 1;
+              hcell[ q ] = C.ring_index( slot_rel, ch, local );
+              hslot[ q ] = cache_insert( ckey, hcell[ q ] );
+            }
+          }
+          double hv[ HQ ];
+#pragma unroll
+          for ( int q = 0; q < HQ; ++q )
+            hv[ q ] = cell_load( rb_rsrc, hslot[ q ] != NONE ? hcell[ q ] * 8u : NO_CELL );
+#pragma unroll
+          for ( int q = 0; q < HQ; ++q )
+            if ( hslot[ q ] != NONE )
+              cval[ hslot[ q ] ] = hv[ q ];
+        }
+        __syncthreads();
+      }
       // rounds of R events; the synapse records of round r + 1 are loaded while round r
       // is bucketed and applied
       SynRec rec[ RQ ];
@@ -646,12 +756,14 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         uint32_t key[ RQ ], loc[ RQ ], dst[ RQ ], ci[ RQ ], goff[ RQ ];
         double val[ RQ ], cv[ RQ ];
         bool lw[ RQ ];  // the event's cell is in the LDS window
+        bool lc[ RQ ];  // the event's cell is in the cell cache (ci = its cache slot)
         const uint32_t eb = e0 + t * RQ;
 #pragma unroll
         for ( int q = 0; q < RQ; ++q )
         {
           loc[ q ] = NONE;
           goff[ q ] = NO_CELL;
+          lc[ q ] = false;
           if ( eb + q < E )
           {
             const double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
@@ -684,7 +796,14 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               if ( slot >= C.L )
                 slot -= C.L;
               ci[ q ] = slot * C.nct + ch * tb + local;
-              goff[ q ] = ci[ q ] * 8u;
+              const uint32_t hs = cache ? cache_find( ckey, ci[ q ] ) : NONE;
+              if ( hs != NONE )
+              {
+                lc[ q ] = true;
+                ci[ q ] = hs;
+              }
+              else
+                goff[ q ] = ci[ q ] * 8u;
             }
           }
         }
@@ -770,7 +889,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             goff[ q ] = NO_CELL;
             continue;
           }
-          double v = lw[ q ] ? S.win[ ci[ q ] ] : cv[ q ];
+          double v = lw[ q ] ? S.win[ ci[ q ] ] : lc[ q ] ? cval[ ci[ q ] ] : cv[ q ];
           v = v + val[ q ];
           for ( uint32_t i = srt[ q ] + 1; i < hi; ++i )
           {
@@ -786,11 +905,23 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         {
           if ( loc[ q ] != NONE and lw[ q ] )
             S.win[ ci[ q ] ] = cv[ q ];
+          else if ( loc[ q ] != NONE and lc[ q ] )
+            cval[ ci[ q ] ] = cv[ q ];
           cell_store( rb_rsrc, goff[ q ], cv[ q ] );  // NO_CELL unless a global run leader
         }
         NG_PHASE( 5 );  // apply
       }
       }  // passes
+    }
+    if ( cache )
+    {
+      // write the cached cells back to the ring: the update and the drive read them there
+      __syncthreads();
+      for ( uint32_t i = t; i < CACHE_SLOTS; i += BS )
+      {
+        const uint32_t k = ckey[ i ];
+        cell_store( rb_rsrc, k != EMPTY_KEY ? k * 8u : NO_CELL, cval[ i ] );
+      }
     }
     if ( t == 0 and events )
       g_add( &d.counters[ 1 ], events );

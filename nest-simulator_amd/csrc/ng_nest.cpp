@@ -529,9 +529,9 @@ Kernel::set_kernel_status( const Json& d )
     else if ( k == "delivery_mode" )
     {
       const std::string v = kv.second.string();
-      const int m = v == "auto" ? 0 : v == "window" ? 1 : v == "global" ? 2 : -1;
+      const int m = v == "auto" ? 0 : v == "window" ? 1 : v == "global" ? 2 : v == "cache" ? 3 : -1;
       if ( m < 0 )
-        throw std::runtime_error( "BadProperty: delivery_mode must be 'auto', 'window' or 'global'" );
+        throw std::runtime_error( "BadProperty: delivery_mode must be 'auto', 'window', 'global' or 'cache'" );
       if ( simulated_ and m != delivery_mode_ )
         throw std::runtime_error( "KernelException: delivery_mode cannot change after Simulate" );
       delivery_mode_ = m;
@@ -633,8 +633,10 @@ Kernel::get_kernel_status_json() const
     << ", \"time_gather_spike_data\": " << time_kernel_ms_[ K_FLATTEN ] * 1e-3
     << ", \"time_deliver_spike_data\": 0, \"fused_deliver_update\": true"
     << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().tb ) : targets_per_block_ )
-    << ", \"delivery_mode\": \"" << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : "global" ) << "\""
+    << ", \"delivery_mode\": \""
+    << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : delivery_mode_ == 2 ? "global" : "cache" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
+    << ", \"cache_delivery\": " << ( engine_ and engine_->built() and engine_->cache() ? "true" : "false" )
     << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )
     << ", \"window_slots\": " << ( engine_ and engine_->built() ? engine_->net().wn : window_slots_ )
     << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" )
@@ -899,7 +901,9 @@ Kernel::create( const std::string& model, int64_t n, const Json& params )
     throw std::runtime_error( "BadProperty: node parameters must be a dictionary" );
   if ( n < 1 )
     throw std::runtime_error( "BadProperty: Create requires n >= 1." );
-  if ( simulated_ )
+  // recording devices may join after Simulate (they change what is read back, not the network
+  // on the device); neurons and generators may not
+  if ( simulated_ and model != "spike_recorder" and model != "multimeter" and model != "voltmeter" )
     throw std::runtime_error( "NotImplemented: creating nodes after Simulate is not supported on the device path." );
   const int64_t first = static_cast< int64_t >( nodes_.size() ) + 1;
   int model_id = -1;

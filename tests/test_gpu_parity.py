@@ -18,7 +18,8 @@ LAYOUTS = {"auto": {}, "win48": {"delivery_mode": "window", "targets_per_block":
            "win512": {"delivery_mode": "window", "targets_per_block": 512},
            "win2": {"delivery_mode": "window", "window_slots": 2},
            "glob32": {"delivery_mode": "global", "targets_per_block": 32},
-           "glob64p3": {"delivery_mode": "global", "targets_per_block": 64, "delivery_passes": 3}}
+           "glob64p3": {"delivery_mode": "global", "targets_per_block": 64, "delivery_passes": 3},
+           "cache": {"delivery_mode": "cache"}, "cache32": {"delivery_mode": "cache", "targets_per_block": 32}}
 layouts = pytest.mark.parametrize("layout", list(LAYOUTS))
 
 
@@ -103,7 +104,7 @@ def test_brunel_delta_refractory_input(layout):
     _compare(desc, portable_math=True, layout=layout)
 
 
-@pytest.mark.parametrize("layout", ["auto", "glob32", "win48", "win512", "win2", "glob64p3"])
+@pytest.mark.parametrize("layout", ["auto", "glob32", "win48", "win512", "win2", "glob64p3", "cache", "cache32"])
 def test_bernoulli_min_delay_one_step(layout):
     # C3-like: pairwise_bernoulli, normal delays clipped at dt -> min_delay = 1 step
     params = {"tau_m": 10.0, "tau_syn_ex": 0.5, "tau_syn_in": 0.5, "C_m": 250.0, "V_th": -55.0,
@@ -160,12 +161,12 @@ def _compare_tol(desc, fields, rel=1e-6, layout="auto"):
     return k, o
 
 
-@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "glob64p3"])
+@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "glob64p3", "cache"])
 def test_aeif_network(layout):
     _compare_tol(nb.aeif_random(N=1000, T=100.0), fields=("V_m", "w", "g_ex", "g_in"), layout=layout)
 
 
-@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "win2", "glob32", "glob64p3"])
+@pytest.mark.parametrize("layout", ["auto", "win48", "win512", "win2", "glob32", "glob64p3", "cache"])
 def test_mixed_models(layout):
     desc = nb.mixed_models(T=100.0)
     k, pops, sr = nb.build_gpu(desc, layout=LAYOUTS[layout])

@@ -43,7 +43,7 @@ def test_multimeter_exp_network_against_oracle(layout):
     desc = nb.exp_random(N=1200, KE=80, KI=20, T=60.0, seed=17, n_vp=2)
     fields = ("V_m", "I_syn_ex", "I_syn_in")
     targets = np.arange(0, 1200, 37)
-    k, mm, o = _sampled(desc, targets, {"interval": 0.3, "offset": 0.2}, fields, layout, steps=[25.05, 34.95])
+    k, mm, o = _sampled(desc, targets, {"interval": 0.3, "offset": 0.2}, fields, layout, steps=[25.1, 34.9])
     ev = mm.get("events")
     g, st, vals = _oracle_events(o, fields, "iaf_psc_exp")
     print("samples device %d oracle %d" % (len(ev["senders"]), len(g)), flush=True)

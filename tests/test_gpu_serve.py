@@ -110,7 +110,7 @@ def test_http_sessions_through_served_form_equal_plain():
         rng = np.random.default_rng(seed)
         v0 = rng.uniform(-70.0, -55.0, n).tolist()
         call("SetKernelStatus", {"params": {"rng_seed": seed, "total_num_virtual_procs": 2}}, name)
-        pop = call("Create", {"model": "iaf_psc_exp", "n": n, "params": {"I_e": 370.0}}, name)
+        pop = call("Create", {"model": "iaf_psc_exp", "n": n, "params": {"I_e": 450.0}}, name)
         call("SetStatus", {"nodes": pop, "params": [{"V_m": v} for v in v0]}, name)
         call("Connect", {"pre": pop, "post": pop, "conn_spec": {"rule": "fixed_indegree", "indegree": 50},
                          "syn_spec": {"weight": {"normal": {"mean": 40.0, "std": 5.0}}, "delay": 1.5}}, name)
@@ -132,7 +132,7 @@ def test_http_sessions_through_served_form_equal_plain():
         v0, ev, vm = results[name]
         k = nestgpu.Kernel(0)
         k.SetKernelStatus({"rng_seed": seed, "total_num_virtual_procs": 2})
-        pop = k.Create("iaf_psc_exp", n, {"I_e": 370.0})
+        pop = k.Create("iaf_psc_exp", n, {"I_e": 450.0})
         pop.set(V_m=np.array(v0))
         k.Connect(pop, pop, {"rule": "fixed_indegree", "indegree": 50},
                   {"weight": nestgpu.Parameter({"normal": {"mean": 40.0, "std": 5.0}}), "delay": 1.5})

@@ -76,7 +76,8 @@ def _c3_gpu(scale, n_vp, layout=None):
     return k, pops, sr
 
 
-@pytest.mark.parametrize("layout", [None, {"delivery_mode": "global", "targets_per_block": 64}])
+@pytest.mark.parametrize("layout", [None, {"delivery_mode": "global", "targets_per_block": 64},
+                                    {"delivery_mode": "cache"}], ids=["auto", "global", "cache"])
 def test_circuit_scaled_against_oracle(layout):
     T, n_vp = 100.0, 4
     k, pops, sr = _c3_gpu(0.1, n_vp, layout)
@@ -110,12 +111,13 @@ def test_circuit_full_size_against_oracle():
 
 
 def test_circuit_full_size_layouts_agree():
-    # full size over 100 ms: the automatic layout (LDS window) and global delivery in blocks of
-    # 128 give identical spikes and bit-identical states
+    # full size over 100 ms: the automatic layout (LDS window), global delivery in blocks of
+    # 128 and cache delivery give identical spikes and bit-identical states
     T = 100.0
     res = []
     for layout in ({"keep_host_connections": False},
-                   {"keep_host_connections": False, "delivery_mode": "global", "targets_per_block": 128}):
+                   {"keep_host_connections": False, "delivery_mode": "global", "targets_per_block": 128},
+                   {"keep_host_connections": False, "delivery_mode": "cache"}):
         k, pops, sr = _c3_gpu(1.0, 8, layout)
         k.Simulate(T)
         ev = sr.get("events")
@@ -124,8 +126,9 @@ def test_circuit_full_size_layouts_agree():
         print(layout, "spikes", len(ev["senders"]), "window", k.GetKernelStatus("window_delivery"), flush=True)
         del k, pops, sr
     assert len(res[0][0]) > 0
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(a, b)
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(a, b)
 
 
 def _instance_gpu(sp):
