@@ -104,6 +104,18 @@ int nestgpu_node_status( nestgpu_kernel* k, int64_t id, char* buf, size_t cap, s
  * one kernel per process). t_ms[i] is the simulation time of kernel i. */
 int nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms );
 
+/* Variant of nestgpu_batch_simulate that allocates nothing (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Every kernel must already be prepared
+ * (nestgpu_prepare, or an earlier Simulate) with no change since; the call then makes no
+ * hipMalloc, hipFree, host-pinned or graph allocation: the launch records, block prefix and
+ * status words of the slice kernels live in a fixed device array of the library, one per
+ * device, set up at the first call on that device, and the slices are launched directly on
+ * the device's null stream. At most 64 kernels per call (larger n is an error). Results are
+ * identical to nestgpu_batch_simulate. Calls that overlap on one device share that array and
+ * race; calls on one device must not run concurrently. */
+int nestgpu_batch_simulate_noalloc( nestgpu_kernel** ks, int n, const double* t_ms );
+
 /* Layout autotuner of the batched form (no counterpart in the reference): groups the kernels
  * (none simulated yet) into classes by neuron models and neuron count (one per octave), times
  * probe runs of probe_ms on up to 4 instances of each class for every candidate

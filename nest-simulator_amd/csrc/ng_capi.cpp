@@ -323,6 +323,49 @@ nestgpu_batch_simulate( nestgpu_kernel** ks, int n, const double* t_ms )
 }
 
 int
+nestgpu_batch_simulate_noalloc( nestgpu_kernel** ks, int n, const double* t_ms )
+{
+  try
+  {
+    if ( n <= 0 )
+      return 0;
+    if ( ks == nullptr or t_ms == nullptr )
+      throw std::runtime_error( "null argument" );
+    if ( n > nestgpu::NOALLOC_MAX_NETS )
+      throw std::runtime_error( "batch_simulate_noalloc: " + std::to_string( n ) + " kernels, at most "
+        + std::to_string( nestgpu::NOALLOC_MAX_NETS ) + " per call" );
+    std::vector< nestgpu::Engine* > engines;
+    std::vector< int64_t > steps;
+    const int dev = ks[ 0 ]->device;
+    for ( int i = 0; i < n; ++i )
+    {
+      if ( ks[ i ] == nullptr )
+        throw std::runtime_error( "null kernel handle" );
+      if ( ks[ i ]->device != dev )
+        throw std::runtime_error( "batched kernels must share one device" );
+      if ( not ks[ i ]->k->device_copy_current() )
+        throw std::runtime_error( "KernelException: batch_simulate_noalloc needs every kernel prepared, with no "
+                                  "change since its Prepare (Prepare allocates the network's device memory)" );
+      steps.push_back( ks[ i ]->k->checked_steps( t_ms[ i ] ) );
+    }
+    for ( int i = 0; i < n; ++i )
+    {
+      ks[ i ]->k->prepare_for_batch();  // the device copy is current: only the Prepare flags change
+      engines.push_back( &ks[ i ]->k->engine() );
+    }
+    nestgpu::run_noalloc( dev, engines, steps, 64 );
+    for ( int i = 0; i < n; ++i )
+      ks[ i ]->k->after_batch_run( steps[ i ] );
+    return 0;
+  }
+  catch ( const std::exception& e )
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int
 nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand, int* chosen )
 {
   try

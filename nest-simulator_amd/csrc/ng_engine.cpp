@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 
 namespace nestgpu
@@ -621,6 +622,96 @@ Runner::run_chunk( const std::vector< Engine* >& engines, std::vector< int64_t >
   last_ms_ += ms;
   finish_chunk( engines, h_nets_, h_status_, remaining );
   return true;
+}
+
+// ------------------------------------------------------------------ run without allocation
+// The launch records, block prefix and status words of run_noalloc live in this fixed
+// device array (each device holds its own copy until the process ends). Host code never
+// uses the array's name as an address: it takes the device address from hipGetSymbolAddress
+// once per device (noalloc_scratch) and passes that to kernels and copies.
+namespace
+{
+struct NoAllocArea
+{
+  NetDev nets[ NOALLOC_MAX_NETS ];
+  uint32_t prefix[ NOALLOC_MAX_NETS + 1 ];
+  uint32_t pad_;
+  unsigned long long status[ 2 * NOALLOC_MAX_NETS ];
+};
+}  // namespace
+
+static __device__ __align__( 256 ) unsigned char scratch[ sizeof( NoAllocArea ) ];
+
+namespace
+{
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+NoAllocArea* g_noalloc_ptr[ NOALLOC_MAX_DEVICES ] = {};
+
+// Device address of this device's scratch array; the device must be current.
+NoAllocArea*
+noalloc_scratch( int device )
+{
+  if ( device < 0 or device >= NOALLOC_MAX_DEVICES )
+    throw std::runtime_error( "run without allocation: device index out of range" );
+  std::lock_guard< std::mutex > lock( g_noalloc_mu );
+  if ( g_noalloc_ptr[ device ] == nullptr )
+  {
+    void* p = nullptr;
+    hip_check( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ), "hipGetSymbolAddress(scratch)" );
+    if ( p == nullptr )
+      throw std::runtime_error( "run without allocation: no device address for the scratch array" );
+    g_noalloc_ptr[ device ] = static_cast< NoAllocArea* >( p );
+  }
+  return g_noalloc_ptr[ device ];
+}
+}  // namespace
+
+void
+run_noalloc( int device, const std::vector< Engine* >& engines, const std::vector< int64_t >& steps, int S )
+{
+  const int n = static_cast< int >( engines.size() );
+  if ( n > NOALLOC_MAX_NETS )
+    throw std::runtime_error( "run without allocation: " + std::to_string( n ) + " instances, at most "
+      + std::to_string( NOALLOC_MAX_NETS ) + " fit the fixed scratch array" );
+  if ( S < 1 )
+    throw std::runtime_error( "run without allocation: slices per chunk must be at least 1" );
+  for ( Engine* E : engines )
+    if ( not E->built() )
+      throw std::runtime_error( "run without allocation: every kernel must be prepared first (Prepare allocates "
+                                "the network's device memory)" );
+  if ( n == 0 )
+    return;
+  hip_check( hipSetDevice( device ), "hipSetDevice" );
+  NoAllocArea* area = noalloc_scratch( device );
+  NetDev* d_nets = area->nets;  // device addresses inside the scratch array
+  uint32_t* d_prefix = area->prefix;
+  unsigned long long* d_status = area->status;
+  const hipStream_t st = nullptr;  // the device's null stream: nothing to create
+  NetDev h_nets[ NOALLOC_MAX_NETS ];
+  unsigned long long h_status[ 2 * NOALLOC_MAX_NETS ];
+  std::vector< int64_t > remaining = steps;
+  std::vector< uint32_t > prefix;
+  for ( ;; )
+  {
+    int mask = 0;
+    bool win = false, split = false;
+    if ( not stage_chunk( engines, remaining, S, h_nets, prefix, mask, win, split ) )
+      break;
+    const uint32_t total = prefix[ n ];
+    // every word the kernels read is written here for each chunk, so nothing left in the
+    // array by an earlier call is used
+    hip_check( hipMemcpyAsync( d_prefix, prefix.data(), ( n + 1 ) * sizeof( uint32_t ), hipMemcpyHostToDevice, st ),
+      "prefix" );
+    hip_check( hipMemcpyAsync( d_nets, h_nets, n * sizeof( NetDev ), hipMemcpyHostToDevice, st ), "nets H2D" );
+    launch_chunk_direct( h_nets, d_nets, n, S, split, st,
+      [ & ]( int s ) { hip_check( launch_slice( d_nets, d_prefix, n, total, mask, win, s, 0, st ), "launch_slice" ); } );
+    hip_check( launch_status( d_nets, n, d_status, st ), "launch_status" );
+    hip_check( hipMemcpyAsync( h_status, d_status, 2 * n * sizeof( unsigned long long ), hipMemcpyDeviceToHost, st ),
+      "status D2H" );
+    hip_check( hipStreamSynchronize( st ), "hipStreamSynchronize" );
+    finish_chunk( engines, h_nets, h_status, remaining );
+  }
 }
 
 }  // namespace nestgpu

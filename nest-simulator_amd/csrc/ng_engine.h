@@ -269,4 +269,15 @@ private:
 };
 
 void hip_check( hipError_t e, const char* what );
+
+// Chunked slice loop of prepared instances that allocates nothing: the launch records, the
+// block prefix and the status words live in a fixed device array of the library (one copy
+// per device, its address taken once per device at the first call there) instead of
+// per-runner allocations; slices are launched directly on the device's null stream, without
+// graph capture. At most NOALLOC_MAX_NETS instances per call. Calls that overlap on one device
+// share that array and race.
+constexpr int NOALLOC_MAX_NETS = 64;
+void run_noalloc( int device, const std::vector< Engine* >& engines, const std::vector< int64_t >& steps,
+  int slices_per_chunk );
+
 }  // namespace nestgpu

@@ -376,6 +376,11 @@ public:
     return device_;
   }
   void prepare_for_batch();  // prepare without running (idempotent)
+  // The device copy is built and matches the host tables: Prepare would allocate nothing.
+  bool device_copy_current() const
+  {
+    return engine_ and engine_->built() and not store_dirty_ and not released_;
+  }
   // Steps of a Run of t_ms, with Run's checks (non-negative, a multiple of the resolution).
   int64_t checked_steps( double t_ms ) const;
   // Discard the device copy and every simulation result, back to the state before the

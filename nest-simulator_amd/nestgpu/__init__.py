@@ -54,6 +54,7 @@ _SIGS = {
     "nestgpu_node_status": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_size_t)]),
     "nestgpu_batch_simulate": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
+    "nestgpu_batch_simulate_noalloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
     "nestgpu_batch_autotune": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double,
                                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nestgpu_server_new": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -546,6 +547,16 @@ def batch_simulate(kernels, times):
     arr = (_vp * len(kernels))(*[k._h for k in kernels])
     t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (len(kernels),)))
     if _lib.nestgpu_batch_simulate(arr, len(kernels), _ptr(t, ctypes.c_double)) != 0:
+        raise NESTError(_lib.nestgpu_last_error(None).decode())
+
+
+def batch_simulate_noalloc(kernels, times):
+    """batch_simulate for prepared kernels that allocates no device memory (the launch
+    records and status words live in a fixed device array of the library, at most 64 kernels
+    per call). Calls on one device must not overlap."""
+    arr = (_vp * len(kernels))(*[k._h for k in kernels])
+    t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (len(kernels),)))
+    if _lib.nestgpu_batch_simulate_noalloc(arr, len(kernels), _ptr(t, ctypes.c_double)) != 0:
         raise NESTError(_lib.nestgpu_last_error(None).decode())
 
 
