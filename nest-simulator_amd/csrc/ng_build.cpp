@@ -71,6 +71,156 @@ Kernel::window_slots( int max_delay, int nch, uint32_t tb )
     std::min< uint64_t >( max_delay, WIN_CAP / ( static_cast< uint64_t >( nch ) * std::max< uint32_t >( tb, 1 ) ) ) );
 }
 
+// Contiguous blocks of device indices balancing the work of their targets: at most p_max
+// blocks of at most cap targets each, the largest block work as small as greedy filling under
+// a bisected bound makes it (a block closes when the next target would exceed the bound or
+// the block is full). start has one entry per block and the end.
+static std::vector< uint32_t >
+partition_by_work( const std::vector< double >& work, uint32_t p_max, uint32_t cap )
+{
+  const uint32_t N = static_cast< uint32_t >( work.size() );
+  double total = 0, heaviest = 0;
+  for ( double x : work )
+  {
+    total += x;
+    heaviest = std::max( heaviest, x );
+  }
+  auto fill = [ & ]( double bound, std::vector< uint32_t >* out )
+  {
+    uint32_t blocks = 0, size = 0;
+    double sum = 0;
+    if ( out )
+      out->assign( 1, 0 );
+    for ( uint32_t i = 0; i < N; ++i )
+    {
+      if ( size > 0 and ( size == cap or sum + work[ i ] > bound ) )
+      {
+        ++blocks;
+        if ( out )
+          out->push_back( i );
+        size = 0;
+        sum = 0;
+      }
+      ++size;
+      sum += work[ i ];
+    }
+    ++blocks;
+    if ( out )
+      out->push_back( N );
+    return blocks;
+  };
+  // at the total every block is cut by size only: ceil(N / cap) <= p_max blocks
+  double lo = std::max( total / p_max, heaviest ), hi = total;
+  for ( int it = 0; it < 50 and hi - lo > 1e-9 * hi; ++it )
+  {
+    const double mid = 0.5 * ( lo + hi );
+    if ( fill( mid, nullptr ) <= p_max )
+      hi = mid;
+    else
+      lo = mid;
+  }
+  std::vector< uint32_t > start;
+  fill( hi, &start );
+  return start;
+}
+
+static double
+largest_block_work( const std::vector< double >& work, const std::vector< uint32_t >& start )
+{
+  double m = 0;
+  for ( size_t q = 0; q + 1 < start.size(); ++q )
+  {
+    double sum = 0;
+    for ( uint32_t i = start[ q ]; i < start[ q + 1 ]; ++i )
+      sum += work[ i ];
+    m = std::max( m, sum );
+  }
+  return m;
+}
+
+std::vector< uint32_t >
+Kernel::balanced_partition( uint32_t N, uint32_t tb ) const
+{
+  // in-degree of every device index: the events it receives per spike of all its sources
+  const int nw = static_cast< int >( std::max< size_t >( 1, std::min< size_t >( conns_.size(), 16 ) ) );
+  std::vector< std::vector< uint32_t > > part( nw, std::vector< uint32_t >( N, 0 ) );
+  {
+    std::vector< std::thread > th;
+    for ( int w = 0; w < nw; ++w )
+      th.emplace_back(
+        [ &, w ]
+        {
+          for ( size_t v = w; v < conns_.size(); v += nw )
+            for ( uint32_t g : conns_[ v ].tgt )
+              ++part[ w ][ gid_to_dev_[ g ] ];
+        } );
+    for ( auto& x : th )
+      x.join();
+  }
+  std::vector< double > work( N, 0.0 );
+  double total = 0;
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    uint64_t k = 0;
+    for ( int w = 0; w < nw; ++w )
+      k += part[ w ][ i ];
+    work[ i ] = static_cast< double >( k );
+    total += work[ i ];
+  }
+  // a target's update costs about as much as delivering a fifth of the mean in-degree
+  const double upd = 0.2 * total / N + 1.0;
+  for ( double& x : work )
+    x += upd;
+  return partition_by_work( work, ( N + tb - 1 ) / tb, block_cap( tb ) );
+}
+
+uint32_t
+Kernel::block_cap( uint32_t tb )
+{
+  return std::min< uint32_t >( MAX_TB, ( 2 * tb + 7 ) / 8 * 8 );
+}
+
+void
+Kernel::rebalance_blocks()
+{
+  // Target blocks balanced by the events their targets received so far (measured by the slice
+  // kernel), once, at the first Run boundary that ends a slice after at least 64 slices
+  // counted. A target's update costs about as much as delivering 4 events per slice (the
+  // update and the delivery phases of an iaf_psc_exp circuit on the MI355X). The layout
+  // changes only if the largest block work drops by 5 % or more; results do not depend on it.
+  Engine& E = *engine_;
+  const int64_t slices = rx_steps_ / std::max< long >( min_delay_, 1 );
+  if ( slices < 64 or E.part != 0 or E.from_step != 0 )
+    return;
+  rebalanced_ = true;
+  std::vector< uint32_t > rx;
+  E.read_received( rx );
+  const uint32_t N = static_cast< uint32_t >( rx.size() );
+  std::vector< double > work( N );
+  double events = 0;
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    work[ i ] = rx[ i ] + 4.0 * static_cast< double >( slices );
+    events += rx[ i ];
+  }
+  if ( events < static_cast< double >( N ) * slices )  // under one event per target and slice
+    return;
+  uint32_t tb_nom = 0;
+  bool win = false;
+  choose_layout( N, static_cast< int >( max_delay_ ), E.net().nch, tb_nom, win );
+  std::vector< uint32_t > start = partition_by_work( work, ( N + tb_nom - 1 ) / tb_nom, block_cap( tb_nom ) );
+  if ( largest_block_work( work, start ) > 0.95 * largest_block_work( work, E.block_start() ) )
+    return;
+  const std::vector< uint32_t > before = block_start_;
+  block_start_ = start;
+  BuildSpec b;
+  build_spec( b );
+  if ( E.relayout( b ) )
+    ++block_relayouts_;
+  else
+    block_start_ = before;
+}
+
 void
 Kernel::build_spec( BuildSpec& b )
 {
@@ -85,6 +235,51 @@ Kernel::build_spec( BuildSpec& b )
     all_delta = all_delta and Pm.model == M_IAF_PSC_DELTA;
   b.nch = all_delta ? 1 : 2;
   choose_layout( N, b.max_delay, b.nch, b.tb, b.win );
+  // ---- device index = position in NEST's receive order (VP, then node id)
+  std::vector< uint32_t > order( N );
+  for ( uint32_t i = 0; i < N; ++i )
+    order[ i ] = i;
+  std::stable_sort( order.begin(), order.end(),
+    [ this ]( uint32_t a, uint32_t c ) { return vp_of( neuron_gid_[ a ] ) < vp_of( neuron_gid_[ c ] ); } );
+  std::vector< uint32_t > dev_of( N );
+  for ( uint32_t o = 0; o < N; ++o )
+    dev_of[ order[ o ] ] = o;
+  b.vp_begin.assign( n_vp_ + 1, 0 );
+  for ( uint32_t o = 0; o < N; ++o )
+    b.vp_begin[ vp_of( neuron_gid_[ order[ o ] ] ) + 1 ]++;
+  for ( int v = 0; v < n_vp_; ++v )
+    b.vp_begin[ v + 1 ] += b.vp_begin[ v ];
+  gid_to_dev_.assign( nodes_.size() + 1, -1 );
+  dev_to_gid_.assign( N, 0 );
+  for ( uint32_t i = 0; i < N; ++i )
+  {
+    gid_to_dev_[ neuron_gid_[ i ] ] = static_cast< int32_t >( dev_of[ i ] );
+    dev_to_gid_[ dev_of[ i ] ] = neuron_gid_[ i ];
+  }
+
+  // ---- target blocks: contiguous device-index ranges. With the block size left to the
+  // library, the blocks are balanced by in-degree (no more blocks than uniform ones of the
+  // chosen size, each at most twice as large); a rebuild after Simulate keeps the partition,
+  // which the ring buffers and pending spike lists are laid out by (the measured balance
+  // moves them explicitly).
+  if ( simulated_ and not block_start_.empty() and block_start_.back() == N )
+    b.bstart = block_start_;
+  else if ( targets_per_block_ == 0 and balanced_blocks_ and N > 0 )
+    b.bstart = balanced_partition( N, b.tb );
+  else
+  {
+    b.bstart.resize( ( N + b.tb - 1 ) / b.tb + 1 );
+    for ( size_t q = 0; q < b.bstart.size(); ++q )
+      b.bstart[ q ] = static_cast< uint32_t >( std::min< uint64_t >( static_cast< uint64_t >( q ) * b.tb, N ) );
+  }
+  {
+    uint32_t largest = 0;
+    for ( size_t q = 0; q + 1 < b.bstart.size(); ++q )
+      largest = std::max( largest, b.bstart[ q + 1 ] - b.bstart[ q ] );
+    if ( targets_per_block_ == 0 )
+      b.tb = std::max< uint32_t >( 8, ( largest + 7 ) / 8 * 8 );
+  }
+  block_start_ = b.bstart;
   const uint32_t P = b.n_blocks();
   if ( b.win )
   {
@@ -140,28 +335,6 @@ Kernel::build_spec( BuildSpec& b )
     b.np = std::max( 1, std::min( delivery_passes_, b.max_delay ) );
     b.ws = ( b.max_delay + b.np - 1 ) / b.np;
   }
-  // ---- device index = position in NEST's receive order (VP, then node id)
-  std::vector< uint32_t > order( N );
-  for ( uint32_t i = 0; i < N; ++i )
-    order[ i ] = i;
-  std::stable_sort( order.begin(), order.end(),
-    [ this ]( uint32_t a, uint32_t c ) { return vp_of( neuron_gid_[ a ] ) < vp_of( neuron_gid_[ c ] ); } );
-  std::vector< uint32_t > dev_of( N );
-  for ( uint32_t o = 0; o < N; ++o )
-    dev_of[ order[ o ] ] = o;
-  b.vp_begin.assign( n_vp_ + 1, 0 );
-  for ( uint32_t o = 0; o < N; ++o )
-    b.vp_begin[ vp_of( neuron_gid_[ order[ o ] ] ) + 1 ]++;
-  for ( int v = 0; v < n_vp_; ++v )
-    b.vp_begin[ v + 1 ] += b.vp_begin[ v ];
-  gid_to_dev_.assign( nodes_.size() + 1, -1 );
-  dev_to_gid_.assign( N, 0 );
-  for ( uint32_t i = 0; i < N; ++i )
-  {
-    gid_to_dev_[ neuron_gid_[ i ] ] = static_cast< int32_t >( dev_of[ i ] );
-    dev_to_gid_[ dev_of[ i ] ] = neuron_gid_[ i ];
-  }
-
   // ---- drive: at most one spike_generator per neuron
   std::vector< int64_t > gen_of( N, -1 );
   std::vector< double > drive_w( N, 0 );
@@ -474,7 +647,10 @@ Kernel::build_spec( BuildSpec& b )
     } );
   start = std::vector< uint32_t >();
   src_dev = HostArray< uint32_t >();
-  const uint32_t tbs = b.tb;
+  // block of every device index
+  std::vector< uint32_t > blk_of( N );
+  for ( uint32_t q = 0; q < P; ++q )
+    std::fill( blk_of.begin() + b.bstart[ q ], blk_of.begin() + b.bstart[ q + 1 ], q );
   b.boff.resize( static_cast< size_t >( N ) * ( P + 1 ) );
   parallel(
     [ & ]( int w )
@@ -496,7 +672,7 @@ Kernel::build_spec( BuildSpec& b )
         uint32_t* bo = &b.boff[ static_cast< size_t >( s ) * ( P + 1 ) ];
         std::fill( bo, bo + P + 1, 0u );
         for ( uint64_t j = r0; j < r1; ++j )
-          ++bo[ b.syn[ j ].word / tbs + 1 ];
+          ++bo[ blk_of[ b.syn[ j ].word ] + 1 ];
         bo[ 0 ] = static_cast< uint32_t >( r0 );
         for ( uint32_t q = 0; q < P; ++q )
         {
@@ -508,11 +684,11 @@ Kernel::build_spec( BuildSpec& b )
         for ( size_t j = 0; j < row_syn.size(); ++j )
         {
           const uint32_t t = row_syn[ j ].word;
-          const uint32_t blk = t / tbs;
+          const uint32_t blk = blk_of[ t ];
           SynRec& rec = b.syn[ cur[ blk ]++ ];
           rec.w_lo = row_syn[ j ].w_lo;
           rec.w_hi = row_syn[ j ].w_hi;
-          rec.word = ( t - blk * tbs ) | ( static_cast< uint32_t >( row_delay[ j ] ) << 16 );
+          rec.word = ( t - b.bstart[ blk ] ) | ( static_cast< uint32_t >( row_delay[ j ] ) << 16 );
         }
         for ( uint32_t q = 0; q < P; ++q )
         {

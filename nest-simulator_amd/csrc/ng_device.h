@@ -83,8 +83,8 @@ static_assert( CACHE_SLOTS * ( 4 + 8 ) <= WIN_CAP * 8, "the cell cache must fit 
 struct NetDev
 {
   // ---- sizes
-  uint32_t n;         // neurons
-  uint32_t tb;        // targets per block
+  uint32_t n;         // neurons (below 2^24: spike list entries carry the device index)
+  uint32_t tb;        // largest block: the stride of a block's ring-buffer slot and spike list
   uint32_t n_blocks;  // blocks of this instance
   uint32_t blk0;      // first global block index of this instance in a launch
   int32_t min_delay, max_delay, L;  // steps; ring length L = min + max
@@ -106,9 +106,17 @@ struct NetDev
   const uint8_t* model;  // per neuron model id (used when uniform_model < 0)
   const ParamSet* psets;
   double* rb;         // [block][slot][channel][tb], slot = absolute step % L
-  // ---- spikes: per block lists of (local << 16 | part << 8 | lag), two slice parities
+  // ---- blocks: block b owns device indices [bstart[b], bstart[b + 1]), sizes balanced by
+  //      the expected delivery work (in-degree) of their targets
+  const uint32_t* bstart;
+  // ---- spikes: per block lists of (device index << 8 | lag) and the run part of each entry,
+  //      two slice parities
   uint32_t* blk_cnt;  // [2][n_blocks]
   uint32_t* blk_spk;  // [2][n_blocks][spk_cap]
+  uint8_t* blk_part;  // [2][n_blocks][spk_cap]
+  // ---- received events per target (device index), counted while set: the measured
+  //      delivery work that target blocks are balanced by
+  uint32_t* rx;
   uint32_t* flat_src; // exact receive order of a slice split across Run calls
   uint32_t* flat_lag;
   uint32_t* flat_meta;  // [0..1] flat_n per parity, [2..3] flat_ready per parity

@@ -37,7 +37,15 @@ void
 Engine::free_list( std::vector< void* >& l )
 {
   for ( void* p : l )
+  {
     hipFree( p );
+    const auto it = sizes_.find( p );
+    if ( it != sizes_.end() )
+    {
+      bytes_ -= it->second;
+      sizes_.erase( it );
+    }
+  }
   l.clear();
 }
 
@@ -51,6 +59,7 @@ Engine::alloc( size_t bytes )
   hip_check( hipMalloc( &p, bytes ), "hipMalloc" );
   hip_check( hipMemset( p, 0, bytes ), "hipMemset" );
   bytes_ += bytes;
+  sizes_[ p ] = bytes;
   return p;
 }
 
@@ -67,14 +76,28 @@ Engine::upload( const std::vector< T, A >& v )
 void
 Engine::build( const BuildSpec& b, bool keep_state )
 {
+  build_layout( b, keep_state, false );
+}
+
+void
+Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
+{
   hip_check( hipSetDevice( device_ ), "hipSetDevice" );
-  if ( keep_state and ( not built_ or net_.n != b.n or net_.tb != b.tb or net_.min_delay != b.min_delay
-                        or net_.max_delay != b.max_delay or net_.nch != b.nch ) )
+  if ( keep_state
+       and ( not built_ or net_.n != b.n or net_.min_delay != b.min_delay or net_.max_delay != b.max_delay
+             or net_.nch != b.nch or ( not relayout and ( net_.tb != b.tb or bstart_ != b.bstart ) ) ) )
     throw std::runtime_error( "device store rebuild changed the network shape after Simulate" );
   if ( b.min_delay > 256 )
     throw std::runtime_error( "NotImplemented: min_delay above 256 steps on the device path" );
   if ( b.tb == 0 or b.tb > MAX_TB )
     throw std::runtime_error( "targets_per_block must be in [1, " + std::to_string( MAX_TB ) + "]" );
+  if ( b.n >= ( 1u << 24 ) )
+    throw std::runtime_error( "NotImplemented: 2^24 or more neurons in one instance" );
+  if ( b.bstart.size() != b.n_blocks() + 1 or b.bstart.front() != 0 or b.bstart.back() != b.n )
+    throw std::runtime_error( "target blocks do not cover the neurons" );
+  for ( size_t q = 0; q + 1 < b.bstart.size(); ++q )
+    if ( b.bstart[ q + 1 ] <= b.bstart[ q ] or b.bstart[ q + 1 ] - b.bstart[ q ] > b.tb )
+      throw std::runtime_error( "target blocks must be non-empty and at most targets_per_block large" );
   if ( b.nch != 1 and b.nch != 2 )
     throw std::runtime_error( "ring-buffer channels must be 1 or 2" );
   const uint32_t P = b.n_blocks();
@@ -126,6 +149,8 @@ Engine::build( const BuildSpec& b, bool keep_state )
   net_.pidx = static_cast< const uint16_t* >( S( upload( b.pidx ) ) );
   net_.model = static_cast< const uint8_t* >( S( upload( b.model ) ) );
   net_.vp_begin = static_cast< const uint32_t* >( S( upload( b.vp_begin ) ) );
+  net_.bstart = static_cast< const uint32_t* >( S( upload( b.bstart ) ) );
+  bstart_ = b.bstart;
   net_.drive_counts = static_cast< const uint8_t* >( S( upload( b.drive_counts ) ) );
   net_.drive_cols = b.drive_cols;
   net_.drive_idx = static_cast< const uint32_t* >( S( upload( b.drive_idx ) ) );
@@ -145,6 +170,8 @@ Engine::build( const BuildSpec& b, bool keep_state )
     net_.blk_cnt = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * 2 * std::max< uint32_t >( P, 1 ) ) ) );
     net_.blk_spk = static_cast< uint32_t* >(
       T( alloc( sizeof( uint32_t ) * 2 * static_cast< size_t >( std::max< uint32_t >( P, 1 ) ) * net_.spk_cap ) ) );
+    net_.blk_part = static_cast< uint8_t* >(
+      T( alloc( sizeof( uint8_t ) * 2 * static_cast< size_t >( std::max< uint32_t >( P, 1 ) ) * net_.spk_cap ) ) );
     const size_t flat_cap = static_cast< size_t >( std::max< uint32_t >( b.n, 1 ) ) * b.min_delay;
     net_.flat_src = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * flat_cap ) ) );
     net_.flat_lag = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * flat_cap ) ) );
@@ -152,9 +179,30 @@ Engine::build( const BuildSpec& b, bool keep_state )
     net_.log_cap = 0;
     net_.log_n = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) ) ) );
     net_.counters = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) * N_COUNTERS ) ) );
+    rx_ = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * std::max< uint32_t >( b.n, 1 ) ) ) );
+    net_.rx = nullptr;
     clock = 0;
     from_step = 0;
     part = 0;
+  }
+  if ( keep_state and relayout )
+  {
+    // new ring buffers and spike lists for the new blocks (the caller moves their contents)
+    auto replace = [ this ]( void* old, size_t bytes )
+    {
+      void* p = alloc( bytes );
+      for ( auto& q : state_allocs_ )
+        if ( q == old )
+          q = p;
+      std::vector< void* > one( 1, old );
+      free_list( one );
+      return p;
+    };
+    const size_t P1 = std::max< uint32_t >( P, 1 );
+    net_.rb = static_cast< double* >( replace( net_.rb, sizeof( double ) * static_cast< size_t >( P ) * net_.L * b.nch * b.tb ) );
+    net_.blk_cnt = static_cast< uint32_t* >( replace( net_.blk_cnt, sizeof( uint32_t ) * 2 * P1 ) );
+    net_.blk_spk = static_cast< uint32_t* >( replace( net_.blk_spk, sizeof( uint32_t ) * 2 * P1 * net_.spk_cap ) );
+    net_.blk_part = static_cast< uint8_t* >( replace( net_.blk_part, sizeof( uint8_t ) * 2 * P1 * net_.spk_cap ) );
   }
   if ( b.log_cap != net_.log_cap )
   {
@@ -200,6 +248,97 @@ Engine::build( const BuildSpec& b, bool keep_state )
   net_.todo = 0;
   net_.active = 0;
   built_ = true;
+}
+
+bool
+Engine::relayout( const BuildSpec& b )
+{
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  if ( not built_ or part != 0 or from_step != 0 or b.n != net_.n or b.nch != net_.nch or b.min_delay != net_.min_delay
+       or b.max_delay != net_.max_delay or b.bstart.size() < 2 )
+    return false;
+  hip_check( hipDeviceSynchronize(), "relayout: synchronize" );
+  const uint32_t P0 = net_.n_blocks, tb0 = net_.tb, cap0 = net_.spk_cap;
+  const size_t cells = static_cast< size_t >( net_.L ) * net_.nch;  // ring cells per target
+  const std::vector< uint32_t > start0 = bstart_;
+  std::vector< uint32_t > cnt0( 2 * static_cast< size_t >( P0 ) ), spk0( cnt0.size() * cap0 );
+  std::vector< uint8_t > part0( spk0.size() );
+  std::vector< double > rb0( static_cast< size_t >( P0 ) * cells * tb0 );
+  auto d2h = []( void* dst, const void* src, size_t bytes )
+  {
+    if ( bytes )
+      hip_check( hipMemcpy( dst, src, bytes, hipMemcpyDeviceToHost ), "relayout: read" );
+  };
+  d2h( cnt0.data(), net_.blk_cnt, cnt0.size() * sizeof( uint32_t ) );
+  d2h( spk0.data(), net_.blk_spk, spk0.size() * sizeof( uint32_t ) );
+  d2h( part0.data(), net_.blk_part, part0.size() );
+  d2h( rb0.data(), net_.rb, rb0.size() * sizeof( double ) );
+  build_layout( b, true, true );
+  const uint32_t P1 = net_.n_blocks, tb1 = net_.tb, cap1 = net_.spk_cap;
+  auto block_of = [ this ]( uint32_t i )
+  { return static_cast< uint32_t >( std::upper_bound( bstart_.begin(), bstart_.end(), i ) - bstart_.begin() ) - 1; };
+  // ring cells: target i keeps its cells, at its place in its new block
+  std::vector< double > rb1( static_cast< size_t >( P1 ) * cells * tb1, 0.0 );
+  for ( uint32_t q0 = 0; q0 < P0; ++q0 )
+    for ( uint32_t i = start0[ q0 ]; i < start0[ q0 + 1 ]; ++i )
+    {
+      const uint32_t q1 = block_of( i );
+      const size_t l0 = i - start0[ q0 ], l1 = i - bstart_[ q1 ];
+      for ( size_t c = 0; c < cells; ++c )
+        rb1[ ( q1 * cells + c ) * tb1 + l1 ] = rb0[ ( q0 * cells + c ) * tb0 + l0 ];
+    }
+  // spike lists: the lists of consecutive blocks, read in block order, hold their entries in
+  // device-index order; cutting that sequence at the new bounds keeps it
+  std::vector< uint32_t > cnt1( 2 * static_cast< size_t >( P1 ), 0 ), spk1( cnt1.size() * cap1, 0 );
+  std::vector< uint8_t > part1( spk1.size(), 0 );
+  for ( size_t par = 0; par < 2; ++par )
+    for ( uint32_t q0 = 0; q0 < P0; ++q0 )
+    {
+      const size_t l0 = par * P0 + q0;
+      for ( uint32_t k = 0; k < std::min( cnt0[ l0 ], cap0 ); ++k )
+      {
+        const uint32_t e = spk0[ l0 * cap0 + k ];
+        const size_t l1 = par * P1 + block_of( e >> 8 );
+        const uint32_t o = cnt1[ l1 ]++;
+        if ( o >= cap1 )
+          throw std::runtime_error( "relayout: spike list of a new block overflows" );
+        spk1[ l1 * cap1 + o ] = e;
+        part1[ l1 * cap1 + o ] = part0[ l0 * cap0 + k ];
+      }
+    }
+  auto h2d = []( void* dst, const void* src, size_t bytes )
+  {
+    if ( bytes )
+      hip_check( hipMemcpy( dst, src, bytes, hipMemcpyHostToDevice ), "relayout: write" );
+  };
+  h2d( net_.rb, rb1.data(), rb1.size() * sizeof( double ) );
+  h2d( net_.blk_cnt, cnt1.data(), cnt1.size() * sizeof( uint32_t ) );
+  h2d( net_.blk_spk, spk1.data(), spk1.size() * sizeof( uint32_t ) );
+  h2d( net_.blk_part, part1.data(), part1.size() );
+  return true;
+}
+
+void
+Engine::count_received( bool on )
+{
+  net_.rx = on ? rx_ : nullptr;
+}
+
+void
+Engine::read_received( std::vector< uint32_t >& out ) const
+{
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  out.resize( net_.n );
+  if ( net_.n )
+    hip_check( hipMemcpy( out.data(), rx_, net_.n * sizeof( uint32_t ), hipMemcpyDeviceToHost ), "read received" );
+}
+
+void
+Engine::clear_received()
+{
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  if ( net_.n )
+    hip_check( hipMemset( rx_, 0, net_.n * sizeof( uint32_t ) ), "clear received" );
 }
 
 void

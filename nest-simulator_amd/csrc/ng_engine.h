@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 #include "ng_device.h"
@@ -107,10 +108,12 @@ struct BuildSpec
   uint32_t mm_cols = 0;
   int32_t mm_int = 0;
   int64_t mm_off = 0;
+  // target blocks: block q holds device indices [bstart[q], bstart[q + 1]), at most tb each
+  std::vector< uint32_t > bstart;
   uint32_t
   n_blocks() const
   {
-    return n == 0 ? 0 : ( n + tb - 1 ) / tb;
+    return bstart.empty() ? ( n == 0 ? 0 : ( n + tb - 1 ) / tb ) : static_cast< uint32_t >( bstart.size() - 1 );
   }
 };
 
@@ -125,6 +128,22 @@ public:
   // First call allocates everything; later calls (connections changed) replace the
   // synapse store and inputs and keep state, ring buffers and pending spikes.
   void build( const BuildSpec& b, bool keep_state );
+
+  // Same network over new target blocks (b differs from the current layout in bstart and tb
+  // only): the store is replaced and the ring buffers and pending spike lists are moved to
+  // the new blocks. Only at a slice boundary (part == 0) whose pending lists hold one run part;
+  // returns false, changing nothing, otherwise.
+  bool relayout( const BuildSpec& b );
+  const std::vector< uint32_t >&
+  block_start() const
+  {
+    return bstart_;
+  }
+
+  // received events per target, counted by the slice kernel while switched on
+  void count_received( bool on );
+  void read_received( std::vector< uint32_t >& out ) const;
+  void clear_received();
 
   void set_state( int field, uint32_t dev, double v );
   void get_state( int field, std::vector< double >& out );
@@ -197,12 +216,16 @@ private:
   template < typename T, typename A >
   T* upload( const std::vector< T, A >& v );
   void free_list( std::vector< void* >& l );
+  void build_layout( const BuildSpec& b, bool keep_state, bool relayout );
 
   int device_;
   bool built_ = false;
   int mask_ = 0;
   NetDev net_ {};
   std::vector< void* > state_allocs_, store_allocs_, log_allocs_, mm_allocs_;
+  std::vector< uint32_t > bstart_;  // target-block bounds of the current layout
+  uint32_t* rx_ = nullptr;          // received-event counts (state allocation)
+  std::unordered_map< void*, size_t > sizes_;  // bytes of every live allocation
   uint64_t bytes_ = 0;
 };
 

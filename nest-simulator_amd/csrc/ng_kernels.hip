@@ -496,6 +496,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   const SlicePhase ph = slice_phase( d, s );
   if ( not ph.active or b >= d.n_blocks )
     return;
+  // the block's targets: device indices dev0 .. dev0 + nb - 1
+  const uint32_t dev0 = glob( d.bstart )[ b ];
+  const uint32_t nb = glob( d.bstart )[ b + 1 ] - dev0;
   // phase timing (counters[4..6]): thread 0 reads the 100 MHz wall clock at the start, after
   // delivery and at the end of the workgroup's work
   const bool ptime = d.phase_timing != 0;
@@ -575,6 +578,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     __syncthreads();
     NG_PHASE( 0 );
     unsigned long long events = 0;
+    // events received by this thread's target (local index t), counted while d.rx is set
+    uint32_t rxc = 0;
     for ( uint32_t k0 = 0; k0 < K; k0 += CH )
     {
       // each thread resolves SPT consecutive spikes of the chunk: (source, lag), then its
@@ -604,7 +609,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               ++blk;
             const uint32_t e =
               glob( d.blk_spk )[ ( static_cast< size_t >( ppar ) * P + blk ) * d.spk_cap + ( k - pref[ blk ] ) ];
-            src[ q ] = blk * tb + ( e >> 16 );
+            src[ q ] = e >> 8;
             lag[ q ] = e & 0xffu;
           }
         }
@@ -704,7 +709,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
                 : MASK == 2           ? M_IAF_PSC_EXP
                 : MASK == 4           ? M_AEIF_COND_EXP
                 : umodel >= 0         ? umodel
-                                      : glob( d.model )[ b * tb + local ];
+                                      : glob( d.model )[ dev0 + local ];
               uint32_t ch = 0;
               if ( m == M_IAF_PSC_EXP )
                 ch = w > 0 ? 1u : 0u;
@@ -773,7 +778,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               : MASK == 2           ? M_IAF_PSC_EXP
               : MASK == 4           ? M_AEIF_COND_EXP
               : umodel >= 0         ? umodel
-                                    : glob( d.model )[ b * tb + local ];
+                                    : glob( d.model )[ dev0 + local ];
             uint32_t ch = 0;
             double v = w;  // static_synapse: weight * multiplicity 1
             if ( m == M_IAF_PSC_EXP )
@@ -841,6 +846,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             S.bst[ BS ] = run;
         }
         S.bcnt[ cur ^ 1 ][ t ] = 0;
+        if ( d.rx )
+          rxc += S.bcnt[ cur ][ t ];
         __syncthreads();
         NG_PHASE( 3 );  // bucket starts
 #pragma unroll
@@ -925,24 +932,26 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     }
     if ( t == 0 and events )
       g_add( &d.counters[ 1 ], events );
+    // the block owns its targets' counts: one plain read-modify-write per target and slice
+    if ( d.rx and t < nb and rxc )
+      glob( d.rx )[ dev0 + t ] += rxc;
   }
   __syncthreads();  // window staged and delivered
   if ( ptime and t == 0 )
     tp1 = wall_clock64();
 
   // ================================================================ 2. update
-  // Threads [0, tb) update one neuron each. The spike_generator drive of the slice lands in
+  // Threads [0, nb) update one neuron each. The spike_generator drive of the slice lands in
   // cells at least min_delay ahead (every delay, the drive's included, is >= min_delay), which
-  // the update does not touch, and in distinct cells per lag: when tb <= BS / 2 the spare
-  // threads add it concurrently, G groups of tb threads splitting the lags.
+  // the update does not touch, and in distinct cells per lag: when nb <= BS / 2 the spare
+  // threads add it concurrently, G groups of nb threads splitting the lags.
   const uint32_t local = t;
-  const uint32_t dev = b * tb + local;
-  const uint32_t G = ( static_cast< uint32_t >( BS ) - tb ) / tb;
+  const uint32_t dev = dev0 + local;
+  const uint32_t G = ( static_cast< uint32_t >( BS ) - nb ) / nb;
   uint64_t bits[ 4 ] = { 0, 0, 0, 0 };
-  if ( local >= tb and local < tb + G * tb )
+  if ( local >= nb and local < nb + G * nb )
   {
-    const uint32_t dl = ( local - tb ) % tb, g = ( local - tb ) / tb, ddev = b * tb + dl;
-    if ( ddev < d.n )
+    const uint32_t dl = ( local - nb ) % nb, g = ( local - nb ) / nb, ddev = dev0 + dl;
     {
       gref< const ParamSet > dps = glob( d.psets )[ glob( d.pidx )[ ddev ] ];
       const int dm = ( MASK == 1 ) ? M_IAF_PSC_DELTA
@@ -952,7 +961,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       drive_adds( d, dps, dm, ddev, dl, C, ph.origin, ph.from + static_cast< int >( g ), static_cast< int >( G ), ph.to );
     }
   }
-  if ( local < tb and dev < d.n )
+  if ( local < nb )
   {
     gref< const ParamSet > ps = glob( d.psets )[ glob( d.pidx )[ dev ] ];
     // multimeter: the next step of this part whose state is sampled (ph.to: none)
@@ -1208,6 +1217,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     uint32_t o = S.cnt_base + ex;
     unsigned long long li = S.log_base + ex;
     gptr< uint32_t > lst = glob( d.blk_spk ) + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+    gptr< uint8_t > lpart = glob( d.blk_part ) + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
     for ( int w = 0; w < 4; ++w )
     {
       unsigned long long x = bits[ w ];
@@ -1216,7 +1226,10 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
         const uint32_t lag = w * 64 + ( __ffsll( static_cast< long long >( x ) ) - 1 );
         x &= x - 1;
         if ( o < d.spk_cap )
-          lst[ o ] = ( local << 16 ) | ( static_cast< uint32_t >( ph.part ) << 8 ) | lag;
+        {
+          lst[ o ] = ( dev << 8 ) | lag;
+          lpart[ o ] = static_cast< uint8_t >( ph.part );
+        }
         ++o;
         if ( log_on )
         {
@@ -1282,23 +1295,25 @@ __global__ void __launch_bounds__( FLATTEN_BS )
     const uint32_t lo = d.vp_begin[ vp ], hi = d.vp_begin[ vp + 1 ];
     if ( lo >= hi )
       continue;
-    const uint32_t b0 = lo / d.tb;
-    const uint32_t b1 = min( ( hi - 1 ) / d.tb + 1, P );
+    // the blocks holding device indices lo .. hi - 1
+    const uint32_t b0 = upper_bound_u32( d.bstart, P + 1, lo ) - 1;
+    const uint32_t b1 = upper_bound_u32( d.bstart, P + 1, hi - 1 );
     for ( int p = 0; p < n_parts; ++p )
       for ( uint32_t c0 = b0; c0 < b1; c0 += FLATTEN_BS )
       {
         const uint32_t b = c0 + t;
         uint32_t n = 0, cnt = 0;
         const uint32_t* lst = nullptr;
+        const uint8_t* lpart = nullptr;
         if ( b < b1 )
         {
           n = min( d.blk_cnt[ static_cast< size_t >( par ) * P + b ], d.spk_cap );
           lst = d.blk_spk + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
+          lpart = d.blk_part + ( static_cast< size_t >( par ) * P + b ) * d.spk_cap;
           for ( uint32_t q = 0; q < n; ++q )
           {
-            const uint32_t e = lst[ q ];
-            const uint32_t dev = b * d.tb + ( e >> 16 );
-            cnt += static_cast< int >( ( e >> 8 ) & 0xffu ) == p and dev >= lo and dev < hi ? 1u : 0u;
+            const uint32_t dev = lst[ q ] >> 8;
+            cnt += lpart[ q ] == p and dev >= lo and dev < hi ? 1u : 0u;
           }
         }
         uint32_t ex;
@@ -1307,8 +1322,8 @@ __global__ void __launch_bounds__( FLATTEN_BS )
         for ( uint32_t q = 0; q < n; ++q )
         {
           const uint32_t e = lst[ q ];
-          const uint32_t dev = b * d.tb + ( e >> 16 );
-          if ( static_cast< int >( ( e >> 8 ) & 0xffu ) != p or dev < lo or dev >= hi )
+          const uint32_t dev = e >> 8;
+          if ( lpart[ q ] != p or dev < lo or dev >= hi )
             continue;
           d.flat_src[ w ] = dev;
           d.flat_lag[ w ] = e & 0xffu;

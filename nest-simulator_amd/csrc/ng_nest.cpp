@@ -441,6 +441,11 @@ Kernel::reset()
   rec_events_.clear();
   spike_count_ = 0;
   targets_per_block_ = 0;
+  balanced_blocks_ = true;
+  block_start_.clear();
+  rebalanced_ = false;
+  rx_steps_ = 0;
+  block_relayouts_ = 0;
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
   param_states_.normals.clear();
@@ -524,6 +529,14 @@ Kernel::set_kernel_status( const Json& d )
       if ( simulated_ )
         throw std::runtime_error( "KernelException: targets_per_block cannot change after Simulate" );
       targets_per_block_ = v;
+      store_dirty_ = true;
+    }
+    else if ( k == "balanced_blocks" )
+    {
+      const bool v = kv.second.number() != 0;
+      if ( simulated_ and v != balanced_blocks_ )
+        throw std::runtime_error( "KernelException: balanced_blocks cannot change after Simulate" );
+      balanced_blocks_ = v;
       store_dirty_ = true;
     }
     else if ( k == "delivery_mode" )
@@ -633,6 +646,9 @@ Kernel::get_kernel_status_json() const
     << ", \"time_gather_spike_data\": " << time_kernel_ms_[ K_FLATTEN ] * 1e-3
     << ", \"time_deliver_spike_data\": 0, \"fused_deliver_update\": true"
     << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().tb ) : targets_per_block_ )
+    << ", \"balanced_blocks\": " << ( balanced_blocks_ ? "true" : "false" )
+    << ", \"block_relayouts\": " << block_relayouts_
+    << ", \"target_blocks\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().n_blocks ) : 0 )
     << ", \"delivery_mode\": \""
     << ( delivery_mode_ == 0 ? "auto" : delivery_mode_ == 1 ? "window" : delivery_mode_ == 2 ? "global" : "cache" ) << "\""
     << ", \"window_delivery\": " << ( engine_ and engine_->built() and engine_->window() ? "true" : "false" )
@@ -2112,6 +2128,8 @@ Kernel::reset_device()
   prepared_ = false;
   simulated_ = false;
   store_dirty_ = true;
+  rebalanced_ = false;
+  rx_steps_ = 0;
   for ( auto& e : rec_events_ )
     e.clear();
   for ( Meter& m : meters_ )
@@ -2157,11 +2175,14 @@ Kernel::estimate_device_bytes() const
     tb = std::max< uint32_t >( 32, std::min< uint32_t >( MAX_TB, static_cast< uint32_t >( ( N + 511 ) / 512 ) ) );
   }
   const uint64_t P = ( N + tb - 1 ) / tb;
+  // balanced blocks: as many blocks at most, each up to twice as large
+  if ( targets_per_block_ == 0 and balanced_blocks_ )
+    tb = block_cap( tb );
   const uint64_t L = dmin + dmax;
   uint64_t b = static_cast< uint64_t >( E ) * sizeof( SynRec ) + N * ( P + 1 ) * 4  // store
     + P * L * nch * tb * 8                                                          // ring buffers
     + N * ( 5 * 8 + 4 + 2 + 1 + 4 )                                                 // state, indices
-    + 2 * P * tb * dmin * 4 + 2 * N * dmin * 4                                     // spike lists
+    + 2 * P * tb * dmin * 5 + 2 * N * dmin * 4                                     // spike lists
     + ( std::any_of( recorders_.begin(), recorders_.end(), []( const auto& r ) { return not r.empty(); } )
           ? std::max< uint64_t >( 1u << 20, N * 16 ) * 12
           : 0 );
@@ -2262,9 +2283,16 @@ Kernel::run( double t_ms )
     kl0[ q ] = runner_->kernel_launches[ q ];
   }
   const int64_t gc0 = runner_->graph_chunks, dc0 = runner_->direct_chunks, cf0 = runner_->capture_failures;
+  // received events are counted for the measured block balance until it has been applied
+  const bool measure = runner_ == own_runner_.get() and balanced_blocks_ and targets_per_block_ == 0
+    and not rebalanced_ and not host_connections_freed_;
+  engine_->count_received( measure );
   const auto t0 = std::chrono::steady_clock::now();
   runner_->run( { engine_.get() }, { steps } );
   const auto t1 = std::chrono::steady_clock::now();
+  engine_->count_received( false );
+  if ( measure )
+    rx_steps_ += steps;
   last_wall_ms_ = std::chrono::duration< double, std::milli >( t1 - t0 ).count();
   time_simulate_ms_ += last_wall_ms_;
   last_kernel_ms_ = runner_->last_kernel_ms();
@@ -2277,6 +2305,14 @@ Kernel::run( double t_ms )
   direct_chunks_ += runner_->direct_chunks - dc0;
   capture_failures_ += runner_->capture_failures - cf0;
   collect_log();
+  if ( measure )
+  {
+    const auto t2 = std::chrono::steady_clock::now();
+    rebalance_blocks();
+    const double ms = std::chrono::duration< double, std::milli >( std::chrono::steady_clock::now() - t2 ).count();
+    last_wall_ms_ += ms;
+    time_simulate_ms_ += ms;
+  }
 }
 
 void

@@ -475,6 +475,10 @@ private:
   void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
   void build_spec( BuildSpec& b );
   void choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const;
+  // block bounds balancing in-degree over at most ceil(n / tb) blocks
+  std::vector< uint32_t > balanced_partition( uint32_t n, uint32_t tb ) const;
+  static uint32_t block_cap( uint32_t tb );  // largest balanced block for a nominal size tb
+  void rebalance_blocks();
   static int window_slots( int max_delay, int nch, uint32_t tb );
   void collect_log();
 
@@ -547,6 +551,12 @@ private:
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
   int targets_per_block_ = 0;  // 0: chosen from the network size
+  bool balanced_blocks_ = true;  // automatic block size: blocks balanced by in-degree, then by
+                                 // the measured received events
+  std::vector< uint32_t > block_start_;  // target-block bounds of the device copy
+  bool rebalanced_ = false;              // the measured balance has been applied (or declined)
+  int64_t rx_steps_ = 0;                 // steps run with received events counted
+  int64_t block_relayouts_ = 0;
   int delivery_mode_ = 0;      // 0 automatic, 1 window (LDS ring-buffer slots), 2 global
   bool keep_host_connections_ = true;  // false: the host copy is freed once the device holds it
   bool host_connections_freed_ = false;
