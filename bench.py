@@ -464,7 +464,7 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
     k = nestgpu.Kernel(0)
     k.SetKernelStatus({"delivery_mode": args.delivery_mode, "targets_per_block": args.targets_per_block,
                        "workgroups_per_cu": args.workgroups_per_cu, "window_slots": args.window_slots,
-                       "keep_host_connections": False, "graph_launch": bool(args.graph_launch)})
+                       "graph_launch": bool(args.graph_launch)})
     if workload == "c2":
         build_c2(k, nestgpu, args.n, args.ke, args.ki, args.n_vp, args.seed, args.drive_seed)
         n_neurons = args.n
@@ -481,9 +481,11 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
         k.Run(args.t_step)
         print("%s warmup %d done" % (workload, i), file=sys.stderr, flush=True)
     st0 = k.GetKernelStatus()
+    kernel_ms = 0.0  # GPU time of the slice launches: HIP events around every chunk on the run's stream
     t0 = time.perf_counter()
     for i in range(steps):
         k.Run(args.t_step)  # Run synchronizes the device before it returns
+        kernel_ms += k.GetKernelStatus("time_simulate_kernel_ms")
     wall = time.perf_counter() - t0
     st = k.GetKernelStatus()
     events = st["synaptic_events"] - st0["synaptic_events"]
@@ -496,9 +498,23 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
     rate = spikes / n_neurons / bio_s
     print("%s timed: %d steps, %.3f s wall, RTF %.4f, %.3g events/s, rate %.2f Hz" % (
         workload, steps, wall, rtf, value, rate), file=sys.stderr, flush=True)
+    min_delay = st["min_delay_steps"]
+    steps_t = steps * int(round(args.t_step / 0.1))
+    slices_t = steps_t // min_delay
+    # algorithmic bytes of the fused slice kernel, per unit:
+    #   28 B per recurrent synaptic event (12 B synapse record, 16 B ring-buffer cell RMW)
+    #   32 B per neuron-step (two ring-buffer channels read and cleared)
+    #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
+    #   16 B per nonzero drive count (ring-buffer RMW; counts are generated in the kernel)
+    def alg_bytes(ev, n_steps, n_slices):
+        return 28.0 * ev + 32.0 * n_neurons * n_steps + 56.0 * n_neurons * n_slices + 16.0 * drive_nz_per_step * n_steps
 
-    # ---- roofline pass: HIP events around every slice launch on the simulation stream, and
-    # the workgroups' own clock for the delivery phase
+    avg_us = 1e3 * kernel_ms / slices_t
+    alg_per_launch = alg_bytes(events, steps_t, slices_t) / slices_t
+    achieved = alg_per_launch / (avg_us * 1e-6) / 1e9
+
+    # ---- second pass: HIP events around every single slice launch, and the workgroups' own
+    # clock for the delivery phase
     k.SetKernelStatus({"kernel_timing": True, "phase_timing": True})
     evr0 = k.GetKernelStatus("synaptic_events")
     k.Run(args.roofline_ms)
@@ -507,29 +523,17 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
     events_r = st2["synaptic_events"] - evr0
     kms = st2["kernel_ms"]
     kl = st2["kernel_launches"]
-    steps_r = int(round(args.roofline_ms / 0.1))
-    min_delay = st2["min_delay_steps"]
-    slices_r = steps_r // min_delay
-    # algorithmic bytes of the fused slice kernel, per unit:
-    #   28 B per recurrent synaptic event (12 B synapse record, 16 B ring-buffer cell RMW)
-    #   32 B per neuron-step (two ring-buffer channels read and cleared)
-    #   56 B per neuron per slice (V, I_ex, I_in, r read once and written once)
-    #   16 B per nonzero drive count (ring-buffer RMW; counts are generated in the kernel)
-    nz = drive_nz_per_step * steps_r
     launches = kl["slice"]
-    deliv_bytes = 28.0 * events_r
-    alg_total = deliv_bytes + 32.0 * n_neurons * steps_r + 56.0 * n_neurons * slices_r + 16.0 * nz
-    alg_per_launch = alg_total / launches
-    avg_us = 1e3 * kms["slice"] / launches
-    achieved = alg_per_launch / (avg_us * 1e-6) / 1e9
     ticks = st2["phase_ticks"]
     deliv_share = ticks["deliver"] / ticks["total"] if ticks["total"] else 0.0
     deliv_us = avg_us * deliv_share
-    deliv_achieved = (deliv_bytes / launches) / (deliv_us * 1e-6) / 1e9 if deliv_us > 0 else 0.0
+    deliv_bytes = 28.0 * events / slices_t
+    deliv_achieved = deliv_bytes / (deliv_us * 1e-6) / 1e9 if deliv_us > 0 else 0.0
     per_kernel = {n: {"avg_us": 1e3 * ms / kl[n], "launches": kl[n], "total_ms": ms}
                   for n, ms in kms.items() if kl.get(n)}
-    print("%s slice kernel: %.1f us avg over %d launches (delivery %.1f us, %.0f %%), %.3g events in roofline pass"
-          % (workload, avg_us, launches, deliv_us, 100 * deliv_share, events_r), file=sys.stderr, flush=True)
+    print("%s slice kernel: %.1f us per launch over the timed region (delivery %.1f us, %.0f %%); %.1f us between "
+          "events around single launches" % (workload, avg_us, deliv_us, 100 * deliv_share,
+                                            1e3 * kms["slice"] / launches), file=sys.stderr, flush=True)
     layout = {"delivery": "window" if st2["window_delivery"] else "cache" if st2["cache_delivery"] else "global",
               "targets_per_block": st2["targets_per_block"], "window_slots": st2["window_slots"]}
     del k
@@ -563,14 +567,76 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_source_digest": traffic_digest, "source_digest": source_digest(),
                      "avg_launch_us": avg_us, "alg_bytes_per_launch": alg_per_launch,
-                     "events_per_launch": events_r / launches, "kernels": per_kernel,
+                     "events_per_launch": events / slices_t, "launches": slices_t,
+                     "method": "timed region: GPU time of the slice launches (HIP events around every chunk on the "
+                               "simulation stream) / launches",
+                     "single_launch_pass": {"kernels": per_kernel, "events": events_r,
+                                            "note": "HIP events around each launch, one at a time; includes the "
+                                                    "event and launch overhead"},
                      "delivery": {"achieved": deliv_achieved, "frac": deliv_achieved / HBM_PEAK_GBS,
                                   "avg_us": deliv_us, "share_of_slice": deliv_share,
-                                  "alg_bytes_per_launch": deliv_bytes / launches,
+                                  "alg_bytes_per_launch": deliv_bytes,
                                   "method": "28 B per recurrent event / (slice launch time x the workgroups' mean "
                                             "delivery-phase share of their run time, 100 MHz clock)"}},
         "cpu_baseline": cpu,
     }
+
+
+# ---------------------------------------------------------------- rocprofv3 summaries
+def _csv_rows(path):
+    import csv
+
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def summarize_trace(path, skip, count, kernel="k_slice"):
+    """Per-kernel totals of a rocprofv3 --kernel-trace CSV, and the mean duration of the
+    dispatches [skip, skip + count) of the named kernel (the timed region of a bench run)."""
+    rows = sorted(_csv_rows(path), key=lambda r: int(r["Start_Timestamp"]))
+    per = {}
+    for r in rows:
+        per.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    total = sum(sum(v) for v in per.values())
+    out = ["%-64s %7s %12s %10s %7s" % ("kernel", "calls", "total_ms", "avg_us", "pct")]
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:6]:
+        out.append("%-64s %7d %12.3f %10.2f %7.2f" % (name[:64], len(v), sum(v) * 1e-3, sum(v) / len(v),
+                                                       100.0 * sum(v) / total))
+    ks = [d for r, d in ((r, (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3) for r in rows)
+          if kernel in r["Kernel_Name"]]
+    sel = ks[skip:skip + count]
+    out.append("")
+    out.append("%s: dispatches %d-%d of %d (the timed region): avg %.2f us, median %.2f us" % (
+        kernel, skip, skip + len(sel) - 1, len(ks), float(np.mean(sel)), float(np.median(sel))))
+    return "\n".join(out), float(np.mean(sel))
+
+
+def _counter(path, counter, kernel=None):
+    rows = [r for r in _csv_rows(path) if r["Counter_Name"] == counter and (kernel is None or kernel in r["Kernel_Name"])]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return np.array([float(r["Counter_Value"]) for r in rows])
+
+
+def summarize_pmc(fetch, write, copy_fetch, copy_write, skip, count, kernel="k_slice"):
+    """HBM bytes per dispatch of the named kernel over dispatches [skip, skip + count) from
+    FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for the plain 512 MiB
+    device copy of bench.py --calibrate-copy (its three largest dispatches)."""
+    copy_bytes = 512.0 * 1024 * 1024
+    out, tot = [], 0.0
+    for name, path, cpath in (("FETCH_SIZE", fetch, copy_fetch), ("WRITE_SIZE", write, copy_write)):
+        c = np.sort(_counter(cpath, name))[-3:]
+        scale = copy_bytes / (float(np.median(c)) * 1024.0)
+        v = _counter(path, name, kernel)
+        sel = v[skip:skip + count]
+        per = float(np.mean(sel)) * 1024.0 * scale
+        tot += per
+        out.append("%s: copy reads %.0f KiB for 512 MiB (scale %.3f); %s dispatches %d-%d of %d: mean %.0f KiB, "
+                   "median %.0f KiB -> %.1f MB per launch" % (name, float(np.median(c)), scale, kernel, skip,
+                                                              skip + len(sel) - 1, len(v), float(np.mean(sel)),
+                                                              float(np.median(sel)), per * 1e-6))
+    out.append("traffic_bytes_per_launch: %d" % int(round(tot)))
+    out.append("source_digest: %s" % source_digest())
+    return "\n".join(out)
 
 
 def main():
@@ -618,7 +684,21 @@ def main():
     ap.add_argument("--graph-launch", type=int, default=1, choices=[0, 1],
                     help="1: replay each chunk of slice launches from a captured hipGraph (default); 0: launch "
                          "every slice directly")
+    ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
+                    help="print the kernel-time summary of a rocprofv3 --kernel-trace CSV (with --skip/--count)")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="print the HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE CSVs of a "
+                         "bench run and of --calibrate-copy (with --skip/--count)")
+    ap.add_argument("--skip", type=int, default=0, help="summaries: k_slice dispatches before the timed region")
+    ap.add_argument("--count", type=int, default=1 << 30, help="summaries: k_slice dispatches of the timed region")
     args = ap.parse_args()
+
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.skip, args.count)[0])
+        return
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc, args.skip, args.count))
+        return
 
     if args.calibrate_copy:
         import torch
