@@ -66,10 +66,21 @@ SeedSeqFe128::SeedSeqFe128( const std::vector< uint32_t >& entropy )
       e = mix( e, hash( entropy[ next ] ) );
 }
 
-VpRng::VpRng( const std::vector< uint32_t >& seed )
+VpRng::VpRng( const std::vector< uint32_t >& seed, int rng_type )
+  : type( rng_type )
 {
   SeedSeqFe128 sseq( seed );
-  engine.seed( sseq );
+  if ( type == RNG_PHILOX_32 )
+  {
+    // r123::Engine's seeding: the key is the first two words of the seed sequence, the
+    // counter zero (Random123/array.h seed(), conventional/Engine.hpp)
+    uint32_t k[ 2 ];
+    sseq.generate( k, k + 2 );
+    philox.k0 = k[ 0 ];
+    philox.k1 = k[ 1 ];
+  }
+  else
+    engine.seed( sseq );
 }
 
 // ------------------------------------------------------------------ parameters
@@ -411,6 +422,7 @@ Kernel::reset()
   tb_ = TimeBase();
   n_vp_ = 1;
   seed_ = 143202461;
+  rng_type_ = RNG_MT19937_64;
   rngs_.clear();
   rng_dirty_ = true;
   nodes_.clear();
@@ -467,7 +479,7 @@ Kernel::init_rngs()
   // seed_seq_fe128{ base_seed, THREAD_SPECIFIC_SEEDER_, vp }
   rngs_.clear();
   for ( int vp = 0; vp < n_vp_; ++vp )
-    rngs_.emplace_back( std::vector< uint32_t > { seed_, 0xb84c9baeu, static_cast< uint32_t >( vp ) } );
+    rngs_.emplace_back( std::vector< uint32_t > { seed_, 0xb84c9baeu, static_cast< uint32_t >( vp ) }, rng_type_ );
   rng_dirty_ = false;
 }
 
@@ -602,8 +614,12 @@ Kernel::set_kernel_status( const Json& d )
     }
     else if ( k == "rng_type" )
     {
-      if ( kv.second.string() != "mt19937_64" )
-        throw std::runtime_error( "BadProperty: only rng_type mt19937_64 is supported." );
+      // RandomManager::set_status (random_manager.cpp:160-185): the streams restart
+      const std::string v = kv.second.string();
+      if ( v != "mt19937_64" and v != "Philox_32" )
+        throw std::runtime_error( "BadProperty: rng_type must be 'mt19937_64' or 'Philox_32'." );
+      rng_type_ = v == "Philox_32" ? RNG_PHILOX_32 : RNG_MT19937_64;
+      rng_dirty_ = true;
     }
     else if ( k == "print_time" or k == "overwrite_files" or k == "data_path" )
       continue;
@@ -632,7 +648,8 @@ Kernel::get_kernel_status_json() const
   const long dmin = simulated_ ? min_delay_ : ( d_min_seen_ < 0 ? 1 : d_min_seen_ );
   const long dmax = simulated_ ? max_delay_ : std::max( d_max_seen_, 1L );
   o << "{\"resolution\": " << tb_.resolution_ms() << ", \"tics_per_step\": " << tb_.tics_per_step
-    << ", \"rng_seed\": " << seed_ << ", \"total_num_virtual_procs\": " << n_vp_
+    << ", \"rng_seed\": " << seed_ << ", \"rng_type\": \"" << ( rng_type_ == RNG_PHILOX_32 ? "Philox_32" : "mt19937_64" )
+    << "\", \"rng_types\": [\"mt19937_64\", \"Philox_32\"], \"total_num_virtual_procs\": " << n_vp_
     << ", \"local_num_threads\": " << n_vp_ << ", \"min_delay\": " << dmin * tb_.ms_per_step
     << ", \"max_delay\": " << dmax * tb_.ms_per_step << ", \"min_delay_steps\": " << dmin
     << ", \"max_delay_steps\": " << dmax << ", \"num_connections\": " << num_connections()

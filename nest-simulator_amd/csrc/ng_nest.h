@@ -19,6 +19,7 @@
 
 #include "ng_device.h"
 #include "ng_engine.h"
+#include "ng_philox.h"
 #include "ng_json.h"
 
 namespace nestgpu
@@ -89,21 +90,70 @@ struct SeedSeqFe128
   }
 };
 
-// RandomGenerator< std::mt19937_64 > (random_generators.h:141-297)
+// Philox_32 stream as a uniform random bit generator over 32-bit words (ng_philox.h): the
+// key and the position of the next word; the current block is kept
+struct PhiloxStream
+{
+  typedef uint32_t result_type;
+  uint32_t k0 = 0, k1 = 0;
+  uint64_t pos = 0;
+  uint64_t cached = ~uint64_t( 0 );  // block held in v
+  uint32_t v[ 4 ] = { 0, 0, 0, 0 };
+  static constexpr result_type
+  min()
+  {
+    return 0;
+  }
+  static constexpr result_type
+  max()
+  {
+    return 0xffffffffu;
+  }
+  result_type
+  operator()()
+  {
+    const uint64_t b = pos >> 2;
+    if ( b != cached )
+    {
+      philox_block( k0, k1, b, v );
+      cached = b;
+    }
+    const uint32_t w = v[ 3u - static_cast< uint32_t >( pos & 3u ) ];
+    ++pos;
+    return w;
+  }
+};
+
+enum RngType
+{
+  RNG_MT19937_64 = 0,
+  RNG_PHILOX_32 = 1
+};
+
+// RandomGenerator< std::mt19937_64 > or RandomGenerator< r123::Engine< r123::Philox4x32 > >
+// (random_generators.h:141-297), the draws the builders and parameters make.
 // Per-VP objects are written on every draw or connection by their own thread: each sits on
 // cache lines of its own so that the VP threads of Connect do not share lines.
 struct alignas( 64 ) VpRng
 {
+  int type = RNG_MT19937_64;
   std::mt19937_64 engine;
+  PhiloxStream philox;
   std::uniform_int_distribution< unsigned long > ulong_dist;
-  explicit VpRng( const std::vector< uint32_t >& seed );
-  // std::generate_canonical<double, 53>(engine) in closed form: for a 64-bit engine it takes
-  // one output and divides it by 2^64, clamping a result that rounds to 1 to the double below
-  // (libstdc++ bits/random.tcc). The library's generic form evaluates two long-double
-  // logarithms per draw unless the compiler folds them, which hipcc does not.
+  VpRng( const std::vector< uint32_t >& seed, int rng_type );
+  // std::generate_canonical<double, 53>(engine) in closed form (libstdc++ bits/random.tcc):
+  // a 64-bit engine gives one word divided by 2^64, a 32-bit engine two words summed (the
+  // first low) and scaled by 2^-64; a result that rounds to 1 is clamped to the double below.
+  // The library's generic form evaluates two long-double logarithms per draw unless the
+  // compiler folds them, which hipcc does not.
   double
   canonical()
   {
+    if ( type == RNG_PHILOX_32 )
+    {
+      const uint32_t lo = philox();
+      return philox_canonical( lo, philox() );
+    }
     const double u = static_cast< double >( engine() ) * 0x1p-64;
     return u < 1.0 ? u : std::nextafter( 1.0, 0.0 );
   }
@@ -117,7 +167,7 @@ struct alignas( 64 ) VpRng
   ulrand( unsigned long n )
   {
     std::uniform_int_distribution< unsigned long >::param_type p( 0, n - 1 );
-    return ulong_dist( engine, p );
+    return type == RNG_PHILOX_32 ? ulong_dist( philox, p ) : ulong_dist( engine, p );
   }
 };
 
@@ -488,6 +538,7 @@ private:
   uint32_t seed_ = 143202461;
   std::vector< VpRng > rngs_;
   bool rng_dirty_ = true;
+  int rng_type_ = RNG_MT19937_64;
   std::vector< NodeRec > nodes_;  // index gid - 1
   std::vector< NeuronParams > params_;
   std::vector< NeuronState > state_;

@@ -10,8 +10,11 @@
 // Reference files followed (paths relative to the NEST source tree):
 //   time base ........ nestkernel/nest_time.h:306-311,508-522,539-542; nest_time.cpp:88-103;
 //                      libnestutil/numerics.cpp:41-44 (ld_round)
-//   RNG .............. nestkernel/random_manager.cpp:42-48,95; random_generators.h:141-297;
-//                      thirdparty/randutils.hpp:215-385 (seed_seq_fe128, restated below)
+//   RNG .............. nestkernel/random_manager.cpp:42-48,63-96; random_generators.h:141-297;
+//                      thirdparty/randutils.hpp:215-385 (seed_seq_fe128, restated below);
+//                      Philox_32: thirdparty/Random123/philox.h:240-330 and
+//                      conventional/Engine.hpp (r123::Engine<Philox4x32>, restated below
+//                      from the published Philox4x32-10 algorithm, Salmon et al., SC11)
 //   parameters ....... nestkernel/parameter.h:131-286,468-600,878-990; parameter.cpp:95-117
 //   connect .......... nestkernel/conn_builder.cpp:379-437 (single_connect_), 960-1031 (one_to_one),
 //                      1170-1255 (all_to_all), 1448-1545 (fixed_indegree), 1881-1965 (bernoulli);
@@ -171,27 +174,123 @@ struct SeedSeqFe128
   }
 };
 
-// nestkernel/random_generators.h:141-297: engine + the three draws the builders use.
+// Philox4x32 block function with R rounds (philox.h:296-330): each round multiplies words 0
+// and 2 of the counter by the two Philox constants (64-bit products split in high and low
+// halves), mixes the high halves with words 1 and 3 and the key, and the key is bumped by
+// the two Weyl constants between rounds.
+inline void
+philox4x32( int rounds, const uint32_t ctr_in[ 4 ], const uint32_t key_in[ 2 ], uint32_t out[ 4 ] )
+{
+  uint32_t c[ 4 ] = { ctr_in[ 0 ], ctr_in[ 1 ], ctr_in[ 2 ], ctr_in[ 3 ] };
+  uint32_t k0 = key_in[ 0 ], k1 = key_in[ 1 ];
+  for ( int r = 0; r < rounds; ++r )
+  {
+    if ( r > 0 )
+    {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = static_cast< uint64_t >( 0xD2511F53u ) * c[ 0 ];
+    const uint64_t p1 = static_cast< uint64_t >( 0xCD9E8D57u ) * c[ 2 ];
+    const uint32_t n0 = static_cast< uint32_t >( p1 >> 32 ) ^ c[ 1 ] ^ k0;
+    const uint32_t n1 = static_cast< uint32_t >( p1 );
+    const uint32_t n2 = static_cast< uint32_t >( p0 >> 32 ) ^ c[ 3 ] ^ k1;
+    const uint32_t n3 = static_cast< uint32_t >( p0 );
+    c[ 0 ] = n0;
+    c[ 1 ] = n1;
+    c[ 2 ] = n2;
+    c[ 3 ] = n3;
+  }
+  for ( int i = 0; i < 4; ++i )
+    out[ i ] = c[ i ];
+}
+
+// r123::Engine< r123::Philox4x32 > (conventional/Engine.hpp) as a uniform random bit
+// generator: the key is two words of the seed sequence, the counter starts at zero and is
+// incremented (with carry) before each block; a block's four words are returned last word
+// first.
+struct PhiloxEngine
+{
+  typedef uint32_t result_type;
+  uint32_t key[ 2 ] = { 0, 0 };
+  uint32_t ctr[ 4 ] = { 0, 0, 0, 0 };
+  uint32_t v[ 4 ] = { 0, 0, 0, 0 };
+  int left = 0;  // words of the current block not yet returned
+  static constexpr result_type
+  min()
+  {
+    return 0;
+  }
+  static constexpr result_type
+  max()
+  {
+    return 0xffffffffu;
+  }
+  template < typename SeedSeq >
+  void
+  seed( SeedSeq& s )
+  {
+    s.generate( key, key + 2 );
+    for ( auto& c : ctr )
+      c = 0;
+    left = 0;
+  }
+  result_type
+  operator()()
+  {
+    if ( left == 0 )
+    {
+      for ( int i = 0; i < 4 and ++ctr[ i ] == 0; ++i )
+      {
+      }
+      philox4x32( 10, ctr, key, v );
+      left = 4;
+    }
+    return v[ --left ];
+  }
+};
+
+enum RngKind
+{
+  RNG_MT19937_64 = 0,
+  RNG_PHILOX_32 = 1
+};
+
+// nestkernel/random_generators.h:141-297: engine + the three draws the builders use; the
+// engine is mt19937_64 (NEST's default rng_type) or Philox_32 (random_manager.cpp:69).
 struct Rng
 {
+  int kind = RNG_MT19937_64;
   std::mt19937_64 eng;
+  PhiloxEngine ph;
   std::uniform_int_distribution< unsigned long > ul;
   std::uniform_real_distribution<> u01 { 0.0, 1.0 };
-  explicit Rng( const std::vector< uint32_t >& seed )
+  explicit Rng( const std::vector< uint32_t >& seed, int k = RNG_MT19937_64 )
+    : kind( k )
   {
     SeedSeqFe128 sseq( seed );
-    eng.seed( sseq );
+    if ( kind == RNG_PHILOX_32 )
+      ph.seed( sseq );
+    else
+      eng.seed( sseq );
+  }
+  // runs f on the engine in use (the std distributions are templates over it)
+  template < typename F >
+  auto
+  with( F&& f )
+  {
+    return kind == RNG_PHILOX_32 ? f( ph ) : f( eng );
   }
   double
   drand()
   {
-    return u01( eng );
+    return with( [ this ]( auto& e ) { return u01( e ); } );
   }
   unsigned long
   ulrand( unsigned long n )
   {
     std::uniform_int_distribution< unsigned long >::param_type p( 0, n - 1 );
-    return ul( eng, p );
+    return with( [ & ]( auto& e ) { return ul( e, p ); } );
   }
 };
 
@@ -357,7 +456,7 @@ struct Param
     case 3:
       if ( b == 0 )
         return a;
-      return ( *normals )[ vp ]( rng.eng );
+      return rng.with( [ & ]( auto& e ) { return ( *normals )[ vp ]( e ); } );
     case 4:
     {
       const double x = l->value( rng, vp );
@@ -376,7 +475,7 @@ struct Param
       // LognormalParameter::value (parameter.cpp:147-157)
       if ( b == 0 )
         return std::exp( a );
-      return ( *lognormals )[ vp ]( rng.eng );
+      return rng.with( [ & ]( auto& e ) { return ( *lognormals )[ vp ]( e ); } );
     case 10:
       // ExponentialParameter::value (parameter.h:339-342)
       return a * ( -std::log( 1 - rng.drand() ) );
@@ -759,11 +858,19 @@ struct Kernel
   {
     if ( res > 0 )
       tb.set_resolution( res );
-    for ( int vp = 0; vp < n_vp; ++vp )
-      vp_rng.emplace_back( std::vector< uint32_t > { seed, 0xb84c9baeu, static_cast< uint32_t >( vp ) } );
+    seed_rngs( RNG_MT19937_64 );
     conns.resize( n_vp );
     reg.resize( n_vp );
     gid2idx.push_back( -1 );  // gid 0 unused
+  }
+
+  // per-VP streams seeded with {rng_seed, THREAD_SPECIFIC_SEEDER_, vp} (random_manager.cpp:95)
+  void
+  seed_rngs( int kind )
+  {
+    vp_rng.clear();
+    for ( int vp = 0; vp < n_vp; ++vp )
+      vp_rng.emplace_back( std::vector< uint32_t > { seed, 0xb84c9baeu, static_cast< uint32_t >( vp ) }, kind );
   }
 
   int
@@ -1780,6 +1887,43 @@ or_seed_seq_generate( const uint32_t* seed, int n_seed, uint32_t* out, int n_out
   std::vector< uint32_t > s( seed, seed + n_seed );
   oracle::SeedSeqFe128 ss( s );
   ss.generate( out, out + n_out );
+  return 0;
+}
+
+// rng_type: 0 mt19937_64, 1 Philox_32; reseeds the per-VP streams (before any draw)
+int
+or_set_rng_type( void* k, int kind )
+{
+  OR_TRY( if ( kind != oracle::RNG_MT19937_64 and kind != oracle::RNG_PHILOX_32 ) throw std::runtime_error(
+            "unknown rng type" );
+          K->seed_rngs( kind ); )
+}
+
+int
+or_philox4x32( int rounds, const uint32_t* ctr, const uint32_t* key, uint32_t* out )
+{
+  oracle::philox4x32( rounds, ctr, key, out );
+  return 0;
+}
+
+// n draws of one family from a stream seeded with the given words: 0 engine words,
+// 1 drand, 2 ulrand(range), 3 normal(0, 1) (std::normal_distribution, one object)
+int
+or_rng_draws( int kind, const uint32_t* seed, int n_seed, int family, unsigned long range, int n, double* out )
+{
+  oracle::Rng r( std::vector< uint32_t >( seed, seed + n_seed ), kind );
+  std::normal_distribution<> nd( 0.0, 1.0 );
+  for ( int i = 0; i < n; ++i )
+  {
+    if ( family == 0 )
+      out[ i ] = static_cast< double >( r.with( []( auto& e ) { return static_cast< uint64_t >( e() ); } ) );
+    else if ( family == 1 )
+      out[ i ] = r.drand();
+    else if ( family == 2 )
+      out[ i ] = static_cast< double >( r.ulrand( range ) );
+    else
+      out[ i ] = r.with( [ & ]( auto& e ) { return nd( e ); } );
+  }
   return 0;
 }
 

@@ -48,7 +48,7 @@ def lib():
         L.or_iaf_propagator.restype = ctypes.c_double
         L.or_iaf_propagator.argtypes = [ctypes.c_double] * 4
         L.or_num_samples.restype = ctypes.c_long
-        for name in ("or_set_sampling", "or_get_samples", "or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
+        for name in ("or_set_rng_type", "or_set_sampling", "or_get_samples", "or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
                      "or_add_connections", "or_get_connections", "or_add_drive", "or_simulate", "or_run_steps",
                      "or_get_spikes", "or_clear_spikes", "or_kernel_status"):
             getattr(L, name).restype = ctypes.c_int
@@ -128,9 +128,13 @@ class OracleError(RuntimeError):
 
 
 class Oracle:
-    def __init__(self, n_vp=1, seed=143202461, resolution=0.1, threads=1, portable_math=False):
+    RNG_TYPES = {"mt19937_64": 0, "Philox_32": 1}
+
+    def __init__(self, n_vp=1, seed=143202461, resolution=0.1, threads=1, portable_math=False, rng_type="mt19937_64"):
         self.L = lib()
         self.k = self.L.or_new(n_vp, seed, resolution)
+        if rng_type != "mt19937_64":
+            self._chk(self.L.or_set_rng_type(ctypes.c_void_p(self.k), self.RNG_TYPES[rng_type]))
         if threads != 1:
             self._chk(self.L.or_set_threads(ctypes.c_void_p(self.k), threads))
         if portable_math:

@@ -71,6 +71,8 @@ def build_gpu(desc, device=0, kernel=None, layout=None):
     """layout: extra kernel status for the device layout (delivery_mode, targets_per_block)."""
     k = kernel or nestgpu.Kernel(device)
     st = {"total_num_virtual_procs": desc.get("n_vp", 1), "rng_seed": desc.get("seed", 143202461)}
+    if "rng_type" in desc:
+        st["rng_type"] = desc["rng_type"]
     st.update(layout or {})
     if "resolution" in desc:
         st["resolution"] = desc["resolution"]
@@ -110,7 +112,7 @@ def build_gpu(desc, device=0, kernel=None, layout=None):
 
 def build_oracle(desc, threads=1, portable_math=False):
     o = Oracle(n_vp=desc.get("n_vp", 1), seed=desc.get("seed", 143202461), resolution=desc.get("resolution", 0.1),
-               threads=threads, portable_math=portable_math)
+               threads=threads, portable_math=portable_math, rng_type=desc.get("rng_type", "mt19937_64"))
     pops = []
     for model, n, params in desc["pops"]:
         pops.append(o.create(model, n, params))

@@ -61,12 +61,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("rng_type", ["mt19937_64", "Philox_32"])
 @pytest.mark.parametrize("case", CASES, ids=[c[4] + "-vp%d" % c[0] for c in CASES])
-def test_connectivity_identical_to_oracle(case):
+def test_connectivity_identical_to_oracle(case, rng_type):
+    # both of NEST's per-VP stream types: mt19937_64 (the default) and Philox_32
     n_vp, seed, npre, npost, rule, rp, w, d, aut, mult, same = case
     pops = [("iaf_psc_delta", npre, {})] if same else [("iaf_psc_delta", npre, {}), ("iaf_psc_exp", npost, {})]
     post = (0, 0, npre) if same else (1, 0, npost)
-    desc = dict(n_vp=n_vp, seed=seed, pops=pops, conns=[((0, 0, npre), post, rule, rp, w, d, aut, mult)])
+    desc = dict(n_vp=n_vp, seed=seed, pops=pops, conns=[((0, 0, npre), post, rule, rp, w, d, aut, mult)],
+                rng_type=rng_type)
     k, kp, sr = nb.build_gpu(desc)
     o, op = nb.build_oracle(desc)
     s1, t1, w1, d1 = _conns_gpu(k)

@@ -730,3 +730,64 @@ def test_sampled_state_is_after_the_step():
     np.testing.assert_allclose(f[:, 0], -70.0 + (1.0 - np.exp(-t / 10.0)) * (10.0 / 250.0) * I, rtol=0, atol=1e-9)
     # the synaptic currents of an unstimulated neuron stay at zero
     assert np.all(f[:, 1:3] == 0.0)
+
+
+# ------------------------------------------------------------------ Philox_32 streams
+PHILOX_PROBE = os.path.join(ROOT, "oracle", "_ref", "philox_probe")
+
+
+def _philox_draws(family, n, rng, seed):
+    import ctypes
+
+    L = lib()
+    L.or_rng_draws.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, ctypes.c_int,
+                               ctypes.c_ulong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    s = (ctypes.c_uint32 * len(seed))(*seed)
+    out = (ctypes.c_double * n)()
+    assert L.or_rng_draws(1, s, len(seed), family, rng, n, out) == 0
+    return list(out)
+
+
+def test_philox4x32_known_answers():
+    # Random123's own known-answer vectors (thirdparty/Random123/tests/kat_vectors), 7 and 10 rounds
+    import ctypes
+
+    L = lib()
+    with open(os.path.join(GOLDEN, "philox4x32_kat.json")) as f:
+        kat = json.load(f)
+    assert len(kat) == 6
+    for v in kat:
+        c = (ctypes.c_uint32 * 4)(*v["ctr"])
+        k = (ctypes.c_uint32 * 2)(*v["key"])
+        o = (ctypes.c_uint32 * 4)()
+        L.or_philox4x32(v["rounds"], c, k, o)
+        assert list(o) == v["out"], v
+
+
+def test_philox_engine_draws_match_reference_vectors():
+    # engine words, uniform_real, uniform_int (a range rejecting about half of the words
+    # included) and normal draws of r123::Engine<Philox4x32> seeded as NEST seeds a VP stream,
+    # from the reference's own headers (tests/golden/make_philox_vectors.py)
+    with open(os.path.join(GOLDEN, "philox_engine_draws.json")) as f:
+        cases = json.load(f)
+    assert len(cases) == 24
+    for c in cases:
+        got = _philox_draws(c["family"], len(c["values"]), c["range"], c["seed"])
+        if c["family"] in (1, 3):
+            assert got == c["values"], (c["seed"], c["family"])
+        else:
+            assert [int(x) for x in got] == c["values"], (c["seed"], c["family"], c["range"])
+
+
+@pytest.mark.skipif(not os.path.exists(PHILOX_PROBE), reason="reference probe not built (oracle/ref)")
+def test_philox_engine_draws_match_reference_probe_live():
+    rng = np.random.default_rng(11)
+    for _ in range(12):
+        seed = [int(rng.integers(1, 2 ** 32)), 0xB84C9BAE, int(rng.integers(0, 64))]
+        fam = int(rng.integers(0, 4))
+        r = int(rng.choice([2, 1000, 77169, 3000000001]))
+        txt = subprocess.run([PHILOX_PROBE, str(fam), "300", str(r)] + [str(x) for x in seed], check=True,
+                             capture_output=True, text=True).stdout.split()
+        got = _philox_draws(fam, 300, r, seed)
+        want = [float(x) for x in txt]
+        assert got == want, (seed, fam, r)
