@@ -1,5 +1,6 @@
 // ng_nest.cpp -- host kernel: NEST's Create/Connect/Simulate surface over the device engine.
 #include "ng_nest.h"
+#include "ng_conngen.h"
 
 #include <algorithm>
 #include <chrono>
@@ -454,6 +455,8 @@ Kernel::reset()
   spike_count_ = 0;
   targets_per_block_ = 0;
   balanced_blocks_ = true;
+  connect_on_device_ = false;
+  device_connect_calls_ = 0;
   block_start_.clear();
   rebalanced_ = false;
   rx_steps_ = 0;
@@ -543,6 +546,8 @@ Kernel::set_kernel_status( const Json& d )
       targets_per_block_ = v;
       store_dirty_ = true;
     }
+    else if ( k == "connect_on_device" )
+      connect_on_device_ = kv.second.number() != 0;
     else if ( k == "balanced_blocks" )
     {
       const bool v = kv.second.number() != 0;
@@ -664,6 +669,8 @@ Kernel::get_kernel_status_json() const
     << ", \"time_deliver_spike_data\": 0, \"fused_deliver_update\": true"
     << ", \"targets_per_block\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().tb ) : targets_per_block_ )
     << ", \"balanced_blocks\": " << ( balanced_blocks_ ? "true" : "false" )
+    << ", \"connect_on_device\": " << ( connect_on_device_ ? "true" : "false" )
+    << ", \"device_connect_calls\": " << device_connect_calls_
     << ", \"block_relayouts\": " << block_relayouts_
     << ", \"target_blocks\": " << ( engine_ and engine_->built() ? static_cast< int64_t >( engine_->net().n_blocks ) : 0 )
     << ", \"delivery_mode\": \""
@@ -1820,6 +1827,28 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     return;
   }
 
+  // neuron -> neuron on the device: the builders whose draws take a known number of words of
+  // a counter-based stream, with constant weight and delay
+  const bool const_params = ( not wp or wp->is_const() ) and ( not dp or dp->is_const() );
+  if ( connect_on_device_ and const_params
+       and ( rule == "pairwise_bernoulli" or ( rule == "fixed_indegree" and autapses and multapses ) ) )
+  {
+    if ( rng_type_ != RNG_PHILOX_32 )
+      throw std::runtime_error( "BadProperty: connect_on_device draws from counter-based streams: set rng_type "
+                                "'Philox_32' first" );
+    // single_connect_ with constant parameters: delay, then weight (conn_builder.cpp:379-437)
+    const double delay = dp ? dp->const_value() : 1.0;
+    const double weight = wp ? wp->const_value() : 1.0;
+    const long ds = tb_.delay_ms_to_steps( delay );
+    check_delay( ds );
+    if ( ds > 65535 )
+      throw std::runtime_error( "BadDelay: delays above 65535 steps are not supported" );
+    device_connect( pre, post, rule, rule == "fixed_indegree" ? std::round( indegree ) : p, autapses, weight,
+      static_cast< uint16_t >( ds ) );
+    store_dirty_ = true;
+    return;
+  }
+
   // neuron -> neuron: every VP runs the builder for its local targets with its own stream
   // (FixedInDegreeBuilder::inner_connect_ conn_builder.cpp:1508-1545, BernoulliBuilder
   // :1937-1965, OneToOne :960-1031, AllToAll :1170-1255)
@@ -1954,6 +1983,78 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     }
   }
   store_dirty_ = true;
+}
+
+void
+Kernel::device_connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const std::string& rule,
+  double param, bool autapses, double weight, uint16_t delay_steps )
+{
+  // each VP's local targets in builder order, as in the host worker of connect()
+  std::vector< uint32_t > pre32( pre.size() );
+  for ( size_t i = 0; i < pre.size(); ++i )
+    pre32[ i ] = static_cast< uint32_t >( pre[ i ] );
+  std::vector< GenStream > streams( n_vp_ );
+  for ( int vp = 0; vp < n_vp_; ++vp )
+    streams[ vp ] = GenStream { rngs_[ vp ].philox.k0, rngs_[ vp ].philox.k1, rngs_[ vp ].philox.pos };
+  std::vector< std::vector< uint32_t > > src( n_vp_ ), tgt( n_vp_ );
+  if ( rule == "pairwise_bernoulli" )
+  {
+    // the source index of each target when it is also a source (its autapse is skipped)
+    std::map< int64_t, int32_t > index_in_pre;
+    if ( not autapses )
+      for ( size_t i = 0; i < pre.size(); ++i )
+        index_in_pre.emplace( pre[ i ], static_cast< int32_t >( i ) );
+    std::vector< GenRow > rows;
+    std::vector< uint64_t > draws( n_vp_, 0 );
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      for ( const int64_t t : post )
+      {
+        if ( vp_of( t ) != vp )
+          continue;
+        int32_t aut = -1;
+        if ( not autapses )
+        {
+          const auto it = index_in_pre.find( t );
+          if ( it != index_in_pre.end() )
+            aut = it->second;
+        }
+        rows.push_back( GenRow { static_cast< uint32_t >( t ), aut, draws[ vp ], static_cast< uint32_t >( vp ) } );
+        draws[ vp ] += pre.size() - ( aut >= 0 ? 1 : 0 );
+      }
+    gen_pairwise_bernoulli( device_, pre32, rows, streams, param, src, tgt );
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      rngs_[ vp ].philox.pos += 2 * draws[ vp ];  // one drand, two words, per visited pair
+  }
+  else
+  {
+    const uint32_t K = static_cast< uint32_t >( std::max( 0.0, param ) );
+    std::vector< std::vector< uint32_t > > rows( n_vp_ );
+    for ( const int64_t t : post )
+      rows[ vp_of( t ) ].push_back( static_cast< uint32_t >( t ) );
+    gen_fixed_indegree( device_, pre32, rows, K, streams, src, tgt );
+    for ( int vp = 0; vp < n_vp_; ++vp )
+      rngs_[ vp ].philox.pos = streams[ vp ].pos;
+  }
+  for ( int vp = 0; vp < n_vp_; ++vp )
+  {
+    ConnList& C = conns_[ vp ];
+    const size_t n = src[ vp ].size();
+    C.reserve_more( n );
+    C.src.insert( C.src.end(), src[ vp ].begin(), src[ vp ].end() );
+    C.tgt.insert( C.tgt.end(), tgt[ vp ].begin(), tgt[ vp ].end() );
+    C.w.insert( C.w.end(), n, weight );
+    C.d.insert( C.d.end(), n, delay_steps );
+    if ( C.size() > static_cast< size_t >( ( 1u << 27 ) - 2 ) )
+      throw std::runtime_error( "KernelException: Too many connections: at most 134217726 connections supported per "
+                                "virtual process and synapse model." );
+    if ( n )
+    {
+      if ( d_min_seen_ < 0 or delay_steps < d_min_seen_ )
+        d_min_seen_ = delay_steps;
+      d_max_seen_ = std::max< long >( d_max_seen_, delay_steps );
+    }
+  }
+  ++device_connect_calls_;
 }
 
 void

@@ -523,6 +523,10 @@ private:
   void apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool creating );
   void check_delay( long steps );
   void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
+  // pairwise_bernoulli (param p) or fixed_indegree (param K, autapses and multapses allowed)
+  // with constant weight and delay, drawn on the device from the Philox_32 streams
+  void device_connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const std::string& rule,
+    double param, bool autapses, double weight, uint16_t delay_steps );
   void build_spec( BuildSpec& b );
   void choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const;
   // block bounds balancing in-degree over at most ceil(n / tb) blocks
@@ -602,6 +606,8 @@ private:
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
   int targets_per_block_ = 0;  // 0: chosen from the network size
+  bool connect_on_device_ = false;  // eligible Connect calls draw on the device
+  int64_t device_connect_calls_ = 0;
   bool balanced_blocks_ = true;  // automatic block size: blocks balanced by in-degree, then by
                                  // the measured received events
   std::vector< uint32_t > block_start_;  // target-block bounds of the device copy

@@ -651,3 +651,18 @@ def test_multimeter_parameters_and_checks():
     k.Connect(vm, delta)
     with pytest.raises(nestgpu.NESTError, match="IllegalConnection"):
         k.Connect(exp, mm)
+
+
+def test_connect_on_device_needs_counter_based_streams():
+    # device draws need rng_type Philox_32; a call with random parameters draws on the host
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"connect_on_device": True})
+    a = k.Create("iaf_psc_exp", 20)
+    with pytest.raises(nestgpu.NESTError, match="Philox_32"):
+        k.Connect(a, a, {"rule": "pairwise_bernoulli", "p": 0.5}, {"weight": 1.0, "delay": 1.0})
+    assert len(k.GetConnections()["source"]) == 0
+    k.SetKernelStatus({"rng_type": "Philox_32"})
+    k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 3}, {"weight": nestgpu.random.normal(0.0, 1.0)})
+    k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 3, "allow_multapses": False}, {"weight": 1.0})
+    assert k.GetKernelStatus("device_connect_calls") == 0
+    assert len(k.GetConnections()["source"]) == 120
