@@ -161,12 +161,14 @@ def instance_params(sp):
                 rate_hz=sp["eta"] * 10000.0)
 
 
-def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True):
+def build_instance(k, nestgpu, sp, n_vp=4, keep_host_connections=True, rng_type="mt19937_64", connect_on_device=False):
     """One batched-sweep instance: N neurons (80 % excitatory), fixed_indegree 0.8 K from the
     excitatory and 0.2 K from the inhibitory neurons, delay 1.5 ms, seeded Poisson drive through
-    one spike_generator per neuron (instance_params). Returns the population."""
+    one spike_generator per neuron (instance_params). With rng_type Philox_32 and
+    connect_on_device the two fixed_indegree calls draw on the device. Returns the population."""
     k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": sp["seed"],
-                       "keep_host_connections": keep_host_connections})
+                       "keep_host_connections": keep_host_connections, "rng_type": rng_type,
+                       "connect_on_device": connect_on_device})
     N, K = sp["N"], sp["K"]
     KE = int(0.8 * K)
     KI = K - KE
@@ -195,7 +197,8 @@ def run_batched(args, nestgpu):
     kernels = [nestgpu.Kernel(0) for _ in specs]
     workers = max(1, min(16, (os.cpu_count() or 4) // 2))
     with ThreadPoolExecutor(workers) as ex:
-        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False), zip(kernels, specs)))
+        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False, rng_type=args.rng_type,
+                                   connect_on_device=bool(args.connect_on_device)), zip(kernels, specs)))
     n_neurons = sum(sp["N"] for sp in specs)
     n_syn = sum(k.GetKernelStatus("num_connections") for k in kernels) - n_neurons
     print("built %d instances: %d neurons, %d synapses in %.1f s" % (len(specs), n_neurons, n_syn,
@@ -223,6 +226,7 @@ def run_batched(args, nestgpu):
     t0 = time.perf_counter()
     for i in range(args.steps):
         nestgpu.batch_simulate(kernels, args.t_step)
+        print("batched step %d done" % i, file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ev_b = events() - ev0
     value = ev_b / wall
@@ -236,7 +240,8 @@ def run_batched(args, nestgpu):
     sub = specs[::4]
     plain = [nestgpu.Kernel(0) for _ in sub]
     with ThreadPoolExecutor(workers) as ex:
-        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False), zip(plain, sub)))
+        list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False, rng_type=args.rng_type,
+                                   connect_on_device=bool(args.connect_on_device)), zip(plain, sub)))
     for k in plain:
         k.Prepare()
         k.Run(args.plain_warm_ms)
@@ -261,6 +266,7 @@ def run_batched(args, nestgpu):
                                "of every instance" % (len(specs), models["iaf_psc_exp"], models["aeif_cond_exp"],
                                                        args.t_step),
                    "n_instances": len(specs), "n_neurons": n_neurons, "n_synapses": n_syn, "build_s": build_s,
+                   "rng_type": args.rng_type, "connect_on_device": bool(args.connect_on_device),
                    "targets_per_block": sorted(set(chosen)) if chosen else "library default"},
         "batched_vs_plain": value / plain_value,
         "plain_path": {"value": plain_value, "wall_s": plain_wall, "ms_per_instance": args.plain_ms,
@@ -305,7 +311,8 @@ def run_served(args, nestgpu):
     def handle(i):
         r = reqs[i]
         k = nestgpu.Kernel(0)
-        build_instance(k, nestgpu, r["spec"], keep_host_connections=False)
+        build_instance(k, nestgpu, r["spec"], keep_host_connections=False, rng_type=args.rng_type,
+                       connect_on_device=bool(args.connect_on_device))
         built = time.perf_counter() - t_origin
         rid = srv.submit(k, r["calls"])
         info = srv.wait(rid)
@@ -346,7 +353,8 @@ def run_served(args, nestgpu):
     for r in reqs:
         t0 = time.perf_counter()
         k = nestgpu.Kernel(0)
-        build_instance(k, nestgpu, r["spec"], keep_host_connections=False)
+        build_instance(k, nestgpu, r["spec"], keep_host_connections=False, rng_type=args.rng_type,
+                       connect_on_device=bool(args.connect_on_device))
         for c in r["calls"]:
             k.Simulate(c)
         service.append(time.perf_counter() - t0)
@@ -372,7 +380,7 @@ def run_served(args, nestgpu):
                                "N log-uniform [2k, 100k], T uniform [100, 2000] ms in 1-4 Simulate calls, HBM "
                                "budget %g GiB, %d host builders" % (len(reqs), args.qps, args.hbm_budget_gib,
                                                                    args.builders),
-                   "server": stats},
+                   "server": stats, "rng_type": args.rng_type, "connect_on_device": bool(args.connect_on_device)},
         "serial_plain_path": {"value": serial_value, "makespan_s": serial_makespan,
                               "latency_s": {"p50": float(np.percentile(lat_serial, 50)),
                                             "p99": float(np.percentile(lat_serial, 99))}},
@@ -678,6 +686,10 @@ def main():
     ap.add_argument("--builders", type=int, default=6, help="c5: host threads building request networks")
     ap.add_argument("--hbm-budget-gib", type=float, default=128.0, help="c5: device memory budget of the server")
     ap.add_argument("--delivery-mode", default="auto", choices=["auto", "window", "global", "cache"])
+    ap.add_argument("--rng-type", default="mt19937_64", choices=["mt19937_64", "Philox_32"],
+                    help="c4/c5: per-VP stream type of the instances")
+    ap.add_argument("--connect-on-device", type=int, default=0, choices=[0, 1],
+                    help="c4/c5: draw the instances' connections on the device (needs --rng-type Philox_32)")
     ap.add_argument("--targets-per-block", type=int, default=0, help="0: chosen by the library")
     ap.add_argument("--window-slots", type=int, default=0, help="cap on ring slots in LDS; 0: as many as fit")
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="0: as many as fit")

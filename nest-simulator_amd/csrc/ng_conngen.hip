@@ -184,48 +184,59 @@ __global__ void __launch_bounds__( GEN_BS )
   }
 }
 
-template < typename T >
-T*
-dev_alloc( size_t n )
+// Device memory of one call: stream-ordered allocations on a non-blocking stream of the
+// calling thread, so that a call neither waits for nor holds up simulations that other
+// threads run on the device (hipMalloc / hipFree synchronize the whole device).
+struct CallMem
 {
-  void* p = nullptr;
-  hip_check( hipMalloc( &p, std::max< size_t >( n, 1 ) * sizeof( T ) ), "conngen: hipMalloc" );
-  return static_cast< T* >( p );
-}
-
-template < typename T >
-T*
-dev_upload( const std::vector< T >& v )
-{
-  T* p = dev_alloc< T >( v.size() );
-  if ( not v.empty() )
-    hip_check( hipMemcpy( p, v.data(), v.size() * sizeof( T ), hipMemcpyHostToDevice ), "conngen: upload" );
-  return p;
-}
-
-template < typename T >
-void
-dev_download( std::vector< T >& v, const T* p, size_t n )
-{
-  v.resize( n );
-  if ( n )
-    hip_check( hipMemcpy( v.data(), p, n * sizeof( T ), hipMemcpyDeviceToHost ), "conngen: download" );
-}
-
-struct DevFree
-{
+  hipStream_t st = nullptr;
   std::vector< void* > ptrs;
-  ~DevFree()
+  explicit CallMem( int device )
+  {
+    thread_local hipStream_t streams[ 64 ] = {};
+    if ( device < 0 or device >= 64 )
+      throw std::runtime_error( "conngen: device index out of range" );
+    if ( not streams[ device ] )
+      hip_check( hipStreamCreateWithFlags( &streams[ device ], hipStreamNonBlocking ), "conngen: stream" );
+    st = streams[ device ];
+  }
+  ~CallMem()
   {
     for ( void* p : ptrs )
-      hipFree( p );
+      hipFreeAsync( p, st );
+    hipStreamSynchronize( st );
   }
   template < typename T >
   T*
-  operator()( T* p )
+  alloc( size_t n )
   {
+    void* p = nullptr;
+    hip_check( hipMallocAsync( &p, std::max< size_t >( n, 1 ) * sizeof( T ), st ), "conngen: hipMallocAsync" );
     ptrs.push_back( p );
+    return static_cast< T* >( p );
+  }
+  template < typename T >
+  T*
+  upload( const std::vector< T >& v )
+  {
+    T* p = alloc< T >( v.size() );
+    if ( not v.empty() )
+      hip_check( hipMemcpyAsync( p, v.data(), v.size() * sizeof( T ), hipMemcpyHostToDevice, st ), "conngen: upload" );
     return p;
+  }
+  // queued copy to host; valid after sync()
+  template < typename T >
+  void
+  download( std::vector< T >& v, const T* p, size_t n )
+  {
+    v.resize( n );
+    if ( n )
+      hip_check( hipMemcpyAsync( v.data(), p, n * sizeof( T ), hipMemcpyDeviceToHost, st ), "conngen: download" );
+  }
+  void
+  sync()
+  {
+    hip_check( hipStreamSynchronize( st ), "conngen: synchronize" );
   }
 };
 
@@ -248,16 +259,18 @@ gen_pairwise_bernoulli( int device, const std::vector< uint32_t >& pre, const st
     return;
   if ( n_seg > 0x7fffffffull )
     throw std::runtime_error( "NotImplemented: pairwise_bernoulli on the device: too many (target, source) segments" );
-  DevFree F;
-  const uint32_t* d_pre = F( dev_upload( pre ) );
-  const GenRow* d_rows = F( dev_upload( rows ) );
-  const GenStream* d_st = F( dev_upload( streams ) );
-  uint32_t* d_cnt = F( dev_alloc< uint32_t >( n_seg ) );
+  CallMem M( device );
+  const uint32_t* d_pre = M.upload( pre );
+  const GenRow* d_rows = M.upload( rows );
+  const GenStream* d_st = M.upload( streams );
+  uint32_t* d_cnt = M.alloc< uint32_t >( n_seg );
   const uint32_t grid = static_cast< uint32_t >( n_seg );
-  k_bernoulli< false ><<< grid, GEN_BS >>>( d_pre, n_src, d_rows, d_st, n_chunks, p, d_cnt, nullptr, nullptr, nullptr );
+  k_bernoulli< false ><<< grid, GEN_BS, 0, M.st >>>( d_pre, n_src, d_rows, d_st, n_chunks, p, d_cnt, nullptr, nullptr,
+    nullptr );
   hip_check( hipGetLastError(), "conngen: count launch" );
   std::vector< uint32_t > cnt;
-  dev_download( cnt, d_cnt, n_seg );
+  M.download( cnt, d_cnt, n_seg );
+  M.sync();
   std::vector< uint64_t > off( n_seg );
   uint64_t total = 0;
   for ( uint64_t s = 0; s < n_seg; ++s )
@@ -265,14 +278,15 @@ gen_pairwise_bernoulli( int device, const std::vector< uint32_t >& pre, const st
     off[ s ] = total;
     total += cnt[ s ];
   }
-  const uint64_t* d_off = F( dev_upload( off ) );
-  uint32_t* d_src = F( dev_alloc< uint32_t >( total ) );
-  uint32_t* d_tgt = F( dev_alloc< uint32_t >( total ) );
-  k_bernoulli< true ><<< grid, GEN_BS >>>( d_pre, n_src, d_rows, d_st, n_chunks, p, nullptr, d_off, d_src, d_tgt );
+  const uint64_t* d_off = M.upload( off );
+  uint32_t* d_src = M.alloc< uint32_t >( total );
+  uint32_t* d_tgt = M.alloc< uint32_t >( total );
+  k_bernoulli< true ><<< grid, GEN_BS, 0, M.st >>>( d_pre, n_src, d_rows, d_st, n_chunks, p, nullptr, d_off, d_src, d_tgt );
   hip_check( hipGetLastError(), "conngen: emit launch" );
   std::vector< uint32_t > all_src, all_tgt;
-  dev_download( all_src, d_src, total );
-  dev_download( all_tgt, d_tgt, total );
+  M.download( all_src, d_src, total );
+  M.download( all_tgt, d_tgt, total );
+  M.sync();
   // rows are grouped by VP in builder order: each VP's connections are contiguous
   uint64_t at = 0;
   for ( size_t r = 0; r < rows.size(); )
@@ -301,69 +315,99 @@ gen_fixed_indegree( int device, const std::vector< uint32_t >& pre,
     throw std::runtime_error( "NotImplemented: fixed_indegree on the device needs 1 to 2^32 - 1 sources" );
   // the share of words Lemire's method rejects for this range: threshold / 2^32
   const double reject_rate = static_cast< double >( ( 0u - range ) % range ) / 4294967296.0;
-  DevFree F;
-  const uint32_t* d_pre = F( dev_upload( pre ) );
-  for ( size_t vp = 0; vp < streams.size(); ++vp )
+  const size_t n_vp = streams.size();
+  std::vector< uint64_t > Q( n_vp ), slack( n_vp );
+  for ( size_t vp = 0; vp < n_vp; ++vp )
   {
     src[ vp ].clear();
     tgt[ vp ].clear();
-    const uint64_t Q = static_cast< uint64_t >( row_tgt[ vp ].size() ) * K;
-    if ( Q == 0 )
-      continue;
-    GenStream& S = streams[ vp ];
-    // rejected words over a window the draws reach: Q words plus room for the rejections
-    std::vector< uint64_t > rej;
-    uint64_t slack = 1024 + static_cast< uint64_t >( 2.0 * reject_rate * static_cast< double >( Q ) ) + Q / 4096;
-    while ( true )
+    Q[ vp ] = static_cast< uint64_t >( row_tgt[ vp ].size() ) * K;
+    slack[ vp ] = 1024 + static_cast< uint64_t >( 2.0 * reject_rate * static_cast< double >( Q[ vp ] ) ) + Q[ vp ] / 4096;
+  }
+  CallMem M( device );
+  const uint32_t* d_pre = M.upload( pre );
+  // 1. the rejected words over a window each VP's draws reach: Q words plus room for the
+  //    rejections; a window holding more rejections than its room is searched again, larger
+  std::vector< std::vector< uint64_t > > rej( n_vp );
+  std::vector< bool > done( n_vp, false );
+  for ( size_t vp = 0; vp < n_vp; ++vp )
+    done[ vp ] = Q[ vp ] == 0;
+  while ( true )
+  {
+    std::vector< size_t > todo;
+    for ( size_t vp = 0; vp < n_vp; ++vp )
+      if ( not done[ vp ] )
+        todo.push_back( vp );
+    if ( todo.empty() )
+      break;
+    uint32_t* d_n = M.alloc< uint32_t >( n_vp );
+    hip_check( hipMemsetAsync( d_n, 0, n_vp * sizeof( uint32_t ), M.st ), "conngen: clear" );
+    std::vector< uint64_t* > d_rej( n_vp, nullptr );
+    std::vector< uint32_t > cap( n_vp, 0 );
+    for ( size_t vp : todo )
     {
-      const uint64_t n_words = Q + slack;
-      const uint32_t cap = static_cast< uint32_t >( std::min< uint64_t >( slack + 1, 0x7fffffffull ) );
-      uint64_t* d_rej = dev_alloc< uint64_t >( cap );
-      uint32_t* d_n = dev_alloc< uint32_t >( 1 );
-      hip_check( hipMemset( d_n, 0, sizeof( uint32_t ) ), "conngen: clear" );
+      const GenStream& S = streams[ vp ];
+      const uint64_t n_words = Q[ vp ] + slack[ vp ];
+      cap[ vp ] = static_cast< uint32_t >( std::min< uint64_t >( slack[ vp ] + 1, 0x7fffffffull ) );
+      d_rej[ vp ] = M.alloc< uint64_t >( cap[ vp ] );
       const uint64_t blocks = ( ( S.pos + n_words + 3 ) >> 2 ) - ( S.pos >> 2 );
       const uint64_t grid = ( blocks + GEN_BS - 1 ) / GEN_BS;
       if ( grid > 0x7fffffffull )
-      {
-        hipFree( d_rej );
-        hipFree( d_n );
         throw std::runtime_error( "NotImplemented: fixed_indegree on the device: too many draws in one call" );
-      }
-      k_rejected<<< static_cast< uint32_t >( grid ), GEN_BS >>>( S, S.pos, n_words, range, d_rej, d_n, cap );
+      k_rejected<<< static_cast< uint32_t >( grid ), GEN_BS, 0, M.st >>>( S, S.pos, n_words, range, d_rej[ vp ],
+        d_n + vp, cap[ vp ] );
       hip_check( hipGetLastError(), "conngen: rejection launch" );
-      uint32_t n_rej = 0;
-      hip_check( hipMemcpy( &n_rej, d_n, sizeof n_rej, hipMemcpyDeviceToHost ), "conngen: rejections" );
-      // the window holds every word the draws take when it holds at most `slack` rejections
-      const bool fits = n_rej <= slack and n_rej <= cap;
-      if ( fits )
-        dev_download( rej, d_rej, n_rej );
-      hipFree( d_rej );
-      hipFree( d_n );
-      if ( fits )
-        break;
-      slack *= 4;
     }
-    std::sort( rej.begin(), rej.end() );
-    std::vector< uint64_t > rk( rej.size() );
-    for ( size_t i = 0; i < rej.size(); ++i )
-      rk[ i ] = rej[ i ] - S.pos - i;
-    const uint64_t* d_rk = F( dev_upload( rk ) );
-    const uint32_t* d_rows = F( dev_upload( row_tgt[ vp ] ) );
-    uint32_t* d_src = F( dev_alloc< uint32_t >( Q ) );
-    uint32_t* d_tgt = F( dev_alloc< uint32_t >( Q ) );
-    const uint64_t threads = ( Q + QPT - 1 ) / QPT;
+    std::vector< uint32_t > n_rej;
+    M.download( n_rej, d_n, n_vp );
+    M.sync();
+    for ( size_t vp : todo )
+    {
+      // the window holds every word the draws take when it holds at most `slack` rejections
+      if ( n_rej[ vp ] <= slack[ vp ] and n_rej[ vp ] <= cap[ vp ] )
+      {
+        M.download( rej[ vp ], d_rej[ vp ], n_rej[ vp ] );
+        done[ vp ] = true;
+      }
+      else
+        slack[ vp ] *= 4;
+    }
+    M.sync();
+  }
+  // 2. every draw's word, all VPs queued together
+  std::vector< std::vector< uint64_t > > rk( n_vp );
+  for ( size_t vp = 0; vp < n_vp; ++vp )
+  {
+    if ( Q[ vp ] == 0 )
+      continue;
+    const GenStream& S = streams[ vp ];
+    std::sort( rej[ vp ].begin(), rej[ vp ].end() );
+    rk[ vp ].resize( rej[ vp ].size() );
+    for ( size_t i = 0; i < rej[ vp ].size(); ++i )
+      rk[ vp ][ i ] = rej[ vp ][ i ] - S.pos - i;
+    const uint64_t* d_rk = M.upload( rk[ vp ] );
+    const uint32_t* d_rows = M.upload( row_tgt[ vp ] );
+    uint32_t* d_src = M.alloc< uint32_t >( Q[ vp ] );
+    uint32_t* d_tgt = M.alloc< uint32_t >( Q[ vp ] );
+    const uint64_t threads = ( Q[ vp ] + QPT - 1 ) / QPT;
     const uint64_t grid = ( threads + GEN_BS - 1 ) / GEN_BS;
     if ( grid > 0x7fffffffull )
       throw std::runtime_error( "NotImplemented: fixed_indegree on the device: too many draws in one call" );
-    k_fixed_indegree<<< static_cast< uint32_t >( grid ), GEN_BS >>>(
-      S, Q, K, d_pre, range, d_rows, d_rk, static_cast< uint32_t >( rk.size() ), d_src, d_tgt );
+    k_fixed_indegree<<< static_cast< uint32_t >( grid ), GEN_BS, 0, M.st >>>(
+      S, Q[ vp ], K, d_pre, range, d_rows, d_rk, static_cast< uint32_t >( rk[ vp ].size() ), d_src, d_tgt );
     hip_check( hipGetLastError(), "conngen: draw launch" );
-    dev_download( src[ vp ], d_src, Q );
-    dev_download( tgt[ vp ], d_tgt, Q );
-    // the stream continues after the last draw's word
-    const uint64_t m_last = static_cast< uint64_t >( std::upper_bound( rk.begin(), rk.end(), Q - 1 ) - rk.begin() );
-    S.pos += Q + m_last;
+    M.download( src[ vp ], d_src, Q[ vp ] );
+    M.download( tgt[ vp ], d_tgt, Q[ vp ] );
   }
+  M.sync();
+  for ( size_t vp = 0; vp < n_vp; ++vp )
+    if ( Q[ vp ] )
+    {
+      // the stream continues after the last draw's word
+      const uint64_t m_last =
+        static_cast< uint64_t >( std::upper_bound( rk[ vp ].begin(), rk[ vp ].end(), Q[ vp ] - 1 ) - rk[ vp ].begin() );
+      streams[ vp ].pos += Q[ vp ] + m_last;
+    }
 }
 
 }  // namespace nestgpu
