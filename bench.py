@@ -207,8 +207,11 @@ def run_batched(args, nestgpu):
     chosen = None
     if args.autotune:
         t_at = time.perf_counter()
-        chosen = nestgpu.batch_autotune(kernels, probe_ms=args.probe_ms)
-        print("autotune: %.1f s, targets_per_block per size class %s" % (
+        caps = tuple(int(x) for x in args.autotune_windows.split(","))
+        cands = tuple(int(x) for x in args.autotune_blocks.split(","))
+        chosen = nestgpu.batch_autotune(kernels, probe_ms=args.probe_ms, candidates=cands, window_caps=caps,
+                                        with_windows=True)
+        print("autotune: %.1f s, (targets_per_block, window cap) per size class %s" % (
             time.perf_counter() - t_at, sorted(set(zip([sp["N"].bit_length() for sp in specs], chosen)))),
             file=sys.stderr, flush=True)
     with ThreadPoolExecutor(4) as ex:
@@ -243,6 +246,7 @@ def run_batched(args, nestgpu):
         list(ex.map(lambda a: build_instance(a[0], nestgpu, a[1], keep_host_connections=False, rng_type=args.rng_type,
                                    connect_on_device=bool(args.connect_on_device)), zip(plain, sub)))
     for k in plain:
+        k.SetKernelStatus({"graph_launch": bool(args.graph_launch)})
         k.Prepare()
         k.Run(args.plain_warm_ms)
     ev0 = sum(k.GetKernelStatus("synaptic_events") for k in plain)
@@ -267,13 +271,14 @@ def run_batched(args, nestgpu):
                                                        args.t_step),
                    "n_instances": len(specs), "n_neurons": n_neurons, "n_synapses": n_syn, "build_s": build_s,
                    "rng_type": args.rng_type, "connect_on_device": bool(args.connect_on_device),
-                   "targets_per_block": sorted(set(chosen)) if chosen else "library default"},
+                   "layout": sorted(set(chosen)) if chosen else "library default",
+                   "layout_key": "(targets_per_block, cap on LDS ring slots, 0 = as many as fit)"},
         "batched_vs_plain": value / plain_value,
         "plain_path": {"value": plain_value, "wall_s": plain_wall, "ms_per_instance": args.plain_ms,
                        "instances": len(plain), "layout": "library default", "warm_ms": args.plain_warm_ms},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def c5_requests(n, seed, qps):
@@ -667,6 +672,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="c3: skip the extra c2 measurement")
+    ap.add_argument("--no-c4", action="store_true", help="c3: skip the extra batched-sweep (c4) measurement")
     ap.add_argument("--c2-steps", type=int, default=5, help="c3: timed steps of the extra c2 measurement")
     ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"],
                     help="c3: 77k-neuron 8-population circuit with min_delay one step (default); c2: 100k-neuron "
@@ -678,6 +684,10 @@ def main():
                     help="c4: untimed biological ms per instance before the plain-path timing")
     ap.add_argument("--autotune", action="store_true", help="c4: pick targets_per_block per size class first")
     ap.add_argument("--probe-ms", type=float, default=30.0, help="c4: autotuner probe length")
+    ap.add_argument("--autotune-blocks", default="64,128,256,512",
+                    help="c4: candidate targets_per_block for the autotuner")
+    ap.add_argument("--autotune-windows", default="0,4",
+                    help="c4: candidate caps on the LDS ring slots for the autotuner (0: as many as fit)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (scale for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -738,7 +748,10 @@ def main():
     if args.workload in ("c4", "c5"):
         if args.seed == 20241:
             args.seed = 4242
-        (run_batched if args.workload == "c4" else run_served)(args, nestgpu)
+        if args.workload == "c4":
+            print(json.dumps(run_batched(args, nestgpu)), flush=True)
+        else:
+            run_served(args, nestgpu)
         return
 
     res = run_plain(args, nestgpu, args.workload, args.steps, args.warmup, not args.no_cpu_baseline)
@@ -751,6 +764,14 @@ def main():
         line["c2"] = {k: c2[k] for k in ("value", "unit", "steps", "ms_per_step", "rtf", "config")}
         line["c2"]["roofline"] = {k: c2["roofline"][k] for k in ("achieved", "frac", "avg_launch_us", "traffic",
                                                                  "delivery")}
+    if args.workload == "c3" and not args.no_c4:
+        # the batched sweep, autotuned, at 100 ms steps: its batched / plain ratio
+        a4 = argparse.Namespace(**vars(args))
+        a4.seed, a4.t_step, a4.steps, a4.warmup, a4.plain_ms = 4242, 100.0, 2, 1, 50.0
+        a4.autotune, a4.autotune_blocks = True, "128,256,512"
+        c4 = run_batched(a4, nestgpu)
+        line["c4"] = {k: c4[k] for k in ("value", "unit", "steps", "ms_per_step", "rtf", "config", "batched_vs_plain",
+                                         "plain_path")}
     print(json.dumps(line), flush=True)
 
 

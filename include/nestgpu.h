@@ -118,14 +118,16 @@ int nestgpu_batch_simulate_noalloc( nestgpu_kernel** ks, int n, const double* t_
 
 /* Layout autotuner of the batched form (no counterpart in the reference): groups the kernels
  * (none simulated yet) into classes by neuron models and neuron count (one per octave), times
- * probe runs of probe_ms on up to 4 instances of each class for every candidate
- * targets_per_block, and sets the cheapest on every instance of the class (each keeps its
- * delivery_mode). A probe's cost is its block-time, wall time x min(blocks, 2 x compute units):
- * its share of a batch that fills the device. Probe runs are discarded: every kernel returns
- * to its state before the first Prepare. chosen[i] (may be NULL) receives the value picked for
- * kernel i. */
+ * probe runs of probe_ms on up to 4 instances of each class for every pair of a candidate
+ * targets_per_block (the target tile of a workgroup) and a candidate cap on the ring-buffer
+ * slots staged in LDS (window_caps, 0 = as many as fit; NULL or n_win 0: that default only),
+ * and sets the cheapest pair on every instance of the class (each keeps its delivery_mode). A
+ * probe's cost is its block-time, wall time x min(blocks, 2 x compute units): its share of a
+ * batch that fills the device. Probe runs are discarded: every kernel returns to its state
+ * before the first Prepare. chosen[i] and chosen_win[i] (either may be NULL) receive the
+ * values picked for kernel i. */
 int nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand,
-  int* chosen );
+  const int* window_caps, int n_win, int* chosen, int* chosen_win );
 
 /* Served form (counterpart of nest-server, pynest/nest/server/hl_api_server.py, which runs
  * one kernel per request and serialises calls): a request loop on one device. Admitted

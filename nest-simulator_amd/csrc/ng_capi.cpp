@@ -366,7 +366,8 @@ nestgpu_batch_simulate_noalloc( nestgpu_kernel** ks, int n, const double* t_ms )
 }
 
 int
-nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand, int* chosen )
+nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* candidates, int n_cand,
+  const int* window_caps, int n_win, int* chosen, int* chosen_win )
 {
   try
   {
@@ -403,11 +404,15 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
       const size_t ns = std::min< size_t >( 4, members.size() );
       for ( size_t q = 0; q < ns; ++q )
         sample.push_back( members[ q * members.size() / ns ] );
-      int best = -1;
+      int best = -1, best_win = 0;
       double best_ms = 0;
-      for ( int c = 0; c < n_cand; ++c )
+      const int nw = window_caps and n_win > 0 ? n_win : 1;
+      for ( int c = 0; c < n_cand * nw; ++c )
       {
-        const int tb = candidates[ c ];
+        const int tb = candidates[ c / nw ];
+        const int wcap = window_caps and n_win > 0 ? window_caps[ c % nw ] : 0;
+        if ( wcap < 0 )
+          throw std::runtime_error( "batch_autotune: window slot caps must be >= 0 (0: as many as fit)" );
         bool ok = tb >= 1 and tb <= static_cast< int >( nestgpu::MAX_TB );
         for ( int i : sample )
           ok = ok and ( ks[ i ]->k->num_neurons() + tb - 1 ) / tb <= nestgpu::PMAX_GLOB;
@@ -421,7 +426,7 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
           keep.push_back( ks[ i ]->k->keep_host_connections() );
           ks[ i ]->k->set_keep_host_connections( true );  // probes rebuild the device copy
           ks[ i ]->k->reset_device();
-          ks[ i ]->k->set_layout( tb, ks[ i ]->k->delivery_mode() );
+          ks[ i ]->k->set_layout( tb, ks[ i ]->k->delivery_mode(), wcap );
           ks[ i ]->k->prepare_for_batch();
           engines.push_back( &ks[ i ]->k->engine() );
           steps.push_back( ks[ i ]->k->steps_for( probe_ms ) );
@@ -437,6 +442,7 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
         if ( best < 0 or ms < best_ms )
         {
           best = tb;
+          best_win = wcap;
           best_ms = ms;
         }
         for ( size_t q = 0; q < sample.size(); ++q )
@@ -450,9 +456,11 @@ nestgpu_batch_autotune( nestgpu_kernel** ks, int n, double probe_ms, const int* 
       for ( int i : members )
       {
         ks[ i ]->k->reset_device();
-        ks[ i ]->k->set_layout( best, ks[ i ]->k->delivery_mode() );
+        ks[ i ]->k->set_layout( best, ks[ i ]->k->delivery_mode(), best_win );
         if ( chosen )
           chosen[ i ] = best;
+        if ( chosen_win )
+          chosen_win[ i ] = best_win;
       }
     }
     return 0;

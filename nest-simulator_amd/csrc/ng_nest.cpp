@@ -2340,12 +2340,13 @@ Kernel::release_device()
 }
 
 void
-Kernel::set_layout( int targets_per_block, int delivery_mode )
+Kernel::set_layout( int targets_per_block, int delivery_mode, int window_slots )
 {
   if ( simulated_ )
     throw std::runtime_error( "KernelException: the device layout cannot change after Simulate" );
   targets_per_block_ = targets_per_block;
   delivery_mode_ = delivery_mode;
+  window_slots_ = window_slots;
   store_dirty_ = true;
 }
 

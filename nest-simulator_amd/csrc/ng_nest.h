@@ -436,7 +436,7 @@ public:
   // Discard the device copy and every simulation result, back to the state before the
   // first Prepare (used by the layout autotuner's probe runs).
   void reset_device();
-  void set_layout( int targets_per_block, int delivery_mode );
+  void set_layout( int targets_per_block, int delivery_mode, int window_slots = 0 );
   // Device memory Prepare will allocate, estimated from the host tables (serving admission).
   uint64_t estimate_device_bytes() const;
   // Copy the device state into the host tables, collect the spike log and free the device

@@ -56,7 +56,9 @@ _SIGS = {
     "nestgpu_batch_simulate": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
     "nestgpu_batch_simulate_noalloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _f64p]),
     "nestgpu_batch_autotune": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double,
-                                              ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+                                              ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)]),
     "nestgpu_server_new": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
     "nestgpu_server_free": (None, [_vp]),
     "nestgpu_server_submit": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, _i64p]),
@@ -560,16 +562,20 @@ def batch_simulate_noalloc(kernels, times):
         raise NESTError(_lib.nestgpu_last_error(None).decode())
 
 
-def batch_autotune(kernels, probe_ms=20.0, candidates=(64, 128, 256, 512)):
-    """Pick targets_per_block per class (neuron models, size octave) of not yet simulated
-    kernels from probe runs (discarded), scored by block-time. Returns the value chosen for
-    each kernel."""
+def batch_autotune(kernels, probe_ms=20.0, candidates=(64, 128, 256, 512), window_caps=(0,), with_windows=False):
+    """Pick targets_per_block and the cap on LDS ring slots (0: as many as fit) per class
+    (neuron models, size octave) of not yet simulated kernels from probe runs (discarded) of
+    every pair, scored by block-time. Returns the targets_per_block chosen for each kernel, or
+    with with_windows the list of (targets_per_block, window cap) pairs."""
     arr = (_vp * len(kernels))(*[k._h for k in kernels])
     cand = (ctypes.c_int * len(candidates))(*[int(c) for c in candidates])
+    wc = (ctypes.c_int * len(window_caps))(*[int(c) for c in window_caps])
     out = (ctypes.c_int * len(kernels))()
-    if _lib.nestgpu_batch_autotune(arr, len(kernels), float(probe_ms), cand, len(candidates), out) != 0:
+    out_w = (ctypes.c_int * len(kernels))()
+    if _lib.nestgpu_batch_autotune(arr, len(kernels), float(probe_ms), cand, len(candidates), wc, len(window_caps),
+                                   out, out_w) != 0:
         raise NESTError(_lib.nestgpu_last_error(None).decode())
-    return list(out)
+    return list(zip(out, out_w)) if with_windows else list(out)
 
 
 class Server:
