@@ -26,6 +26,9 @@
 //   deliver .......... event_delivery_manager.cpp:599-776; connector_base.h:399-428;
 //                      models/static_synapse.h:160-169; nestkernel/event.h:1006-1013
 //   ring buffers ..... nestkernel/ring_buffer.h:157-203,395-408
+//   plasticity ....... models/stdp_synapse.h:187-198,222-283 (send, facilitate_, depress_);
+//                      nestkernel/archiving_node.cpp:38-48,85-113,165-230 (get_K_value,
+//                      get_history, set_spiketime); connection_manager.cpp:78 (stdp_eps)
 //   models ........... models/iaf_psc_delta.cpp:69-343; models/iaf_psc_exp.cpp:71-363;
 //                      libnestutil/iaf_propagator.cpp:42-74; models/aeif_cond_exp.cpp:79-541
 //   drive ............ models/spike_generator.cpp:321-389
@@ -552,6 +555,12 @@ struct Neuron
   // reads them) taken after each sampled step (UniversalDataLogger::record_data)
   bool sampled = false;
   std::vector< std::array< double, 5 > > samples;
+  // ArchivingNode (archiving_node.cpp:38-48): tau_minus, and for a neuron with incoming
+  // stdp synapses its post-spike history (time, Kminus) and trace
+  double tau_minus = 20.0;
+  bool stdp_in = false;
+  double Kminus = 0.0, last_spike = -1.0;
+  std::vector< std::pair< double, double > > hist;
   // state
   double y0 = 0, y3 = 0, refr_buf = 0;          // delta
   double V = 0, i_ex = 0, i_in = 0, i0 = 0, i1 = 0;  // exp
@@ -802,7 +811,15 @@ struct Conn
 {
   long src, tgt;
   double w;
-  long d;  // steps
+  long d;             // steps
+  int syn = -1;       // stdp parameter set, -1 for static_synapse
+  double kplus = 0;   // stdp_synapse Kplus
+};
+
+// stdp_synapse parameters (stdp_synapse.h:200-210)
+struct StdpPrm
+{
+  double tau_plus = 20.0, lambda = 0.01, alpha = 1.0, mu_plus = 1.0, mu_minus = 1.0, Wmax = 100.0;
 };
 
 struct Drive
@@ -846,9 +863,17 @@ struct Kernel
     std::vector< int32_t > tgt;
     std::vector< int32_t > d;
     std::vector< double > w;
+    // stdp_synapse state per position, and the connection each position holds
+    std::vector< int32_t > syn;
+    std::vector< double > kplus, t_last;
+    std::vector< std::pair< int, long > > ref;
   };
   std::vector< Part > parts;
   bool store_valid = false;
+  std::vector< StdpPrm > stdp_prm;
+  int next_syn = -1;          // synapse of the next Connect calls: -1 static, else stdp set
+  double next_kplus = 0.0;
+  static constexpr double stdp_eps = 1.0e-6;  // ConnectionManager::stdp_eps_
   // drive lookup per neuron idx: (drive index, position) or -1
   std::vector< std::pair< int, long > > drive_of;
 
@@ -893,6 +918,12 @@ struct Kernel
   {
     auto has = [ &d ]( const char* k ) { return d.count( k ) > 0; };
     auto get = [ &d ]( const char* k ) { return d.at( k ); };
+    if ( has( "tau_minus" ) )
+    {
+      if ( get( "tau_minus" ) <= 0 )
+        throw std::runtime_error( "BadProperty: tau_minus must be positive." );
+      n.tau_minus = get( "tau_minus" );
+    }
     if ( n.model == AEIF_COND_EXP )
     {
       const char* keys[] = { "V_peak", "V_reset", "t_ref", "g_L", "C_m", "E_ex", "E_in", "E_L", "Delta_T", "tau_w",
@@ -1087,7 +1118,14 @@ struct Kernel
       throw std::runtime_error( "BadDelay: Delay must be greater than or equal to resolution" );
     lo = std::min( lo, ds );
     hi = std::max( hi, ds );
-    conns[ vp ].push_back( Conn { s, t, weight, ds } );
+    if ( next_syn >= 0 )
+    {
+      // stdp_synapse::set_status (stdp_synapse.h:303-314): weight and Wmax of the same sign
+      const double Wmax = stdp_prm[ next_syn ].Wmax;
+      if ( not( ( ( weight >= 0 ) - ( weight < 0 ) ) == ( ( Wmax >= 0 ) - ( Wmax < 0 ) ) ) )
+        throw std::runtime_error( "BadProperty: Weight and Wmax must have same sign." );
+    }
+    conns[ vp ].push_back( Conn { s, t, weight, ds, next_syn, next_kplus } );
   }
 
   void
@@ -1305,20 +1343,31 @@ struct Kernel
       p.tgt.resize( p.ptr[ G ] );
       p.d.resize( p.ptr[ G ] );
       p.w.resize( p.ptr[ G ] );
+      p.syn.resize( p.ptr[ G ] );
+      p.kplus.resize( p.ptr[ G ] );
+      p.t_last.resize( p.ptr[ G ] );
+      p.ref.resize( p.ptr[ G ] );
     }
     std::vector< std::vector< long > > cur( n_threads );
     for ( int i = 0; i < n_threads; ++i )
       cur[ i ].assign( parts[ i ].ptr.begin(), parts[ i ].ptr.end() - 1 );
     // stable: VP order, then creation order within a VP; per target this is creation order
     for ( int vp = 0; vp < n_vp; ++vp )
-      for ( const auto& c : conns[ vp ] )
+      for ( size_t ci = 0; ci < conns[ vp ].size(); ++ci )
       {
+        const Conn& c = conns[ vp ][ ci ];
         const long ti = gid2idx[ c.tgt ];
         const int o = owner( ti );
         const long pos = cur[ o ][ c.src ]++;
         parts[ o ].tgt[ pos ] = static_cast< int32_t >( ti );
         parts[ o ].d[ pos ] = static_cast< int32_t >( c.d );
         parts[ o ].w[ pos ] = c.w;
+        parts[ o ].syn[ pos ] = c.syn;
+        parts[ o ].kplus[ pos ] = c.kplus;
+        parts[ o ].t_last[ pos ] = 0.0;  // stdp_synapse: t_lastspike_ starts at 0
+        parts[ o ].ref[ pos ] = std::make_pair( vp, static_cast< long >( ci ) );
+        if ( c.syn >= 0 )
+          neurons[ ti ].stdp_in = true;  // ArchivingNode::n_incoming_ > 0
       }
     drive_of.assign( neurons.size(), std::make_pair( -1, 0L ) );
     for ( size_t k = 0; k < drives.size(); ++k )
@@ -1513,6 +1562,16 @@ struct Kernel
         try
         {
           update_neuron( neurons[ i ], origin, from_step, to_step, &masks[ i * nw ], dummy );
+          for ( long w = 0; w < nw; ++w )
+          {
+            uint64_t m = masks[ i * nw + w ];
+            while ( m )
+            {
+              const long lag = w * 64 + __builtin_ctzll( m );
+              m &= m - 1;
+              set_spiketime( neurons[ i ], origin + lag + 1 );
+            }
+          }
           const auto& dv = drive_of[ i ];
           if ( dv.first >= 0 )
           {
@@ -1601,7 +1660,11 @@ struct Kernel
         // stamp = clock + lag + 1 - min_delay; slot step = stamp + d - 1
         const long base = clock + lag - min_delay;
         for ( long k = p.ptr[ s ]; k < p.ptr[ s + 1 ]; ++k )
+        {
+          if ( p.syn[ k ] >= 0 )
+            stdp_send( const_cast< Part& >( p ), k, neurons[ p.tgt[ k ] ], base + 1 );
           handle( neurons[ p.tgt[ k ] ], base + p.d[ k ], p.w[ k ], 1.0 );
+        }
         delivered += p.ptr[ s + 1 ] - p.ptr[ s ];
       }
 #pragma omp atomic
@@ -1609,6 +1672,87 @@ struct Kernel
     }
   }
   long events_delivered = 0;
+
+  // Time::get_ms of a step stamp: MS_PER_TIC * tics (nest_time.h:494-505)
+  double
+  stamp_ms( long step ) const
+  {
+    return tb.ms_per_tic * static_cast< double >( step * tb.tics_per_step );
+  }
+
+  // stdp_synapse::send (stdp_synapse.h:222-283) for the connection at position k of part p,
+  // before the event is handled; the spike's stamp is the step `stamp`
+  void
+  stdp_send( Part& p, long k, Neuron& tgt, long stamp )
+  {
+    const StdpPrm& P = stdp_prm[ p.syn[ k ] ];
+    const double t_spike = stamp_ms( stamp );
+    const double dendritic_delay = p.d[ k ] * tb.ms_per_step;  // Time::delay_steps_to_ms
+    const double t_last = p.t_last[ k ];
+    double w = p.w[ k ];
+    const double kplus = p.kplus[ k ];
+    auto powm = [ this ]( double x, double m ) { return m == 1.0 ? x : m == 0.0 ? 1.0 : mm.pow( x, m ); };
+    // facilitate_ / depress_ (stdp_synapse.h:187-198)
+    auto facilitate = [ & ]( double w0, double kp )
+    {
+      const double norm_w = ( w0 / P.Wmax ) + ( P.lambda * powm( 1.0 - ( w0 / P.Wmax ), P.mu_plus ) * kp );
+      return norm_w < 1.0 ? norm_w * P.Wmax : P.Wmax;
+    };
+    auto depress = [ & ]( double w0, double km )
+    {
+      const double norm_w = ( w0 / P.Wmax ) - ( P.alpha * P.lambda * powm( w0 / P.Wmax, P.mu_minus ) * km );
+      return norm_w > 0.0 ? norm_w * P.Wmax : 0.0;
+    };
+    // get_history( t_last - d, t_spike - d ): entries with t1 + eps <= t < t2 + eps, oldest
+    // first (archiving_node.cpp:165-191)
+    const double t1_lim = ( t_last - dendritic_delay ) + stdp_eps;
+    const double t2_lim = ( t_spike - dendritic_delay ) + stdp_eps;
+    for ( const auto& h : tgt.hist )
+      if ( h.first >= t1_lim and h.first < t2_lim )
+      {
+        const double minus_dt = t_last - ( h.first + dendritic_delay );
+        w = facilitate( w, kplus * mm.exp( minus_dt / P.tau_plus ) );
+      }
+    // get_K_value( t_spike - d ) (archiving_node.cpp:85-113)
+    const double t = t_spike - dendritic_delay;
+    double K = 0.0;
+    for ( long i = static_cast< long >( tgt.hist.size() ) - 1; i >= 0; --i )
+      if ( t - tgt.hist[ i ].first > stdp_eps )
+      {
+        K = tgt.hist[ i ].second * mm.exp( ( tgt.hist[ i ].first - t ) * ( 1. / tgt.tau_minus ) );
+        break;
+      }
+    w = depress( w, K );
+    p.w[ k ] = w;
+    p.kplus[ k ] = kplus * mm.exp( ( t_last - t_spike ) / P.tau_plus ) + 1.0;
+    p.t_last[ k ] = t_spike;
+  }
+
+  // ArchivingNode::set_spiketime (archiving_node.cpp:196-230) for a neuron with incoming stdp
+  // synapses: the history is never pruned here, which changes no value read from it
+  void
+  set_spiketime( Neuron& n, long stamp )
+  {
+    const double t_sp_ms = stamp_ms( stamp );
+    if ( n.stdp_in )
+    {
+      n.Kminus = n.Kminus * mm.exp( ( n.last_spike - t_sp_ms ) * ( 1. / n.tau_minus ) ) + 1.0;
+      n.last_spike = t_sp_ms;
+      n.hist.emplace_back( t_sp_ms, n.Kminus );
+    }
+    else
+      n.last_spike = t_sp_ms;
+  }
+
+  // the current weights of stdp connections back into the connection lists
+  void
+  sync_weights()
+  {
+    for ( const Part& p : parts )
+      for ( size_t k = 0; k < p.syn.size(); ++k )
+        if ( p.syn[ k ] >= 0 )
+          conns[ p.ref[ k ].first ][ p.ref[ k ].second ].w = p.w[ k ];
+  }
 
   double
   get_state( long gid, const std::string& f )
@@ -1780,7 +1924,7 @@ or_num_connections( void* k )
 int
 or_get_connections( void* k, long* src, long* tgt, double* w, long* d )
 {
-  OR_TRY( long i = 0; for ( const auto& v
+  OR_TRY( K->sync_weights(); long i = 0; for ( const auto& v
                             : K->conns ) for ( const auto& c
                                                : v ) {
     src[ i ] = c.src;
@@ -1888,6 +2032,26 @@ or_seed_seq_generate( const uint32_t* seed, int n_seed, uint32_t* out, int n_out
   oracle::SeedSeqFe128 ss( s );
   ss.generate( out, out + n_out );
   return 0;
+}
+
+// synapse model of the next Connect calls: kind 0 static_synapse, 1 stdp_synapse with the
+// parameters (tau_plus, lambda, alpha, mu_plus, mu_minus, Wmax) and the initial Kplus
+int
+or_set_synapse( void* k, int kind, const double* prm, double kplus )
+{
+  OR_TRY( if ( kind == 0 ) K->next_syn = -1; else {
+    if ( kplus < 0 ) throw std::runtime_error( "BadProperty: Kplus must be non-negative." );
+    oracle::StdpPrm P;
+    P.tau_plus = prm[ 0 ];
+    P.lambda = prm[ 1 ];
+    P.alpha = prm[ 2 ];
+    P.mu_plus = prm[ 3 ];
+    P.mu_minus = prm[ 4 ];
+    P.Wmax = prm[ 5 ];
+    K->stdp_prm.push_back( P );
+    K->next_syn = static_cast< int >( K->stdp_prm.size() ) - 1;
+    K->next_kplus = kplus;
+  } )
 }
 
 // rng_type: 0 mt19937_64, 1 Philox_32; reseeds the per-VP streams (before any draw)

@@ -48,7 +48,8 @@ def lib():
         L.or_iaf_propagator.restype = ctypes.c_double
         L.or_iaf_propagator.argtypes = [ctypes.c_double] * 4
         L.or_num_samples.restype = ctypes.c_long
-        for name in ("or_set_rng_type", "or_set_sampling", "or_get_samples", "or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
+        L.or_set_synapse.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double]
+        for name in ("or_set_synapse", "or_set_rng_type", "or_set_sampling", "or_get_samples", "or_set_threads", "or_set_portable_math", "or_set_status", "or_get_state", "or_connect",
                      "or_add_connections", "or_get_connections", "or_add_drive", "or_simulate", "or_run_steps",
                      "or_get_spikes", "or_clear_spikes", "or_kernel_status"):
             getattr(L, name).restype = ctypes.c_int
@@ -177,8 +178,23 @@ class Oracle:
                                       field.encode(), _p(out, ctypes.c_double)))
         return out
 
+    STDP_KEYS = ("tau_plus", "lambda", "alpha", "mu_plus", "mu_minus", "Wmax")
+    STDP_DEFAULTS = {"tau_plus": 20.0, "lambda": 0.01, "alpha": 1.0, "mu_plus": 1.0, "mu_minus": 1.0, "Wmax": 100.0}
+
     def connect(self, pre, post, rule="all_to_all", rule_param=0.0, weight=None, delay=None,
-                allow_autapses=True, allow_multapses=True):
+                allow_autapses=True, allow_multapses=True, stdp=None):
+        """stdp: None for static_synapse, else a dict of stdp_synapse parameters (and Kplus)."""
+        if stdp is not None:
+            prm = dict(self.STDP_DEFAULTS, **{k: v for k, v in stdp.items() if k != "Kplus"})
+            arr = (ctypes.c_double * 6)(*[float(prm[k]) for k in self.STDP_KEYS])
+            self._chk(self.L.or_set_synapse(ctypes.c_void_p(self.k), 1, arr, ctypes.c_double(stdp.get("Kplus", 0.0))))
+        try:
+            self._connect(pre, post, rule, rule_param, weight, delay, allow_autapses, allow_multapses)
+        finally:
+            if stdp is not None:
+                self._chk(self.L.or_set_synapse(ctypes.c_void_p(self.k), 0, None, ctypes.c_double(0.0)))
+
+    def _connect(self, pre, post, rule, rule_param, weight, delay, allow_autapses, allow_multapses):
         pre = np.ascontiguousarray(pre, dtype=np.int64)
         post = np.ascontiguousarray(post, dtype=np.int64)
         ws, ds = spec(weight), spec(delay)

@@ -791,3 +791,82 @@ def test_philox_engine_draws_match_reference_probe_live():
         got = _philox_draws(fam, 300, r, seed)
         want = [float(x) for x in txt]
         assert got == want, (seed, fam, r)
+
+
+# ------------------------------------------------------------------ stdp_synapse
+def _stdp_reproduce(pre, post, w0, Wmax, lam, alpha, tau_plus, tau_minus, mu_plus, mu_minus):
+    """Final weight of one stdp_synapse from its pre- and postsynaptic spike trains (post times
+    already shifted by the dendritic delay), processed event by event in time order: a
+    presynaptic spike depresses with the postsynaptic trace and then raises its own trace; a
+    postsynaptic spike facilitates with the presynaptic trace and then raises its own. This is
+    the reproduction testsuite/pytests/synapses/test_stdp_synapse_basic.py:120-212 makes."""
+    kp = km = 0.0
+    last_pre = last_post = 0.0
+    w = w0 / Wmax
+    i = j = 0
+    pre_t = pre[0] if len(pre) else np.inf
+    post_t = post[0] if len(post) else np.inf
+
+    def fac():
+        nonlocal w
+        w = min(1.0, w + lam * (1.0 - w) ** mu_plus * kp * np.exp((last_pre - post_t) / tau_plus))
+
+    def dep():
+        nonlocal w
+        w = max(0.0, w - lam * alpha * w ** mu_minus * km * np.exp((last_post - pre_t) / tau_minus))
+
+    while j < len(pre) or i < len(post):
+        if pre_t == post_t:
+            if last_post != post_t:
+                fac()
+            if last_pre != pre_t:
+                dep()
+            if j + 1 >= len(pre):
+                break
+            kp = kp * np.exp((last_pre - pre_t) / tau_plus) + 1.0
+            j += 1
+            last_pre, pre_t = pre_t, pre[j]
+            if i + 1 < len(post):
+                km = km * np.exp((last_post - post_t) / tau_minus) + 1.0
+            i += 1
+            last_post, post_t = post_t, (post[i] if i < len(post) else np.inf)
+        elif pre_t < post_t:
+            dep()
+            kp = kp * np.exp((last_pre - pre_t) / tau_plus) + 1.0
+            if j + 1 >= len(pre):
+                break
+            j += 1
+            last_pre, pre_t = pre_t, pre[j]
+        else:
+            fac()
+            km = km * np.exp((last_post - post_t) / tau_minus) + 1.0
+            i += 1
+            last_post, post_t = post_t, (post[i] if i < len(post) else np.inf)
+    return w * Wmax
+
+
+@pytest.mark.parametrize("delay,mu", [(1.0, (1.0, 1.0)), (2.3, (1.0, 1.0)), (1.0, (0.0, 1.0)), (1.5, (1.0, 0.0))])
+def test_stdp_synapse_reproduced_from_spike_trains(delay, mu):
+    # one stdp_synapse from an irregularly firing neuron onto a driven one (the reference's
+    # test_stdp_synapse_basic setup, with iaf_psc_exp neurons and seeded Poisson drive)
+    T, h = 10000.0, 0.1
+    o = Oracle(resolution=h)
+    pre = o.create("iaf_psc_exp", 1, {"I_e": 300.0})
+    post = o.create("iaf_psc_exp", 1, {"I_e": 330.0, "tau_minus": 30.0})
+    n_steps = int(round(T / h))
+    rng = np.random.default_rng(int(delay * 10) + int(mu[0]) * 3 + int(mu[1]))
+    o.add_drive(pre[0], rng.poisson(0.04, size=(n_steps, 1)).astype(np.uint8), 900.0, 1.0)
+    o.add_drive(post[0], rng.poisson(0.03, size=(n_steps, 1)).astype(np.uint8), 700.0, 1.0)
+    prm = {"tau_plus": 20.0, "lambda": 0.01, "alpha": 1.1, "mu_plus": mu[0], "mu_minus": mu[1], "Wmax": 90.0}
+    o.connect(pre, post, "one_to_one", weight=45.0, delay=delay, stdp=prm)
+    o.simulate(T)
+    g, s = o.spikes()
+    t_pre = s[g == pre[0]] * h
+    t_post = s[g == post[0]] * h
+    # the last slice's spikes are not delivered before the run ends
+    t_pre = t_pre[t_pre <= T - 1.0]
+    assert len(t_pre) > 50 and len(t_post) > 50
+    w_sim = o.connections()[2][0]
+    w_rep = _stdp_reproduce(t_pre, t_post + delay, 45.0, 90.0, 0.01, 1.1, 20.0, 30.0, mu[0], mu[1])
+    assert w_sim != 45.0
+    assert w_sim == pytest.approx(w_rep, abs=1e-12), (w_sim, w_rep)
