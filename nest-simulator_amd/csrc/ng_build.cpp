@@ -189,6 +189,12 @@ Kernel::rebalance_blocks()
   // update and the delivery phases of an iaf_psc_exp circuit on the MI355X). The layout
   // changes only if the largest block work drops by 5 % or more; results do not depend on it.
   Engine& E = *engine_;
+  if ( not stdp_sets_.empty() )
+  {
+    // the move rebuilds the store from the host table, which holds no stdp_synapse state
+    rebalanced_ = true;
+    return;
+  }
   const int64_t slices = rx_steps_ / std::max< long >( min_delay_, 1 );
   if ( slices < 64 or E.part != 0 or E.from_step != 0 )
     return;
@@ -592,6 +598,21 @@ Kernel::build_spec( BuildSpec& b )
   };
   HostArray< uint32_t > src_dev( E );
   HostArray< uint16_t > delay( E );
+  // stdp_synapse: the synapse of every connection in creation order, and the creation index
+  // each store position holds (carried through the passes)
+  bool plastic = false;
+  for ( const auto& C : conns_ )
+    plastic = plastic or not C.syn.empty();
+  std::vector< uint16_t > syn_flat;
+  HostArray< uint32_t > cidx;
+  if ( plastic )
+  {
+    syn_flat.assign( static_cast< size_t >( E ), 0 );
+    for ( size_t v = 0; v < conns_.size(); ++v )
+      for ( size_t i = 0; i < conns_[ v ].syn.size(); ++i )
+        syn_flat[ vp_first[ v ] + i ] = conns_[ v ].syn[ i ];
+    cidx = HostArray< uint32_t >( E );
+  }
   // offsets fit 32 bits: fewer than 2^32 connections
   std::vector< uint32_t > start( static_cast< size_t >( n_workers ) * N, 0 );
   parallel(
@@ -642,7 +663,11 @@ Kernel::build_spec( BuildSpec& b )
         rec.w_lo = static_cast< uint32_t >( wbits );
         rec.w_hi = static_cast< uint32_t >( wbits >> 32 );
         rec.word = static_cast< uint32_t >( gid_to_dev_[ C.tgt[ i ] ] );  // device target, for now
+        if ( C.d[ i ] - min_delay_ > static_cast< long >( DELAY_MASK ) )
+          throw std::runtime_error( "NotImplemented: delays more than 32767 steps above min_delay" );
         delay[ pos ] = static_cast< uint16_t >( C.d[ i ] - min_delay_ );
+        if ( plastic )
+          cidx[ pos ] = static_cast< uint32_t >( e );
       }
     } );
   start = std::vector< uint32_t >();
@@ -666,6 +691,13 @@ Kernel::build_spec( BuildSpec& b )
       std::vector< SynRec > row_syn;
       std::vector< uint16_t > row_delay;
       std::vector< uint32_t > cur( P );
+      struct RecC
+      {
+        SynRec r;
+        uint32_t c;
+      };
+      std::vector< RecC > row_rc;
+      std::vector< uint32_t > row_c;
       for ( uint32_t s = s_lo; s < s_hi; ++s )
       {
         const uint64_t r0 = row_ptr[ s ], r1 = row_ptr[ s + 1 ];
@@ -681,6 +713,33 @@ Kernel::build_spec( BuildSpec& b )
         }
         row_syn.assign( b.syn.begin() + r0, b.syn.begin() + r1 );
         row_delay.assign( delay.begin() + r0, delay.begin() + r1 );
+        if ( plastic )
+        {
+          // the same grouping and delay order, each record with its creation index
+          row_c.assign( cidx.begin() + r0, cidx.begin() + r1 );
+          row_rc.resize( r1 - r0 );
+          for ( size_t j = 0; j < row_syn.size(); ++j )
+          {
+            const uint32_t t = row_syn[ j ].word;
+            const uint32_t blk = blk_of[ t ];
+            RecC& x = row_rc[ cur[ blk ]++ - r0 ];
+            x.r.w_lo = row_syn[ j ].w_lo;
+            x.r.w_hi = row_syn[ j ].w_hi;
+            x.r.word = ( t - b.bstart[ blk ] ) | ( static_cast< uint32_t >( row_delay[ j ] ) << 16 )
+              | ( syn_flat[ row_c[ j ] ] ? PLASTIC_BIT : 0u );
+            x.c = row_c[ j ];
+          }
+          for ( uint32_t q = 0; q < P; ++q )
+            std::stable_sort( row_rc.begin() + ( bo[ q ] - r0 ), row_rc.begin() + ( bo[ q + 1 ] - r0 ),
+              []( const RecC& x, const RecC& y )
+              { return ( ( x.r.word >> 16 ) & DELAY_MASK ) < ( ( y.r.word >> 16 ) & DELAY_MASK ); } );
+          for ( size_t k = 0; k < row_rc.size(); ++k )
+          {
+            b.syn[ r0 + k ] = row_rc[ k ].r;
+            cidx[ r0 + k ] = row_rc[ k ].c;
+          }
+          continue;
+        }
         for ( size_t j = 0; j < row_syn.size(); ++j )
         {
           const uint32_t t = row_syn[ j ].word;
@@ -696,7 +755,8 @@ Kernel::build_spec( BuildSpec& b )
           const uint32_t len = bo[ q + 1 ] - bo[ q ];
           if ( len <= 1 )
             continue;
-          auto by_delay = []( const SynRec& x, const SynRec& y ) { return ( x.word >> 16 ) < ( y.word >> 16 ); };
+          auto by_delay = []( const SynRec& x, const SynRec& y )
+          { return ( ( x.word >> 16 ) & DELAY_MASK ) < ( ( y.word >> 16 ) & DELAY_MASK ); };
           if ( len > 32 )
           {
             std::stable_sort( run, run + len, by_delay );
@@ -713,6 +773,47 @@ Kernel::build_spec( BuildSpec& b )
         }
       }
     } );
+  // stdp_synapse: per-record state, where each connection went, and the neurons' flags
+  b.stdp_prm.clear();
+  b.stdp_syn = HostArray< StdpSyn >();
+  b.stdp_in.clear();
+  b.stdp_out.clear();
+  syn_pos_.clear();
+  if ( plastic )
+  {
+    for ( const StdpSet& q : stdp_sets_ )
+      b.stdp_prm.push_back( StdpPrm { q.tau_plus, q.lambda, q.alpha, q.mu_plus, q.mu_minus, q.Wmax } );
+    b.stdp_syn = HostArray< StdpSyn >( E );
+    syn_pos_.resize( static_cast< size_t >( E ) );
+    parallel(
+      [ & ]( int w )
+      {
+        const int64_t lo = E * w / n_workers, hi = E * ( w + 1 ) / n_workers;
+        for ( int64_t pos = lo; pos < hi; ++pos )
+        {
+          const uint32_t c = cidx[ pos ];
+          syn_pos_[ c ] = static_cast< uint32_t >( pos );
+          const uint16_t sv = syn_flat[ c ];
+          // stdp_synapse state at creation: Kplus from the syn_spec, t_lastspike_ 0
+          b.stdp_syn[ pos ] = StdpSyn { sv ? stdp_sets_[ sv - 1 ].Kplus : 0.0, 0.0, sv ? sv - 1u : 0u, 0u };
+        }
+      } );
+    b.stdp_in.assign( N, 0 );
+    b.stdp_out.assign( N, 0 );
+    for ( size_t v = 0; v < conns_.size(); ++v )
+      for ( size_t i = 0; i < conns_[ v ].syn.size(); ++i )
+        if ( conns_[ v ].syn[ i ] )
+        {
+          b.stdp_in[ gid_to_dev_[ conns_[ v ].tgt[ i ] ] ] = 1;
+          b.stdp_out[ gid_to_dev_[ conns_[ v ].src[ i ] ] ] = 1;
+        }
+    b.tau_minus_inv.assign( N, 0.0 );
+    for ( uint32_t i = 0; i < N; ++i )
+      b.tau_minus_inv[ dev_of[ i ] ] = 1. / params_[ i ].tau_minus;
+  }
+  b.ms_per_tic = tb_.ms_per_tic;
+  b.ms_per_step = tb_.ms_per_step;
+  b.tics_per_step = tb_.tics_per_step;
   // spikes are logged only while a spike_recorder observes some neuron
   bool recording = false;
   for ( const auto& rec : recorders_ )

@@ -49,6 +49,27 @@ struct ParamSet
   int64_t drive_steps;  // emission steps covered
 };
 
+// stdp_synapse (models/stdp_synapse.h:200-215): a parameter set, and the state of one synapse
+// record (Kplus, the time of its last presynaptic spike, its parameter set); the weight is the
+// record's own
+struct StdpPrm
+{
+  double tau_plus, lambda, alpha, mu_plus, mu_minus, Wmax;
+};
+struct StdpSyn
+{
+  double kplus;
+  double t_last;  // ms
+  uint32_t prm;
+  uint32_t pad;
+};
+// synapse record word: local target (bits 0-15), delay - min_delay (bits 16-30), stdp (bit 31)
+constexpr uint32_t PLASTIC_BIT = 0x80000000u;
+constexpr uint32_t DELAY_MASK = 0x7fffu;
+// post-spike history kept per neuron with incoming stdp synapses (ArchivingNode::history_):
+// the last STDP_HIST spikes (time, Kminus)
+constexpr uint32_t STDP_HIST = 64;
+
 struct PoissonTable
 {
   double cdf[ 64 ];
@@ -117,6 +138,23 @@ struct NetDev
   // ---- received events per target (device index), counted while set: the measured
   //      delivery work that target blocks are balanced by
   uint32_t* rx;
+  // ---- stdp_synapse (stdp != 0): parameter sets, per-record state, per-neuron flags,
+  //      1/tau_minus and post-spike history [neuron][STDP_HIST] with its count, trace and last
+  //      spike time; the time base turns step stamps into ms as nest::Time does
+  uint32_t stdp;
+  const StdpPrm* sprm;
+  StdpSyn* sst;
+  const uint8_t* stdp_in;   // has incoming stdp synapses (ArchivingNode::n_incoming_ > 0)
+  const uint8_t* stdp_out;  // has outgoing stdp synapses
+  const double* tau_minus_inv;
+  double* hist_t;
+  double* hist_k;
+  uint32_t* hist_n;
+  double* kminus;
+  double* last_spike;
+  double ms_per_tic;
+  double ms_per_step;
+  int64_t tics_per_step;
   uint32_t* flat_src; // exact receive order of a slice split across Run calls
   uint32_t* flat_lag;
   uint32_t* flat_meta;  // [0..1] flat_n per parity, [2..3] flat_ready per parity
@@ -191,7 +229,9 @@ enum ErrorFlag : unsigned long long
   ERR_SPIKE_LIST = 4,
   ERR_LOG = 8,
   ERR_AEIF_STEPS = 16,
-  ERR_SAMPLES = 32
+  ERR_SAMPLES = 32,
+  ERR_STDP_HISTORY = 64,  // a post-spike history ring held too few entries for an stdp synapse
+  ERR_STDP_TWICE = 128    // a neuron with outgoing stdp synapses spiked twice in one slice
 };
 
 // First step of [from, to) whose stamp origin + lag + 1 is a sampling stamp of the instance's

@@ -142,6 +142,8 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
   mask_ = 0;
   for ( const auto& ps : b.psets )
     mask_ |= 1 << ps.model;
+  if ( not b.stdp_prm.empty() )
+    mask_ |= 8;  // the launch needs the stdp_synapse instantiation of the slice kernel
   // ---- store and inputs (replaced on every build)
   net_.boff = static_cast< const uint32_t* >( S( upload( b.boff ) ) );
   net_.syn = static_cast< const SynRec* >( S( upload( b.syn ) ) );
@@ -151,6 +153,30 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
   net_.vp_begin = static_cast< const uint32_t* >( S( upload( b.vp_begin ) ) );
   net_.bstart = static_cast< const uint32_t* >( S( upload( b.bstart ) ) );
   bstart_ = b.bstart;
+  n_syn_ = b.syn.size();
+  // stdp_synapse: parameter sets, record state and neuron flags come with the store; the
+  // post-spike histories are state
+  net_.stdp = b.stdp_prm.empty() ? 0u : 1u;
+  net_.ms_per_tic = b.ms_per_tic;
+  net_.ms_per_step = b.ms_per_step;
+  net_.tics_per_step = b.tics_per_step;
+  net_.sprm = nullptr;
+  net_.sst = nullptr;
+  net_.stdp_in = net_.stdp_out = nullptr;
+  net_.tau_minus_inv = nullptr;
+  if ( net_.stdp )
+  {
+    if ( keep_state and not relayout )
+      throw std::runtime_error( "NotImplemented: rebuilding the device copy of a network with stdp_synapse after Simulate" );
+    if ( b.stdp_syn.size() != b.syn.size() or b.stdp_in.size() != b.n or b.stdp_out.size() != b.n
+         or b.tau_minus_inv.size() != b.n )
+      throw std::runtime_error( "stdp_synapse: per-record state and per-neuron flags expected" );
+    net_.sprm = static_cast< const StdpPrm* >( S( upload( b.stdp_prm ) ) );
+    net_.sst = static_cast< StdpSyn* >( S( upload( b.stdp_syn ) ) );
+    net_.stdp_in = static_cast< const uint8_t* >( S( upload( b.stdp_in ) ) );
+    net_.stdp_out = static_cast< const uint8_t* >( S( upload( b.stdp_out ) ) );
+    net_.tau_minus_inv = static_cast< const double* >( S( upload( b.tau_minus_inv ) ) );
+  }
   net_.drive_counts = static_cast< const uint8_t* >( S( upload( b.drive_counts ) ) );
   net_.drive_cols = b.drive_cols;
   net_.drive_idx = static_cast< const uint32_t* >( S( upload( b.drive_idx ) ) );
@@ -181,6 +207,18 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
     net_.counters = static_cast< unsigned long long* >( T( alloc( sizeof( unsigned long long ) * N_COUNTERS ) ) );
     rx_ = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * std::max< uint32_t >( b.n, 1 ) ) ) );
     net_.rx = nullptr;
+    net_.hist_t = net_.hist_k = net_.kminus = net_.last_spike = nullptr;
+    net_.hist_n = nullptr;
+    if ( net_.stdp )
+    {
+      // ArchivingNode state: no history, Kminus 0, last spike -1 ms (archiving_node.cpp:38-48)
+      const size_t nh = static_cast< size_t >( b.n ) * STDP_HIST;
+      net_.hist_t = static_cast< double* >( T( alloc( sizeof( double ) * nh ) ) );
+      net_.hist_k = static_cast< double* >( T( alloc( sizeof( double ) * nh ) ) );
+      net_.hist_n = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * b.n ) ) );
+      net_.kminus = static_cast< double* >( T( alloc( sizeof( double ) * b.n ) ) );
+      net_.last_spike = static_cast< double* >( T( upload( std::vector< double >( b.n, -1.0 ) ) ) );
+    }
     clock = 0;
     from_step = 0;
     part = 0;
@@ -316,6 +354,21 @@ Engine::relayout( const BuildSpec& b )
   h2d( net_.blk_spk, spk1.data(), spk1.size() * sizeof( uint32_t ) );
   h2d( net_.blk_part, part1.data(), part1.size() );
   return true;
+}
+
+void
+Engine::read_weights( std::vector< double >& w ) const
+{
+  hip_check( hipSetDevice( device_ ), "hipSetDevice" );
+  std::vector< SynRec > recs( n_syn_ );
+  if ( n_syn_ )
+    hip_check( hipMemcpy( recs.data(), net_.syn, n_syn_ * sizeof( SynRec ), hipMemcpyDeviceToHost ), "read weights" );
+  w.resize( n_syn_ );
+  for ( size_t i = 0; i < n_syn_; ++i )
+  {
+    const uint64_t bits = ( static_cast< uint64_t >( recs[ i ].w_hi ) << 32 ) | recs[ i ].w_lo;
+    std::memcpy( &w[ i ], &bits, sizeof bits );
+  }
 }
 
 void
@@ -699,6 +752,11 @@ finish_chunk( const std::vector< Engine* >& engines, const NetDev* h_nets, const
         why += " spike log capacity exceeded;";
       if ( flags & ERR_SAMPLES )
         why += " multimeter sample buffer exceeded;";
+      if ( flags & ERR_STDP_HISTORY )
+        why += " a neuron spiked more than " + std::to_string( STDP_HIST )
+          + " times between two spikes of a presynaptic stdp_synapse neuron (post-spike history);";
+      if ( flags & ERR_STDP_TWICE )
+        why += " a neuron with outgoing stdp_synapse connections spiked twice within one min_delay slice;";
       throw std::runtime_error( "device run failed:" + why );
     }
     const int64_t total_steps = E.from_step + d.todo;

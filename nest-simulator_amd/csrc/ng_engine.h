@@ -103,6 +103,14 @@ struct BuildSpec
   std::vector< PoissonTable > ptables;
   uint64_t log_cap = 1 << 20;
   bool log_enabled = false;
+  // stdp_synapse: parameter sets, state per synapse record (empty without stdp), neurons
+  // with incoming / outgoing stdp synapses, 1/tau_minus per neuron; the time base
+  std::vector< StdpPrm > stdp_prm;
+  HostArray< StdpSyn > stdp_syn;
+  std::vector< uint8_t > stdp_in, stdp_out;
+  std::vector< double > tau_minus_inv;
+  double ms_per_tic = 0.001, ms_per_step = 0.1;
+  int64_t tics_per_step = 100;
   // multimeter sampling: per neuron sample column (~0 none), interval and offset in steps
   std::vector< uint32_t > mm_col;
   uint32_t mm_cols = 0;
@@ -139,6 +147,9 @@ public:
   {
     return bstart_;
   }
+
+  // weights of the synapse records in store order (stdp_synapse changes them)
+  void read_weights( std::vector< double >& w ) const;
 
   // received events per target, counted by the slice kernel while switched on
   void count_received( bool on );
@@ -225,6 +236,7 @@ private:
   std::vector< void* > state_allocs_, store_allocs_, log_allocs_, mm_allocs_;
   std::vector< uint32_t > bstart_;  // target-block bounds of the current layout
   uint32_t* rx_ = nullptr;          // received-event counts (state allocation)
+  size_t n_syn_ = 0;                // synapse records of the store
   std::unordered_map< void*, size_t > sizes_;  // bytes of every live allocation
   uint64_t bytes_ = 0;
 };

@@ -170,7 +170,7 @@ run_split( gptr< const SynRec > syn, uint32_t g0, uint32_t len, int t )
   while ( lo < hi )
   {
     const uint32_t mid = ( lo + hi ) >> 1;
-    if ( static_cast< int >( syn[ g0 + mid ].word >> 16 ) < t )
+    if ( static_cast< int >( ( syn[ g0 + mid ].word >> 16 ) & DELAY_MASK ) < t )
       lo = mid + 1;
     else
       hi = mid;
@@ -466,7 +466,124 @@ __device__ unsigned long long g_phase_ticks[ 8 ];
 
 // Two workgroups of 8 waves per CU need at most 128 VGPRs (4 waves per SIMD). The aeif
 // integrator needs more; its kernels run one workgroup per CU.
-template < int MASK, bool WIN >
+// ---------------------------------------------------------------- stdp_synapse
+// stdp_synapse::send (models/stdp_synapse.h:222-283) for the event of record ridx, emitted at
+// step stamp `stamp`, to device target tdev over a delay of dsteps: facilitation by the
+// target's spikes since the synapse's last presynaptic spike, depression by its trace, then
+// the synapse's own trace. Returns the new weight, which the record keeps. Every record is
+// delivered at most once per slice (a source with stdp synapses spikes at most once per slice,
+// checked in the gather phase), so no other thread touches its state meanwhile; the target's
+// history is written only in the update phase.
+constexpr double STDP_EPS = 1.0e-6;  // ConnectionManager::stdp_eps_ (connection_manager.cpp:78)
+
+__device__ inline double
+stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t stamp, uint32_t dsteps )
+{
+  gref< StdpSyn > st = glob( d.sst )[ ridx ];
+  const double kplus = st.kplus, t_last = st.t_last;
+  gref< const StdpPrm > P = glob( d.sprm )[ st.prm ];
+  const double tau_plus = P.tau_plus, lambda = P.lambda, alpha = P.alpha, Wmax = P.Wmax;
+  const bool mu_p = P.mu_plus != 0.0, mu_m = P.mu_minus != 0.0;  // exponents 1 or 0 (host-checked)
+  const double t_spike = d.ms_per_tic * static_cast< double >( stamp * d.tics_per_step );  // Time::get_ms
+  const double dd = static_cast< double >( dsteps ) * d.ms_per_step;  // Time::delay_steps_to_ms
+  // ArchivingNode::get_history( t_last - d, t_spike - d ): entries t1 + eps <= t < t2 + eps
+  const double t1_lim = ( t_last - dd ) + STDP_EPS;
+  const double t2_lim = ( t_spike - dd ) + STDP_EPS;
+  const uint32_t n = glob( d.hist_n )[ tdev ];
+  const uint32_t first = n > STDP_HIST ? n - STDP_HIST : 0u;
+  gptr< const double > ht = glob( d.hist_t ) + static_cast< size_t >( tdev ) * STDP_HIST;
+  gptr< const double > hk = glob( d.hist_k ) + static_cast< size_t >( tdev ) * STDP_HIST;
+  // spikes dropped from the ring may lie in the range (or hold the trace before t2 - eps)
+  if ( first > 0 and ht[ first % STDP_HIST ] >= t1_lim )
+    g_or( &d.counters[ 0 ], ERR_STDP_HISTORY );
+  for ( uint32_t i = first; i < n; ++i )
+  {
+    const double th = ht[ i % STDP_HIST ];
+    if ( th >= t1_lim and th < t2_lim )
+    {
+      // facilitate_ (stdp_synapse.h:187-191)
+      const double minus_dt = t_last - ( th + dd );
+      const double kp = kplus * ng_exp( minus_dt / tau_plus );
+      const double norm_w = ( w / Wmax ) + ( lambda * ( mu_p ? 1.0 - ( w / Wmax ) : 1.0 ) * kp );
+      w = norm_w < 1.0 ? norm_w * Wmax : Wmax;
+    }
+  }
+  // ArchivingNode::get_K_value( t_spike - d ) (archiving_node.cpp:85-113): the latest spike
+  // strictly before
+  const double tq = t_spike - dd;
+  double K = 0.0;
+  for ( uint32_t i = n; i > first; --i )
+  {
+    const double th = ht[ ( i - 1 ) % STDP_HIST ];
+    if ( tq - th > STDP_EPS )
+    {
+      K = hk[ ( i - 1 ) % STDP_HIST ] * ng_exp( ( th - tq ) * glob( d.tau_minus_inv )[ tdev ] );
+      break;
+    }
+  }
+  // depress_ (stdp_synapse.h:193-198)
+  const double norm_w = ( w / Wmax ) - ( alpha * lambda * ( mu_m ? w / Wmax : 1.0 ) * K );
+  w = norm_w > 0.0 ? norm_w * Wmax : 0.0;
+  st.kplus = kplus * ng_exp( ( t_last - t_spike ) / tau_plus ) + 1.0;
+  st.t_last = t_spike;
+  uint64_t bits;
+  __builtin_memcpy( &bits, &w, sizeof bits );
+  gref< SynRec > r = glob( const_cast< SynRec* >( d.syn ) )[ ridx ];
+  r.w_lo = static_cast< uint32_t >( bits );
+  r.w_hi = static_cast< uint32_t >( bits >> 32 );
+  return w;
+}
+
+// ArchivingNode::set_spiketime (archiving_node.cpp:196-230) for the spikes of neuron dev at the
+// set bits of `bits` (lags of the slice at origin, in order); a neuron with outgoing stdp
+// synapses must spike at most once per slice
+__device__ inline void
+stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, const uint64_t* bits )
+{
+  const bool in = glob( d.stdp_in )[ dev ] != 0, out = glob( d.stdp_out )[ dev ] != 0;
+  if ( not in and not out )
+    return;
+  double last = glob( d.last_spike )[ dev ];
+  const double slice_start = d.ms_per_tic * static_cast< double >( ( origin + 1 ) * d.tics_per_step );
+  int count = 0;
+  double km = in ? glob( d.kminus )[ dev ] : 0.0;
+  uint32_t n = in ? glob( d.hist_n )[ dev ] : 0u;
+  for ( int w = 0; w < 4; ++w )
+  {
+    uint64_t x = bits[ w ];
+    while ( x )
+    {
+      const int lag = w * 64 + ( __ffsll( static_cast< long long >( x ) ) - 1 );
+      x &= x - 1;
+      const double t_sp = d.ms_per_tic * static_cast< double >( ( origin + lag + 1 ) * d.tics_per_step );
+      if ( out and ( count > 0 or last >= slice_start ) )
+        g_or( &d.counters[ 0 ], ERR_STDP_TWICE );
+      ++count;
+      if ( in )
+      {
+        km = km * ng_exp( ( last - t_sp ) * glob( d.tau_minus_inv )[ dev ] ) + 1.0;
+        const size_t at = static_cast< size_t >( dev ) * STDP_HIST + n % STDP_HIST;
+        glob( d.hist_t )[ at ] = t_sp;
+        glob( d.hist_k )[ at ] = km;
+        ++n;
+      }
+      last = t_sp;
+    }
+  }
+  if ( count )
+  {
+    glob( d.last_spike )[ dev ] = last;
+    if ( in )
+    {
+      glob( d.kminus )[ dev ] = km;
+      glob( d.hist_n )[ dev ] = n;
+    }
+  }
+}
+
+// STDP: some instance of the launch has stdp_synapse connections (its own instantiation, so
+// that networks without them carry none of that code or its registers)
+template < int MASK, bool WIN, bool STDP = false >
 __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
   k_slice( const NetDev* __restrict__ nets, const uint32_t* __restrict__ blk_prefix, int n_nets, int s )
 {
@@ -704,7 +821,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
             {
               const double w = __hiloint2double( static_cast< int >( hr[ q ].w_hi ), static_cast< int >( hr[ q ].w_lo ) );
               const uint32_t local = hr[ q ].word & 0xffffu;
-              const uint32_t slot_rel = hl[ q ] + ( hr[ q ].word >> 16 );
+              const uint32_t slot_rel = hl[ q ] + ( ( hr[ q ].word >> 16 ) & DELAY_MASK );
               const int m = MASK == 1 ? M_IAF_PSC_DELTA
                 : MASK == 2           ? M_IAF_PSC_EXP
                 : MASK == 4           ? M_AEIF_COND_EXP
@@ -771,9 +888,17 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           lc[ q ] = false;
           if ( eb + q < E )
           {
-            const double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
+            double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
             const uint32_t local = rec[ q ].word & 0xffffu;
-            const uint32_t slot_rel = rlag[ q ] + ( rec[ q ].word >> 16 );
+            if ( STDP and ( rec[ q ].word & PLASTIC_BIT ) )
+            {
+              // the record's index, found again from the event's place in the chunk
+              const uint32_t jr = upper_bound_u32( S.c_off, CH + 1, eb + q ) - 1;
+              w = stdp_event( d, S.c_g0[ jr ] + ( eb + q - S.c_off[ jr ] ), w, dev0 + local,
+                ph.origin - d.min_delay + rlag[ q ] + 1,
+                ( ( rec[ q ].word >> 16 ) & DELAY_MASK ) + static_cast< uint32_t >( d.min_delay ) );
+            }
+            const uint32_t slot_rel = rlag[ q ] + ( ( rec[ q ].word >> 16 ) & DELAY_MASK );
             const int m = MASK == 1 ? M_IAF_PSC_DELTA
               : MASK == 2           ? M_IAF_PSC_EXP
               : MASK == 4           ? M_AEIF_COND_EXP
@@ -1181,6 +1306,8 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     if ( G == 0 )
       drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, 1, ph.to );
   }
+  if ( STDP and d.stdp and local < nb )
+    stdp_spikes( d, dev, ph.origin, bits );
   NG_PHASE( 6 );
 
   // ================================================================ 3. spike lists
@@ -1345,21 +1472,27 @@ static void
 launch_mask( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, int s,
   size_t dyn_lds, hipStream_t st )
 {
-  switch ( mask )
+  // mask: the models of the launch (bits 0-2) and stdp_synapse (bit 3)
+  const bool stdp = ( mask & 8 ) != 0;
+#define NG_LAUNCH( M )                                                                                        \
+  ( stdp ? k_slice< M, WIN, true ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s ) \
+         : k_slice< M, WIN, false ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s ) )
+  switch ( mask & 7 )
   {
   case 1:
-    k_slice< 1, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    NG_LAUNCH( 1 );
     break;
   case 2:
-    k_slice< 2, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    NG_LAUNCH( 2 );
     break;
   case 4:
-    k_slice< 4, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    NG_LAUNCH( 4 );
     break;
   default:
-    k_slice< 7, WIN ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s );
+    NG_LAUNCH( 7 );
     break;
   }
+#undef NG_LAUNCH
 }
 
 // Dynamic LDS that limits the slice kernel to wg_per_cu resident workgroups per CU (0: no

@@ -461,6 +461,8 @@ Kernel::reset()
   rebalanced_ = false;
   rx_steps_ = 0;
   block_relayouts_ = 0;
+  stdp_sets_.clear();
+  syn_pos_.clear();
   delivery_mode_ = 0;
   wg_per_cu_ = 0;
   param_states_.normals.clear();
@@ -735,6 +737,7 @@ Kernel::default_params( int model )
 {
   NeuronParams p {};
   p.model = model;
+  p.tau_minus = 20.0;  // ArchivingNode (archiving_node.cpp:40)
   if ( model == M_IAF_PSC_DELTA )  // models/iaf_psc_delta.cpp:69-88
   {
     p.tau_m = 10.0;
@@ -799,6 +802,15 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
     return false;
   };
   auto get = [ & ]( const char* k ) { return d.at( k ).number(); };
+  // ArchivingNode::set_status (archiving_node.cpp:238-257): the trace time constant of stdp
+  // synapses onto this neuron, checked first and set once the model's own checks pass
+  double tau_minus = p.tau_minus;
+  if ( has( "tau_minus" ) )
+  {
+    tau_minus = get( "tau_minus" );
+    if ( tau_minus <= 0.0 )
+      throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
+  }
   if ( p.model == M_AEIF_COND_EXP )
   {
     const char* keys[] = { "V_peak", "V_reset", "t_ref", "g_L", "C_m", "E_ex", "E_in", "E_L", "Delta_T", "tau_w", "a",
@@ -932,6 +944,7 @@ Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
   for ( const auto& kv : d.obj )
     if ( not used.count( kv.first ) )
       throw std::runtime_error( "UnaccessedDictionaryEntry: '" + kv.first + "' is not a parameter of this model" );
+  p.tau_minus = tau_minus;
 }
 
 int64_t
@@ -1512,6 +1525,8 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       out[ i ] = p.tau_m;
     else if ( key == "t_ref" )
       out[ i ] = p.t_ref;
+    else if ( key == "tau_minus" )
+      out[ i ] = p.tau_minus;
     else if ( key == "I_e" )
       out[ i ] = p.I_e;
     else if ( key == "tau_syn_ex" and p.model != M_IAF_PSC_DELTA )
@@ -1560,7 +1575,7 @@ Kernel::check_delay( long steps )
 }
 
 void
-Kernel::single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp )
+Kernel::single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp, uint16_t syn )
 {
   // BipartiteConnBuilder::single_connect_ (conn_builder.cpp:379-437): delay is drawn before
   // the weight when both are given; defaults of static_synapse are weight 1, delay 1 ms
@@ -1576,6 +1591,13 @@ Kernel::single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Par
     weight = wp->value( rng, vp );
   const long ds = tb_.delay_ms_to_steps( delay );
   check_delay( ds );
+  if ( syn )
+  {
+    // stdp_synapse::set_status (stdp_synapse.h:303-314): weight and Wmax of the same sign
+    const double Wmax = stdp_sets_[ syn - 1 ].Wmax;
+    if ( not( ( ( weight >= 0 ) - ( weight < 0 ) ) == ( ( Wmax >= 0 ) - ( Wmax < 0 ) ) ) )
+      throw std::runtime_error( "BadProperty: Weight and Wmax must have same sign." );
+  }
   ConnList& C = conns_[ vp ];
   C.src.push_back( static_cast< uint32_t >( s ) );
   C.tgt.push_back( static_cast< uint32_t >( t ) );
@@ -1583,6 +1605,11 @@ Kernel::single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Par
   if ( ds > 65535 )
     throw std::runtime_error( "BadDelay: delays above 65535 steps are not supported" );
   C.d.push_back( static_cast< uint16_t >( ds ) );
+  if ( syn )
+  {
+    C.syn.resize( C.size(), 0 );
+    C.syn.back() = syn;
+  }
 }
 
 void
@@ -1629,13 +1656,36 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     }
   }
   std::unique_ptr< Param > wp, dp;
+  // stdp_synapse (models/stdp_synapse.h:200-215): its parameters, constant per Connect call
+  bool stdp = false;
+  if ( syn_spec.has( "synapse_model" ) )
+  {
+    const std::string& sm = syn_spec.at( "synapse_model" ).string();
+    if ( sm == "stdp_synapse" )
+      stdp = true;
+    else if ( sm != "static_synapse" and sm != "static_synapse_hpc" )
+      throw std::runtime_error( "UnknownSynapseType: '" + sm + "' is not available on the device path." );
+  }
+  StdpSet sset;
   for ( const auto& kv : syn_spec.obj )
   {
     if ( kv.first == "synapse_model" )
+      continue;
+    double* sdst = kv.first == "tau_plus" ? &sset.tau_plus
+      : kv.first == "lambda"              ? &sset.lambda
+      : kv.first == "alpha"               ? &sset.alpha
+      : kv.first == "mu_plus"             ? &sset.mu_plus
+      : kv.first == "mu_minus"            ? &sset.mu_minus
+      : kv.first == "Wmax"                ? &sset.Wmax
+      : kv.first == "Kplus"               ? &sset.Kplus
+                                          : nullptr;
+    if ( sdst )
     {
-      const std::string& sm = kv.second.string();
-      if ( sm != "static_synapse" and sm != "static_synapse_hpc" )
-        throw std::runtime_error( "UnknownSynapseType: '" + sm + "' is not available on the device path." );
+      if ( not stdp )
+        throw std::runtime_error( "UnaccessedDictionaryEntry: syn_spec key '" + kv.first + "'" );
+      if ( kv.second.kind != Json::NUM )
+        throw std::runtime_error( "NotImplemented: stdp_synapse parameter '" + kv.first + "' must be a number" );
+      *sdst = kv.second.number();
     }
     else if ( kv.first == "weight" )
       wp = Param::from_json( kv.second, n_vp_, true, &param_states_ );
@@ -1657,8 +1707,24 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       else
         check( kv.second );
     }
-    else
+    else if ( not sdst )
       throw std::runtime_error( "UnaccessedDictionaryEntry: syn_spec key '" + kv.first + "'" );
+  }
+  uint16_t syn = 0;
+  if ( stdp )
+  {
+    if ( sset.Kplus < 0 )
+      throw std::runtime_error( "BadProperty: Kplus must be non-negative." );
+    // facilitate_/depress_ raise (1 - w/Wmax) and w/Wmax to mu_plus and mu_minus: 0 and 1 (the
+    // defaults) are exact without pow, which device and host would round differently
+    if ( ( sset.mu_plus != 0.0 and sset.mu_plus != 1.0 ) or ( sset.mu_minus != 0.0 and sset.mu_minus != 1.0 ) )
+      throw std::runtime_error( "NotImplemented: stdp_synapse with mu_plus or mu_minus other than 0 or 1" );
+    if ( not( sset.tau_plus > 0 ) or sset.Wmax == 0 )
+      throw std::runtime_error( "BadProperty: stdp_synapse needs tau_plus > 0 and Wmax != 0" );
+    if ( simulated_ )
+      throw std::runtime_error( "NotImplemented: stdp_synapse connections after Simulate" );
+    if ( stdp_sets_.size() >= 65534 )
+      throw std::runtime_error( "NotImplemented: more than 65534 stdp_synapse Connect calls" );
   }
   if ( pre.empty() or post.empty() )
     return;
@@ -1724,6 +1790,13 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
     }
     store_dirty_ = true;
     return;
+  }
+  if ( stdp and ( pre_kind != 0 or post_kind != 0 ) )
+    throw std::runtime_error( "NotImplemented: stdp_synapse connects neurons only on the device path" );
+  if ( stdp )
+  {
+    stdp_sets_.push_back( sset );
+    syn = static_cast< uint16_t >( stdp_sets_.size() );
   }
   if ( pre_kind == 2 )
     throw std::runtime_error( "IllegalConnection: spike_recorder cannot be a source" );
@@ -1830,7 +1903,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
   // neuron -> neuron on the device: the builders whose draws take a known number of words of
   // a counter-based stream, with constant weight and delay
   const bool const_params = ( not wp or wp->is_const() ) and ( not dp or dp->is_const() );
-  if ( connect_on_device_ and const_params
+  if ( connect_on_device_ and const_params and not stdp
        and ( rule == "pairwise_bernoulli" or ( rule == "fixed_indegree" and autapses and multapses ) ) )
   {
     if ( rng_type_ != RNG_PHILOX_32 )
@@ -1901,7 +1974,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
           const int64_t s = pre[ ti ];
           if ( s == t and not autapses )
             continue;
-          single_connect( s, t, vp, rng, wp.get(), dp.get() );
+          single_connect( s, t, vp, rng, wp.get(), dp.get(), syn );
         }
         else if ( rule == "all_to_all" )
         {
@@ -1913,7 +1986,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
                 skip( vp, 1 );  // conn_builder.cpp:1244-1251
               continue;
             }
-            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+            single_connect( s, t, vp, rng, wp.get(), dp.get(), syn );
           }
         }
         else if ( rule == "fixed_indegree" )
@@ -1933,7 +2006,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
             } while ( skip_autapse or skip_multapse );
             if ( not multapses )
               chosen.push_back( s_id );
-            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+            single_connect( s, t, vp, rng, wp.get(), dp.get(), syn );
           }
         }
         else
@@ -1944,7 +2017,7 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
               continue;
             if ( rng.drand() >= p )
               continue;
-            single_connect( s, t, vp, rng, wp.get(), dp.get() );
+            single_connect( s, t, vp, rng, wp.get(), dp.get(), syn );
           }
         }
       }
@@ -2165,7 +2238,12 @@ Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int3
 {
   if ( host_connections_freed_ )
     throw std::runtime_error( "KernelException: the host connection table was released (keep_host_connections false)" );
-  // neuron connections per VP in creation order, then device connections
+  // neuron connections per VP in creation order, then device connections; stdp_synapse
+  // weights as the device holds them after a run
+  std::vector< double > dev_w;
+  if ( not stdp_sets_.empty() and engine_ and engine_->built() and simulated_ and not released_
+       and syn_pos_.size() == static_cast< size_t >( num_connections() - static_cast< int64_t >( dev_conns_.size() ) ) )
+    engine_->read_weights( dev_w );
   int64_t k = 0;
   for ( int v = 0; v < n_vp_; ++v )
   {
@@ -2174,7 +2252,7 @@ Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int3
     {
       src[ k ] = C.src[ i ];
       tgt[ k ] = C.tgt[ i ];
-      w[ k ] = C.w[ i ];
+      w[ k ] = not dev_w.empty() and C.syn_at( i ) ? dev_w[ syn_pos_[ k ] ] : C.w[ i ];
       d[ k ] = C.d[ i ];
       if ( vp )
         vp[ k ] = v;

@@ -297,6 +297,7 @@ struct NeuronParams
   double tau_ex, tau_in;
   double V_peak, g_L, E_ex, E_in, Delta_T, tau_w, a, b, tol;  // aeif: absolute
   double delta, rho;
+  double tau_minus;  // ArchivingNode: trace time constant of incoming stdp synapses
   int32_t refr_input;
 };
 
@@ -348,6 +349,14 @@ struct alignas( 64 ) ConnList
   std::vector< uint32_t > src, tgt;
   std::vector< double > w;
   std::vector< uint16_t > d;
+  // synapse of each connection: 0 static_synapse, k an stdp_synapse of parameter set k - 1;
+  // shorter than the list while its tail is static (empty when every connection is static)
+  std::vector< uint16_t > syn;
+  uint16_t
+  syn_at( size_t i ) const
+  {
+    return i < syn.size() ? syn[ i ] : 0;
+  }
   size_t
   size() const
   {
@@ -522,7 +531,7 @@ private:
   static NeuronParams default_params( int model );
   void apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool creating );
   void check_delay( long steps );
-  void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp );
+  void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp, uint16_t syn = 0 );
   // pairwise_bernoulli (param p) or fixed_indegree (param K, autapses and multapses allowed)
   // with constant weight and delay, drawn on the device from the Philox_32 streams
   void device_connect( const std::vector< int64_t >& pre, const std::vector< int64_t >& post, const std::string& rule,
@@ -606,6 +615,13 @@ private:
   double time_kernel_ms_[ N_KINDS ] = { 0, 0 };
   int64_t kernel_launches_[ N_KINDS ] = { 0, 0 };
   int targets_per_block_ = 0;  // 0: chosen from the network size
+  // stdp_synapse parameter sets, one per Connect call that makes stdp_synapse connections
+  struct StdpSet
+  {
+    double tau_plus = 20.0, lambda = 0.01, alpha = 1.0, mu_plus = 1.0, mu_minus = 1.0, Wmax = 100.0, Kplus = 0.0;
+  };
+  std::vector< StdpSet > stdp_sets_;
+  std::vector< uint32_t > syn_pos_;  // store position of every neuron connection (creation order), with stdp
   bool connect_on_device_ = false;  // eligible Connect calls draw on the device
   int64_t device_connect_calls_ = 0;
   bool balanced_blocks_ = true;  // automatic block size: blocks balanced by in-degree, then by

@@ -4,9 +4,10 @@ A description is a dict:
   n_vp, seed, resolution
   pops:  [(model, n, params)]                  node ids are assigned in this order
   v0:    {pop_index: array}                     initial V_m per neuron (optional)
-  conns: [(pre, post, rule, rule_param, weight, delay, autapses, multapses)]
+  conns: [(pre, post, rule, rule_param, weight, delay, autapses, multapses, stdp)]
          pre/post are (pop_index, start, stop) slices; weight/delay use the oracle's
-         tuple encoding (numbers, ("normal", m, s), ("sum", a, b), ...)
+         tuple encoding (numbers, ("normal", m, s), ("sum", a, b), ...); stdp (optional) is
+         None for static_synapse or a dict of stdp_synapse parameters (and Kplus)
   drive: [(pop_index, counts[n_steps, n], weight, delay_ms)]
 """
 
@@ -92,7 +93,7 @@ def build_gpu(desc, device=0, kernel=None, layout=None):
             cs["indegree"] = int(rp)
         elif rule == "pairwise_bernoulli":
             cs["p"] = float(rp)
-        syn = {}
+        syn = {} if len(c) < 9 or c[8] is None else dict(c[8], synapse_model="stdp_synapse")
         if w is not None:
             syn["weight"] = to_param(w)
         if d is not None:
@@ -123,7 +124,7 @@ def build_oracle(desc, threads=1, portable_math=False):
         aut = c[6] if len(c) > 6 else True
         mult = c[7] if len(c) > 7 else True
         o.connect(_ids(pops, pre), _ids(pops, post), rule, rp, weight=w, delay=d, allow_autapses=aut,
-                  allow_multapses=mult)
+                  allow_multapses=mult, stdp=c[8] if len(c) > 8 else None)
     for pi, counts, w, d in desc.get("drive", []):
         o.add_drive(pops[pi][0], counts, w, d)
     return o, pops

@@ -666,3 +666,27 @@ def test_connect_on_device_needs_counter_based_streams():
     k.Connect(a, a, {"rule": "fixed_indegree", "indegree": 3, "allow_multapses": False}, {"weight": 1.0})
     assert k.GetKernelStatus("device_connect_calls") == 0
     assert len(k.GetConnections()["source"]) == 120
+
+
+def test_stdp_synapse_connect_checks():
+    # stdp_synapse::set_status (stdp_synapse.h:303-314) and this path's limits
+    k = nestgpu.Kernel(0)
+    a = k.Create("iaf_psc_exp", 10, {"tau_minus": 25.0})
+    assert np.all(a.get("tau_minus") == 25.0)
+    with pytest.raises(nestgpu.NESTError, match="same sign"):
+        k.Connect(a, a, "all_to_all", {"synapse_model": "stdp_synapse", "weight": -1.0, "Wmax": 50.0})
+    with pytest.raises(nestgpu.NESTError, match="Kplus"):
+        k.Connect(a, a, "all_to_all", {"synapse_model": "stdp_synapse", "Kplus": -1.0})
+    with pytest.raises(nestgpu.NESTError, match="mu_plus"):
+        k.Connect(a, a, "all_to_all", {"synapse_model": "stdp_synapse", "mu_plus": 0.5})
+    with pytest.raises(nestgpu.NESTError, match="UnaccessedDictionaryEntry"):
+        k.Connect(a, a, "all_to_all", {"weight": 1.0, "tau_plus": 10.0})
+    with pytest.raises(nestgpu.NESTError, match="strictly positive"):
+        a.set(tau_minus=0.0)
+    assert len(k.GetConnections()["source"]) == 0
+    k.Connect(a[:5], a[5:], "all_to_all", {"synapse_model": "stdp_synapse", "weight": 30.0, "Wmax": 60.0,
+                                             "lambda": 0.02, "delay": 1.5})
+    k.Connect(a[5:], a[:5], "all_to_all", {"weight": 2.0})
+    c = k.GetConnections()
+    assert len(c["source"]) == 50
+    assert np.array_equal(np.sort(np.unique(c["weight"])), [2.0, 30.0])
