@@ -76,12 +76,18 @@ def c3_description(drive_seed, scale=1.0):
     return dict(sizes=sizes, v0=v0, prob=prob, k_ext=k_ext, drive_seed=drive_seed)
 
 
-def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9, scale=1.0):
+# stdp_synapse parameters of the plastic variant of the circuit (excitatory to excitatory
+# connections); Wmax twice the mean weight
+C3_STDP = {"tau_plus": 20.0, "lambda": 0.01, "alpha": 1.0, "mu_plus": 1.0, "mu_minus": 1.0, "Wmax": 2 * C3_W}
+
+
+def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9, scale=1.0, stdp=None):
     """8-population layered circuit: 77,169 iaf_psc_exp (scale 1), pairwise_bernoulli with an 8x8
     probability matrix drawn from the seed (uniform in [0, 0.14]), weights normal(87.8, 10 %)
     and x -4 from inhibitory populations, delays normal (E 1.5 ms, I 0.75 ms, sd 50 %)
     clipped at dt, background Poisson 8 Hz x K_ext with K_ext uniform in [1500, 2500] per
-    population. Returns (populations, total neurons, expected drive events per step)."""
+    population. stdp: None, or stdp_synapse parameters for the connections between excitatory
+    populations. Returns (populations, total neurons, expected drive events per step)."""
     c = c3_description(drive_seed, scale)
     k.SetKernelStatus({"total_num_virtual_procs": n_vp, "rng_seed": seed})
     pops = [k.Create("iaf_psc_exp", n, NEURON) for n in c["sizes"]]
@@ -92,7 +98,10 @@ def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9, scale=1.0):
         w = nestgpu.random.normal(C3_W, C3_W_SD) if exc else -4.0 * nestgpu.random.normal(C3_W, C3_W_SD)
         d = nestgpu.math.max(nestgpu.random.normal(*C3_DELAY[0 if exc else 1]), 0.1)
         for i, post in enumerate(pops):
-            k.Connect(pre, post, {"rule": "pairwise_bernoulli", "p": float(c["prob"][i, j])}, {"weight": w, "delay": d})
+            syn = {"weight": w, "delay": d}
+            if stdp is not None and exc and i % 2 == 0:
+                syn.update(stdp, synapse_model="stdp_synapse")
+            k.Connect(pre, post, {"rule": "pairwise_bernoulli", "p": float(c["prob"][i, j])}, syn)
     lam = 0.0
     for i, post in enumerate(pops):
         rate = 8.0 * float(c["k_ext"][i])
@@ -103,7 +112,7 @@ def build_c3(k, nestgpu, n_vp, seed, drive_seed, t_stop=1e9, scale=1.0):
     return pops, sum(c["sizes"]), lam
 
 
-def build_c3_oracle(o, nestgpu, drive_seed, n_steps, scale=1.0):
+def build_c3_oracle(o, nestgpu, drive_seed, n_steps, scale=1.0, stdp=None):
     """The same circuit on the oracle (connections from the same per-VP streams; the
     parameter objects that several Connect calls share carry their normal-draw state, as the
     library's do), with the library's seeded Poisson counts of every generator as dense drive
@@ -118,7 +127,8 @@ def build_c3_oracle(o, nestgpu, drive_seed, n_steps, scale=1.0):
         w = wn if exc else ("product", wn, -4.0)
         d = ("max", ("normal",) + C3_DELAY[0 if exc else 1] + (2 * j + 1,), 0.1)
         for i, post in enumerate(pops):
-            o.connect(pre, post, "pairwise_bernoulli", float(c["prob"][i, j]), weight=w, delay=d)
+            o.connect(pre, post, "pairwise_bernoulli", float(c["prob"][i, j]), weight=w, delay=d,
+                      stdp=stdp if exc and i % 2 == 0 else None)
     n_total = sum(c["sizes"])
     gen0 = n_total + 1  # generators are created after every population, population by population
     for i, post in enumerate(pops):
@@ -423,7 +433,7 @@ def cpu_baseline(args, workload, threads):
         o.connect(g[NE:], g, "fixed_indegree", KI, weight=("product", ("normal", 87.8, 8.8), -4.0), delay=delay)
         o.add_drive(g[0], _poisson_counts_c2(nestgpu, N, args.drive_seed, steps), 87.8, 1.0)
     else:
-        build_c3_oracle(o, nestgpu, args.drive_seed, steps)
+        build_c3_oracle(o, nestgpu, args.drive_seed, steps, stdp=C3_STDP if args.stdp else None)
     o.run_steps(0)  # prepare (store build) outside the timed window
     setup_s = time.perf_counter() - t0
     o.simulate(t_warm)
@@ -483,7 +493,10 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
         n_neurons = args.n
         drive_nz_per_step = args.n * (1.0 - np.exp(-8000.0 * 0.1e-3))
     else:
-        _, n_neurons, drive_nz_per_step = build_c3(k, nestgpu, args.n_vp, args.seed, args.drive_seed)
+        if args.stdp:
+            k.SetKernelStatus({"stdp_history_capacity": args.stdp_history})
+        _, n_neurons, drive_nz_per_step = build_c3(k, nestgpu, args.n_vp, args.seed, args.drive_seed,
+                                                   stdp=C3_STDP if args.stdp else None)
     n_syn = k.GetKernelStatus("num_connections") - n_neurons
     k.Prepare()
     build_s = time.perf_counter() - t_build
@@ -567,6 +580,9 @@ def run_plain(args, nestgpu, workload, steps, warmup, with_cpu):
         workload_text = ("8-population layered circuit: 77169 iaf_psc_exp, pairwise_bernoulli (%d synapses), normal "
                          "delays clipped at dt (min_delay one step), Poisson background 8 Hz x K_ext, dt 0.1 ms, one "
                          "step = Run(%g ms)" % (n_syn, args.t_step))
+        if args.stdp:
+            workload_text += ("; connections between excitatory populations are stdp_synapse (tau_plus 20 ms, "
+                              "lambda 0.01, alpha 1, mu 1, Wmax %g pA)" % C3_STDP["Wmax"])
     default_layout = (args.delivery_mode == "auto" and args.targets_per_block == 0 and args.workgroups_per_cu == 0
                       and args.window_slots == 0)
     traffic, traffic_digest = measured_traffic(workload) if default_layout else (None, None)
@@ -673,6 +689,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="c3: skip the extra c2 measurement")
     ap.add_argument("--no-c4", action="store_true", help="c3: skip the extra batched-sweep (c4) measurement")
+    ap.add_argument("--stdp", action="store_true",
+                    help="c3: connections between excitatory populations are stdp_synapse")
+    ap.add_argument("--stdp-history", type=int, default=256, help="c3 --stdp: post-spike history entries per neuron")
     ap.add_argument("--c2-steps", type=int, default=5, help="c3: timed steps of the extra c2 measurement")
     ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"],
                     help="c3: 77k-neuron 8-population circuit with min_delay one step (default); c2: 100k-neuron "

@@ -75,6 +75,10 @@ int nestgpu_set_spike_counts( nestgpu_kernel* k, int64_t first_id, int64_t n, in
 int64_t nestgpu_num_connections( nestgpu_kernel* k );
 int nestgpu_get_connections( nestgpu_kernel* k, int64_t* sources, int64_t* targets, double* weights,
   int64_t* delay_steps, int32_t* vps );
+/* The synapse model of each connection, in nestgpu_get_connections' order: 0 static_synapse,
+ * 1 stdp_synapse (the synapse_model / synapse_modelid a connection's status reports,
+ * nestkernel/connector_model_impl.h:105-106, connection_id.cpp:65). */
+int nestgpu_get_connection_models( nestgpu_kernel* k, int32_t* models );
 
 /* nest::prepare / run / cleanup / simulate (nestkernel/nest.cpp:499-534). */
 int nestgpu_prepare( nestgpu_kernel* k );

@@ -811,6 +811,7 @@ Kernel::build_spec( BuildSpec& b )
     for ( uint32_t i = 0; i < N; ++i )
       b.tau_minus_inv[ dev_of[ i ] ] = 1. / params_[ i ].tau_minus;
   }
+  b.stdp_hcap = stdp_history_;
   b.ms_per_tic = tb_.ms_per_tic;
   b.ms_per_step = tb_.ms_per_step;
   b.tics_per_step = tb_.tics_per_step;

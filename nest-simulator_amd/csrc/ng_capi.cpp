@@ -192,6 +192,12 @@ nestgpu_get_connections( nestgpu_kernel* h, int64_t* sources, int64_t* targets, 
   return guarded( h, [ & ] { h->k->get_connections( sources, targets, weights, delay_steps, vps ); } );
 }
 
+extern "C" int
+nestgpu_get_connection_models( nestgpu_kernel* h, int32_t* models )
+{
+  return guarded( h, [ & ] { h->k->get_connection_models( models ); } );
+}
+
 int
 nestgpu_prepare( nestgpu_kernel* h )
 {

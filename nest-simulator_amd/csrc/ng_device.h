@@ -67,8 +67,9 @@ struct StdpSyn
 constexpr uint32_t PLASTIC_BIT = 0x80000000u;
 constexpr uint32_t DELAY_MASK = 0x7fffu;
 // post-spike history kept per neuron with incoming stdp synapses (ArchivingNode::history_):
-// the last STDP_HIST spikes (time, Kminus)
-constexpr uint32_t STDP_HIST = 64;
+// a ring of the last stdp_hcap spikes (time, Kminus), a power of two
+constexpr uint32_t STDP_HIST_DEFAULT = 256;
+constexpr uint32_t STDP_HIST_MAX = 65536;
 
 struct PoissonTable
 {
@@ -139,9 +140,10 @@ struct NetDev
   //      delivery work that target blocks are balanced by
   uint32_t* rx;
   // ---- stdp_synapse (stdp != 0): parameter sets, per-record state, per-neuron flags,
-  //      1/tau_minus and post-spike history [neuron][STDP_HIST] with its count, trace and last
+  //      1/tau_minus and post-spike history [neuron][stdp_hcap] with its count, trace and last
   //      spike time; the time base turns step stamps into ms as nest::Time does
   uint32_t stdp;
+  uint32_t stdp_hcap;  // history entries per neuron, a power of two
   const StdpPrm* sprm;
   StdpSyn* sst;
   const uint8_t* stdp_in;   // has incoming stdp synapses (ArchivingNode::n_incoming_ > 0)

@@ -212,7 +212,10 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
     if ( net_.stdp )
     {
       // ArchivingNode state: no history, Kminus 0, last spike -1 ms (archiving_node.cpp:38-48)
-      const size_t nh = static_cast< size_t >( b.n ) * STDP_HIST;
+      if ( b.stdp_hcap < 2 or b.stdp_hcap > STDP_HIST_MAX or ( b.stdp_hcap & ( b.stdp_hcap - 1 ) ) )
+        throw std::runtime_error( "stdp_synapse: history capacity must be a power of two in [2, 65536]" );
+      net_.stdp_hcap = b.stdp_hcap;
+      const size_t nh = static_cast< size_t >( b.n ) * b.stdp_hcap;
       net_.hist_t = static_cast< double* >( T( alloc( sizeof( double ) * nh ) ) );
       net_.hist_k = static_cast< double* >( T( alloc( sizeof( double ) * nh ) ) );
       net_.hist_n = static_cast< uint32_t* >( T( alloc( sizeof( uint32_t ) * b.n ) ) );
@@ -753,8 +756,9 @@ finish_chunk( const std::vector< Engine* >& engines, const NetDev* h_nets, const
       if ( flags & ERR_SAMPLES )
         why += " multimeter sample buffer exceeded;";
       if ( flags & ERR_STDP_HISTORY )
-        why += " a neuron spiked more than " + std::to_string( STDP_HIST )
-          + " times between two spikes of a presynaptic stdp_synapse neuron (post-spike history);";
+        why += " a neuron spiked more than " + std::to_string( d.stdp_hcap )
+          + " times between two spikes of a presynaptic stdp_synapse neuron (post-spike history;"
+            " raise stdp_history_capacity);";
       if ( flags & ERR_STDP_TWICE )
         why += " a neuron with outgoing stdp_synapse connections spiked twice within one min_delay slice;";
       throw std::runtime_error( "device run failed:" + why );

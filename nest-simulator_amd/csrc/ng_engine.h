@@ -110,6 +110,7 @@ struct BuildSpec
   std::vector< uint8_t > stdp_in, stdp_out;
   std::vector< double > tau_minus_inv;
   double ms_per_tic = 0.001, ms_per_step = 0.1;
+  uint32_t stdp_hcap = STDP_HIST_DEFAULT;  // post-spike history entries per neuron (power of two)
   int64_t tics_per_step = 100;
   // multimeter sampling: per neuron sample column (~0 none), interval and offset in steps
   std::vector< uint32_t > mm_col;

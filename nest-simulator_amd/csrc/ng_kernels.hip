@@ -490,34 +490,39 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
   const double t1_lim = ( t_last - dd ) + STDP_EPS;
   const double t2_lim = ( t_spike - dd ) + STDP_EPS;
   const uint32_t n = glob( d.hist_n )[ tdev ];
-  const uint32_t first = n > STDP_HIST ? n - STDP_HIST : 0u;
-  gptr< const double > ht = glob( d.hist_t ) + static_cast< size_t >( tdev ) * STDP_HIST;
-  gptr< const double > hk = glob( d.hist_k ) + static_cast< size_t >( tdev ) * STDP_HIST;
-  // spikes dropped from the ring may lie in the range (or hold the trace before t2 - eps)
-  if ( first > 0 and ht[ first % STDP_HIST ] >= t1_lim )
+  const uint32_t cap = d.stdp_hcap, hm = cap - 1;
+  const uint32_t first = n > cap ? n - cap : 0u;  // entries before first left the ring
+  gptr< const double > ht = glob( d.hist_t ) + static_cast< size_t >( tdev ) * cap;
+  gptr< const double > hk = glob( d.hist_k ) + static_cast< size_t >( tdev ) * cap;
+  // the times are increasing: walk back from the newest entry to the range [lo, hi) of
+  // entries t1 + eps <= t < t2 + eps, so that an event costs the target spikes in its range
+  uint32_t hi = n;
+  while ( hi > first and ht[ ( hi - 1 ) & hm ] >= t2_lim )
+    --hi;
+  uint32_t lo = hi;
+  while ( lo > first and ht[ ( lo - 1 ) & hm ] >= t1_lim )
+    --lo;
+  // the range (or the latest spike before t2 - eps) may reach past the entries kept
+  if ( first > 0 and lo == first )
     g_or( &d.counters[ 0 ], ERR_STDP_HISTORY );
-  for ( uint32_t i = first; i < n; ++i )
+  for ( uint32_t i = lo; i < hi; ++i )
   {
-    const double th = ht[ i % STDP_HIST ];
-    if ( th >= t1_lim and th < t2_lim )
-    {
-      // facilitate_ (stdp_synapse.h:187-191)
-      const double minus_dt = t_last - ( th + dd );
-      const double kp = kplus * ng_exp( minus_dt / tau_plus );
-      const double norm_w = ( w / Wmax ) + ( lambda * ( mu_p ? 1.0 - ( w / Wmax ) : 1.0 ) * kp );
-      w = norm_w < 1.0 ? norm_w * Wmax : Wmax;
-    }
+    // facilitate_ (stdp_synapse.h:187-191), oldest entry first
+    const double minus_dt = t_last - ( ht[ i & hm ] + dd );
+    const double kp = kplus * ng_exp( minus_dt / tau_plus );
+    const double norm_w = ( w / Wmax ) + ( lambda * ( mu_p ? 1.0 - ( w / Wmax ) : 1.0 ) * kp );
+    w = norm_w < 1.0 ? norm_w * Wmax : Wmax;
   }
   // ArchivingNode::get_K_value( t_spike - d ) (archiving_node.cpp:85-113): the latest spike
-  // strictly before
+  // strictly before (entries from hi on lie after it)
   const double tq = t_spike - dd;
   double K = 0.0;
-  for ( uint32_t i = n; i > first; --i )
+  for ( uint32_t i = hi; i > first; --i )
   {
-    const double th = ht[ ( i - 1 ) % STDP_HIST ];
+    const double th = ht[ ( i - 1 ) & hm ];
     if ( tq - th > STDP_EPS )
     {
-      K = hk[ ( i - 1 ) % STDP_HIST ] * ng_exp( ( th - tq ) * glob( d.tau_minus_inv )[ tdev ] );
+      K = hk[ ( i - 1 ) & hm ] * ng_exp( ( th - tq ) * glob( d.tau_minus_inv )[ tdev ] );
       break;
     }
   }
@@ -562,7 +567,7 @@ stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, const uint64_t* bits
       if ( in )
       {
         km = km * ng_exp( ( last - t_sp ) * glob( d.tau_minus_inv )[ dev ] ) + 1.0;
-        const size_t at = static_cast< size_t >( dev ) * STDP_HIST + n % STDP_HIST;
+        const size_t at = static_cast< size_t >( dev ) * d.stdp_hcap + ( n & ( d.stdp_hcap - 1 ) );
         glob( d.hist_t )[ at ] = t_sp;
         glob( d.hist_k )[ at ] = km;
         ++n;

@@ -470,6 +470,7 @@ Kernel::reset()
   phase_timing_ = false;
   graph_chunks_ = direct_chunks_ = capture_failures_ = 0;
   window_slots_ = 0;
+  stdp_history_ = STDP_HIST_DEFAULT;
   delivery_passes_ = 0;
   keep_host_connections_ = true;
   host_connections_freed_ = false;
@@ -597,6 +598,19 @@ Kernel::set_kernel_status( const Json& d )
         throw std::runtime_error( "BadProperty: workgroups_per_cu must be in [0, 8] (0 = as many as fit)" );
       wg_per_cu_ = v;
     }
+    else if ( k == "stdp_history_capacity" )
+    {
+      // entries of the post-spike history ring kept per neuron; NEST keeps an unbounded
+      // history (ArchivingNode::history_), a run that needs an entry the ring dropped fails
+      const double v = kv.second.number();
+      const uint32_t u = static_cast< uint32_t >( v );
+      if ( not( v >= 2 and v <= STDP_HIST_MAX ) or static_cast< double >( u ) != v or ( u & ( u - 1 ) ) )
+        throw std::runtime_error( "BadProperty: stdp_history_capacity must be a power of two in [2, 65536]" );
+      if ( simulated_ and u != stdp_history_ )
+        throw std::runtime_error( "KernelException: stdp_history_capacity cannot change after Simulate" );
+      stdp_history_ = u;
+      store_dirty_ = true;
+    }
     else if ( k == "keep_host_connections" )
       keep_host_connections_ = kv.second.number() != 0;
     else if ( k == "kernel_timing" )
@@ -681,6 +695,7 @@ Kernel::get_kernel_status_json() const
     << ", \"cache_delivery\": " << ( engine_ and engine_->built() and engine_->cache() ? "true" : "false" )
     << ", \"delivery_passes\": " << ( engine_ and engine_->built() ? engine_->net().np : delivery_passes_ )
     << ", \"window_slots\": " << ( engine_ and engine_->built() ? engine_->net().wn : window_slots_ )
+    << ", \"stdp_history_capacity\": " << stdp_history_
     << ", \"workgroups_per_cu\": " << wg_per_cu_ << ", \"kernel_timing\": " << ( time_kernels_ ? "true" : "false" )
     << ", \"graph_launch\": " << ( graph_launch_ ? "true" : "false" ) << ", \"graph_chunks\": " << graph_chunks_
     << ", \"direct_chunks\": " << direct_chunks_ << ", \"graph_capture_failures\": " << capture_failures_
@@ -2268,6 +2283,22 @@ Kernel::get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int3
       vp[ k ] = vp_of( c.tgt );
     ++k;
   }
+}
+
+void
+Kernel::get_connection_models( int32_t* model ) const
+{
+  if ( host_connections_freed_ )
+    throw std::runtime_error( "KernelException: the host connection table was released (keep_host_connections false)" );
+  int64_t k = 0;
+  for ( int v = 0; v < n_vp_; ++v )
+  {
+    const ConnList& C = conns_[ v ];
+    for ( size_t i = 0; i < C.size(); ++i )
+      model[ k++ ] = C.syn_at( i ) ? 1 : 0;
+  }
+  for ( size_t i = 0; i < dev_conns_.size(); ++i )
+    model[ k++ ] = 0;
 }
 
 void

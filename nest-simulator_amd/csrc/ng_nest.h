@@ -401,6 +401,8 @@ public:
 
   int64_t num_connections() const;
   void get_connections( int64_t* src, int64_t* tgt, double* w, int64_t* d, int32_t* vp ) const;
+  // 0 static_synapse, 1 stdp_synapse, in get_connections' order
+  void get_connection_models( int32_t* model ) const;
 
   void prepare();
   void run( double t_ms );
@@ -607,6 +609,7 @@ private:
   int slices_per_chunk_ = 64;
   int wg_per_cu_ = 0;
   int window_slots_ = 0;  // cap on the ring slots staged in LDS (0: as many as fit)
+  uint32_t stdp_history_ = STDP_HIST_DEFAULT;  // post-spike history entries per neuron
   int delivery_passes_ = 0;  // global delivery in this many slot-range passes (0, 1: one)
   bool time_kernels_ = false;
   bool phase_timing_ = false;

@@ -45,6 +45,7 @@ _SIGS = {
                                                 ctypes.POINTER(ctypes.c_uint8)]),
     "nestgpu_num_connections": (ctypes.c_int64, [_vp]),
     "nestgpu_get_connections": (ctypes.c_int, [_vp, _i64p, _i64p, _f64p, _i64p, ctypes.POINTER(ctypes.c_int32)]),
+    "nestgpu_get_connection_models": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "nestgpu_prepare": (ctypes.c_int, [_vp]),
     "nestgpu_run": (ctypes.c_int, [_vp, ctypes.c_double]),
     "nestgpu_cleanup": (ctypes.c_int, [_vp]),
@@ -475,15 +476,17 @@ class Kernel:
     def GetConnections(self, source=None, target=None, synapse_model=None):
         """Connections in creation order (VP, then creation within the VP), as arrays. source,
         target: node collections (or ids) that the connections' ends must belong to;
-        synapse_model: only static_synapse exists here (hl_api_connections.py GetConnections)."""
+        synapse_model: "static_synapse" or "stdp_synapse" (hl_api_connections.py GetConnections);
+        "synapse_model" holds each connection's model."""
         c = self._all_connections()
         keep = np.ones(len(c["source"]), bool)
         if source is not None:
             keep &= np.isin(c["source"], source._ids if isinstance(source, NodeCollection) else _i64(source))
         if target is not None:
             keep &= np.isin(c["target"], target._ids if isinstance(target, NodeCollection) else _i64(target))
-        if synapse_model is not None and synapse_model not in ("static_synapse", "static_synapse_hpc"):
-            keep[:] = False
+        if synapse_model is not None:
+            name = "static_synapse" if synapse_model == "static_synapse_hpc" else synapse_model
+            keep &= c["synapse_model"] == name
         if keep.all():
             return c
         return {key: v[keep] for key, v in c.items()}
@@ -498,7 +501,10 @@ class Kernel:
         self._chk(_lib.nestgpu_get_connections(self._h, _ptr(s, ctypes.c_int64), _ptr(t, ctypes.c_int64),
                                                _ptr(w, ctypes.c_double), _ptr(d, ctypes.c_int64),
                                                _ptr(vp, ctypes.c_int32)))
-        return {"source": s, "target": t, "weight": w, "delay_steps": d, "vp": vp}
+        m = np.zeros(n, np.int32)
+        self._chk(_lib.nestgpu_get_connection_models(self._h, _ptr(m, ctypes.c_int32)))
+        return {"source": s, "target": t, "weight": w, "delay_steps": d, "vp": vp,
+                "synapse_model": np.where(m == 1, "stdp_synapse", "static_synapse")}
 
     def Prepare(self):
         self._chk(_lib.nestgpu_prepare(self._h))

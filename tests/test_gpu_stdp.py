@@ -9,7 +9,8 @@ split runs and in the batched form.
 
 This path's limits raise instead of diverging: a neuron with outgoing stdp_synapse
 connections that spikes twice within one min_delay slice, and a target whose post-spike
-history between two presynaptic spikes outgrows the 64 entries kept per neuron.
+history between two presynaptic spikes outgrows the entries kept per neuron
+(stdp_history_capacity).
 """
 
 import numpy as np
@@ -17,6 +18,7 @@ import pytest
 
 import nestgpu
 import netbuild as nb
+from _oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -136,14 +138,76 @@ def test_stdp_source_spiking_twice_in_a_slice_raises():
         k.Simulate(50.0)
 
 
-def test_stdp_history_overflow_raises():
-    # the source's first spike comes after about 105 ms; the target spiked about 80 times by
-    # then, more than the 64 history entries kept per neuron
+def _late_source(cap=None, T=200.0):
+    """The source's first spike comes after about 105 ms; the target spikes about 80 times
+    by then."""
     k = nestgpu.Kernel(0)
+    if cap:
+        k.SetKernelStatus({"stdp_history_capacity": cap})
     pre = k.Create("iaf_psc_exp", 1, {"I_e": 375.01, "V_m": -70.0})
     post = k.Create("iaf_psc_exp", 1, {"I_e": 5000.0, "t_ref": 0.5})
     k.Connect(pre, post, "one_to_one", {"synapse_model": "stdp_synapse", "weight": 10.0, "delay": 1.0})
     sr = k.Create("spike_recorder")
     k.Connect(pre + post, sr)
+    return k, (pre, post), sr
+
+
+def test_stdp_history_capacity():
+    # 64 entries do not reach back to the synapse's range: the run fails
+    k, _, _ = _late_source(cap=64)
     with pytest.raises(nestgpu.NESTError, match="history"):
         k.Simulate(200.0)
+    # the default capacity holds them: same spikes, states and weight as the oracle's
+    # unbounded history
+    k, (pre, post), sr = _late_source()
+    assert k.GetKernelStatus("stdp_history_capacity") == 256
+    o = Oracle(portable_math=True)
+    opre = o.create("iaf_psc_exp", 1, {"I_e": 375.01, "V_m": -70.0})
+    opost = o.create("iaf_psc_exp", 1, {"I_e": 5000.0, "t_ref": 0.5})
+    o.connect(opre, opost, "one_to_one", weight=10.0, delay=1.0, stdp={})
+    k.Simulate(200.0)
+    o.simulate(200.0)
+    ev = sr.get("events")
+    og, os_ = nb.sorted_spikes_oracle(o)
+    print("spikes gpu=%d oracle=%d (source %d)" % (len(ev["senders"]), len(og), np.count_nonzero(og == opre[0])),
+          flush=True)
+    assert np.count_nonzero(og == opre[0]) >= 1 and np.count_nonzero(og == opost[0]) > 64
+    assert np.array_equal(ev["senders"], og)
+    assert np.allclose(ev["times"], os_ * 0.1, rtol=0, atol=1e-9)
+    for p, op in ((pre, opre), (post, opost)):
+        for f in FIELDS:
+            assert np.array_equal(np.atleast_1d(p.get(f)), o.get(op, f)), f
+    c = k.GetConnections(source=pre, target=post)
+    w = o.connections()[2]
+    print("weight gpu %r oracle %r" % (c["weight"][0], w[0]), flush=True)
+    assert c["weight"][0] == w[0] and w[0] != 10.0
+
+
+def test_stdp_circuit_scaled():
+    # the benchmark circuit at scale 0.1 with stdp_synapse between its excitatory populations
+    # (min_delay one step, normal delays), against the oracle
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    T, n_vp = 100.0, 4
+    k = nestgpu.Kernel(0)
+    pops, n, _ = bench.build_c3(k, nestgpu, n_vp, 20241, 777, scale=0.1, stdp=bench.C3_STDP)
+    sr = k.Create("spike_recorder")
+    allp = pops[0]
+    for p in pops[1:]:
+        allp = allp + p
+    k.Connect(allp, sr)
+    w0 = _weights_gpu(k, pops)[2].copy()
+    n_stdp = len(k.GetConnections(synapse_model="stdp_synapse")["source"])
+    k.Simulate(T)
+    print("circuit scale 0.1: %d neurons, %d connections (%d stdp_synapse), min_delay %d" % (
+        n, k.GetKernelStatus("num_connections"), n_stdp, k.GetKernelStatus("min_delay_steps")), flush=True)
+    assert k.GetKernelStatus("min_delay_steps") == 1 and n_stdp > 0
+    o = Oracle(n_vp=n_vp, seed=20241, portable_math=True)
+    opops = bench.build_c3_oracle(o, nestgpu, 777, int(round(T / 0.1)), scale=0.1, stdp=bench.C3_STDP)
+    o.simulate(T)
+    w1 = _check(k, pops, sr, o, opops)
+    print("stdp weights changed: %d of %d" % (np.count_nonzero(w1 != w0), n_stdp), flush=True)
+    assert np.count_nonzero(w1 != w0) > n_stdp // 10

@@ -690,3 +690,15 @@ def test_stdp_synapse_connect_checks():
     c = k.GetConnections()
     assert len(c["source"]) == 50
     assert np.array_equal(np.sort(np.unique(c["weight"])), [2.0, 30.0])
+    # the synapse model of each connection, and the filter on it
+    assert np.array_equal(c["synapse_model"] == "stdp_synapse", c["weight"] == 30.0)
+    s = k.GetConnections(synapse_model="stdp_synapse")
+    assert len(s["source"]) == 25 and np.all(s["weight"] == 30.0) and np.all(np.isin(s["target"], a[5:].tolist()))
+    assert len(k.GetConnections(synapse_model="static_synapse")["source"]) == 25
+    # post-spike history capacity: a power of two in [2, 65536]
+    assert k.GetKernelStatus("stdp_history_capacity") == 256
+    for bad in (100, 1, 131072, 0.5):
+        with pytest.raises(nestgpu.NESTError, match="power of two"):
+            k.SetKernelStatus({"stdp_history_capacity": bad})
+    k.SetKernelStatus({"stdp_history_capacity": 1024})
+    assert k.GetKernelStatus("stdp_history_capacity") == 1024
