@@ -773,40 +773,109 @@ Kernel::build_spec( BuildSpec& b )
         }
       }
     } );
-  // stdp_synapse: per-record state, where each connection went, and the neurons' flags
+  // stdp_synapse: the presynaptic trace rows, each record's row, where each connection went,
+  // and the neurons with incoming stdp synapses
   b.stdp_prm.clear();
-  b.stdp_syn = HostArray< StdpSyn >();
+  b.stdp_pre.clear();
+  b.stdp_pre_row0.clear();
+  b.stdp_srow = HostArray< uint32_t >();
   b.stdp_in.clear();
-  b.stdp_out.clear();
   syn_pos_.clear();
   if ( plastic )
   {
     for ( const StdpSet& q : stdp_sets_ )
       b.stdp_prm.push_back( StdpPrm { q.tau_plus, q.lambda, q.alpha, q.mu_plus, q.mu_minus, q.Wmax } );
-    b.stdp_syn = HostArray< StdpSyn >( E );
+    // the parameter sets of each source's stdp synapses (the records of source s are the
+    // positions [row_ptr[s], row_ptr[s + 1]) in every pass): one, or a sorted list
+    std::vector< uint16_t > one_set( N, 0 );
+    std::vector< std::vector< std::pair< uint32_t, std::vector< uint16_t > > > > several( n_workers );
+    auto source_range = [ & ]( int w, uint32_t& s_lo, uint32_t& s_hi )
+    {
+      const uint64_t e_lo = static_cast< uint64_t >( E ) * w / n_workers;
+      const uint64_t e_hi = static_cast< uint64_t >( E ) * ( w + 1 ) / n_workers;
+      s_lo = w == 0 ? 0
+                    : static_cast< uint32_t >( std::lower_bound( row_ptr.begin(), row_ptr.end() - 1, e_lo ) - row_ptr.begin() );
+      s_hi = w == n_workers - 1
+        ? N
+        : static_cast< uint32_t >( std::lower_bound( row_ptr.begin(), row_ptr.end() - 1, e_hi ) - row_ptr.begin() );
+    };
+    parallel(
+      [ & ]( int w )
+      {
+        uint32_t s_lo, s_hi;
+        source_range( w, s_lo, s_hi );
+        std::vector< uint16_t > sets;
+        for ( uint32_t src = s_lo; src < s_hi; ++src )
+        {
+          sets.clear();
+          for ( uint64_t pos = row_ptr[ src ]; pos < row_ptr[ src + 1 ]; ++pos )
+          {
+            const uint16_t sv = syn_flat[ cidx[ pos ] ];
+            if ( sv and ( sets.empty() or sets.back() != sv ) )
+              sets.push_back( sv );
+          }
+          std::sort( sets.begin(), sets.end() );
+          sets.erase( std::unique( sets.begin(), sets.end() ), sets.end() );
+          if ( sets.size() == 1 )
+            one_set[ src ] = sets[ 0 ];
+          else if ( sets.size() > 1 )
+            several[ w ].emplace_back( src, sets );
+        }
+      } );
+    std::vector< const std::vector< uint16_t >* > sets_of( N, nullptr );
+    for ( const auto& list : several )
+      for ( const auto& ps : list )
+        sets_of[ ps.first ] = &ps.second;
+    b.stdp_pre_row0.assign( N + 1, 0 );
+    for ( uint32_t src = 0; src < N; ++src )
+      b.stdp_pre_row0[ src + 1 ] =
+        b.stdp_pre_row0[ src ] + ( sets_of[ src ] ? static_cast< uint32_t >( sets_of[ src ]->size() ) : one_set[ src ] ? 1u : 0u );
+    // rows at creation: Kplus from the syn_spec, t_lastspike_ 0 (stdp_synapse.h:200-215)
+    b.stdp_pre.assign( b.stdp_pre_row0[ N ], StdpPre {} );
+    for ( uint32_t src = 0; src < N; ++src )
+      for ( uint32_t r = b.stdp_pre_row0[ src ]; r < b.stdp_pre_row0[ src + 1 ]; ++r )
+      {
+        const uint16_t sv = sets_of[ src ] ? ( *sets_of[ src ] )[ r - b.stdp_pre_row0[ src ] ] : one_set[ src ];
+        StdpPre& R = b.stdp_pre[ r ];
+        const StdpSet& q = stdp_sets_[ sv - 1 ];
+        R.cur_kp = q.Kplus;
+        R.cur_tl = 0.0;
+        R.tau_plus = q.tau_plus;
+        R.lambda = q.lambda;
+        R.alpha = q.alpha;
+        R.Wmax = q.Wmax;
+        R.mu_plus = q.mu_plus != 0.0 ? 1u : 0u;
+        R.mu_minus = q.mu_minus != 0.0 ? 1u : 0u;
+      }
+    b.stdp_srow = HostArray< uint32_t >( E );
     syn_pos_.resize( static_cast< size_t >( E ) );
     parallel(
       [ & ]( int w )
       {
-        const int64_t lo = E * w / n_workers, hi = E * ( w + 1 ) / n_workers;
-        for ( int64_t pos = lo; pos < hi; ++pos )
-        {
-          const uint32_t c = cidx[ pos ];
-          syn_pos_[ c ] = static_cast< uint32_t >( pos );
-          const uint16_t sv = syn_flat[ c ];
-          // stdp_synapse state at creation: Kplus from the syn_spec, t_lastspike_ 0
-          b.stdp_syn[ pos ] = StdpSyn { sv ? stdp_sets_[ sv - 1 ].Kplus : 0.0, 0.0, sv ? sv - 1u : 0u, 0u };
-        }
+        uint32_t s_lo, s_hi;
+        source_range( w, s_lo, s_hi );
+        for ( uint32_t src = s_lo; src < s_hi; ++src )
+          for ( uint64_t pos = row_ptr[ src ]; pos < row_ptr[ src + 1 ]; ++pos )
+          {
+            const uint32_t c = cidx[ pos ];
+            syn_pos_[ c ] = static_cast< uint32_t >( pos );
+            const uint16_t sv = syn_flat[ c ];
+            uint32_t row = 0;
+            if ( sv )
+            {
+              row = b.stdp_pre_row0[ src ];
+              if ( sets_of[ src ] )
+                row += static_cast< uint32_t >(
+                  std::lower_bound( sets_of[ src ]->begin(), sets_of[ src ]->end(), sv ) - sets_of[ src ]->begin() );
+            }
+            b.stdp_srow[ pos ] = row;
+          }
       } );
     b.stdp_in.assign( N, 0 );
-    b.stdp_out.assign( N, 0 );
     for ( size_t v = 0; v < conns_.size(); ++v )
       for ( size_t i = 0; i < conns_[ v ].syn.size(); ++i )
         if ( conns_[ v ].syn[ i ] )
-        {
           b.stdp_in[ gid_to_dev_[ conns_[ v ].tgt[ i ] ] ] = 1;
-          b.stdp_out[ gid_to_dev_[ conns_[ v ].src[ i ] ] ] = 1;
-        }
     b.tau_minus_inv.assign( N, 0.0 );
     for ( uint32_t i = 0; i < N; ++i )
       b.tau_minus_inv[ dev_of[ i ] ] = 1. / params_[ i ].tau_minus;

@@ -49,19 +49,27 @@ struct ParamSet
   int64_t drive_steps;  // emission steps covered
 };
 
-// stdp_synapse (models/stdp_synapse.h:200-215): a parameter set, and the state of one synapse
-// record (Kplus, the time of its last presynaptic spike, its parameter set); the weight is the
-// record's own
+// stdp_synapse (models/stdp_synapse.h:200-215): a parameter set, and the presynaptic trace
+// of one source and parameter set. Every stdp_synapse of a source transmits each of its
+// spikes, so Kplus_ and t_lastspike_ evolve identically in all synapses of the source that
+// share tau_plus and the initial Kplus (one Connect call): the store keeps them once per
+// (source, parameter set), and each synapse record only its weight. The source's update
+// advances the trace at each spike and keeps the state before it in the slot of the spike's
+// slice parity, which the next slice's delivery reads while the source may already spike
+// again into the other slot.
 struct StdpPrm
 {
   double tau_plus, lambda, alpha, mu_plus, mu_minus, Wmax;
 };
-struct StdpSyn
+struct alignas( 128 ) StdpPre
 {
-  double kplus;
-  double t_last;  // ms
-  uint32_t prm;
-  uint32_t pad;
+  double kp[ 2 ];  // Kplus before the source's spike of an even / odd slice
+  double tl[ 2 ];  // the source's previous spike time (ms) before that spike
+  double cur_kp;   // Kplus and last spike time after the source's latest spike
+  double cur_tl;
+  // the parameter set's values, so that an event reads one row
+  double tau_plus, lambda, alpha, Wmax;
+  uint32_t mu_plus, mu_minus;  // exponents, 0 or 1
 };
 // synapse record word: local target (bits 0-15), delay - min_delay (bits 16-30), stdp (bit 31)
 constexpr uint32_t PLASTIC_BIT = 0x80000000u;
@@ -144,10 +152,10 @@ struct NetDev
   //      spike time; the time base turns step stamps into ms as nest::Time does
   uint32_t stdp;
   uint32_t stdp_hcap;  // history entries per neuron, a power of two
-  const StdpPrm* sprm;
-  StdpSyn* sst;
+  StdpPre* spre;             // presynaptic traces, rows [pre_row0[s], pre_row0[s + 1]) of source s
+  const uint32_t* pre_row0;
+  const uint32_t* srow;      // per synapse record: its trace row (stdp records only)
   const uint8_t* stdp_in;   // has incoming stdp synapses (ArchivingNode::n_incoming_ > 0)
-  const uint8_t* stdp_out;  // has outgoing stdp synapses
   const double* tau_minus_inv;
   double* hist_t;
   double* hist_k;

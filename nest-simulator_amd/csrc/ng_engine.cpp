@@ -160,21 +160,21 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
   net_.ms_per_tic = b.ms_per_tic;
   net_.ms_per_step = b.ms_per_step;
   net_.tics_per_step = b.tics_per_step;
-  net_.sprm = nullptr;
-  net_.sst = nullptr;
-  net_.stdp_in = net_.stdp_out = nullptr;
+  net_.spre = nullptr;
+  net_.pre_row0 = net_.srow = nullptr;
+  net_.stdp_in = nullptr;
   net_.tau_minus_inv = nullptr;
   if ( net_.stdp )
   {
     if ( keep_state and not relayout )
       throw std::runtime_error( "NotImplemented: rebuilding the device copy of a network with stdp_synapse after Simulate" );
-    if ( b.stdp_syn.size() != b.syn.size() or b.stdp_in.size() != b.n or b.stdp_out.size() != b.n
-         or b.tau_minus_inv.size() != b.n )
-      throw std::runtime_error( "stdp_synapse: per-record state and per-neuron flags expected" );
-    net_.sprm = static_cast< const StdpPrm* >( S( upload( b.stdp_prm ) ) );
-    net_.sst = static_cast< StdpSyn* >( S( upload( b.stdp_syn ) ) );
+    if ( b.stdp_srow.size() != b.syn.size() or b.stdp_in.size() != b.n or b.stdp_pre_row0.size() != b.n + 1
+         or b.stdp_pre.size() != b.stdp_pre_row0.back() or b.tau_minus_inv.size() != b.n )
+      throw std::runtime_error( "stdp_synapse: trace rows, per-record rows and per-neuron flags expected" );
+    net_.spre = static_cast< StdpPre* >( S( upload( b.stdp_pre ) ) );
+    net_.pre_row0 = static_cast< const uint32_t* >( S( upload( b.stdp_pre_row0 ) ) );
+    net_.srow = static_cast< const uint32_t* >( S( upload( b.stdp_srow ) ) );
     net_.stdp_in = static_cast< const uint8_t* >( S( upload( b.stdp_in ) ) );
-    net_.stdp_out = static_cast< const uint8_t* >( S( upload( b.stdp_out ) ) );
     net_.tau_minus_inv = static_cast< const double* >( S( upload( b.tau_minus_inv ) ) );
   }
   net_.drive_counts = static_cast< const uint8_t* >( S( upload( b.drive_counts ) ) );

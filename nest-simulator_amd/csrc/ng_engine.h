@@ -106,8 +106,10 @@ struct BuildSpec
   // stdp_synapse: parameter sets, state per synapse record (empty without stdp), neurons
   // with incoming / outgoing stdp synapses, 1/tau_minus per neuron; the time base
   std::vector< StdpPrm > stdp_prm;
-  HostArray< StdpSyn > stdp_syn;
-  std::vector< uint8_t > stdp_in, stdp_out;
+  std::vector< StdpPre > stdp_pre;       // presynaptic trace rows by source
+  std::vector< uint32_t > stdp_pre_row0;  // n + 1 row starts by device index
+  HostArray< uint32_t > stdp_srow;        // trace row of each synapse record
+  std::vector< uint8_t > stdp_in;
   std::vector< double > tau_minus_inv;
   double ms_per_tic = 0.001, ms_per_step = 0.1;
   uint32_t stdp_hcap = STDP_HIST_DEFAULT;  // post-spike history entries per neuron (power of two)

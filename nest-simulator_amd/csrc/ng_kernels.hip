@@ -468,22 +468,23 @@ __device__ unsigned long long g_phase_ticks[ 8 ];
 // integrator needs more; its kernels run one workgroup per CU.
 // ---------------------------------------------------------------- stdp_synapse
 // stdp_synapse::send (models/stdp_synapse.h:222-283) for the event of record ridx, emitted at
-// step stamp `stamp`, to device target tdev over a delay of dsteps: facilitation by the
-// target's spikes since the synapse's last presynaptic spike, depression by its trace, then
-// the synapse's own trace. Returns the new weight, which the record keeps. Every record is
-// delivered at most once per slice (a source with stdp synapses spikes at most once per slice,
-// checked in the gather phase), so no other thread touches its state meanwhile; the target's
-// history is written only in the update phase.
+// step stamp `stamp` in a slice of parity par, to device target tdev over a delay of dsteps:
+// facilitation by the target's spikes since the synapse's last presynaptic spike, depression
+// by the target's trace. The presynaptic trace and last spike time before this spike come
+// from the source's trace row (the source advanced it when it spiked). Returns the new
+// weight, which the record keeps. Every record is delivered at most once per slice (a source
+// with stdp synapses spikes at most once per slice, checked in the update phase), so no other
+// thread touches its weight meanwhile; the target's history is written only in the update
+// phase.
 constexpr double STDP_EPS = 1.0e-6;  // ConnectionManager::stdp_eps_ (connection_manager.cpp:78)
 
 __device__ inline double
-stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t stamp, uint32_t dsteps )
+stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t stamp, uint32_t dsteps, uint32_t par )
 {
-  gref< StdpSyn > st = glob( d.sst )[ ridx ];
-  const double kplus = st.kplus, t_last = st.t_last;
-  gref< const StdpPrm > P = glob( d.sprm )[ st.prm ];
-  const double tau_plus = P.tau_plus, lambda = P.lambda, alpha = P.alpha, Wmax = P.Wmax;
-  const bool mu_p = P.mu_plus != 0.0, mu_m = P.mu_minus != 0.0;  // exponents 1 or 0 (host-checked)
+  gref< const StdpPre > pre = glob( const_cast< const StdpPre* >( d.spre ) )[ glob( d.srow )[ ridx ] ];
+  const double kplus = pre.kp[ par ], t_last = pre.tl[ par ];
+  const double tau_plus = pre.tau_plus, lambda = pre.lambda, alpha = pre.alpha, Wmax = pre.Wmax;
+  const bool mu_p = pre.mu_plus != 0u, mu_m = pre.mu_minus != 0u;
   const double t_spike = d.ms_per_tic * static_cast< double >( stamp * d.tics_per_step );  // Time::get_ms
   const double dd = static_cast< double >( dsteps ) * d.ms_per_step;  // Time::delay_steps_to_ms
   // ArchivingNode::get_history( t_last - d, t_spike - d ): entries t1 + eps <= t < t2 + eps
@@ -494,13 +495,14 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
   const uint32_t first = n > cap ? n - cap : 0u;  // entries before first left the ring
   gptr< const double > ht = glob( d.hist_t ) + static_cast< size_t >( tdev ) * cap;
   gptr< const double > hk = glob( d.hist_k ) + static_cast< size_t >( tdev ) * cap;
+  auto t_at = [ & ]( uint32_t i ) -> double { return ht[ i & hm ]; };
   // the times are increasing: walk back from the newest entry to the range [lo, hi) of
   // entries t1 + eps <= t < t2 + eps, so that an event costs the target spikes in its range
   uint32_t hi = n;
-  while ( hi > first and ht[ ( hi - 1 ) & hm ] >= t2_lim )
+  while ( hi > first and t_at( hi - 1 ) >= t2_lim )
     --hi;
   uint32_t lo = hi;
-  while ( lo > first and ht[ ( lo - 1 ) & hm ] >= t1_lim )
+  while ( lo > first and t_at( lo - 1 ) >= t1_lim )
     --lo;
   // the range (or the latest spike before t2 - eps) may reach past the entries kept
   if ( first > 0 and lo == first )
@@ -508,7 +510,7 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
   for ( uint32_t i = lo; i < hi; ++i )
   {
     // facilitate_ (stdp_synapse.h:187-191), oldest entry first
-    const double minus_dt = t_last - ( ht[ i & hm ] + dd );
+    const double minus_dt = t_last - ( t_at( i ) + dd );
     const double kp = kplus * ng_exp( minus_dt / tau_plus );
     const double norm_w = ( w / Wmax ) + ( lambda * ( mu_p ? 1.0 - ( w / Wmax ) : 1.0 ) * kp );
     w = norm_w < 1.0 ? norm_w * Wmax : Wmax;
@@ -519,7 +521,7 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
   double K = 0.0;
   for ( uint32_t i = hi; i > first; --i )
   {
-    const double th = ht[ ( i - 1 ) & hm ];
+    const double th = t_at( i - 1 );
     if ( tq - th > STDP_EPS )
     {
       K = hk[ ( i - 1 ) & hm ] * ng_exp( ( th - tq ) * glob( d.tau_minus_inv )[ tdev ] );
@@ -529,8 +531,6 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
   // depress_ (stdp_synapse.h:193-198)
   const double norm_w = ( w / Wmax ) - ( alpha * lambda * ( mu_m ? w / Wmax : 1.0 ) * K );
   w = norm_w > 0.0 ? norm_w * Wmax : 0.0;
-  st.kplus = kplus * ng_exp( ( t_last - t_spike ) / tau_plus ) + 1.0;
-  st.t_last = t_spike;
   uint64_t bits;
   __builtin_memcpy( &bits, &w, sizeof bits );
   gref< SynRec > r = glob( const_cast< SynRec* >( d.syn ) )[ ridx ];
@@ -540,12 +540,16 @@ stdp_event( const NetDev& d, uint32_t ridx, double w, uint32_t tdev, int64_t sta
 }
 
 // ArchivingNode::set_spiketime (archiving_node.cpp:196-230) for the spikes of neuron dev at the
-// set bits of `bits` (lags of the slice at origin, in order); a neuron with outgoing stdp
-// synapses must spike at most once per slice
+// set bits of `bits` (lags of the slice at origin, of parity par, in order), and the source
+// side of stdp_synapse::send (stdp_synapse.h:276-279): each of the neuron's presynaptic trace
+// rows keeps its state before the spike in slot par and advances Kplus_ and t_lastspike_. A
+// neuron with outgoing stdp synapses must spike at most once per slice.
 __device__ inline void
-stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, const uint64_t* bits )
+stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, uint32_t par, const uint64_t* bits )
 {
-  const bool in = glob( d.stdp_in )[ dev ] != 0, out = glob( d.stdp_out )[ dev ] != 0;
+  const bool in = glob( d.stdp_in )[ dev ] != 0;
+  const uint32_t row0 = glob( d.pre_row0 )[ dev ], row1 = glob( d.pre_row0 )[ dev + 1 ];
+  const bool out = row1 > row0;
   if ( not in and not out )
     return;
   double last = glob( d.last_spike )[ dev ];
@@ -571,6 +575,15 @@ stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, const uint64_t* bits
         glob( d.hist_t )[ at ] = t_sp;
         glob( d.hist_k )[ at ] = km;
         ++n;
+      }
+      for ( uint32_t r = row0; r < row1; ++r )
+      {
+        gref< StdpPre > pre = glob( d.spre )[ r ];
+        const double kp = pre.cur_kp, tl = pre.cur_tl;
+        pre.kp[ par ] = kp;
+        pre.tl[ par ] = tl;
+        pre.cur_kp = kp * ng_exp( ( tl - t_sp ) / pre.tau_plus ) + 1.0;
+        pre.cur_tl = t_sp;
       }
       last = t_sp;
     }
@@ -856,6 +869,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       // is bucketed and applied
       SynRec rec[ RQ ];
       uint32_t rlag[ RQ ];
+      uint32_t rid[ STDP ? RQ : 1 ];  // stdp_synapse: the records' indices
       // Every thread issues RQ record loads (events past the end reload the last record):
       // with a fixed number of loads in flight the waits count exactly, instead of draining
       // all outstanding memory operations.
@@ -869,13 +883,18 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           const uint32_t e = eb + q < E ? eb + q : E - 1;
           while ( j + 1 < static_cast< uint32_t >( CH ) and S.c_off[ j + 1 ] <= e )
             ++j;
-          rec[ q ] = load_rec( glob( d.syn ) + S.c_g0[ j ] + ( e - S.c_off[ j ] ) );
+          const uint32_t ri = S.c_g0[ j ] + ( e - S.c_off[ j ] );
+          rec[ q ] = load_rec( glob( d.syn ) + ri );
           rlag[ q ] = S.c_lag[ j ];
+          if constexpr ( STDP )
+            rid[ q ] = ri;
         }
       };
       if ( E )
         load_round( 0 );
       const int umodel = d.uniform_model;
+      const uint32_t dpar = static_cast< uint32_t >( ph.parity ^ 1 );  // the delivered spikes' slice
+      (void)dpar;
       for ( uint32_t e0 = 0; e0 < E; e0 += R, cur ^= 1 )
       {
         // stage: thread t holds events e0 + t*RQ .. + RQ - 1 (key order = round index) and
@@ -895,13 +914,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
           {
             double w = __hiloint2double( static_cast< int >( rec[ q ].w_hi ), static_cast< int >( rec[ q ].w_lo ) );
             const uint32_t local = rec[ q ].word & 0xffffu;
-            if ( STDP and ( rec[ q ].word & PLASTIC_BIT ) )
+            if constexpr ( STDP )
             {
-              // the record's index, found again from the event's place in the chunk
-              const uint32_t jr = upper_bound_u32( S.c_off, CH + 1, eb + q ) - 1;
-              w = stdp_event( d, S.c_g0[ jr ] + ( eb + q - S.c_off[ jr ] ), w, dev0 + local,
-                ph.origin - d.min_delay + rlag[ q ] + 1,
-                ( ( rec[ q ].word >> 16 ) & DELAY_MASK ) + static_cast< uint32_t >( d.min_delay ) );
+              if ( rec[ q ].word & PLASTIC_BIT )
+                w = stdp_event( d, rid[ q ], w, dev0 + local, ph.origin - d.min_delay + rlag[ q ] + 1,
+                  ( ( rec[ q ].word >> 16 ) & DELAY_MASK ) + static_cast< uint32_t >( d.min_delay ), dpar );
             }
             const uint32_t slot_rel = rlag[ q ] + ( ( rec[ q ].word >> 16 ) & DELAY_MASK );
             const int m = MASK == 1 ? M_IAF_PSC_DELTA
@@ -1312,7 +1329,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, 1, ph.to );
   }
   if ( STDP and d.stdp and local < nb )
-    stdp_spikes( d, dev, ph.origin, bits );
+    stdp_spikes( d, dev, ph.origin, static_cast< uint32_t >( ph.parity ), bits );
   NG_PHASE( 6 );
 
   // ================================================================ 3. spike lists

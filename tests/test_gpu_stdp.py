@@ -114,6 +114,25 @@ def test_stdp_split_runs_and_readback(mu):
     _check(k, pops, sr, o, opops, w0)
 
 
+def test_stdp_sources_with_several_parameter_sets():
+    # every excitatory source has stdp synapses of two Connect calls with different tau_plus
+    # and initial Kplus (two presynaptic traces per source), and static ones
+    desc = stdp_net(N=1200, seed=41, drive_seed=6)
+    N = 1200
+    pre, w, d, stdp = desc["conns"][0][0], desc["conns"][0][4], desc["conns"][0][5], desc["conns"][0][8]
+    other = dict(stdp, tau_plus=25.0, Kplus=0.0, **{"lambda": 0.03}, mu_plus=0.0)
+    desc["conns"] = [(pre, (0, 0, N // 2), "fixed_indegree", 60, w, d, True, True, stdp),
+                     (pre, (0, N // 2, N), "fixed_indegree", 60, w, d, True, True, other),
+                     (pre, (0, 0, N), "fixed_indegree", 20, w, d),
+                     desc["conns"][1]]
+    k, pops, sr = nb.build_gpu(desc)
+    o, opops = nb.build_oracle(desc, portable_math=True)
+    w0 = _weights_gpu(k, pops)[2].copy()
+    k.Simulate(desc["T"])
+    o.simulate(desc["T"])
+    _check(k, pops, sr, o, opops, w0)
+
+
 def test_stdp_batched():
     # instances with and without stdp_synapse share the slice launches
     descs = [stdp_net(N=800, seed=31, drive_seed=1), nb.exp_random(N=900, T=200.0, seed=5, n_vp=2),
