@@ -16,25 +16,6 @@
 namespace nestgpu
 {
 
-// Propagator P32 of IAFPropagatorExp::evaluate (libnestutil/iaf_propagator.cpp:42-74)
-static double
-iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
-{
-  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
-  const double gamma = beta / c_m;
-  const double inv_tau_syn = 1 / tau_syn;
-  const double inv_tau_m = 1 / tau_m;
-  const double inv_c_m = 1 / c_m;
-  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
-  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
-  const double expm1_h_tau = std::expm1( h * inv_beta );
-  const double P32 = gamma * exp_h_tau_syn * expm1_h_tau;
-  if ( std::isnormal( P32 ) and P32 > 0 )
-    return P32;
-  const double exp_h_tau = std::exp( -h * inv_tau_m );
-  return h * inv_c_m * exp_h_tau;
-}
-
 void
 Kernel::choose_layout( uint32_t n, int max_delay, int nch, uint32_t& tb, bool& win ) const
 {
@@ -236,10 +217,10 @@ Kernel::build_spec( BuildSpec& b )
   b.min_delay = static_cast< int >( min_delay_ );
   b.max_delay = static_cast< int >( max_delay_ );
   b.n = N;
-  bool all_delta = true;
+  // ring-buffer channels: the most any model here uses
+  b.nch = 1;
   for ( const auto& Pm : params_ )
-    all_delta = all_delta and Pm.model == M_IAF_PSC_DELTA;
-  b.nch = all_delta ? 1 : 2;
+    b.nch = std::max( b.nch, ModelManager::instance().get( Pm.model ).channels() );
   choose_layout( N, b.max_delay, b.nch, b.tb, b.win );
   // ---- device index = position in NEST's receive order (VP, then node id)
   std::vector< uint32_t > order( N );
@@ -445,51 +426,8 @@ Kernel::build_spec( BuildSpec& b )
     std::memset( &ps, 0, sizeof ps );
     ps.model = Pm.model;
     ps.refractory_counts = static_cast< int32_t >( tb_.ms_to_steps( Pm.t_ref ) );  // Time( Time::ms( t_ref ) ).get_steps()
-    if ( Pm.model == M_IAF_PSC_DELTA )
-    {
-      // iaf_psc_delta::pre_run_hook (iaf_psc_delta.cpp:234-270)
-      const double P33 = std::exp( -h / Pm.tau_m );
-      const double P30 = 1 / Pm.c_m * ( 1 - P33 ) * Pm.tau_m;
-      ps.p[ 0 ] = P30;
-      ps.p[ 1 ] = P33;
-      ps.p[ 2 ] = 0.0 + Pm.I_e;
-      ps.p[ 3 ] = Pm.V_th;
-      ps.p[ 4 ] = Pm.V_reset;
-      ps.p[ 5 ] = Pm.V_min;
-      // refractory_input: input arriving while refractory decays by exp(-r h / tau_m) and is
-      // added when the neuron becomes active again (iaf_psc_delta.cpp:284-299)
-      ps.p[ 6 ] = Pm.refr_input ? 1.0 : 0.0;
-      ps.p[ 7 ] = h;
-      ps.p[ 8 ] = Pm.tau_m;
-    }
-    else if ( Pm.model == M_IAF_PSC_EXP )
-    {
-      // iaf_psc_exp::pre_run_hook (iaf_psc_exp.cpp:244-286)
-      const double P11ex = std::exp( -h / Pm.tau_ex );
-      const double P11in = std::exp( -h / Pm.tau_in );
-      const double P22 = std::exp( -h / Pm.tau_m );
-      const double P21ex = iaf_propagator_exp( Pm.tau_ex, Pm.tau_m, Pm.c_m, h );
-      const double P21in = iaf_propagator_exp( Pm.tau_in, Pm.tau_m, Pm.c_m, h );
-      const double P20 = Pm.tau_m / Pm.c_m * ( 1.0 - P22 );
-      ps.p[ 0 ] = P22;
-      ps.p[ 1 ] = P21ex;
-      ps.p[ 2 ] = P21in;
-      ps.p[ 3 ] = P20;
-      ps.p[ 4 ] = P11ex;
-      ps.p[ 5 ] = P11in;
-      ps.p[ 6 ] = Pm.I_e + 0.0;
-      ps.p[ 7 ] = Pm.V_th;
-      ps.p[ 8 ] = Pm.V_reset;
-      ps.p[ 9 ] = ( 1. - P11ex ) * 0.0;
-    }
-    else
-    {
-      // aeif_cond_exp::pre_run_hook (aeif_cond_exp.cpp:420-435)
-      const double vals[] = { Pm.V_peak, Pm.V_reset, Pm.g_L, Pm.c_m, Pm.E_ex, Pm.E_in, Pm.E_L, Pm.Delta_T, Pm.tau_w,
-        Pm.a, Pm.b, Pm.V_th, Pm.tau_ex, Pm.tau_in, Pm.I_e, Pm.tol, Pm.Delta_T > 0. ? Pm.V_peak : Pm.V_th, h };
-      for ( int q = 0; q < 18; ++q )
-        ps.p[ q ] = vals[ q ];
-    }
+    const NodeModel& M = ModelManager::instance().get( Pm.model );
+    M.pre_run_hook( Pm, h, ps );
     if ( gen_of[ i ] >= 0 )
     {
       const SpikeGen& G = gens_[ node( gen_of[ i ] ).idx ];
@@ -536,7 +474,8 @@ Kernel::build_spec( BuildSpec& b )
       first_model = Pm.model;
     uniform = uniform and Pm.model == first_model;
     b.r[ dv ] = S.r;
-    const double sv[] = { S.v0, S.v1, S.v2, S.v3, Pm.model == M_AEIF_COND_EXP ? h : 0.0 };
+    double sv[ 5 ];
+    M.device_state( Pm, S, h, sv );
     for ( int f = 0; f < 5; ++f )
       b.v[ f ][ dv ] = sv[ f ];
   }

@@ -340,6 +340,9 @@ cache_insert( lptr< uint32_t > keys, uint32_t cell )
   return NONE;  // no free slot within the probe limit: the cell stays in the ring
 }
 
+// handle( SpikeEvent& ) of model m (the models' device halves, ng_models_dev.h)
+__device__ __forceinline__ void node_receive( int m, double w, uint32_t& ch, double& v );
+
 // spike_generator drive of this part: handle() of the target with weight * multiplicity.
 // Each lag adds into a different cell, so the cells of LB lags are read together.
 template < bool WIN >
@@ -367,29 +370,10 @@ drive_adds( const NetDev& d, gref< const ParamSet > ps, int m, uint32_t dev, uin
     for ( int i = 0; i < KC; ++i )
       PC.c[ i ] = PC.T->cdf[ i ];
   }
-  // the channel and multiplier of the weight: models/*::handle(SpikeEvent&)
+  // the channel and multiplier of the weight: the model's handle( SpikeEvent& )
   uint32_t ch;
   double sw;
-  if ( m == M_IAF_PSC_DELTA )
-  {
-    ch = 0;
-    sw = w;
-  }
-  else if ( m == M_IAF_PSC_EXP )
-  {
-    ch = w > 0.0 ? 1 : 0;
-    sw = w;
-  }
-  else if ( w > 0.0 )
-  {
-    ch = 0;
-    sw = w;
-  }
-  else
-  {
-    ch = 1;
-    sw = -w;
-  }
+  node_receive( m, w, ch, sw );
   for ( int l0 = from; l0 < to; l0 += LB * stride )
   {
     double add[ LB ];
@@ -441,6 +425,14 @@ store_sample( const NetDev& d, uint32_t col, int64_t stamp, double f0, double f1
   p[ 2 ] = f2;
   p[ 3 ] = f3;
 }
+
+}  // namespace nestgpu
+
+// the neuron models' device halves: update and handle( SpikeEvent& ) per model id
+#include "ng_models_dev.h"
+
+namespace nestgpu
+{
 
 // Phase timers of the slice kernel, compiled in only with -DNG_PROFILE: per phase, the sum
 // over workgroups of wall-clock ticks (read back with nestgpu_profile_read).
@@ -845,11 +837,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
                 : MASK == 4           ? M_AEIF_COND_EXP
                 : umodel >= 0         ? umodel
                                       : glob( d.model )[ dev0 + local ];
-              uint32_t ch = 0;
-              if ( m == M_IAF_PSC_EXP )
-                ch = w > 0 ? 1u : 0u;
-              else if ( m == M_AEIF_COND_EXP and not( w > 0.0 ) )
-                ch = 1;
+              uint32_t ch;
+              double v;
+              node_receive( m, w, ch, v );
               hcell[ q ] = C.ring_index( slot_rel, ch, local );
               hslot[ q ] = cache_insert( ckey, hcell[ q ] );
             }
@@ -926,15 +916,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               : MASK == 4           ? M_AEIF_COND_EXP
               : umodel >= 0         ? umodel
                                     : glob( d.model )[ dev0 + local ];
-            uint32_t ch = 0;
-            double v = w;  // static_synapse: weight * multiplicity 1
-            if ( m == M_IAF_PSC_EXP )
-              ch = w > 0 ? 1u : 0u;
-            else if ( m == M_AEIF_COND_EXP and not( w > 0.0 ) )
-            {
-              ch = 1;
-              v = -w;
-            }
+            uint32_t ch;
+            double v;
+            node_receive( m, w, ch, v );  // static_synapse: weight * multiplicity 1
             loc[ q ] = local;
             val[ q ] = v;
             key[ q ] = ( ( slot_rel * d.nch + ch ) << IDX_BITS ) | static_cast< uint32_t >( t * RQ + q );
@@ -1119,212 +1103,11 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
       : ( MASK == 4 )           ? M_AEIF_COND_EXP
                                 : ps.model;
     if ( ( MASK & 1 ) and m == M_IAF_PSC_DELTA )
-    {
-      // models/iaf_psc_delta.cpp:272-331
-      const double P30 = ps.p[ 0 ], P33 = ps.p[ 1 ], y0I = ps.p[ 2 ], V_th = ps.p[ 3 ], V_reset = ps.p[ 4 ],
-                   V_min = ps.p[ 5 ];
-      const bool refr_input = ps.p[ 6 ] != 0.0;
-      const double h = ps.p[ 7 ], tau_m = ps.p[ 8 ];
-      double y3 = glob( d.v[ 0 ] )[ dev ];
-      double refr_buf = refr_input ? glob( d.v[ 1 ] )[ dev ] : 0.0;  // refr_spikes_buffer_
-      int32_t r = glob( d.r )[ dev ];
-      // ring-buffer cells are read LB lags at a time (independent loads), then consumed in
-      // lag order and cleared
-      for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
-      {
-        double win_[ LB ];
-#pragma unroll
-        for ( int q = 0; q < LB; ++q )
-          if ( l0 + q < ph.to )
-            win_[ q ] = C.get( l0 + q, 0, local );
-#pragma unroll
-        for ( int q = 0; q < LB; ++q )
-          if ( l0 + q < ph.to )
-          {
-            const int lag = l0 + q;
-            C.put( lag, 0, local, 0.0 );
-            if ( r == 0 )
-            {
-              y3 = P30 * y0I + P33 * y3 + win_[ q ];
-              if ( refr_input and refr_buf != 0.0 )
-              {
-                y3 += refr_buf;
-                refr_buf = 0.0;
-              }
-              y3 = ( y3 < V_min ? V_min : y3 );
-            }
-            else
-            {
-              if ( refr_input )
-                refr_buf += win_[ q ] * ng_exp( -static_cast< double >( r ) * h / tau_m );
-              --r;
-            }
-            if ( y3 >= V_th )
-            {
-              r = ps.refractory_counts;
-              y3 = V_reset;
-              bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
-            }
-            if ( lag == next_sample )
-            {
-              store_sample( d, mcol, ph.origin + lag + 1, y3, 0.0, 0.0, 0.0 );
-              next_sample += d.mm_int;
-            }
-          }
-      }
-      glob( d.v[ 0 ] )[ dev ] = y3;
-      if ( refr_input )
-        glob( d.v[ 1 ] )[ dev ] = refr_buf;
-      glob( d.r )[ dev ] = r;
-    }
+      DeviceNode< M_IAF_PSC_DELTA >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
     if ( ( MASK & 2 ) and m == M_IAF_PSC_EXP )
-    {
-      // models/iaf_psc_exp.cpp:288-349 (delta = 0: deterministic threshold)
-      const double P22 = ps.p[ 0 ], P21ex = ps.p[ 1 ], P21in = ps.p[ 2 ], P20 = ps.p[ 3 ], P11ex = ps.p[ 4 ],
-                   P11in = ps.p[ 5 ], IeI0 = ps.p[ 6 ], Theta = ps.p[ 7 ], V_reset = ps.p[ 8 ], exI1 = ps.p[ 9 ];
-      double V = glob( d.v[ 0 ] )[ dev ];
-      double iex = glob( d.v[ 1 ] )[ dev ];
-      double iin = glob( d.v[ 2 ] )[ dev ];
-      int32_t r = glob( d.r )[ dev ];
-      for ( int l0 = ph.from; l0 < ph.to; l0 += LB )
-      {
-        double w_in[ LB ], w_ex[ LB ];
-#pragma unroll
-        for ( int q = 0; q < LB; ++q )
-          if ( l0 + q < ph.to )
-          {
-            w_in[ q ] = C.get( l0 + q, 0, local );
-            w_ex[ q ] = C.get( l0 + q, 1, local );
-          }
-#pragma unroll
-        for ( int q = 0; q < LB; ++q )
-          if ( l0 + q < ph.to )
-          {
-            const int lag = l0 + q;
-            if ( r == 0 )
-              V = V * P22 + iex * P21ex + iin * P21in + IeI0 * P20;
-            else
-              --r;
-            iex *= P11ex;
-            iin *= P11in;
-            iex += exI1;
-            iex += w_ex[ q ];
-            iin += w_in[ q ];
-            if ( V >= Theta )
-            {
-              r = ps.refractory_counts;
-              V = V_reset;
-              bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
-            }
-            C.put( lag, 0, local, 0.0 );
-            C.put( lag, 1, local, 0.0 );
-            if ( lag == next_sample )
-            {
-              store_sample( d, mcol, ph.origin + lag + 1, V, iex, iin, 0.0 );
-              next_sample += d.mm_int;
-            }
-          }
-      }
-      glob( d.v[ 0 ] )[ dev ] = V;
-      glob( d.v[ 1 ] )[ dev ] = iex;
-      glob( d.v[ 2 ] )[ dev ] = iin;
-      glob( d.r )[ dev ] = r;
-    }
+      DeviceNode< M_IAF_PSC_EXP >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
     if ( ( MASK & 4 ) and m == M_AEIF_COND_EXP )
-    {
-      // models/aeif_cond_exp.cpp:443-523
-      AeifPar A;
-      A.V_peak = ps.p[ 0 ];
-      A.V_reset = ps.p[ 1 ];
-      A.g_L = ps.p[ 2 ];
-      A.C_m = ps.p[ 3 ];
-      A.E_ex = ps.p[ 4 ];
-      A.E_in = ps.p[ 5 ];
-      A.E_L = ps.p[ 6 ];
-      A.Delta_T = ps.p[ 7 ];
-      A.tau_w = ps.p[ 8 ];
-      A.a = ps.p[ 9 ];
-      A.b = ps.p[ 10 ];
-      A.V_th = ps.p[ 11 ];
-      A.tau_ex = ps.p[ 12 ];
-      A.tau_in = ps.p[ 13 ];
-      A.I_e = ps.p[ 14 ];
-      A.tol = ps.p[ 15 ];
-      const double V_peak_eff = ps.p[ 16 ];
-      const double hstep = ps.p[ 17 ];
-      double y[ 4 ] = { glob( d.v[ 0 ] )[ dev ], glob( d.v[ 1 ] )[ dev ], glob( d.v[ 2 ] )[ dev ],
-        glob( d.v[ 3 ] )[ dev ] };
-      double hint = glob( d.v[ 4 ] )[ dev ];
-      int32_t r = glob( d.r )[ dev ];
-      // One loop over (lag, accepted step): each iteration makes one adaptive step of the
-      // lane's neuron, or, once the neuron has reached the end of its lag, closes the lag.
-      // Lanes advance through their own lags independently, so a wave costs its slowest
-      // neuron's total number of steps, not the sum over lags of the slowest per lag. Each
-      // neuron's sequence of operations is that of the lag loop with an inner step loop.
-      // More than AEIF_MAX_STEPS accepted steps within one time step stop the lane's lag and
-      // raise ERR_AEIF_STEPS, which fails the run (the oracle stops with the same limit).
-      int lag = ph.from;
-      double tt = 0.0;
-      int guard = 0;
-      while ( lag < ph.to )
-      {
-        if ( tt < hstep and guard >= AEIF_MAX_STEPS )
-        {
-          g_or( &d.counters[ 0 ], ERR_AEIF_STEPS );
-          tt = hstep;
-        }
-        if ( tt < hstep )
-        {
-          ++guard;
-          aeif_evolve( A, r > 0, &tt, hstep, &hint, y );
-          if ( y[ 0 ] < -1e3 or y[ 3 ] < -1e6 or y[ 3 ] > 1e6 )
-          {
-            g_or( &d.counters[ 0 ], ERR_AEIF_INSTABILITY );
-            tt = hstep;
-          }
-          else if ( r > 0 )
-            y[ 0 ] = A.V_reset;
-          else if ( y[ 0 ] >= V_peak_eff )
-          {
-            y[ 0 ] = A.V_reset;
-            y[ 3 ] += A.b;
-            r = ps.refractory_counts > 0 ? ps.refractory_counts + 1 : 0;
-            const uint64_t bit = 1ull << ( lag & 63 );
-            uint64_t had = 0;
-#pragma unroll
-            for ( int w = 0; w < 4; ++w )  // lag differs per lane: no dynamic register index
-              if ( ( lag >> 6 ) == w )
-              {
-                had = bits[ w ] & bit;
-                bits[ w ] |= bit;
-              }
-            if ( had )
-              g_or( &d.counters[ 0 ], ERR_AEIF_DOUBLE_SPIKE );
-          }
-        }
-        else
-        {
-          if ( r > 0 )
-            --r;
-          y[ 1 ] += C.get( lag, 0, local );
-          y[ 2 ] += C.get( lag, 1, local );
-          C.put( lag, 0, local, 0.0 );
-          C.put( lag, 1, local, 0.0 );
-          if ( lag == next_sample )
-          {
-            store_sample( d, mcol, ph.origin + lag + 1, y[ 0 ], y[ 1 ], y[ 2 ], y[ 3 ] );
-            next_sample += d.mm_int;
-          }
-          ++lag;
-          tt = 0.0;
-          guard = 0;
-        }
-      }
-      for ( int q = 0; q < 4; ++q )
-        glob( d.v[ q ] )[ dev ] = y[ q ];
-      glob( d.v[ 4 ] )[ dev ] = hint;
-      glob( d.r )[ dev ] = r;
-    }
+      DeviceNode< M_AEIF_COND_EXP >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
     if ( G == 0 )
       drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, 1, ph.to );
   }

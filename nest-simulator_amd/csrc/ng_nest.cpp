@@ -670,7 +670,15 @@ Kernel::get_kernel_status_json() const
   const long dmax = simulated_ ? max_delay_ : std::max( d_max_seen_, 1L );
   o << "{\"resolution\": " << tb_.resolution_ms() << ", \"tics_per_step\": " << tb_.tics_per_step
     << ", \"rng_seed\": " << seed_ << ", \"rng_type\": \"" << ( rng_type_ == RNG_PHILOX_32 ? "Philox_32" : "mt19937_64" )
-    << "\", \"rng_types\": [\"mt19937_64\", \"Philox_32\"], \"total_num_virtual_procs\": " << n_vp_
+    << "\", \"rng_types\": [\"mt19937_64\", \"Philox_32\"]";
+  // ModelManager::get_status (model_manager.cpp:116-131): the registered neuron models, then
+  // the devices; the synapse models
+  o << ", \"node_models\": [";
+  for ( const std::string& nm : ModelManager::instance().names() )
+    o << "\"" << nm << "\", ";
+  o << "\"spike_generator\", \"spike_recorder\", \"multimeter\", \"voltmeter\"]"
+    << ", \"synapse_models\": [\"static_synapse\", \"stdp_synapse\"]"
+    << ", \"total_num_virtual_procs\": " << n_vp_
     << ", \"local_num_threads\": " << n_vp_ << ", \"min_delay\": " << dmin * tb_.ms_per_step
     << ", \"max_delay\": " << dmax * tb_.ms_per_step << ", \"min_delay_steps\": " << dmin
     << ", \"max_delay_steps\": " << dmax << ", \"num_connections\": " << num_connections()
@@ -743,219 +751,26 @@ Kernel::model_name( int64_t gid ) const
     return "spike_recorder";
   if ( r.kind == 3 )
     return meters_[ r.idx ].voltmeter ? "voltmeter" : "multimeter";
-  static const char* names[] = { "iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp" };
-  return names[ params_[ r.idx ].model ];
+  return ModelManager::instance().get( params_[ r.idx ].model ).name();
 }
 
-NeuronParams
-Kernel::default_params( int model )
-{
-  NeuronParams p {};
-  p.model = model;
-  p.tau_minus = 20.0;  // ArchivingNode (archiving_node.cpp:40)
-  if ( model == M_IAF_PSC_DELTA )  // models/iaf_psc_delta.cpp:69-88
-  {
-    p.tau_m = 10.0;
-    p.c_m = 250.0;
-    p.t_ref = 2.0;
-    p.E_L = -70.0;
-    p.I_e = 0.0;
-    p.V_th = -55.0 - p.E_L;
-    p.V_min = -std::numeric_limits< double >::max();
-    p.V_reset = -70.0 - p.E_L;
-  }
-  else if ( model == M_IAF_PSC_EXP )  // models/iaf_psc_exp.cpp:71-94
-  {
-    p.tau_m = 10.0;
-    p.c_m = 250.0;
-    p.t_ref = 2.0;
-    p.E_L = -70.0;
-    p.I_e = 0.0;
-    p.V_th = -55.0 - p.E_L;
-    p.V_reset = -70.0 - p.E_L;
-    p.tau_ex = 2.0;
-    p.tau_in = 2.0;
-    p.rho = 0.01;
-    p.delta = 0.0;
-  }
-  else  // models/aeif_cond_exp.cpp:132-151
-  {
-    p.V_peak = 0.0;
-    p.V_reset = -60.0;
-    p.t_ref = 0.0;
-    p.g_L = 30.0;
-    p.c_m = 281.0;
-    p.E_ex = 0.0;
-    p.E_in = -85.0;
-    p.E_L = -70.6;
-    p.Delta_T = 2.0;
-    p.tau_w = 144.0;
-    p.a = 4.0;
-    p.b = 80.5;
-    p.V_th = -50.4;
-    p.tau_ex = 0.2;
-    p.tau_in = 2.0;
-    p.I_e = 0.0;
-    p.tol = 1e-6;
-  }
-  return p;
-}
-
-// Parameters_::set and State_::set of the three models: E_L first, then the values stored
-// relative to E_L (iaf), then the checks the reference makes.
+// ArchivingNode::set_status (archiving_node.cpp:238-257) for every neuron, then the model's
+// Parameters_::set and State_::set; a key that neither reads fails the call
 void
 Kernel::apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool )
 {
   std::set< std::string > used;
-  auto has = [ & ]( const char* k )
-  {
-    if ( d.has( k ) )
-    {
-      used.insert( k );
-      return true;
-    }
-    return false;
-  };
-  auto get = [ & ]( const char* k ) { return d.at( k ).number(); };
-  // ArchivingNode::set_status (archiving_node.cpp:238-257): the trace time constant of stdp
-  // synapses onto this neuron, checked first and set once the model's own checks pass
+  // the trace time constant of stdp synapses onto this neuron, checked first and set once the
+  // model's own checks pass
   double tau_minus = p.tau_minus;
-  if ( has( "tau_minus" ) )
+  if ( d.has( "tau_minus" ) )
   {
-    tau_minus = get( "tau_minus" );
+    used.insert( "tau_minus" );
+    tau_minus = d.at( "tau_minus" ).number();
     if ( tau_minus <= 0.0 )
       throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
   }
-  if ( p.model == M_AEIF_COND_EXP )
-  {
-    const char* keys[] = { "V_peak", "V_reset", "t_ref", "g_L", "C_m", "E_ex", "E_in", "E_L", "Delta_T", "tau_w", "a",
-      "b", "V_th", "tau_syn_ex", "tau_syn_in", "I_e", "gsl_error_tol" };
-    double* dst[] = { &p.V_peak, &p.V_reset, &p.t_ref, &p.g_L, &p.c_m, &p.E_ex, &p.E_in, &p.E_L, &p.Delta_T, &p.tau_w,
-      &p.a, &p.b, &p.V_th, &p.tau_ex, &p.tau_in, &p.I_e, &p.tol };
-    NeuronParams q = p;
-    double* qdst[] = { &q.V_peak, &q.V_reset, &q.t_ref, &q.g_L, &q.c_m, &q.E_ex, &q.E_in, &q.E_L, &q.Delta_T,
-      &q.tau_w, &q.a, &q.b, &q.V_th, &q.tau_ex, &q.tau_in, &q.I_e, &q.tol };
-    for ( size_t i = 0; i < sizeof( keys ) / sizeof( keys[ 0 ] ); ++i )
-      if ( has( keys[ i ] ) )
-        *qdst[ i ] = get( keys[ i ] );
-    (void) dst;
-    // Parameters_::set and State_::set (aeif_cond_exp.cpp:233-304), checked in that order;
-    // nothing is changed unless every check passes
-    if ( q.V_peak < q.V_th )
-      throw std::runtime_error( "BadProperty: V_peak >= V_th required." );
-    if ( q.Delta_T < 0. )
-      throw std::runtime_error( "BadProperty: Delta_T must be positive." );
-    if ( q.Delta_T > 0. )
-    {
-      // the exponential term must not overflow at spike time, with a margin of 1e20
-      const double max_exp_arg = std::log( std::numeric_limits< double >::max() / 1e20 );
-      if ( ( q.V_peak - q.V_th ) / q.Delta_T >= max_exp_arg )
-        throw std::runtime_error( "BadProperty: The current combination of V_peak, V_th and Delta_T will lead to "
-                                  "numerical overflow at spike time; try for instance to increase Delta_T or to "
-                                  "reduce V_peak to avoid this problem." );
-    }
-    if ( q.V_reset >= q.V_peak )
-      throw std::runtime_error( "BadProperty: Ensure that: V_reset < V_peak ." );
-    if ( q.c_m <= 0 )
-      throw std::runtime_error( "BadProperty: Ensure that C_m >0" );
-    if ( q.t_ref < 0 )
-      throw std::runtime_error( "BadProperty: Refractory time cannot be negative." );
-    if ( q.tau_ex <= 0 or q.tau_in <= 0 or q.tau_w <= 0 )
-      throw std::runtime_error( "BadProperty: All time constants must be strictly positive." );
-    if ( q.tol <= 0. )
-      throw std::runtime_error( "BadProperty: The gsl_error_tol must be strictly positive." );
-    const double g_ex = has( "g_ex" ) ? get( "g_ex" ) : s.v1;
-    const double g_in = has( "g_in" ) ? get( "g_in" ) : s.v2;
-    if ( g_ex < 0 or g_in < 0 )
-      throw std::runtime_error( "BadProperty: Conductances must not be negative." );
-    p = q;
-    if ( has( "V_m" ) )
-      s.v0 = get( "V_m" );
-    s.v1 = g_ex;
-    s.v2 = g_in;
-    if ( has( "w" ) )
-      s.v3 = get( "w" );
-  }
-  else
-  {
-    NeuronParams q = p;
-    const double ELold = q.E_L;
-    if ( has( "E_L" ) )
-      q.E_L = get( "E_L" );
-    const double delta_EL = q.E_L - ELold;
-    if ( has( "V_reset" ) )
-      q.V_reset = get( "V_reset" ) - q.E_L;
-    else
-      q.V_reset -= delta_EL;
-    if ( has( "V_th" ) )
-      q.V_th = get( "V_th" ) - q.E_L;
-    else
-      q.V_th -= delta_EL;
-    if ( q.model == M_IAF_PSC_DELTA )
-    {
-      if ( has( "V_min" ) )
-        q.V_min = get( "V_min" ) - q.E_L;
-      else
-        q.V_min -= delta_EL;
-    }
-    if ( has( "I_e" ) )
-      q.I_e = get( "I_e" );
-    if ( has( "C_m" ) )
-      q.c_m = get( "C_m" );
-    if ( has( "tau_m" ) )
-      q.tau_m = get( "tau_m" );
-    if ( q.model == M_IAF_PSC_EXP )
-    {
-      if ( has( "tau_syn_ex" ) )
-        q.tau_ex = get( "tau_syn_ex" );
-      if ( has( "tau_syn_in" ) )
-        q.tau_in = get( "tau_syn_in" );
-    }
-    if ( has( "t_ref" ) )
-      q.t_ref = get( "t_ref" );
-    if ( q.V_reset >= q.V_th )
-      throw std::runtime_error( "BadProperty: Reset potential must be smaller than threshold." );
-    if ( q.model == M_IAF_PSC_EXP )
-    {
-      // iaf_psc_exp::Parameters_::set (iaf_psc_exp.cpp:149-164)
-      if ( q.c_m <= 0 )
-        throw std::runtime_error( "BadProperty: Capacitance must be strictly positive." );
-      if ( q.tau_m <= 0 or q.tau_ex <= 0 or q.tau_in <= 0 )
-        throw std::runtime_error( "BadProperty: Membrane and synapse time constants must be strictly positive." );
-      if ( q.t_ref < 0 )
-        throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
-    }
-    else
-    {
-      // iaf_psc_delta::Parameters_::set (iaf_psc_delta.cpp:148-163)
-      if ( q.c_m <= 0 )
-        throw std::runtime_error( "BadProperty: Capacitance must be > 0." );
-      if ( q.t_ref < 0 )
-        throw std::runtime_error( "BadProperty: Refractory time must not be negative." );
-      if ( q.tau_m <= 0 )
-        throw std::runtime_error( "BadProperty: Membrane time constant must be > 0." );
-    }
-    if ( q.model == M_IAF_PSC_DELTA and has( "refractory_input" ) )
-      q.refr_input = d.at( "refractory_input" ).number() != 0;
-    if ( q.model == M_IAF_PSC_EXP )
-    {
-      if ( has( "rho" ) )
-        q.rho = get( "rho" );
-      if ( has( "delta" ) )
-        q.delta = get( "delta" );
-      if ( q.rho < 0 )
-        throw std::runtime_error( "BadProperty: Stochastic firing intensity must not be negative." );
-      if ( q.delta < 0 )
-        throw std::runtime_error( "BadProperty: Width of threshold region must not be negative." );
-      if ( q.delta > 1e-10 )
-        throw std::runtime_error( "NotImplemented: stochastic threshold (delta > 0) is not supported on the device path." );
-    }
-    p = q;
-    if ( has( "V_m" ) )
-      s.v0 = get( "V_m" ) - p.E_L;
-    else
-      s.v0 -= delta_EL;
-  }
+  ModelManager::instance().get( p.model ).set_status( p, s, d, used );
   for ( const auto& kv : d.obj )
     if ( not used.count( kv.first ) )
       throw std::runtime_error( "UnaccessedDictionaryEntry: '" + kv.first + "' is not a parameter of this model" );
@@ -974,19 +789,13 @@ Kernel::create( const std::string& model, int64_t n, const Json& params )
   if ( simulated_ and model != "spike_recorder" and model != "multimeter" and model != "voltmeter" )
     throw std::runtime_error( "NotImplemented: creating nodes after Simulate is not supported on the device path." );
   const int64_t first = static_cast< int64_t >( nodes_.size() ) + 1;
-  int model_id = -1;
-  if ( model == "iaf_psc_delta" )
-    model_id = M_IAF_PSC_DELTA;
-  else if ( model == "iaf_psc_exp" )
-    model_id = M_IAF_PSC_EXP;
-  else if ( model == "aeif_cond_exp" )
-    model_id = M_AEIF_COND_EXP;
-  if ( model_id >= 0 )
+  if ( const NodeModel* M = ModelManager::instance().find( model ) )
   {
-    NeuronParams p = default_params( model_id );
+    NeuronParams p {};
     NeuronState s {};
-    if ( model_id == M_AEIF_COND_EXP )
-      s.v0 = p.E_L;  // State_( const Parameters_& ): y[V_M] = E_L
+    p.model = M->id();
+    p.tau_minus = 20.0;  // ArchivingNode (archiving_node.cpp:40)
+    M->defaults( p, s );
     apply_params( p, s, params, true );
     for ( int64_t i = 0; i < n; ++i )
     {
@@ -1314,34 +1123,39 @@ Kernel::collect_samples()
   {
     if ( m.targets.empty() or m.record_from.empty() )
       continue;
-    std::vector< int > fidx;
-    for ( const std::string& v : m.record_from )
-      fidx.push_back( v == "V_m" ? 0 : ( v == "I_syn_ex" or v == "g_ex" ) ? 1 : ( v == "I_syn_in" or v == "g_in" ) ? 2 : 3 );
+    // per target: the device field of each recordable and the offset its model adds (the
+    // iaf models keep V_m relative to E_L)
+    const size_t nf = m.record_from.size();
     std::vector< size_t > tcol;  // columns of the targets
+    std::vector< int > tfield;
+    std::vector< double > toff;
     for ( int64_t g : m.targets )
     {
       const auto it = std::lower_bound( mm_col_gid_.begin(), mm_col_gid_.end(), g );
-      if ( it != mm_col_gid_.end() and *it == g )
-        tcol.push_back( static_cast< size_t >( it - mm_col_gid_.begin() ) );
+      if ( it == mm_col_gid_.end() or *it != g )
+        continue;
+      tcol.push_back( static_cast< size_t >( it - mm_col_gid_.begin() ) );
+      const NeuronParams& p = params_[ node( g ).idx ];
+      const NodeModel& M = ModelManager::instance().get( p.model );
+      for ( const std::string& v : m.record_from )
+      {
+        const int f = M.state_field( v );
+        tfield.push_back( f < 0 ? 0 : f );
+        toff.push_back( M.state_offset( p, f ) );
+      }
     }
     for ( size_t row = 0; row < stamps.size(); ++row )
     {
       const int64_t stamp = stamps[ row ];
       if ( m.window.set and not m.window.active( stamp ) )
         continue;
-      for ( size_t c : tcol )
+      for ( size_t k = 0; k < tcol.size(); ++k )
       {
-        const int64_t g = mm_col_gid_[ c ];
-        const NeuronParams& p = params_[ node( g ).idx ];
-        m.ev_gid.push_back( g );
+        const size_t c = tcol[ k ];
+        m.ev_gid.push_back( mm_col_gid_[ c ] );
         m.ev_stamp.push_back( stamp );
-        for ( int f : fidx )
-        {
-          double v = data[ ( row * cols + c ) * 4 + f ];
-          if ( f == 0 and p.model != M_AEIF_COND_EXP )
-            v += p.E_L;  // the iaf models keep V_m relative to E_L
-          m.ev_vals.push_back( v );
-        }
+        for ( size_t j = 0; j < nf; ++j )
+          m.ev_vals.push_back( data[ ( row * cols + c ) * 4 + tfield[ k * nf + j ] ] + toff[ k * nf + j ] );
       }
     }
   }
@@ -1422,7 +1236,7 @@ Kernel::set_values( const std::vector< int64_t >& gids, const std::string& key, 
     if ( key == "V_m" and live )
     {
       const int dev = gid_to_dev_[ gids[ i ] ];
-      engine_->set_state( 0, static_cast< uint32_t >( dev ), p.model == M_AEIF_COND_EXP ? v[ i ] : v[ i ] - p.E_L );
+      engine_->set_state( 0, static_cast< uint32_t >( dev ), v[ i ] - ModelManager::instance().get( p.model ).state_offset( p, 0 ) );
       continue;
     }
     Json d;
@@ -1482,20 +1296,8 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       throw std::runtime_error( "BadProperty: '" + key + "' is not a property of " + model_name( gids[ i ] ) );
     const NeuronParams& p = params_[ r.idx ];
     const NeuronState& s = state_[ r.idx ];
-    const bool aeif = p.model == M_AEIF_COND_EXP;
-    int field = -1;
-    if ( key == "V_m" )
-      field = 0;
-    else if ( key == "I_syn_ex" and p.model == M_IAF_PSC_EXP )
-      field = 1;
-    else if ( key == "I_syn_in" and p.model == M_IAF_PSC_EXP )
-      field = 2;
-    else if ( key == "g_ex" and aeif )
-      field = 1;
-    else if ( key == "g_in" and aeif )
-      field = 2;
-    else if ( key == "w" and aeif )
-      field = 3;
+    const NodeModel& M = ModelManager::instance().get( p.model );
+    const int field = M.state_field( key );
     if ( field >= 0 or key == "refractory_steps" )
     {
       double v;
@@ -1525,55 +1327,18 @@ Kernel::get_values( const std::vector< int64_t >& gids, const std::string& key, 
       if ( field < 0 )
         out[ i ] = rr;
       else
-        out[ i ] = ( field == 0 and not aeif ) ? v + p.E_L : v;
+        out[ i ] = v + M.state_offset( p, field );
       continue;
     }
-    if ( key == "E_L" )
-      out[ i ] = p.E_L;
-    else if ( key == "V_th" )
-      out[ i ] = aeif ? p.V_th : p.V_th + p.E_L;
-    else if ( key == "V_reset" )
-      out[ i ] = aeif ? p.V_reset : p.V_reset + p.E_L;
-    else if ( key == "C_m" )
-      out[ i ] = p.c_m;
-    else if ( key == "tau_m" and not aeif )
-      out[ i ] = p.tau_m;
-    else if ( key == "t_ref" )
-      out[ i ] = p.t_ref;
+    double pv;
+    if ( M.get_param( p, key, pv ) )
+      out[ i ] = pv;
     else if ( key == "tau_minus" )
       out[ i ] = p.tau_minus;
-    else if ( key == "I_e" )
-      out[ i ] = p.I_e;
-    else if ( key == "tau_syn_ex" and p.model != M_IAF_PSC_DELTA )
-      out[ i ] = p.tau_ex;
-    else if ( key == "tau_syn_in" and p.model != M_IAF_PSC_DELTA )
-      out[ i ] = p.tau_in;
     else if ( key == "global_id" )
       out[ i ] = static_cast< double >( gids[ i ] );
     else if ( key == "vp" or key == "thread" )
       out[ i ] = vp_of( gids[ i ] );  // one process: the node's thread is its VP (vp_manager_impl.h:43-52)
-    else if ( key == "refractory_input" and p.model == M_IAF_PSC_DELTA )
-      out[ i ] = p.refr_input;
-    else if ( key == "V_min" and p.model == M_IAF_PSC_DELTA )
-      out[ i ] = p.V_min + p.E_L;
-    else if ( aeif and key == "V_peak" )
-      out[ i ] = p.V_peak;
-    else if ( aeif and key == "g_L" )
-      out[ i ] = p.g_L;
-    else if ( aeif and key == "E_ex" )
-      out[ i ] = p.E_ex;
-    else if ( aeif and key == "E_in" )
-      out[ i ] = p.E_in;
-    else if ( aeif and key == "Delta_T" )
-      out[ i ] = p.Delta_T;
-    else if ( aeif and key == "tau_w" )
-      out[ i ] = p.tau_w;
-    else if ( aeif and key == "a" )
-      out[ i ] = p.a;
-    else if ( aeif and key == "b" )
-      out[ i ] = p.b;
-    else if ( aeif and key == "gsl_error_tol" )
-      out[ i ] = p.tol;
     else
       throw std::runtime_error( "BadProperty: unknown property '" + key + "'" );
   }
@@ -1779,14 +1544,13 @@ Kernel::connect( const std::vector< int64_t >& pre, const std::vector< int64_t >
       throw std::runtime_error( "IllegalConnection: multimeter connections use one_to_one or all_to_all" );
     for ( size_t ti = 0; ti < post.size(); ++ti )
     {
-      const int model = params_[ node( post[ ti ] ).idx ].model;
+      const std::vector< std::string > rec =
+        ModelManager::instance().get( params_[ node( post[ ti ] ).idx ].model ).recordables();
       const std::vector< int64_t > meters = rule == "one_to_one" ? std::vector< int64_t > { pre[ ti ] } : pre;
       for ( int64_t mg : meters )
         for ( const std::string& v : meters_[ node( mg ).idx ].record_from )
         {
-          const bool ok = v == "V_m"
-            or ( model == M_IAF_PSC_EXP and ( v == "I_syn_ex" or v == "I_syn_in" ) )
-            or ( model == M_AEIF_COND_EXP and ( v == "g_ex" or v == "g_in" or v == "w" ) );
+          const bool ok = std::find( rec.begin(), rec.end(), v ) != rec.end();
           if ( not ok )
             throw std::runtime_error( "IllegalConnection: Cannot connect with unknown recordable " + v );
         }
@@ -2386,10 +2150,9 @@ Kernel::estimate_device_bytes() const
   E += host_connections_freed_ ? freed_connections_ : 0;
   const long dmin = simulated_ ? min_delay_ : ( d_min_seen_ < 0 ? 1 : d_min_seen_ );
   const long dmax = simulated_ ? max_delay_ : std::max( d_max_seen_, 1L );
-  bool all_delta = true;
+  int nch = 1;
   for ( const auto& Pm : params_ )
-    all_delta = all_delta and Pm.model == M_IAF_PSC_DELTA;
-  const int nch = all_delta ? 1 : 2;
+    nch = std::max( nch, ModelManager::instance().get( Pm.model ).channels() );
   uint32_t tb = 0;
   bool win = false;
   try

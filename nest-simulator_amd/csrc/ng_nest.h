@@ -21,6 +21,7 @@
 #include "ng_engine.h"
 #include "ng_philox.h"
 #include "ng_json.h"
+#include "ng_model.h"
 
 namespace nestgpu
 {
@@ -290,23 +291,6 @@ private:
   std::vector< VpCursor > cursor_;
 };
 
-struct NeuronParams
-{
-  int32_t model;
-  double tau_m, c_m, t_ref, E_L, I_e, V_th, V_min, V_reset;  // iaf: V values relative to E_L
-  double tau_ex, tau_in;
-  double V_peak, g_L, E_ex, E_in, Delta_T, tau_w, a, b, tol;  // aeif: absolute
-  double delta, rho;
-  double tau_minus;  // ArchivingNode: trace time constant of incoming stdp synapses
-  int32_t refr_input;
-};
-
-struct NeuronState
-{
-  double v0, v1, v2, v3, v4;  // delta: y3 | exp: V, i_ex, i_in | aeif: V, g_ex, g_in, w, h_int
-  int32_t r;
-};
-
 // activity window of a device (Device::Parameters_, device.cpp:83-118) in steps: a spike with
 // stamp origin + start < stamp <= origin + stop is recorded or emitted
 // (RecordingDevice::is_active, recording_device.cpp:206-211; StimulationDevice::is_active,
@@ -530,7 +514,6 @@ private:
     return static_cast< int >( gid % n_vp_ );
   }
   void init_rngs();
-  static NeuronParams default_params( int model );
   void apply_params( NeuronParams& p, NeuronState& s, const Json& d, bool creating );
   void check_delay( long steps );
   void single_connect( int64_t s, int64_t t, int vp, VpRng& rng, Param* wp, Param* dp, uint16_t syn = 0 );

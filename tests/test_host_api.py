@@ -702,3 +702,21 @@ def test_stdp_synapse_connect_checks():
             k.SetKernelStatus({"stdp_history_capacity": bad})
     k.SetKernelStatus({"stdp_history_capacity": 1024})
     assert k.GetKernelStatus("stdp_history_capacity") == 1024
+
+
+def test_model_registry_status():
+    # ModelManager::get_status (model_manager.cpp:116-131): the node and synapse models
+    k = nestgpu.Kernel(0)
+    st = k.GetKernelStatus()
+    assert st["node_models"][:3] == ["iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp"]
+    assert set(st["node_models"][3:]) == {"spike_generator", "spike_recorder", "multimeter", "voltmeter"}
+    assert st["synapse_models"] == ["static_synapse", "stdp_synapse"]
+    # every registered neuron model is created through the registry, with its own defaults
+    # and keys
+    for name, key, val in (("iaf_psc_delta", "V_min", -1.7976931348623157e308), ("iaf_psc_exp", "tau_syn_ex", 2.0),
+                           ("aeif_cond_exp", "gsl_error_tol", 1e-6)):
+        n = k.Create(name, 2)
+        assert np.all(np.atleast_1d(n.get(key)) == val)
+        assert np.all(np.atleast_1d(n.get("tau_minus")) == 20.0)
+    with pytest.raises(nestgpu.NESTError, match="UnknownModelName|unknown"):
+        k.Create("hh_psc_alpha", 1)
