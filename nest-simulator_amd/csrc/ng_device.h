@@ -15,13 +15,18 @@
 namespace nestgpu
 {
 
+// device ids of the built-in neuron models (ModelManager registers them in this order)
 enum ModelId : int32_t
 {
   M_IAF_PSC_DELTA = 0,
   M_IAF_PSC_EXP = 1,
   M_AEIF_COND_EXP = 2,
-  N_MODELS = 3
+  M_IAF_PSC_ALPHA = 3,
+  N_MODELS = 4
 };
+// launch mask: bit id for each neuron model of the launch, and this bit when an instance has
+// stdp_synapse connections
+constexpr int MASK_STDP = 1 << 30;
 
 // Per parameter-set constants, computed on the host in pre_run_hook order with glibc libm.
 // Field meaning per model:

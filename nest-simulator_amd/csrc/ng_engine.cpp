@@ -143,7 +143,7 @@ Engine::build_layout( const BuildSpec& b, bool keep_state, bool relayout )
   for ( const auto& ps : b.psets )
     mask_ |= 1 << ps.model;
   if ( not b.stdp_prm.empty() )
-    mask_ |= 8;  // the launch needs the stdp_synapse instantiation of the slice kernel
+    mask_ |= MASK_STDP;  // the launch needs the stdp_synapse instantiation of the slice kernel
   // ---- store and inputs (replaced on every build)
   net_.boff = static_cast< const uint32_t* >( S( upload( b.boff ) ) );
   net_.syn = static_cast< const SynRec* >( S( upload( b.syn ) ) );

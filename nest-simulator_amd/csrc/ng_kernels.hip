@@ -594,7 +594,7 @@ stdp_spikes( const NetDev& d, uint32_t dev, int64_t origin, uint32_t par, const 
 // STDP: some instance of the launch has stdp_synapse connections (its own instantiation, so
 // that networks without them carry none of that code or its registers)
 template < int MASK, bool WIN, bool STDP = false >
-__global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
+__global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & WIDE_MODELS ) ? 1 : 4 )
   k_slice( const NetDev* __restrict__ nets, const uint32_t* __restrict__ blk_prefix, int n_nets, int s )
 {
 #ifdef NG_PROFILE
@@ -832,11 +832,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
               const double w = __hiloint2double( static_cast< int >( hr[ q ].w_hi ), static_cast< int >( hr[ q ].w_lo ) );
               const uint32_t local = hr[ q ].word & 0xffffu;
               const uint32_t slot_rel = hl[ q ] + ( ( hr[ q ].word >> 16 ) & DELAY_MASK );
-              const int m = MASK == 1 ? M_IAF_PSC_DELTA
-                : MASK == 2           ? M_IAF_PSC_EXP
-                : MASK == 4           ? M_AEIF_COND_EXP
-                : umodel >= 0         ? umodel
-                                      : glob( d.model )[ dev0 + local ];
+              const int m = single_model< MASK >() >= 0 ? single_model< MASK >()
+                : umodel >= 0                           ? umodel
+                                                        : glob( d.model )[ dev0 + local ];
               uint32_t ch;
               double v;
               node_receive( m, w, ch, v );
@@ -911,11 +909,9 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
                   ( ( rec[ q ].word >> 16 ) & DELAY_MASK ) + static_cast< uint32_t >( d.min_delay ), dpar );
             }
             const uint32_t slot_rel = rlag[ q ] + ( ( rec[ q ].word >> 16 ) & DELAY_MASK );
-            const int m = MASK == 1 ? M_IAF_PSC_DELTA
-              : MASK == 2           ? M_IAF_PSC_EXP
-              : MASK == 4           ? M_AEIF_COND_EXP
-              : umodel >= 0         ? umodel
-                                    : glob( d.model )[ dev0 + local ];
+            const int m = single_model< MASK >() >= 0 ? single_model< MASK >()
+              : umodel >= 0                           ? umodel
+                                                      : glob( d.model )[ dev0 + local ];
             uint32_t ch;
             double v;
             node_receive( m, w, ch, v );  // static_synapse: weight * multiplicity 1
@@ -1085,10 +1081,7 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     const uint32_t dl = ( local - nb ) % nb, g = ( local - nb ) / nb, ddev = dev0 + dl;
     {
       gref< const ParamSet > dps = glob( d.psets )[ glob( d.pidx )[ ddev ] ];
-      const int dm = ( MASK == 1 ) ? M_IAF_PSC_DELTA
-        : ( MASK == 2 )            ? M_IAF_PSC_EXP
-        : ( MASK == 4 )            ? M_AEIF_COND_EXP
-                                   : dps.model;
+      const int dm = single_model< MASK >() >= 0 ? single_model< MASK >() : dps.model;
       drive_adds( d, dps, dm, ddev, dl, C, ph.origin, ph.from + static_cast< int >( g ), static_cast< int >( G ), ph.to );
     }
   }
@@ -1098,16 +1091,13 @@ __global__ void __launch_bounds__( Tile< WIN >::BS, ( MASK & 4 ) ? 1 : 4 )
     // multimeter: the next step of this part whose state is sampled (ph.to: none)
     const uint32_t mcol = d.mm_int > 0 ? glob( d.mm_col )[ dev ] : NONE;
     int next_sample = mcol != NONE ? first_sample_lag( d, ph.origin, ph.from, ph.to ) : ph.to;
-    const int m = ( MASK == 1 ) ? M_IAF_PSC_DELTA
-      : ( MASK == 2 )           ? M_IAF_PSC_EXP
-      : ( MASK == 4 )           ? M_AEIF_COND_EXP
-                                : ps.model;
-    if ( ( MASK & 1 ) and m == M_IAF_PSC_DELTA )
-      DeviceNode< M_IAF_PSC_DELTA >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
-    if ( ( MASK & 2 ) and m == M_IAF_PSC_EXP )
-      DeviceNode< M_IAF_PSC_EXP >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
-    if ( ( MASK & 4 ) and m == M_AEIF_COND_EXP )
-      DeviceNode< M_AEIF_COND_EXP >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
+    const int m = single_model< MASK >() >= 0 ? single_model< MASK >() : ps.model;
+    // Node::update of the neuron's model (each model of the launch's mask is compiled in)
+#define NG_UPDATE( ID )                                                                 \
+  if ( ( MASK & ( 1 << ( ID ) ) ) and m == ( ID ) )                                   \
+    DeviceNode< ( ID ) >::update( d, ps, dev, local, C, ph, bits, mcol, next_sample );
+    NG_DEVICE_MODELS( NG_UPDATE )
+#undef NG_UPDATE
     if ( G == 0 )
       drive_adds( d, ps, m, dev, local, C, ph.origin, ph.from, 1, ph.to );
   }
@@ -1277,26 +1267,22 @@ static void
 launch_mask( const NetDev* nets, const uint32_t* blk_prefix, int n_nets, uint32_t total_blocks, int mask, int s,
   size_t dyn_lds, hipStream_t st )
 {
-  // mask: the models of the launch (bits 0-2) and stdp_synapse (bit 3)
-  const bool stdp = ( mask & 8 ) != 0;
+  // mask: one bit per model id of the launch, and MASK_STDP for stdp_synapse. A launch of one
+  // model takes that model's own instantiation, any other the one with every model.
+  const bool stdp = ( mask & MASK_STDP ) != 0;
+  const int models = mask & ALL_MODELS;
 #define NG_LAUNCH( M )                                                                                        \
   ( stdp ? k_slice< M, WIN, true ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s ) \
          : k_slice< M, WIN, false ><<< total_blocks, Tile< WIN >::BS, dyn_lds, st >>>( nets, blk_prefix, n_nets, s ) )
-  switch ( mask & 7 )
-  {
-  case 1:
-    NG_LAUNCH( 1 );
-    break;
-  case 2:
-    NG_LAUNCH( 2 );
-    break;
-  case 4:
-    NG_LAUNCH( 4 );
-    break;
-  default:
-    NG_LAUNCH( 7 );
-    break;
+#define NG_ONE( ID )                 \
+  if ( models == ( 1 << ( ID ) ) ) \
+  {                                  \
+    NG_LAUNCH( 1 << ( ID ) );        \
+    return;                          \
   }
+  NG_DEVICE_MODELS( NG_ONE )
+  NG_LAUNCH( ALL_MODELS );
+#undef NG_ONE
 #undef NG_LAUNCH
 }
 
@@ -1308,8 +1294,8 @@ slice_lds_for_occupancy( int wg_per_cu, bool win )
   if ( wg_per_cu <= 0 )
     return 0;
   hipFuncAttributes a;
-  const hipError_t e = win ? hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< 7, true > ) )
-                           : hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< 7, false > ) );
+  const hipError_t e = win ? hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< ALL_MODELS, true > ) )
+                           : hipFuncGetAttributes( &a, reinterpret_cast< const void* >( &k_slice< ALL_MODELS, false > ) );
   if ( e != hipSuccess )
     return 0;
   const size_t lds_per_cu = 160 * 1024;

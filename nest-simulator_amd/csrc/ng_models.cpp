@@ -1,7 +1,8 @@
 // ng_models.cpp -- the host halves of the built-in neuron models and the model registry.
 //
-// iaf_psc_delta (models/iaf_psc_delta.cpp:69-270), iaf_psc_exp (models/iaf_psc_exp.cpp:71-286)
-// and aeif_cond_exp (models/aeif_cond_exp.cpp:132-435): defaults, Parameters_::set /
+// iaf_psc_delta (models/iaf_psc_delta.cpp:69-270), iaf_psc_exp (models/iaf_psc_exp.cpp:71-286),
+// aeif_cond_exp (models/aeif_cond_exp.cpp:132-435) and iaf_psc_alpha
+// (models/iaf_psc_alpha.cpp:68-301): defaults, Parameters_::set /
 // State_::set with the reference's checks and messages, get_status keys, recordables and
 // pre_run_hook. The iaf models keep voltages relative to E_L, as the reference does.
 #include <cmath>
@@ -57,7 +58,39 @@ iaf_propagator_exp( double tau_syn, double tau_m, double c_m, double h )
   return h * inv_c_m * exp_h_tau;
 }
 
-// the parts the two iaf models share: voltages relative to E_L, set in E_L-first order
+// IAFPropagatorAlpha::evaluate (libnestutil/iaf_propagator.cpp:94-116): P31 and P32, the
+// regular form above h_min_regular = 1e-7 tau_m^2 / |tau_m - tau_syn|
+// (iaf_propagator.h:62-66), else their singular limits
+void
+iaf_propagator_alpha( double tau_syn, double tau_m, double c_m, double h, double& P31, double& P32 )
+{
+  const double h_min_regular = 1e-7 * tau_m * tau_m / std::abs( tau_m - tau_syn );
+  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
+  const double gamma = beta / c_m;
+  const double inv_tau_syn = 1 / tau_syn;
+  const double inv_tau_m = 1 / tau_m;
+  const double inv_c_m = 1 / c_m;
+  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
+  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
+  const double expm1_h_tau = std::expm1( h * inv_beta );
+  P32 = gamma * exp_h_tau_syn * expm1_h_tau;
+  double exp_h_tau = std::numeric_limits< double >::quiet_NaN();
+  if ( not( std::isnormal( P32 ) and P32 > 0 ) )
+  {
+    exp_h_tau = std::exp( -h * inv_tau_m );
+    P32 = h * inv_c_m * exp_h_tau;
+  }
+  if ( h > h_min_regular )
+  {
+    P31 = gamma * exp_h_tau_syn * ( beta * expm1_h_tau - h );
+    return;
+  }
+  if ( std::isnan( exp_h_tau ) )
+    exp_h_tau = std::exp( -h * inv_tau_m );
+  P31 = 0.5 * h * h * inv_c_m * exp_h_tau;
+}
+
+// the parts the iaf models share: voltages relative to E_L, set in E_L-first order
 class IafModel : public NodeModel
 {
 public:
@@ -321,6 +354,107 @@ public:
   }
 };
 
+// models/iaf_psc_alpha.cpp:68-301: alpha-shaped synaptic currents, exact integration. The
+// state fields are V_m - E_L, I_syn_ex, I_syn_in, dI_syn_ex, dI_syn_in; the current input
+// y0 of CurrentEvents is zero on this path (no current sources).
+class IafPscAlpha : public IafModel
+{
+public:
+  IafPscAlpha()
+    : IafModel( "iaf_psc_alpha", M_IAF_PSC_ALPHA, 2 )
+  {
+  }
+  void
+  defaults( NeuronParams& p, NeuronState& s ) const override
+  {
+    // Parameters_() (iaf_psc_alpha.cpp:68-80)
+    p.tau_m = 10.0;
+    p.c_m = 250.0;
+    p.t_ref = 2.0;
+    p.E_L = -70.0;
+    p.I_e = 0.0;
+    p.V_reset = -70.0 - p.E_L;
+    p.V_th = -55.0 - p.E_L;
+    p.V_min = -std::numeric_limits< double >::infinity();
+    p.tau_ex = 2.0;
+    p.tau_in = 2.0;
+    s = NeuronState {};
+  }
+  void
+  set_status( NeuronParams& p, NeuronState& s, const Json& d, std::set< std::string >& used ) const override
+  {
+    // Parameters_::set (iaf_psc_alpha.cpp:112-180), State_::set (:188-199)
+    const Keys k { d, used };
+    NeuronParams q = p;
+    const double delta_EL = set_common( q, k );
+    if ( k.has( "V_min" ) )
+      q.V_min = k.get( "V_min" ) - q.E_L;
+    else
+      q.V_min -= delta_EL;
+    set_membrane( q, k );
+    if ( k.has( "tau_syn_ex" ) )
+      q.tau_ex = k.get( "tau_syn_ex" );
+    if ( k.has( "tau_syn_in" ) )
+      q.tau_in = k.get( "tau_syn_in" );
+    if ( k.has( "t_ref" ) )
+      q.t_ref = k.get( "t_ref" );
+    if ( q.c_m <= 0.0 )
+      throw std::runtime_error( "BadProperty: Capacitance must be > 0." );
+    if ( q.tau_m <= 0.0 )
+      throw std::runtime_error( "BadProperty: Membrane time constant must be > 0." );
+    if ( q.tau_ex <= 0.0 or q.tau_in <= 0.0 )
+      throw std::runtime_error( "BadProperty: All synaptic time constants must be > 0." );
+    if ( q.t_ref < 0.0 )
+      throw std::runtime_error( "BadProperty: The refractory time t_ref can't be negative." );
+    if ( q.V_reset >= q.V_th )
+      throw std::runtime_error( "BadProperty: Reset potential must be smaller than threshold." );
+    p = q;
+    set_v_m( p, s, k, delta_EL );
+  }
+  bool
+  get_param( const NeuronParams& p, const std::string& key, double& v ) const override
+  {
+    if ( key == "V_min" )
+      v = p.V_min + p.E_L;
+    else if ( key == "tau_syn_ex" )
+      v = p.tau_ex;
+    else if ( key == "tau_syn_in" )
+      v = p.tau_in;
+    else
+      return IafModel::get_param( p, key, v );
+    return true;
+  }
+  int
+  state_field( const std::string& key ) const override
+  {
+    return key == "V_m" ? 0 : key == "I_syn_ex" ? 1 : key == "I_syn_in" ? 2 : -1;
+  }
+  std::vector< std::string >
+  recordables() const override
+  {
+    return { "V_m", "I_syn_ex", "I_syn_in" };
+  }
+  void
+  pre_run_hook( const NeuronParams& p, double h, ParamSet& ps ) const override
+  {
+    // iaf_psc_alpha::pre_run_hook (iaf_psc_alpha.cpp:248-301), in its order
+    const double P11ex = std::exp( -h / p.tau_ex );
+    const double P11in = std::exp( -h / p.tau_in );
+    const double expm1_tau_m = std::expm1( -h / p.tau_m );
+    const double P30 = -p.tau_m / p.c_m * std::expm1( -h / p.tau_m );
+    const double P21ex = h * P11ex;
+    const double P21in = h * P11in;
+    double P31ex, P32ex, P31in, P32in;
+    iaf_propagator_alpha( p.tau_ex, p.tau_m, p.c_m, h, P31ex, P32ex );
+    iaf_propagator_alpha( p.tau_in, p.tau_m, p.c_m, h, P31in, P32in );
+    const double e = 2.718281828459045;  // numerics::e
+    const double vals[] = { P30, 0.0 + p.I_e, P31ex, P32ex, P31in, P32in, expm1_tau_m, P21ex, P11ex, P21in, P11in,
+      1.0 * e / p.tau_ex, 1.0 * e / p.tau_in, p.V_th, p.V_reset, p.V_min };
+    for ( int q = 0; q < 16; ++q )
+      ps.p[ q ] = vals[ q ];
+  }
+};
+
 class AeifCondExp : public NodeModel
 {
 public:
@@ -480,6 +614,7 @@ ModelManager::ModelManager()
   register_node_model< IafPscDelta >();
   register_node_model< IafPscExp >();
   register_node_model< AeifCondExp >();
+  register_node_model< IafPscAlpha >();
 }
 
 ModelManager&

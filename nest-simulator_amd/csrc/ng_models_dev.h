@@ -20,9 +20,26 @@ namespace nestgpu
 template < int ID >
 struct DeviceNode;
 
+// The device model ids, in the order ModelManager registers them: a model with a DeviceNode
+// specialization joins the slice kernel by its entry here. WIDE marks models whose update needs
+// more than 128 VGPRs (their launches run one workgroup per CU).
+#define NG_DEVICE_MODELS( X ) X( M_IAF_PSC_DELTA ) X( M_IAF_PSC_EXP ) X( M_AEIF_COND_EXP ) X( M_IAF_PSC_ALPHA )
+constexpr int N_DEVICE_MODELS = 4;
+constexpr int ALL_MODELS = ( 1 << N_DEVICE_MODELS ) - 1;
+static_assert( ALL_MODELS < MASK_STDP, "model bits must stay below the stdp bit" );
+
+// the one model id of a launch mask with a single model bit, else -1
+template < int MASK >
+__device__ constexpr int
+single_model()
+{
+  return ( MASK & ( MASK - 1 ) ) == 0 ? __builtin_ctz( MASK ) : -1;
+}
+
 template <>
 struct DeviceNode< M_IAF_PSC_DELTA >
 {
+  static constexpr bool WIDE = false;
   // iaf_psc_delta::handle( SpikeEvent& ) (iaf_psc_delta.cpp:332-341): one channel, the weight
   __device__ __forceinline__ static void
   receive( double w, uint32_t& ch, double& v )
@@ -97,6 +114,7 @@ struct DeviceNode< M_IAF_PSC_DELTA >
 template <>
 struct DeviceNode< M_IAF_PSC_EXP >
 {
+  static constexpr bool WIDE = false;
   // iaf_psc_exp::handle( SpikeEvent& ) (iaf_psc_exp.cpp:351-362): positive weights to the
   // excitatory channel (1), others to the inhibitory one (0), signed
   __device__ __forceinline__ static void
@@ -166,6 +184,7 @@ struct DeviceNode< M_IAF_PSC_EXP >
 template <>
 struct DeviceNode< M_AEIF_COND_EXP >
 {
+  static constexpr bool WIDE = true;  // the RKF45 stepper's state and coefficients
   // aeif_cond_exp::handle( SpikeEvent& ) (aeif_cond_exp.cpp:526-541): positive weights to the
   // excitatory conductance (0), others negated to the inhibitory one (1)
   __device__ __forceinline__ static void
@@ -282,16 +301,108 @@ struct DeviceNode< M_AEIF_COND_EXP >
   }
 };
 
+template <>
+struct DeviceNode< M_IAF_PSC_ALPHA >
+{
+  static constexpr bool WIDE = false;
+  // iaf_psc_alpha::handle( SpikeEvent& ) (iaf_psc_alpha.cpp:373-383): positive weights to
+  // the excitatory channel (1), others to the inhibitory one (0), signed
+  __device__ __forceinline__ static void
+  receive( double w, uint32_t& ch, double& v )
+  {
+    ch = w > 0 ? 1u : 0u;
+    v = w;
+  }
+  template < bool WIN >
+  __device__ __forceinline__ static void
+  update( const NetDev& d, gref< const ParamSet > ps, uint32_t dev, uint32_t local, const Cells< WIN >& C,
+    const SlicePhase& ph, uint64_t* bits, uint32_t mcol, int next_sample )
+  {
+    // models/iaf_psc_alpha.cpp:305-370 (the current input y0 is zero: no current sources)
+    const double P30 = ps.p[ 0 ], y0I = ps.p[ 1 ], P31ex = ps.p[ 2 ], P32ex = ps.p[ 3 ], P31in = ps.p[ 4 ],
+                 P32in = ps.p[ 5 ], expm1_tau_m = ps.p[ 6 ], P21ex = ps.p[ 7 ], P11ex = ps.p[ 8 ],
+                 P21in = ps.p[ 9 ], P11in = ps.p[ 10 ], EPSCInit = ps.p[ 11 ], IPSCInit = ps.p[ 12 ],
+                 Theta = ps.p[ 13 ], V_reset = ps.p[ 14 ], LowerBound = ps.p[ 15 ];
+    double V = glob( d.v[ 0 ] )[ dev ];
+    double iex = glob( d.v[ 1 ] )[ dev ];
+    double iin = glob( d.v[ 2 ] )[ dev ];
+    double diex = glob( d.v[ 3 ] )[ dev ];
+    double diin = glob( d.v[ 4 ] )[ dev ];
+    int32_t r = glob( d.r )[ dev ];
+    // ring-buffer cells read LA lags at a time (its 16 propagators and 5 state variables
+    // leave fewer registers for them than the other iaf models)
+    constexpr int LA = 4;
+    for ( int l0 = ph.from; l0 < ph.to; l0 += LA )
+    {
+      double w_in[ LA ], w_ex[ LA ];
+#pragma unroll
+      for ( int q = 0; q < LA; ++q )
+        if ( l0 + q < ph.to )
+        {
+          w_in[ q ] = C.get( l0 + q, 0, local );
+          w_ex[ q ] = C.get( l0 + q, 1, local );
+        }
+#pragma unroll
+      for ( int q = 0; q < LA; ++q )
+        if ( l0 + q < ph.to )
+        {
+          const int lag = l0 + q;
+          if ( r == 0 )
+          {
+            V = P30 * y0I + P31ex * diex + P32ex * iex + P31in * diin + P32in * iin + expm1_tau_m * V + V;
+            V = ( V < LowerBound ? LowerBound : V );
+          }
+          else
+            --r;
+          iex = P21ex * diex + P11ex * iex;  // P22_ex = P11_ex
+          diex *= P11ex;
+          diex += EPSCInit * w_ex[ q ];
+          iin = P21in * diin + P11in * iin;
+          diin *= P11in;
+          diin += IPSCInit * w_in[ q ];
+          if ( V >= Theta )
+          {
+            r = ps.refractory_counts;
+            V = V_reset;
+            bits[ lag >> 6 ] |= 1ull << ( lag & 63 );
+          }
+          C.put( lag, 0, local, 0.0 );
+          C.put( lag, 1, local, 0.0 );
+          if ( lag == next_sample )
+          {
+            store_sample( d, mcol, ph.origin + lag + 1, V, iex, iin, 0.0 );
+            next_sample += d.mm_int;
+          }
+        }
+    }
+    glob( d.v[ 0 ] )[ dev ] = V;
+    glob( d.v[ 1 ] )[ dev ] = iex;
+    glob( d.v[ 2 ] )[ dev ] = iin;
+    glob( d.v[ 3 ] )[ dev ] = diex;
+    glob( d.v[ 4 ] )[ dev ] = diin;
+    glob( d.r )[ dev ] = r;
+  }
+};
+
+// models whose update needs the wide register budget
+#define NG_WIDE( ID ) | ( DeviceNode< ( ID ) >::WIDE ? ( 1 << ( ID ) ) : 0 )
+constexpr int WIDE_MODELS = 0 NG_DEVICE_MODELS( NG_WIDE );
+#undef NG_WIDE
+
 // handle( SpikeEvent& ) of model m
 __device__ __forceinline__ void
 node_receive( int m, double w, uint32_t& ch, double& v )
 {
-  if ( m == M_IAF_PSC_DELTA )
-    DeviceNode< M_IAF_PSC_DELTA >::receive( w, ch, v );
-  else if ( m == M_IAF_PSC_EXP )
-    DeviceNode< M_IAF_PSC_EXP >::receive( w, ch, v );
-  else
-    DeviceNode< M_AEIF_COND_EXP >::receive( w, ch, v );
+  ch = 0;
+  v = w;
+#define NG_RECEIVE( ID )                           \
+  if ( m == ( ID ) )                               \
+  {                                                \
+    DeviceNode< ( ID ) >::receive( w, ch, v );     \
+    return;                                        \
+  }
+  NG_DEVICE_MODELS( NG_RECEIVE )
+#undef NG_RECEIVE
 }
 
 }  // namespace nestgpu

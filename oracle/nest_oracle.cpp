@@ -537,7 +537,8 @@ enum Model
 {
   IAF_PSC_DELTA = 0,
   IAF_PSC_EXP = 1,
-  AEIF_COND_EXP = 2
+  AEIF_COND_EXP = 2,
+  IAF_PSC_ALPHA = 3
 };
 
 struct Neuron
@@ -563,12 +564,14 @@ struct Neuron
   std::vector< std::pair< double, double > > hist;
   // state
   double y0 = 0, y3 = 0, refr_buf = 0;          // delta
-  double V = 0, i_ex = 0, i_in = 0, i0 = 0, i1 = 0;  // exp
+  double V = 0, i_ex = 0, i_in = 0, i0 = 0, i1 = 0;  // exp; alpha: V, I_ex, I_in
+  double di_ex = 0, di_in = 0;                       // alpha: dI_ex, dI_in
   double ay[ 4 ] = { 0, 0, 0, 0 };               // aeif V, g_ex, g_in, w
   double I_stim = 0, hstep = 0, integ = 0;  // aeif: B_.step_, B_.IntegrationStep_
   long r = 0;
   // propagators (pre_run_hook)
   double P33, P30, P11ex, P11in, P22, P21ex, P21in, P20, V_peak_eff;
+  double P31ex, P32ex, P31in, P32in, expm1_tau_m, EPSCInit, IPSCInit;  // alpha
   long RefractoryCounts;
   // ring buffers: ch 0..3 ([SYN_IN, SYN_EX, I0, I1] for exp; [spikes, currents] for delta;
   // [exc, inh, currents] for aeif), slot-major
@@ -593,6 +596,37 @@ iaf_propagator_p32( double tau_syn, double tau_m, double c_m, double h )
     return P32;
   const double exp_h_tau = std::exp( -h * inv_tau_m );
   return h * inv_c_m * exp_h_tau;
+}
+
+// IAFPropagatorAlpha::evaluate (libnestutil/iaf_propagator.cpp:94-116): (P31, P32), the
+// regular form above h_min_regular = 1e-7 tau_m^2 / |tau_m - tau_syn|, else the singular limits
+static void
+iaf_propagator_alpha( double tau_syn, double tau_m, double c_m, double h, double& P31, double& P32 )
+{
+  const double h_min_regular = 1e-7 * tau_m * tau_m / std::abs( tau_m - tau_syn );
+  const double beta = tau_syn * tau_m / ( tau_m - tau_syn );
+  const double gamma = beta / c_m;
+  const double inv_tau_syn = 1 / tau_syn;
+  const double inv_tau_m = 1 / tau_m;
+  const double inv_c_m = 1 / c_m;
+  const double inv_beta = ( tau_m - tau_syn ) / ( tau_syn * tau_m );
+  const double exp_h_tau_syn = std::exp( -h * inv_tau_syn );
+  const double expm1_h_tau = std::expm1( h * inv_beta );
+  P32 = gamma * exp_h_tau_syn * expm1_h_tau;
+  double exp_h_tau = std::nan( "" );
+  if ( not( std::isnormal( P32 ) and P32 > 0 ) )
+  {
+    exp_h_tau = std::exp( -h * inv_tau_m );
+    P32 = h * inv_c_m * exp_h_tau;
+  }
+  if ( h > h_min_regular )
+  {
+    P31 = gamma * exp_h_tau_syn * ( beta * expm1_h_tau - h );
+    return;
+  }
+  if ( std::isnan( exp_h_tau ) )
+    exp_h_tau = std::exp( -h * inv_tau_m );
+  P31 = 0.5 * h * h * inv_c_m * exp_h_tau;
 }
 
 // ----------------------------------------------------- portable exp/log (fdlibm)
@@ -965,7 +999,7 @@ struct Kernel
       n.V_th = get( "V_th" ) - n.E_L;
     else
       n.V_th -= delta_EL;
-    if ( n.model == IAF_PSC_DELTA )
+    if ( n.model == IAF_PSC_DELTA or n.model == IAF_PSC_ALPHA )
     {
       if ( has( "V_min" ) )
         n.V_min = get( "V_min" ) - n.E_L;
@@ -978,7 +1012,7 @@ struct Kernel
       n.c_m = get( "C_m" );
     if ( has( "tau_m" ) )
       n.tau_m = get( "tau_m" );
-    if ( n.model == IAF_PSC_EXP )
+    if ( n.model == IAF_PSC_EXP or n.model == IAF_PSC_ALPHA )
     {
       if ( has( "tau_syn_ex" ) )
         n.tau_ex = get( "tau_syn_ex" );
@@ -993,7 +1027,7 @@ struct Kernel
       throw std::runtime_error( "Capacitance must be > 0." );
     if ( n.t_ref < 0 )
       throw std::runtime_error( "Refractory time must not be negative." );
-    if ( n.tau_m <= 0 or ( n.model == IAF_PSC_EXP and ( n.tau_ex <= 0 or n.tau_in <= 0 ) ) )
+    if ( n.tau_m <= 0 or ( n.model != IAF_PSC_DELTA and ( n.tau_ex <= 0 or n.tau_in <= 0 ) ) )
       throw std::runtime_error( "Membrane and synapse time constants must be strictly positive." );
     if ( n.model == IAF_PSC_DELTA and has( "refractory_input" ) )
       n.refr_input = get( "refractory_input" ) != 0;
@@ -1034,7 +1068,7 @@ struct Kernel
       n.V_min = -std::numeric_limits< double >::max();
       n.V_reset = -70.0 - n.E_L;
     }
-    else if ( model == IAF_PSC_EXP )  // models/iaf_psc_exp.cpp:71-94
+    else if ( model == IAF_PSC_EXP or model == IAF_PSC_ALPHA )  // iaf_psc_exp.cpp:71-94, iaf_psc_alpha.cpp:68-80
     {
       n.tau_m = 10.0;
       n.c_m = 250.0;
@@ -1045,6 +1079,8 @@ struct Kernel
       n.V_reset = -70.0 - n.E_L;
       n.tau_ex = 2.0;
       n.tau_in = 2.0;
+      if ( model == IAF_PSC_ALPHA )
+        n.V_min = -std::numeric_limits< double >::infinity();
     }
     else  // models/aeif_cond_exp.cpp:132-151
     {
@@ -1312,6 +1348,21 @@ struct Kernel
         n.P21in = iaf_propagator_p32( n.tau_in, n.tau_m, n.c_m, h );
         n.P20 = n.tau_m / n.c_m * ( 1.0 - n.P22 );
       }
+      else if ( n.model == IAF_PSC_ALPHA )
+      {
+        // iaf_psc_alpha::pre_run_hook (iaf_psc_alpha.cpp:248-301)
+        n.P11ex = n.P22 = std::exp( -h / n.tau_ex );  // P22_ex = P11_ex
+        n.P11in = std::exp( -h / n.tau_in );
+        n.P33 = std::exp( -h / n.tau_m );
+        n.expm1_tau_m = std::expm1( -h / n.tau_m );
+        n.P30 = -n.tau_m / n.c_m * std::expm1( -h / n.tau_m );
+        n.P21ex = h * n.P11ex;
+        n.P21in = h * n.P11in;
+        iaf_propagator_alpha( n.tau_ex, n.tau_m, n.c_m, h, n.P31ex, n.P32ex );
+        iaf_propagator_alpha( n.tau_in, n.tau_m, n.c_m, h, n.P31in, n.P32in );
+        n.EPSCInit = 1.0 * 2.718281828459045 / n.tau_ex;  // numerics::e
+        n.IPSCInit = 1.0 * 2.718281828459045 / n.tau_in;
+      }
       else
         n.V_peak_eff = n.Delta_T > 0. ? n.V_peak : n.V_th;
     }
@@ -1389,8 +1440,9 @@ struct Kernel
     const size_t slot = static_cast< size_t >( abs_step % L ) * 4;
     if ( n.model == IAF_PSC_DELTA )
       n.rb[ slot + 0 ] += w * mult;
-    else if ( n.model == IAF_PSC_EXP )
+    else if ( n.model == IAF_PSC_EXP or n.model == IAF_PSC_ALPHA )
     {
+      // iaf_psc_exp.cpp:351-362, iaf_psc_alpha.cpp:373-383 (SYN_IN = 0, SYN_EX = 1)
       const double s = w * mult;
       n.rb[ slot + ( s > 0 ? 1 : 0 ) ] += s;
     }
@@ -1466,6 +1518,32 @@ struct Kernel
         n.i1 = n.rb[ slot + 3 ];
         n.rb[ slot + 0 ] = n.rb[ slot + 1 ] = n.rb[ slot + 2 ] = n.rb[ slot + 3 ] = 0.0;
       }
+      else if ( n.model == IAF_PSC_ALPHA )
+      {
+        // models/iaf_psc_alpha.cpp:305-370 (SYN_IN = 0, SYN_EX = 1, I0 = 2; i0 = y0)
+        if ( n.r == 0 )
+        {
+          n.V = n.P30 * ( n.i0 + n.I_e ) + n.P31ex * n.di_ex + n.P32ex * n.i_ex + n.P31in * n.di_in
+            + n.P32in * n.i_in + n.expm1_tau_m * n.V + n.V;
+          n.V = ( n.V < n.V_min ? n.V_min : n.V );
+        }
+        else
+          --n.r;
+        n.i_ex = n.P21ex * n.di_ex + n.P22 * n.i_ex;
+        n.di_ex *= n.P11ex;
+        n.di_ex += n.EPSCInit * n.rb[ slot + 1 ];
+        n.i_in = n.P21in * n.di_in + n.P11in * n.i_in;
+        n.di_in *= n.P11in;
+        n.di_in += n.IPSCInit * n.rb[ slot + 0 ];
+        if ( n.V >= n.V_th )
+        {
+          n.r = n.RefractoryCounts;
+          n.V = n.V_reset;
+          spiked = true;
+        }
+        n.i0 = n.rb[ slot + 2 ];
+        n.rb[ slot + 0 ] = n.rb[ slot + 1 ] = n.rb[ slot + 2 ] = n.rb[ slot + 3 ] = 0.0;
+      }
       else
       {
         // models/aeif_cond_exp.cpp:443-523
@@ -1511,7 +1589,7 @@ struct Kernel
       {
         if ( n.model == IAF_PSC_DELTA )
           n.samples.push_back( { static_cast< double >( T ), n.y3 + n.E_L, 0.0, 0.0, 0.0 } );
-        else if ( n.model == IAF_PSC_EXP )
+        else if ( n.model == IAF_PSC_EXP or n.model == IAF_PSC_ALPHA )
           n.samples.push_back( { static_cast< double >( T ), n.V + n.E_L, n.i_ex, n.i_in, 0.0 } );
         else
           n.samples.push_back( { static_cast< double >( T ), n.ay[ 0 ], n.ay[ 1 ], n.ay[ 2 ], n.ay[ 3 ] } );
@@ -1765,7 +1843,7 @@ struct Kernel
       if ( f == "refractory" )
         return static_cast< double >( n.r );
     }
-    else if ( n.model == IAF_PSC_EXP )
+    else if ( n.model == IAF_PSC_EXP or n.model == IAF_PSC_ALPHA )
     {
       if ( f == "V_m" )
         return n.V + n.E_L;

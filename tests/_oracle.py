@@ -15,7 +15,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-MODELS = {"iaf_psc_delta": 0, "iaf_psc_exp": 1, "aeif_cond_exp": 2}
+MODELS = {"iaf_psc_delta": 0, "iaf_psc_exp": 1, "aeif_cond_exp": 2, "iaf_psc_alpha": 3}
 RULES = {"one_to_one": 0, "all_to_all": 1, "fixed_indegree": 2, "pairwise_bernoulli": 3}
 
 _lib = None

@@ -708,15 +708,35 @@ def test_model_registry_status():
     # ModelManager::get_status (model_manager.cpp:116-131): the node and synapse models
     k = nestgpu.Kernel(0)
     st = k.GetKernelStatus()
-    assert st["node_models"][:3] == ["iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp"]
-    assert set(st["node_models"][3:]) == {"spike_generator", "spike_recorder", "multimeter", "voltmeter"}
+    assert st["node_models"][:4] == ["iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp", "iaf_psc_alpha"]
+    assert set(st["node_models"][4:]) == {"spike_generator", "spike_recorder", "multimeter", "voltmeter"}
     assert st["synapse_models"] == ["static_synapse", "stdp_synapse"]
     # every registered neuron model is created through the registry, with its own defaults
     # and keys
     for name, key, val in (("iaf_psc_delta", "V_min", -1.7976931348623157e308), ("iaf_psc_exp", "tau_syn_ex", 2.0),
-                           ("aeif_cond_exp", "gsl_error_tol", 1e-6)):
+                           ("aeif_cond_exp", "gsl_error_tol", 1e-6), ("iaf_psc_alpha", "V_min", -np.inf)):
         n = k.Create(name, 2)
         assert np.all(np.atleast_1d(n.get(key)) == val)
         assert np.all(np.atleast_1d(n.get("tau_minus")) == 20.0)
     with pytest.raises(nestgpu.NESTError, match="UnknownModelName|unknown"):
         k.Create("hh_psc_alpha", 1)
+
+
+def test_iaf_psc_alpha_parameters():
+    # iaf_psc_alpha::Parameters_::set and State_::set (iaf_psc_alpha.cpp:112-199): a change of
+    # E_L keeps the absolute V_th, V_reset and V_m (stored relative to E_L), and the
+    # reference's checks and messages
+    k = nestgpu.Kernel(0)
+    n = k.Create("iaf_psc_alpha", 2, {"V_m": -65.0, "tau_syn_ex": 0.5})
+    assert np.all(n.get("V_th") == -55.0) and np.all(n.get("V_reset") == -70.0) and np.all(n.get("V_m") == -65.0)
+    assert np.all(n.get("tau_syn_ex") == 0.5) and np.all(n.get("tau_syn_in") == 2.0)
+    assert np.all(n.get("I_syn_ex") == 0.0) and np.all(n.get("I_syn_in") == 0.0)
+    n.set(E_L=-60.0)
+    assert np.all(n.get("V_th") == -55.0) and np.all(n.get("V_m") == -65.0) and np.all(n.get("V_min") == -np.inf)
+    assert np.all(n.get("E_L") == -60.0)
+    for bad, msg in (({"C_m": 0.0}, "Capacitance must be > 0"), ({"tau_m": -1.0}, "Membrane time constant"),
+                     ({"tau_syn_in": 0.0}, "synaptic time constants"), ({"t_ref": -0.1}, "t_ref can't be negative"),
+                     ({"V_reset": -40.0}, "Reset potential"), ({"g_ex": 1.0}, "UnaccessedDictionaryEntry")):
+        with pytest.raises(nestgpu.NESTError, match=msg):
+            n.set(**bad)
+    assert np.all(n.get("C_m") == 250.0)

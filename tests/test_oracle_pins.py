@@ -596,6 +596,66 @@ def test_iaf_psc_exp_single_psc_closed_form(w):
         assert o.get(n, other)[0] == 0.0
 
 
+@pytest.mark.parametrize("w", [3.5, -2.25])
+def test_iaf_psc_alpha_single_psc_closed_form(w):
+    # One input of weight w read in the update of step k: after that step's V update and
+    # current propagation, dI += w e / tau_s (iaf_psc_alpha.cpp:329-346), so n steps later
+    # I_syn = w (t / tau_s) e^(1 - t / tau_s) and, with a = w e / tau_s, k = 1/tau_s - 1/tau_m,
+    # V_m - E_L = a / C_m e^(-t/tau_m) (1 - e^(-k t) (1 + k t)) / k^2, t = n h: the exact
+    # solution of dV/dt = -V/tau_m + I/C_m. Fails for the channels swapped, a missing factor e,
+    # the input added before the propagation, or P31 and P32 exchanged.
+    tau_m, tau_s, C, E_L, h = 10.0, 2.0, 250.0, -70.0, 0.1
+    o = Oracle()
+    n = o.create("iaf_psc_alpha", 1, {"tau_m": tau_m, "tau_syn_ex": tau_s, "tau_syn_in": tau_s, "C_m": C,
+                                       "E_L": E_L, "V_m": E_L, "V_th": 1e9})
+    counts = np.zeros((10, 1), np.uint8)
+    counts[4, 0] = 1  # emission step 4, delay 1 step: read in the update of step 5
+    o.add_drive(n[0], counts, w, 0.1)
+    o.run_steps(6)  # through step 5
+    ch, other = ("I_syn_ex", "I_syn_in") if w > 0 else ("I_syn_in", "I_syn_ex")
+    assert o.get(n, ch)[0] == 0.0 and o.get(n, "V_m")[0] == E_L
+    a = w * np.e / tau_s
+    k = 1.0 / tau_s - 1.0 / tau_m
+    peak = 0.0
+    for step in range(1, 150):
+        o.run_steps(1)
+        t = step * h
+        i_exp = w * (t / tau_s) * np.exp(1.0 - t / tau_s)
+        v_exp = a / C * np.exp(-t / tau_m) * (1.0 - np.exp(-k * t) * (1.0 + k * t)) / k ** 2
+        assert abs(o.get(n, ch)[0] - i_exp) <= 1e-12 * abs(w)
+        assert abs(o.get(n, "V_m")[0] - E_L - v_exp) <= 1e-11 * (abs(v_exp) + 1e-3)
+        assert o.get(n, other)[0] == 0.0
+        peak = max(peak, abs(o.get(n, ch)[0]))
+    assert peak == pytest.approx(abs(w), rel=1e-12)  # the current peaks at w, at t = tau_s
+
+
+def test_iaf_psc_alpha_dc_and_equal_time_constants():
+    # I_e alone: V_m - E_L = I_e tau_m / C_m (1 - e^(-t/tau_m)) from the first step; with
+    # tau_s = tau_m the propagators take their singular limits and one input still gives the
+    # exact solution V = a / C t^2 / 2 e^(-t/tau_m) (iaf_propagator.cpp:94-116)
+    tau_m, C, E_L, h, I_e = 10.0, 250.0, -70.0, 0.1, 300.0
+    o = Oracle()
+    n = o.create("iaf_psc_alpha", 1, {"tau_m": tau_m, "C_m": C, "E_L": E_L, "V_m": E_L, "V_th": 1e9, "I_e": I_e})
+    for step in range(1, 40):
+        o.run_steps(1)
+        v_exp = I_e * tau_m / C * (1 - np.exp(-step * h / tau_m))
+        assert abs(o.get(n, "V_m")[0] - E_L - v_exp) <= 1e-11 * v_exp
+    w = 4.0
+    o2 = Oracle()
+    n2 = o2.create("iaf_psc_alpha", 1, {"tau_m": tau_m, "tau_syn_ex": tau_m, "C_m": C, "E_L": E_L, "V_m": E_L,
+                                         "V_th": 1e9})
+    counts = np.zeros((3, 1), np.uint8)
+    counts[0, 0] = 1
+    o2.add_drive(n2[0], counts, w, 0.1)
+    o2.run_steps(2)
+    a = w * np.e / tau_m
+    for step in range(1, 60):
+        o2.run_steps(1)
+        t = step * h
+        v_exp = a / C * t * t / 2 * np.exp(-t / tau_m)
+        assert abs(o2.get(n2, "V_m")[0] - E_L - v_exp) <= 1e-10 * v_exp
+
+
 # ------------------------------------------------------------------ receive-order summation (a11)
 B = 2.0 ** 53
 
