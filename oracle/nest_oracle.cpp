@@ -30,7 +30,8 @@
 //                      nestkernel/archiving_node.cpp:38-48,85-113,165-230 (get_K_value,
 //                      get_history, set_spiketime); connection_manager.cpp:78 (stdp_eps)
 //   models ........... models/iaf_psc_delta.cpp:69-343; models/iaf_psc_exp.cpp:71-363;
-//                      libnestutil/iaf_propagator.cpp:42-74; models/aeif_cond_exp.cpp:79-541
+//                      libnestutil/iaf_propagator.cpp:42-116; models/aeif_cond_exp.cpp:79-541;
+//                      models/iaf_psc_alpha.cpp:68-383
 //   drive ............ models/spike_generator.cpp:321-389
 //
 // aeif_cond_exp: NEST integrates it with GSL's rkf45 stepper, the y' error control and
