@@ -128,3 +128,31 @@ def test_alpha_multimeter():
     assert np.allclose(ev["times"], st * 0.1, rtol=0, atol=1e-9)
     for j, name in enumerate(FIELDS):
         assert np.array_equal(ev[name], f[:, j]), name
+
+
+@pytest.mark.parametrize("res,I_e", [(0.1, 1000.0), (0.1, 1450.0), (0.2, 1450.0), (0.5, 1450.0), (1.0, 1450.0)])
+def test_alpha_reference_traces(res, I_e):
+    # testsuite/pytests/neurons/test_iaf_psc_alpha.py (test_iaf_psc_alpha_i0 and
+    # test_iaf_psc_alpha_i0_refractory): the device's voltmeter trace against the reference's
+    # values (rtol 1e-6, as there) and its spike time
+    import test_oracle_pins as pins
+
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"resolution": res})
+    n = k.Create("iaf_psc_alpha", params={"I_e": I_e})
+    vm = k.Create("voltmeter", params={"interval": res})
+    sr = k.Create("spike_recorder")
+    k.Connect(vm, n)
+    k.Connect(n, sr)
+    k.Simulate(8.0)
+    ev = vm.get("events")
+    trace = dict(zip(np.round(ev["times"], 10), ev["V_m"]))
+    expected = pins.ALPHA_I0 if I_e == 1000.0 else pins.alpha_i0_refractory_expected()
+    checked = 0
+    for t, v in expected:
+        kk = round(t / res)
+        if abs(kk * res - t) < 1e-9:
+            assert trace[round(kk * res, 10)] == pytest.approx(v, rel=1e-6), t
+            checked += 1
+    assert checked >= 6
+    assert list(np.round(sr.get("events")["times"], 10))[:1] == [4.8 if I_e == 1000.0 else 3.0]

@@ -230,3 +230,37 @@ def test_stdp_circuit_scaled():
     w1 = _check(k, pops, sr, o, opops)
     print("stdp weights changed: %d of %d" % (np.count_nonzero(w1 != w0), n_stdp), flush=True)
     assert np.count_nonzero(w1 != w0) > n_stdp // 10
+
+
+@pytest.mark.parametrize("delay", [0.1, 1.0, 2.5])
+def test_stdp_reference_trains(delay):
+    # testsuite/pytests/synapses/test_stdp_synapse.py's hard-coded pre/post orders (ties
+    # included) and Poisson tails through relays: the device's final weight equals the
+    # oracle's (portable exp on both sides) and the event-driven reproduction
+    import test_oracle_pins as pins
+
+    T, h = 300.0, 0.1
+    counts = pins.stdp_relay_counts(T, h)
+    k = nestgpu.Kernel(0)
+    pre, post = pins.stdp_relays(k, False, delay)
+    sg = k.Create("spike_generator", 2)
+    k.SetSpikeCounts(sg, counts)
+    k.Connect(sg, pre + post, "one_to_one", {"weight": 1e4, "delay": 0.1})
+    sr = k.Create("spike_recorder")
+    k.Connect(pre + post, sr)
+    k.Simulate(T)
+    o = Oracle(resolution=h, portable_math=True)
+    opre, opost = pins.stdp_relays(o, True, delay)
+    o.add_drive(opre[0], counts, 1e4, 0.1)
+    o.simulate(T)
+    ev = sr.get("events")
+    og, os_ = nb.sorted_spikes_oracle(o)
+    assert np.array_equal(ev["senders"], og) and np.allclose(ev["times"], os_ * h, rtol=0, atol=1e-9)
+    w_dev = k.GetConnections(source=pre, target=post)["weight"][0]
+    w_orc = o.connections()[2][0]
+    t_pre = os_[og == opre[0]] * h
+    t_post = os_[og == opost[0]] * h
+    w_rep = pins._stdp_reproduce(t_pre, t_post + delay, 0.5, 15.0, 0.01, 0.85, 16.8, 33.7, 0.0, 0.0)
+    print("delay %g: device %r oracle %r reproduction %r" % (delay, w_dev, w_orc, w_rep), flush=True)
+    assert w_dev == w_orc
+    assert w_dev == pytest.approx(w_rep, abs=1e-12)

@@ -656,6 +656,51 @@ def test_iaf_psc_alpha_dc_and_equal_time_constants():
         assert abs(o2.get(n2, "V_m")[0] - E_L - v_exp) <= 1e-10 * v_exp
 
 
+# testsuite/pytests/neurons/test_iaf_psc_alpha.py: the reference's V_m traces (rtol 1e-6)
+ALPHA_I0 = [(0.1, -69.602), (0.2, -69.2079), (0.3, -68.8178), (0.4, -68.4316), (0.5, -68.0492), (4.3, -56.0204),
+            (4.4, -55.7615), (4.5, -55.5051), (4.6, -55.2513), (4.7, -55.0001), (4.8, -70), (4.9, -70), (5.0, -70)]
+ALPHA_I0_REFR = [-69.4229, -68.8515, -68.2858, -67.7258, -67.1713, -66.6223, -66.0788, -65.5407, -65.008, -64.4806,
+                 -63.9584, -63.4414, -62.9295, -62.4228, -61.9211, -61.4243, -60.9326, -60.4457, -59.9636, -59.4864,
+                 -59.0139, -58.5461, -58.0829, -57.6244, -57.1704, -56.721, -56.276, -55.8355, -55.3993] + [-70.0] * 21 + [
+                 -69.4229, -68.8515, -68.2858, -67.7258, -67.1713, -66.6223, -66.0788, -65.5407, -65.008, -64.4806,
+                 -63.9584, -63.4414, -62.9295, -62.4228, -61.9211, -61.4243, -60.9326, -60.4457, -59.9636]
+
+
+def alpha_i0_refractory_expected():
+    """(time, V_m) of test_iaf_psc_alpha_i0_refractory, 0.1 ms to 6.9 ms."""
+    return [(round(0.1 * (i + 1), 10), v) for i, v in enumerate(ALPHA_I0_REFR)]
+
+
+def _alpha_trace(res, I_e, T=8.0):
+    o = Oracle(resolution=res)
+    n = o.create("iaf_psc_alpha", 1, {"I_e": I_e})
+    o.set_sampling(n, 1, 0)
+    o.simulate(T)
+    g, st, f = o.samples()
+    g2, s2 = o.spikes()
+    return dict(zip(np.round(st * res, 10), f[:, 0])), s2 * res
+
+
+def test_iaf_psc_alpha_reference_trace_i0():
+    trace, spikes = _alpha_trace(0.1, 1000.0)
+    for t, v in ALPHA_I0:
+        assert trace[round(t, 10)] == pytest.approx(v, rel=1e-6), t
+    assert list(np.round(spikes, 10)) == [4.8]
+
+
+@pytest.mark.parametrize("res", [0.1, 0.2, 0.5, 1.0])
+def test_iaf_psc_alpha_reference_trace_i0_refractory(res):
+    trace, spikes = _alpha_trace(res, 1450.0)
+    n = 0
+    for t, v in alpha_i0_refractory_expected():
+        k = round(t / res)
+        if abs(k * res - t) < 1e-9:
+            assert trace[round(k * res, 10)] == pytest.approx(v, rel=1e-6), (res, t)
+            n += 1
+    assert n >= 6
+    assert np.round(spikes[0], 10) == 3.0
+
+
 # ------------------------------------------------------------------ receive-order summation (a11)
 B = 2.0 ** 53
 
@@ -903,6 +948,65 @@ def _stdp_reproduce(pre, post, w0, Wmax, lam, alpha, tau_plus, tau_minus, mu_plu
             i += 1
             last_post, post_t = post_t, (post[i] if i < len(post) else np.inf)
     return w * Wmax
+
+
+# the spike trains of testsuite/pytests/synapses/test_stdp_synapse.py:76-84: every order of pre
+# and post spikes, ties included, then Poisson trains
+STDP_PRE_MS = [1, 5, 6, 7, 9, 11, 12, 13, 14.5, 16.1, 21, 25, 26, 27, 29, 31, 32, 33, 34.5, 36.1]
+STDP_POST_MS = [2, 3, 4, 8, 9, 10, 12, 12.2, 14.1, 15.4, 22, 23, 24, 28, 29, 30, 32, 33.2, 35.1, 36.4]
+# its synapse: mu 0 (additive), tau_plus 16.8, tau_minus 33.7, Wmax 15, initial weight 0.5
+STDP_REF_PRM = {"tau_plus": 16.8, "lambda": 0.01, "alpha": 0.85, "mu_plus": 0.0, "mu_minus": 0.0, "Wmax": 15.0}
+
+
+def stdp_relay_counts(T=300.0, h=0.1, seed=11):
+    """Dense drive counts that make two relays fire at the hard-coded times (shifted by the
+    relay latency), then at Poisson times (about 50 Hz) up to T - 10 ms."""
+    steps = int(round(T / h))
+    counts = np.zeros((steps, 2), np.uint8)
+    rng = np.random.default_rng(seed)
+    for col, times in ((0, STDP_PRE_MS), (1, STDP_POST_MS)):
+        for t in times:
+            counts[int(round(t / h)), col] = 1
+        tail = np.flatnonzero(rng.random(steps) < 0.005)
+        tail = tail[(tail * h > 40.0) & (tail * h < T - 10.0)]
+        counts[tail, col] = 1
+    return counts
+
+
+def stdp_relays(o_or_k, oracle, delay):
+    """Two iaf_psc_delta relays that fire once per drive count (t_ref 0, a 1e4 mV kick; the
+    postsynaptic relay's threshold is far above what the stdp synapse can add) joined by one
+    stdp_synapse of the reference test's parameters."""
+    relay = {"E_L": 0.0, "V_m": 0.0, "V_reset": 0.0, "t_ref": 0.0, "tau_m": 10.0, "C_m": 250.0, "tau_minus": 33.7}
+    if oracle:
+        pre = o_or_k.create("iaf_psc_delta", 1, dict(relay, V_th=20.0))
+        post = o_or_k.create("iaf_psc_delta", 1, dict(relay, V_th=1000.0))
+        o_or_k.connect(pre, post, "one_to_one", weight=0.5, delay=delay, stdp=STDP_REF_PRM)
+        return pre, post
+    pre = o_or_k.Create("iaf_psc_delta", 1, dict(relay, V_th=20.0))
+    post = o_or_k.Create("iaf_psc_delta", 1, dict(relay, V_th=1000.0))
+    o_or_k.Connect(pre, post, "one_to_one", dict(STDP_REF_PRM, synapse_model="stdp_synapse", weight=0.5,
+                                                 delay=delay))
+    return pre, post
+
+
+@pytest.mark.parametrize("delay", [0.1, 1.0, 2.5])
+def test_stdp_synapse_reference_trains(delay):
+    # the reference test's hard-coded pre/post orders and ties, through relays, against the
+    # event-driven reproduction of the final weight
+    T, h = 300.0, 0.1
+    o = Oracle(resolution=h)
+    pre, post = stdp_relays(o, True, delay)
+    o.add_drive(pre[0], stdp_relay_counts(T, h), 1e4, 0.1)
+    o.simulate(T)
+    g, s = o.spikes()
+    t_pre, t_post = s[g == pre[0]] * h, s[g == post[0]] * h
+    # spike times follow the drive one step later, with the reference test's ties intact
+    assert np.allclose(t_pre[:20], np.array(STDP_PRE_MS) + 0.2) and np.allclose(t_post[:20], np.array(STDP_POST_MS) + 0.2)
+    w_sim = o.connections()[2][0]
+    w_rep = _stdp_reproduce(t_pre, t_post + delay, 0.5, 15.0, 0.01, 0.85, 16.8, 33.7, 0.0, 0.0)
+    assert w_sim != 0.5
+    assert w_sim == pytest.approx(w_rep, abs=1e-12), (w_sim, w_rep)
 
 
 @pytest.mark.parametrize("delay,mu", [(1.0, (1.0, 1.0)), (2.3, (1.0, 1.0)), (1.0, (0.0, 1.0)), (1.5, (1.0, 0.0))])
