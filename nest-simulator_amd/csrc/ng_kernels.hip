@@ -456,8 +456,8 @@ __device__ unsigned long long g_phase_ticks[ 8 ];
   } while ( 0 )
 #endif
 
-// Two workgroups of 8 waves per CU need at most 128 VGPRs (4 waves per SIMD). The aeif
-// integrator needs more; its kernels run one workgroup per CU.
+// Two workgroups of 8 waves per CU need at most 128 VGPRs (4 waves per SIMD); instantiations
+// with a WIDE model run one workgroup per CU instead.
 // ---------------------------------------------------------------- stdp_synapse
 // stdp_synapse::send (models/stdp_synapse.h:222-283) for the event of record ridx, emitted at
 // step stamp `stamp` in a slice of parity par, to device target tdev over a delay of dsteps:

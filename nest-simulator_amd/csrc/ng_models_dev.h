@@ -21,8 +21,9 @@ template < int ID >
 struct DeviceNode;
 
 // The device model ids, in the order ModelManager registers them: a model with a DeviceNode
-// specialization joins the slice kernel by its entry here. WIDE marks models whose update needs
-// more than 128 VGPRs (their launches run one workgroup per CU).
+// specialization joins the slice kernel by its entry here. WIDE marks models whose launches
+// run one workgroup per CU with up to 256 VGPRs; the others are held to 128 VGPRs, so that
+// two workgroups share a CU, spilling what does not fit.
 #define NG_DEVICE_MODELS( X ) X( M_IAF_PSC_DELTA ) X( M_IAF_PSC_EXP ) X( M_AEIF_COND_EXP ) X( M_IAF_PSC_ALPHA )
 constexpr int N_DEVICE_MODELS = 4;
 constexpr int ALL_MODELS = ( 1 << N_DEVICE_MODELS ) - 1;
@@ -184,7 +185,10 @@ struct DeviceNode< M_IAF_PSC_EXP >
 template <>
 struct DeviceNode< M_AEIF_COND_EXP >
 {
-  static constexpr bool WIDE = true;  // the RKF45 stepper's state and coefficients
+  // The RKF45 stepper's state and coefficients want about 220 VGPRs. Held to 128 it spills
+  // about 95 of them to scratch, and two workgroups per CU still run the batched sweep (half
+  // its instances aeif_cond_exp) 37 % faster than one workgroup without spills.
+  static constexpr bool WIDE = false;
   // aeif_cond_exp::handle( SpikeEvent& ) (aeif_cond_exp.cpp:526-541): positive weights to the
   // excitatory conductance (0), others negated to the inhibitory one (1)
   __device__ __forceinline__ static void
