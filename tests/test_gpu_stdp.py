@@ -264,3 +264,44 @@ def test_stdp_reference_trains(delay):
     print("delay %g: device %r oracle %r reproduction %r" % (delay, w_dev, w_orc, w_rep), flush=True)
     assert w_dev == w_orc
     assert w_dev == pytest.approx(w_rep, abs=1e-12)
+
+
+def test_stdp_circuit_full_size():
+    # the benchmark circuit itself (77,169 neurons, about 3.2e8 synapses, 8 VPs) with every
+    # connection between excitatory populations plastic, over 20 ms: spikes and the neurons'
+    # state bit-identical to the oracle (the weights act on the state from the first plastic
+    # event on; reading all 3.2e8 back is left to the scaled test)
+    import os
+    import sys
+    import time
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    T, n_vp = 20.0, 8
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    k = nestgpu.Kernel(0)
+    k.SetKernelStatus({"keep_host_connections": False})
+    pops, n, _ = bench.build_c3(k, nestgpu, n_vp, 20241, 777, stdp=bench.C3_STDP)
+    sr = k.Create("spike_recorder")
+    allp = pops[0]
+    for p in pops[1:]:
+        allp = allp + p
+    k.Connect(allp, sr)
+    k.Simulate(T)
+    print("device: %d connections, %.1f s" % (k.GetKernelStatus("num_connections"), time.time() - t0), flush=True)
+    t0 = time.time()
+    o = Oracle(n_vp=n_vp, seed=20241, threads=threads, portable_math=True)
+    opops = bench.build_c3_oracle(o, nestgpu, 777, int(round(T / 0.1)), stdp=bench.C3_STDP)
+    o.simulate(T)
+    print("oracle: %.1f s" % (time.time() - t0), flush=True)
+    ev = sr.get("events")
+    og, os_ = nb.sorted_spikes_oracle(o)
+    print("spikes gpu=%d oracle=%d" % (len(ev["senders"]), len(og)), flush=True)
+    assert len(og) > 1000
+    assert np.array_equal(ev["senders"], og)
+    assert np.allclose(ev["times"], os_ * 0.1, rtol=0, atol=1e-9)
+    for p, op in zip(pops, opops):
+        for f in FIELDS:
+            a, b = np.atleast_1d(p.get(f)), o.get(op, f)
+            assert np.array_equal(a, b), "%s differs: max abs diff %g" % (f, np.max(np.abs(a - b)))
