@@ -156,3 +156,26 @@ def test_alpha_reference_traces(res, I_e):
             checked += 1
     assert checked >= 6
     assert list(np.round(sr.get("events")["times"], 10))[:1] == [4.8 if I_e == 1000.0 else 3.0]
+
+
+@pytest.mark.parametrize("model", ["iaf_psc_delta", "iaf_psc_exp", "aeif_cond_exp", "iaf_psc_alpha"])
+def test_spike_multiplicity(model):
+    # testsuite/pytests/neurons/test_neurons_handle_multiplicity.py:128-180 checks that a spike
+    # of multiplicity 2 acts like two spikes. A neuron takes one spike_generator here, and one
+    # generator's spike times must differ, so the restatement compares a spike of multiplicity
+    # 2 and weight 1 with one spike of weight 2 (handle( SpikeEvent& ) adds weight *
+    # multiplicity): equal V_m, changed by the input
+    k = nestgpu.Kernel(0)
+    n1 = k.Create(model)
+    n2 = k.Create(model)
+    g1 = k.Create("spike_generator", params={"spike_times": [5.0], "spike_multiplicities": [2]})
+    g2 = k.Create("spike_generator", params={"spike_times": [5.0]})
+    k.Connect(g1, n1, syn_spec={"weight": 1.0})
+    k.Connect(g2, n2, syn_spec={"weight": 2.0})
+    v0 = (float(np.atleast_1d(n1.get("V_m"))[0]), float(np.atleast_1d(n2.get("V_m"))[0]))
+    assert v0[0] == v0[1]
+    k.Simulate(8.0)
+    v1, v2 = float(np.atleast_1d(n1.get("V_m"))[0]), float(np.atleast_1d(n2.get("V_m"))[0])
+    print("%s: V_m %r -> %r, %r" % (model, v0[0], v1, v2), flush=True)
+    assert v1 == v2
+    assert v1 != v0[0]
