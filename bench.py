@@ -691,6 +691,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="c3: skip the extra batched-sweep (c4) measurement")
     ap.add_argument("--stdp", action="store_true",
                     help="c3: connections between excitatory populations are stdp_synapse")
+    ap.add_argument("--no-c3-stdp", action="store_true",
+                    help="c3: skip the extra measurement of the circuit with stdp_synapse between excitatory populations")
     ap.add_argument("--stdp-history", type=int, default=256, help="c3 --stdp: post-spike history entries per neuron")
     ap.add_argument("--c2-steps", type=int, default=5, help="c3: timed steps of the extra c2 measurement")
     ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"],
@@ -783,6 +785,13 @@ def main():
         line["c2"] = {k: c2[k] for k in ("value", "unit", "steps", "ms_per_step", "rtf", "config")}
         line["c2"]["roofline"] = {k: c2["roofline"][k] for k in ("achieved", "frac", "avg_launch_us", "traffic",
                                                                  "delivery")}
+    if args.workload == "c3" and not args.stdp and not args.no_c3_stdp:
+        # the same circuit with every connection between excitatory populations plastic
+        a3 = argparse.Namespace(**vars(args))
+        a3.stdp = True
+        c3s = run_plain(a3, nestgpu, "c3", 1, 1, False)
+        line["c3_stdp"] = {k: c3s[k] for k in ("value", "unit", "steps", "ms_per_step", "rtf", "config")}
+        line["c3_stdp"]["avg_launch_us"] = c3s["roofline"]["avg_launch_us"]
     if args.workload == "c3" and not args.no_c4:
         # the batched sweep, autotuned, at 100 ms steps: its batched / plain ratio
         a4 = argparse.Namespace(**vars(args))
