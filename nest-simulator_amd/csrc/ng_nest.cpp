@@ -2205,6 +2205,18 @@ Kernel::release_device()
     }
     released_events_ = engine_->read_counter( 1 );
     released_spikes_ = engine_->read_counter( 2 );
+    // stdp_synapse weights live on the device: keep them in the host connection table
+    if ( not stdp_sets_.empty() and simulated_ and not host_connections_freed_
+         and syn_pos_.size() == static_cast< size_t >( num_connections() - static_cast< int64_t >( dev_conns_.size() ) ) )
+    {
+      std::vector< double > dev_w;
+      engine_->read_weights( dev_w );
+      int64_t k = 0;
+      for ( auto& C : conns_ )
+        for ( size_t i = 0; i < C.size(); ++i, ++k )
+          if ( C.syn_at( i ) )
+            C.w[ i ] = dev_w[ syn_pos_[ k ] ];
+    }
   }
   engine_.reset();
   prepared_ = false;

@@ -146,3 +146,44 @@ def test_http_sessions_through_served_form_equal_plain():
         assert np.array_equal(np.asarray(ev["senders"]), e2["senders"])
         assert np.array_equal(np.asarray(ev["times"]), e2["times"])
         assert np.array_equal(np.asarray(vm), pop.get("V_m"))
+
+
+def test_served_plastic_and_alpha_requests():
+    # requests with stdp_synapse, with iaf_psc_alpha and with static iaf_psc_exp networks share
+    # the served launches (the stdp instantiation then runs every request; static ones skip
+    # the plastic code): each result, plastic weights included, equals its run alone
+    import netbuild as nb
+    from test_gpu_models import alpha_random
+    from test_gpu_stdp import stdp_net
+
+    descs = [stdp_net(N=600, seed=51, drive_seed=3, T=60.0), nb.exp_random(N=700, T=60.0, seed=6, n_vp=2),
+             alpha_random(N=700, T=60.0, seed=9, n_vp=2), stdp_net(N=900, seed=52, drive_seed=4, mu=(0.0, 1.0),
+                                                                  T=60.0)]
+    calls = [[20.0, 40.0], [60.0], [33.3, 26.7], [10.0, 50.0]]
+
+    def weights(k, pops):
+        c = k.GetConnections()
+        ids = np.concatenate([np.atleast_1d(p.global_id) for p in pops])
+        keep = np.isin(c["source"], ids) & np.isin(c["target"], ids)
+        return c["weight"][keep]
+
+    built = [nb.build_gpu(d) for d in descs]
+    srv = nestgpu.Server(0, slices_per_chunk=8)
+    ids = [srv.submit(k, c) for (k, _, _), c in zip(built, calls)]
+    infos = [srv.wait(r, timeout=120.0) for r in ids]
+    stats = srv.stats()
+    print("server", stats, flush=True)
+    for i, (d, c) in enumerate(zip(descs, calls)):
+        assert infos[i]["state"] == nestgpu.Server.DONE, infos[i]
+        k, pops, sr = nb.build_gpu(d)
+        for t in c:
+            k.Simulate(t)
+        ks, ps, srs = built[i]
+        a = (srs.get("events")["senders"], srs.get("events")["times"], np.atleast_1d(ps[0].get("V_m")),
+             weights(ks, ps))
+        b = (sr.get("events")["senders"], sr.get("events")["times"], np.atleast_1d(pops[0].get("V_m")),
+             weights(k, pops))
+        print(d["pops"][0][0], "spikes", len(a[0]), flush=True)
+        assert len(a[0]) > 0
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), "request %d differs from its run alone" % i
+    srv.close()
