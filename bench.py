@@ -668,6 +668,37 @@ def summarize_pmc(fetch, write, copy_fetch, copy_write, skip, count, kernel="k_s
     return "\n".join(out)
 
 
+def summarize_counters(paths, skip, count, kernel="k_slice"):
+    """Mean per dispatch of every counter in rocprofv3 --pmc CSVs over the named kernel's
+    dispatches [skip, skip + count), and the ratios the SQ counters give: the share of wave
+    cycles spent waiting and issuing, instructions per wave by kind, LDS bank-conflict cycles
+    per LDS-active cycle."""
+    vals = {}
+    n_sel = {}
+    for path in paths:
+        names = sorted({r["Counter_Name"] for r in _csv_rows(path)})
+        for name in names:
+            v = _counter(path, name, kernel)
+            sel = v[skip:skip + count]
+            vals[name] = float(np.mean(sel))
+            n_sel[name] = (len(sel), len(v))
+    out = ["%s dispatches %d-%d" % (kernel, skip, skip + count - 1)]
+    for name in sorted(vals):
+        out.append("%-24s %16.1f   (%d of %d dispatches)" % (name, vals[name], n_sel[name][0], n_sel[name][1]))
+    g = vals.get
+    if g("SQ_WAVE_CYCLES"):
+        for name in ("SQ_WAIT_ANY", "SQ_ACTIVE_INST_ANY", "SQ_WAIT_INST_ANY"):
+            if name in vals:
+                out.append("%s / SQ_WAVE_CYCLES = %.3f" % (name, vals[name] / vals["SQ_WAVE_CYCLES"]))
+    if g("SQ_WAVES"):
+        for name in sorted(vals):
+            if name.startswith("SQ_INSTS_"):
+                out.append("%s per wave = %.1f" % (name, vals[name] / vals["SQ_WAVES"]))
+    if g("SQ_LDS_IDX_ACTIVE") and "SQ_LDS_BANK_CONFLICT" in vals:
+        out.append("SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE = %.3f" % (vals["SQ_LDS_BANK_CONFLICT"] / vals["SQ_LDS_IDX_ACTIVE"]))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -732,6 +763,9 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print the HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE CSVs of a "
                          "bench run and of --calibrate-copy (with --skip/--count)")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="COUNTER_CSV",
+                    help="print the mean per k_slice dispatch of every counter in rocprofv3 --pmc CSVs of a bench "
+                         "run, with the SQ ratios (with --skip/--count)")
     ap.add_argument("--skip", type=int, default=0, help="summaries: k_slice dispatches before the timed region")
     ap.add_argument("--count", type=int, default=1 << 30, help="summaries: k_slice dispatches of the timed region")
     args = ap.parse_args()
@@ -741,6 +775,9 @@ def main():
         return
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc, args.skip, args.count))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters, args.skip, args.count))
         return
 
     if args.calibrate_copy:
